@@ -1,0 +1,208 @@
+#!/usr/bin/env python
+"""Benchmark of the chi2 scan: zchi2 evaluations per second on one MI355X.
+
+Workload (the galaxy petal): 500 GALAXY targets x 1977 redshifts x 10 basis
+vectors over 3 cameras (7958 pixels), synthetic data (redrock_amd.synth).
+One eval = one (target, template, z) chi2 including its coefficient solve.
+A step is one full scan of all targets over all redshifts with every input
+already resident in HBM (template cache built once, as DistTemplate does).
+
+The headline value is the mixed-precision scan (fp16 MFMA, FP64 solve); the
+plain FP64 scan runs on the same inputs for the speedup and the best-z
+agreement.  The CPU baseline is the float64 oracle (a numpy restatement of
+the reference's CPU loop) on a bounded sample of the same targets, one
+process per core, run before the GPU is initialised.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--ntargets 500]
+"""
+import os
+
+for _v in ('OMP_NUM_THREADS', 'OPENBLAS_NUM_THREADS', 'MKL_NUM_THREADS'):
+    os.environ[_v] = '1'      # the CPU baseline runs one single-threaded process per core
+
+import argparse
+import json
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = 'zchi2 evals/sec on 1 GPU: mixed-precision vs FP64 plain at same best-z'
+# reference-form FLOPs per eval (R-apply, weights, Gram, y, solve, model, chi2)
+FLOP_PER_EVAL = {'GALAXY': 3763200, 'QSO': 1137348, 'STAR': 1495337}
+PEAK_FLOPS = {'f16': 2.5e15, 'bf16': 2.5e15, 'f64': 78.6e12}
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+_CPU = {}
+
+
+def _cpu_one(i):
+    from oracle import rr_oracle as orc
+    tm, targets, tdata = _CPU['tm'], _CPU['targets'], _CPU['tdata']
+    orc.calc_zchi2([targets[i]], tdata, tm.nbasis, tm.OIItemplate)
+    return i
+
+
+def cpu_baseline(tm, targets, dwave, seconds):
+    """Oracle scan on host cores: evals/s over a bounded sample of targets."""
+    import multiprocessing as mp
+    from oracle import rr_oracle as orc
+    ncores = min(16, len(os.sched_getaffinity(0)))
+    tdata = orc.template_cache(tm, dwave)
+    _CPU.update(tm=tm, targets=targets, tdata=tdata)
+    t0 = time.perf_counter()
+    _cpu_one(0)
+    one = time.perf_counter() - t0
+    nsample = max(ncores, min(len(targets), int(seconds * ncores / max(one, 1e-3))))
+    nsample = min(nsample, len(targets))
+    ctx = mp.get_context('fork')
+    with ctx.Pool(ncores) as pool:
+        t0 = time.perf_counter()
+        list(pool.imap_unordered(_cpu_one, range(nsample)))
+        wall = time.perf_counter() - t0
+    nz = len(tm.redshifts)
+    return dict(value=nsample * nz / wall, unit='evals/s', cores=ncores, kind='port',
+                sample=f'{nsample} of the workload targets x {nz} z, float64 oracle '
+                       f'(numpy CSR R-apply, normal equations, LAPACK gesv, explicit residual), '
+                       f'one process per core, {wall:.1f} s wall')
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument('--gpus', type=int, default=1)
+    ap.add_argument('--steps', type=int, default=10)
+    ap.add_argument('--warmup', type=int, default=2)
+    ap.add_argument('--ntargets', type=int, default=500)
+    ap.add_argument('--precision', default='f16', choices=('f16', 'bf16'))
+    ap.add_argument('--snr', type=float, default=1.0)
+    ap.add_argument('--seed', type=int, default=0)
+    ap.add_argument('--cpu-seconds', type=float, default=12.0)
+    ap.add_argument('--no-cpu', action='store_true')
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f'bench.py measures one GPU; --gpus {args.gpus} is not supported '
+              f'(targets shard across GPUs with no communication; see the design notes)',
+              file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    from redrock_amd import synth
+
+    tm = synth.galaxy_template()
+    targets, ztrue = synth.make_targets(tm, args.ntargets, seed=args.seed, snr=args.snr)
+    dwave = {s.wavehash: s.wave for s in targets[0].spectra}
+    nt, nz, nb = len(targets), len(tm.redshifts), tm.nbasis
+    evals = nt * nz
+    log(f'workload: {nt} targets x {nz} z x {nb} basis, npix={sum(len(w) for w in dwave.values())}')
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(tm, targets, dwave, args.cpu_seconds)
+        log(f'cpu baseline: {cpu["value"]:.0f} evals/s on {cpu["cores"]} cores')
+
+    import torch
+    from redrock_amd import zscan, _hip
+    from redrock_amd.templates import DistTemplate
+    assert torch.cuda.is_available(), 'bench.py needs a GPU'
+
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    dt = DistTemplate(tm, dwave)
+    batch = zscan.TargetBatch(targets)
+    oii = tm.OIItemplate.sum(axis=1)
+    items = (np.arange(nt), np.zeros(nt), np.full(nt, nz))
+    plans = {p: zscan.ScanPlan(dt.cache, batch, *items, precision=p, oiisum=oii)
+             for p in ('f64', args.precision)}
+    torch.cuda.synchronize()
+    setup_s = time.perf_counter() - t0
+    log(f'setup (template caches, target upload, packing): {setup_s:.2f} s')
+
+    def timed(plan, k):
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(k):
+            plan.run()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t) / k
+
+    def kernel_ms(plan, k):
+        """Average duration of the scan kernel alone (HIP events on its stream)."""
+        evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+               for _ in range(k)]
+        for e0, e1 in evs:
+            e0.record()
+            plan.run(stages=_hip.RR_STAGE_MAIN)
+            e1.record()
+        torch.cuda.synchronize()
+        return sum(e0.elapsed_time(e1) for e0, e1 in evs) / k
+
+    for p, plan in plans.items():
+        for _ in range(args.warmup):
+            plan.run()
+    torch.cuda.synchronize()
+    mix = plans[args.precision]
+    t_mix = timed(mix, args.steps)
+    k_mix = kernel_ms(mix, args.steps)
+    f64 = plans['f64']
+    t_f64 = timed(f64, max(1, args.steps // 2))
+    k_f64 = kernel_ms(f64, max(1, args.steps // 2))
+    # final full runs so the outputs hold complete results
+    mix.run()
+    f64.run()
+    torch.cuda.synchronize()
+
+    c_mix = mix.zchi2.reshape(nt, nz).cpu().numpy()
+    c_f64 = f64.zchi2.reshape(nt, nz).cpu().numpy()
+    bz_mix, bz_f64 = c_mix.argmin(axis=1), c_f64.argmin(axis=1)
+    agree = float(np.mean(bz_mix == bz_f64))
+    rel = np.abs(c_mix - c_f64) / np.abs(c_f64)
+    rel_best = rel[np.arange(nt), bz_f64]
+    zgrid = tm.redshifts
+    zerr_ok = float(np.mean(np.abs(zgrid[bz_f64] - ztrue) < 5e-3 * (1 + ztrue)))
+    log(f'{args.precision}: {t_mix*1e3:.2f} ms/step (kernel {k_mix:.2f} ms); '
+        f'f64: {t_f64*1e3:.2f} ms/step (kernel {k_f64:.2f} ms)')
+    log(f'best-z agreement {agree:.4f}; max rel chi2 err {rel.max():.2e}, at best z {rel_best.max():.2e}; '
+        f'f64 best z within 5e-3(1+z) of truth: {zerr_ok:.3f}')
+
+    flops = FLOP_PER_EVAL['GALAXY'] * evals
+    ach = flops / (k_mix * 1e-3)
+    out = {
+        'metric': METRIC,
+        'value': evals / t_mix,
+        'unit': 'evals/s',
+        'n_gpus': 1,
+        'steps': args.steps,
+        'warmup': args.warmup,
+        'ms_per_step': t_mix * 1e3,
+        'higher_is_better': True,
+        'scaling': 'weak',
+        'vs_baseline': None,
+        'dtype': args.precision,
+        'data': 'synthetic',
+        'config': {'workload': f'galaxy petal: {nt} GALAXY targets x {nz} z x {nb} basis, '
+                               f'3 cameras 7958 px, 11-diagonal R',
+                   'targets': nt, 'redshifts': nz, 'nbasis': nb, 'snr_per_pixel': args.snr,
+                   'solve': 'FP64 Cholesky', 'accumulate': 'fp32'},
+        'fp64': {'value': evals / t_f64, 'ms_per_step': t_f64 * 1e3, 'kernel_ms': k_f64,
+                 'roofline_frac': FLOP_PER_EVAL['GALAXY'] * evals / (k_f64 * 1e-3) / PEAK_FLOPS['f64']},
+        'speedup_vs_fp64': t_f64 / t_mix,
+        'best_z_agreement': agree,
+        'max_rel_chi2_err': float(rel.max()),
+        'max_rel_chi2_err_at_best_z': float(rel_best.max()),
+        'setup_s': setup_s,
+        'roofline': {'bound': 'mfma', 'achieved': ach / 1e12, 'peak': PEAK_FLOPS[args.precision] / 1e12,
+                     'unit': 'TFLOP/s', 'frac': ach / PEAK_FLOPS[args.precision], 'traffic': None,
+                     'kernel': 'zscan_mfma_kernel', 'kernel_ms': k_mix,
+                     'flop_per_eval': FLOP_PER_EVAL['GALAXY'], 'evals_per_launch': evals},
+        'cpu_baseline': cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == '__main__':
+    main()

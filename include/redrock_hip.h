@@ -1,0 +1,216 @@
+/*
+ * redrock_hip.h — C ABI of libredrock_hip.so, the gfx950 implementation of
+ * redrock's redshift chi2 scan.
+ *
+ * Conventions
+ *  - Every pointer argument named d_* is a device pointer (HBM) owned by the
+ *    caller; the library never frees caller memory and allocates nothing.
+ *  - Every entry point is stream-ordered on `stream` (a hipStream_t passed as
+ *    void*; NULL = the null stream) and does not synchronise the device.
+ *  - Every entry point returns 0 on success and a negative code on an
+ *    argument or launch error; rr_last_error() then returns a thread-local
+ *    message.  Numerical failures (a singular normal matrix) are not errors:
+ *    they are reported per (target, redshift) in the status output.
+ *  - Calls that share output buffers on one stream must not overlap.
+ */
+#ifndef REDROCK_HIP_H
+#define REDROCK_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define RR_ABI_VERSION 2
+
+/* Error codes. */
+#define RR_OK 0
+#define RR_EINVAL (-1)
+#define RR_ELAUNCH (-2)
+#define RR_EUNSUPPORTED (-3)
+
+/* Output element types for rr_rebin. */
+#define RR_F64 0
+#define RR_F32 1
+#define RR_F16 2
+#define RR_BF16 3
+
+/* Precision modes of rr_zscan. */
+#define RR_PREC_F64 0   /* plain FP64: vector FP64 FMA, FP64 LU solve           */
+#define RR_PREC_F16 1   /* MFMA fp16 inputs, fp32 accumulate, FP64 Cholesky     */
+#define RR_PREC_BF16 2  /* MFMA bf16 inputs, fp32 accumulate, FP64 Cholesky     */
+
+/* Resolution band: half width 5 (11 diagonals), the DESI layout. */
+#define RR_BAND_HALF 5
+#define RR_BAND_NDIAG 11
+
+/* Largest template basis supported by the scan kernels. */
+#define RR_MAX_NBASIS 12
+
+/* Per-(target, z) status codes written by rr_zscan. */
+#define RR_STATUS_OK 0
+#define RR_STATUS_SINGULAR 1   /* LU found an exactly zero pivot -> HUGE_CHI2 */
+
+int rr_abi_version(void);
+const char *rr_last_error(void);
+
+/*
+ * Lyman-series IGM line table, one entry per line: transmission factor
+ * exp(-A (lambda_obs/line)^B) wherever lambda_obs/(1+z) < line.
+ * Replaces the constants.Lyman_series tables consumed by
+ * transmission_Lyman_CaluraKamble (py/redrock/igm.py:321-364).
+ */
+typedef struct {
+    double line;
+    double A;
+    double B;
+} rr_lyman_line;
+
+#define RR_MAX_LYMAN_LINES 8
+
+/*
+ * rr_rebin — trapezoidal rebin of a template basis to every trial redshift,
+ * with the Lyman-series transmission multiply fused in.
+ *
+ * Replaces rebin.trapz_rebin / _trapz_rebin_batch / batch_trapz_rebin
+ * (py/redrock/rebin.py:20-96,126-209,212-451) followed by the IGM multiply
+ * of DistTemplate.__init__ (py/redrock/templates.py:586-597) and of fitz
+ * (py/redrock/fitz.py:233-243).
+ *
+ *   d_x      (nx)        rest-frame template wavelengths, increasing
+ *   d_y      (nb, y_ld)  template fluxes, basis-major
+ *   d_edges  (nbin+1)    observed-frame bin edges
+ *   d_wave   (nbin)      bin centres for the IGM factor (may be NULL when
+ *                        nlines == 0)
+ *   d_z      (nz)        redshifts
+ *   lines    host array of nlines rr_lyman_line (nlines <= 8; NULL if 0)
+ *   d_out    element (iz, ibin, ib) is written at
+ *            d_out + iz*s_z + ibin*s_pix + ib*s_b (in elements of out_type),
+ *            value multiplied by `scale` before conversion.
+ *
+ * The caller must have checked the edge range (rebin.py:332-341); the kernel
+ * clamps its sample search so that an unchecked call cannot read out of
+ * bounds, but the values are then meaningless.
+ */
+int rr_rebin(const double *d_x, int nx, const double *d_y, int nb, long long y_ld,
+             const double *d_edges, int nbin, const double *d_wave,
+             const double *d_z, int nz, const rr_lyman_line *lines, int nlines,
+             void *d_out, int out_type, long long s_z, long long s_pix, long long s_b,
+             double scale, void *stream);
+
+/*
+ * Scan problem descriptor for rr_zscan.  One "item" is one target scanned
+ * over a contiguous range of the template cache's redshift axis.
+ *
+ * Pixel axis: the target's cameras concatenated in spectrum order
+ * (zscan.py:383-389, spectral_data zscan.py:192-209), padded to npix_pad
+ * (a multiple of 32) with zero weight.  The template cache rows are the same
+ * pixel axis shifted by `halo` rows of zeros.
+ */
+typedef struct {
+    int nitems;        /* number of (target, z-range) items                      */
+    int npix;          /* real pixels per target                                 */
+    int npix_pad;      /* padded pixel count (multiple of 32)                    */
+    int nb;            /* template basis size (<= RR_MAX_NBASIS)                 */
+    int precision;     /* RR_PREC_*                                              */
+    int halo;          /* zero rows before pixel 0 in the cache                  */
+
+    /* Template cache.
+     *  F64  : d_tcache is double, element (b, row, zc) at
+     *         (b*cache_rows + row)*cache_ldz + zc           ([b][row][z])
+     *  F16/BF16: d_tcache is 16-bit, element (zc, b, row) at
+     *         (zc*nb + b)*cache_rows + row                  ([z][b][row])
+     * row = pixel + halo.  cache_nz is the extent of the z axis.            */
+    const void *d_tcache;
+    long long cache_rows;
+    long long cache_ldz;
+    int cache_nz;
+    double tscale;     /* cache values = template * tscale (power of two)        */
+
+    /* Per-target data, indexed by target slot. */
+    const double *d_rband;   /* (ntarget, npix_pad, 11) R[p, p+k-5]             */
+    const double *d_w;       /* (ntarget, npix_pad) ivar                        */
+    const double *d_wf;      /* (ntarget, npix_pad) ivar*flux                   */
+    const double *d_f;       /* (ntarget, npix_pad) flux                        */
+    /* Mixed precision modes only (NULL otherwise), written by rr_pack_mixed: */
+    const void *d_rtile;     /* (ntarget, npix_pad/16, 64, 8) 16-bit MFMA
+                                A-operand tiles of sqrt(w) R                    */
+    const float *d_sfcol;    /* (ntarget, npix_pad, 2) hi/lo of sqrt(w) f       */
+    const double *d_scales;  /* (ntarget, 2) power-of-two scales of R and f     */
+    int uniform_z;           /* 1 if every item has the same z0 and nz: lets a
+                                wavefront share template tiles across 2 items   */
+
+    /* Work list, one entry per item. */
+    const int *d_item_target;  /* target slot                                   */
+    const int *d_item_z0;      /* first cache z index                           */
+    const int *d_item_nz;      /* number of redshifts                           */
+    const long long *d_item_out; /* offset of the item's first row in outputs   */
+
+    /* Optional inputs (NULL = absent). */
+    const double *d_prior;     /* (nb, nb) added to M (zscan.py:711-712,761-762) */
+    const double *d_oiisum;    /* (nb) OIItemplate summed over wavelength        */
+
+    /* Outputs, row r = item_out + iz. */
+    double *d_zchi2;           /* (rows)                                         */
+    double *d_zcoeff;          /* (rows, nb)                                     */
+    double *d_penalty;         /* (rows) or NULL                                 */
+    unsigned char *d_status;   /* (rows) or NULL                                 */
+    int max_item_nz;           /* largest item nz (sizes the grid)               */
+    long long total_rows;      /* number of output rows (> every item_out + nz)  */
+
+    /* Caller-provided device workspace of at least rr_zscan_workspace_bytes()
+     * bytes (per-item statistics and the per-row normal equations).         */
+    void *d_work;
+    long long work_bytes;
+
+    /* Which of the three launches to enqueue: RR_STAGE_* bits, 0 = all.
+     * The stages communicate through the workspace, so a partial call
+     * relies on an earlier call having run the other stages on it.        */
+    int stages;
+} rr_zscan_desc;
+
+#define RR_STAGE_STATS 1    /* per-item f'Wf and sum(w)                     */
+#define RR_STAGE_MAIN 2     /* normal equations (the scan kernel)           */
+#define RR_STAGE_FINISH 4   /* FP64 solve, chi2, penalty, status            */
+
+/* Workspace size rr_zscan needs for this descriptor (nitems, nb, precision,
+ * total_rows are read).  Returns a negative code on a bad descriptor. */
+long long rr_zscan_workspace_bytes(const rr_zscan_desc *desc);
+
+/*
+ * rr_zscan — chi2 of every (item, redshift): R-apply, weighted normal
+ * equations, solve, chi2 and the [OII] penalty, fused in one launch.
+ *
+ * Replaces zscan.calc_zchi2_batch (py/redrock/zscan.py:626-781) including
+ * batch_dot_product_sparse (:45-74,:392-444), batch_dot_product_3d3d
+ * (:76-133,:551-617), solve_matrices PCA (:1107-1152), batch_dot_product_3d2d
+ * (:135-162,:502-541) and calc_negOII_penalty (:783-800); with many items it
+ * also replaces the per-target loop of zscan.calc_zchi2 (:853-887).
+ *
+ * chi2 = f'Wf - c'y (compensated FP64 sum), equal to the reference's
+ * explicit residual sum (:780) at the solution.  Singular M (exactly zero LU
+ * pivot, np.linalg.solve's LinAlgError) gives chi2 = HUGE_CHI2, zero coeffs
+ * and status RR_STATUS_SINGULAR (zscan.py:769-771).  Zero total weight gives
+ * HUGE_CHI2 and zero coeffs with status OK (zscan.py:657-660).
+ *
+ * Mixed modes need halo == 8 and the rr_pack_mixed outputs; F64 needs
+ * cache_ldz >= cache_nz.
+ */
+int rr_zscan(const rr_zscan_desc *desc, void *stream);
+
+/*
+ * rr_pack_mixed — per-target preparation for the mixed precision modes:
+ * packs sqrt(w) R into 16-bit MFMA A-operand tiles (tile q covers output
+ * pixels [16q, 16q+16) and input pixels [16q-8, 16q+24)), splits sqrt(w) f
+ * into 16-bit hi/lo columns and chooses power-of-two scales that keep both in
+ * range.  Inputs are the (ntarget, npix_pad[, 11]) arrays of rr_zscan_desc;
+ * precision is RR_PREC_F16 or RR_PREC_BF16.  Replaces the per-call CSR upload
+ * of zscan.py:428-430 (done once per target here).
+ */
+int rr_pack_mixed(const double *d_rband, const double *d_w, const double *d_f, int ntarget,
+                  int npix_pad, int precision, void *d_rtile, float *d_sfcol,
+                  double *d_scales, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* REDROCK_HIP_H */

@@ -1,0 +1,496 @@
+"""Float64 CPU oracle for redrock's chi2-scan path (TEST INFRASTRUCTURE ONLY).
+
+Only ``tests/``, ``__graft_entry__.smoke()`` and the CPU-baseline leg of
+``bench.py`` may import this module.  It is the checker, never the thing
+measured or shipped.
+
+What pins each function (besides this restatement):
+
+=========================  ====================================================
+function                   independent checks (tests/test_oracle.py)
+=========================  ====================================================
+centers2edges              reference test vectors (test_rebin.py:28-32)
+trapz_rebin / batch        constant density -> exactly 1.0 (test_rebin.py:34-51),
+                           sin integrals (test_rebin.py:72-86), docstring
+                           vectors (rebin.py:222-250), ValueError cases
+                           (test_rebin.py:62-70), brute-force quadrature of
+                           the piecewise-linear interpolant on random grids
+transmission_Lyman         closed form exp(-A (l/l_line)^B), shape/None/T==1
+                           semantics of test_igm.py:32-135
+calc_zchi2_batch           weighted least squares by np.linalg.lstsq on
+                           sqrt(W)A (independent route), dense R @ T, the
+                           quadratic identity chi2 = f'Wf - y'c, HUGE_CHI2
+                           conventions (zero weights, singular M)
+calc_negOII_penalty        closed form
+find_minima                reference docstring vector (fitz.py:52-53)
+minfit                     exact parabola recovery (textbook)
+fitz                       restatement only for the control flow; its pieces
+                           are pinned above.  Control flow: parity unpinned.
+calc_deltachi2             reference vectors (test_zfind.py:141-177,
+sort_zfit_dict             test_zscan.py:101-111, test_zfind.py:86-139)
+=========================  ====================================================
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+import scipy.sparse
+
+HUGE_CHI2 = 9999999.0                       # zscan.py:18
+LYA_WAVELENGTH = 1215.67                    # constants.py:15
+MAX_VELO_DIFF = 1000.0                      # constants.py:7
+MAX_VELO_DIFF_STAR = 100.0                  # constants.py:8
+MIN_DELTACHI2 = 9.0                         # constants.py:11
+C_KMS = 299792.458                          # scipy.constants.speed_of_light/1000 (fitz.py:40)
+
+# Lyman-series tables (constants.py:16-37), ordered as the reference dict.
+LYMAN_SERIES = {
+    'Calura12': [
+        (LYA_WAVELENGTH, 0.0023, 3.64),
+        (1025.72, 0.0023 / 5.2615, 3.64),
+        (972.537, 0.0023 / 14.356, 3.64),
+        (949.7431, 0.0023 / 29.85984, 3.64),
+        (937.8035, 0.0023 / 53.36202, 3.64),
+    ],
+    'Kamble20': [
+        (LYA_WAVELENGTH, 0.00554, 3.182),
+    ],
+}
+
+# zwarning.py:14-56
+ZW_Z_FITLIMIT = 2 ** 5
+ZW_BAD_MINFIT = 2 ** 10
+ZW_SMALL_DELTA_CHI2 = 2 ** 2
+ZW_NEGATIVE_MODEL = 2 ** 3
+ZW_MANY_OUTLIERS = 2 ** 4
+ZW_NEGATIVE_EMISSION = 2 ** 6
+BADFIT_MASK = (ZW_NEGATIVE_MODEL | ZW_MANY_OUTLIERS | ZW_Z_FITLIMIT
+               | ZW_NEGATIVE_EMISSION | ZW_BAD_MINFIT)
+
+# ---------------------------------------------------------------------------
+# C sweep loader
+# ---------------------------------------------------------------------------
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+
+def build_c(force: bool = False) -> str:
+    """Compile oracle/rebin_oracle.c into oracle/liboracle.so (gcc)."""
+    src = os.path.join(_HERE, 'rebin_oracle.c')
+    out = os.path.join(_HERE, 'liboracle.so')
+    if force or not os.path.exists(out) or os.path.getmtime(out) < os.path.getmtime(src):
+        subprocess.check_call(['gcc', '-O2', '-ffp-contract=off', '-fno-fast-math',
+                               '-shared', '-fPIC', '-o', out, src])
+    return out
+
+
+def _lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, 'liboracle.so')
+        if not os.path.exists(path):
+            build_c()
+        lib = ctypes.CDLL(path)
+        f = lib.oracle_trapz_rebin_batch
+        dp = ctypes.POINTER(ctypes.c_double)
+        f.argtypes = [dp, ctypes.c_int, dp, ctypes.c_int, dp, ctypes.c_int,
+                      dp, ctypes.c_int, dp, dp]
+        f.restype = None
+        _LIB = lib
+    return _LIB
+
+
+def _ptr(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+# ---------------------------------------------------------------------------
+# rebin (rebin.py)
+# ---------------------------------------------------------------------------
+def centers2edges(centers):
+    """rebin.py:100-118."""
+    centers = np.asarray(centers)
+    edges = np.zeros(len(centers) + 1)
+    edges[1:-1] = (centers[0:-1] + centers[1:]) / 2.0
+    edges[0] = centers[0] - (centers[1] - edges[1])
+    edges[-1] = centers[-1] + (centers[-1] - edges[-2])
+    return edges
+
+
+def trapz_rebin(x, y, xnew=None, edges=None, myz=None, xmin=None, xmax=None,
+                edge_min=None, edge_max=None):
+    """CPU branch of rebin.py:212-378 (shape rules, range check, sweep)."""
+    if edges is None:
+        edges = centers2edges(xnew)
+    else:
+        edges = np.asarray(edges)
+    edges = np.ascontiguousarray(edges, dtype=np.float64)
+    nbins = len(edges) - 1
+    if edge_min is None:
+        edge_min = edges[0]
+    if edge_max is None:
+        edge_max = edges[-1]
+    scalar_z = False
+    if myz is None:
+        myz = np.array([0], dtype=np.float64)
+        scalar_z = True
+    elif np.isscalar(myz):
+        myz = np.array([myz], dtype=np.float64)
+        scalar_z = True
+    else:
+        myz = np.asarray(myz, dtype=np.float64)
+    nz = len(myz)
+    if nz == 0:
+        return np.zeros((0, nbins, 1), dtype=np.float64)
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    if xmin is None:
+        xmin = x[0] * (1 + myz.max())
+    if xmax is None:
+        xmax = x[-1] * (1 + myz.min())
+    if edge_min < xmin or edge_max > xmax:
+        raise ValueError('edges must be within input x range')
+    y = np.asarray(y, dtype=np.float64)
+    scalar_basis = y.ndim == 1
+    y2 = np.ascontiguousarray(y[None, :] if scalar_basis else y)
+    nbasis = y2.shape[0]
+    result_shape = (nbins,)
+    if not scalar_basis:
+        result_shape += (nbasis,)
+    if not scalar_z:
+        result_shape = (nz,) + result_shape
+    out = np.zeros((nz, nbins, nbasis), dtype=np.float64)
+    work = np.zeros(x.size, dtype=np.float64)
+    _lib().oracle_trapz_rebin_batch(_ptr(x), x.size, _ptr(y2), nbasis, _ptr(edges),
+                                    nbins, _ptr(myz), nz, _ptr(out), _ptr(work))
+    return out.reshape(result_shape)
+
+
+def rebin_template(template, myz, dwave=None, dedges=None, xmin=None, xmax=None):
+    """rebin.py:453-517 (CPU branch)."""
+    myz = np.asarray(myz, dtype=np.float64)
+    if xmin is None:
+        xmin = template.minwave * (1 + myz.max())
+    if xmax is None:
+        xmax = template.maxwave * (1 + myz.min())
+    result = dict()
+    if dwave is not None:
+        for hs, wave in dwave.items():
+            result[hs] = trapz_rebin(template.wave, template.flux, xnew=wave, myz=myz,
+                                     xmin=xmin, xmax=xmax)
+    elif dedges is not None:
+        for hs, edges in dedges.items():
+            minedge = maxedge = None
+            if type(edges) is tuple:
+                edges, minedge, maxedge = edges
+            result[hs] = trapz_rebin(template.wave, template.flux, edges=edges, myz=myz,
+                                     xmin=xmin, xmax=xmax, edge_min=minedge, edge_max=maxedge)
+    return result
+
+
+# ---------------------------------------------------------------------------
+# IGM (igm.py:321-439), CaluraKamble family only
+# ---------------------------------------------------------------------------
+def transmission_Lyman_CaluraKamble(zObj, lObs, lines):
+    """igm.py:321-364 with the line table passed explicitly."""
+    lRest = np.outer(1.0 / (1.0 + zObj), lObs)
+    min_wave = lRest.min()
+    lObs2 = np.tile(lObs, (zObj.size, 1))
+    T = np.ones(lRest.shape)
+    for (line, A, B) in lines:
+        if min_wave > line:
+            continue
+        w = lRest < line
+        zpix = lObs2[w] / line - 1.
+        tauEff = A * (1. + zpix) ** B
+        T[w] *= np.exp(-tauEff)
+    return T
+
+
+def transmission_Lyman(zObj, lObs, model='Calura12', always_return_array=True, lines=None):
+    """igm.py:369-439.  ``lines`` overrides the table (custom models)."""
+    if lines is None and model not in ('Calura12', 'Kamble20', 'None', None):
+        raise ValueError(f'Unrecognized model {model}')
+    if (model == 'None' or model is None) and not always_return_array:
+        return None
+    min_wave = np.min(lObs) / (1. + np.max(zObj))
+    if min_wave > LYA_WAVELENGTH and not always_return_array:
+        return None
+    scalar_z = np.isscalar(zObj)
+    scalar_l = np.isscalar(lObs)
+    zObj = np.atleast_1d(zObj)
+    lObs = np.atleast_1d(lObs)
+    if always_return_array and (min_wave > LYA_WAVELENGTH or model is None or model == 'None'):
+        T = np.ones((len(zObj), len(lObs)))
+    else:
+        T = transmission_Lyman_CaluraKamble(zObj, lObs, lines if lines is not None
+                                            else LYMAN_SERIES[model])
+    if scalar_z and scalar_l:
+        T = T[0, 0]
+    elif scalar_z:
+        T = T[0]
+    elif scalar_l:
+        T = T[:, 0]
+    return T
+
+
+def template_cache(template, dwave, myz=None, lines=None):
+    """DistTemplate.__init__ data (templates.py:584-597): rebin then IGM."""
+    if myz is None:
+        myz = template.redshifts
+    data = rebin_template(template, myz, dwave)
+    for k in list(dwave.keys()):
+        T = transmission_Lyman(myz, dwave[k], always_return_array=False,
+                               model=template.igm_model, lines=lines)
+        if T is None:
+            continue
+        data[k] *= T[:, :, None]
+    return data
+
+
+# ---------------------------------------------------------------------------
+# chi2 scan (zscan.py)
+# ---------------------------------------------------------------------------
+def spectral_data(spectra):
+    """zscan.py:192-209."""
+    weights = np.concatenate([s.ivar for s in spectra])
+    flux = np.concatenate([s.flux for s in spectra])
+    wflux = weights * flux
+    return weights, flux, wflux
+
+
+def dot_product_sparse_one(spectra, tdata, i):
+    """zscan.py:446-467."""
+    Tb = [scipy.sparse.csr_matrix(s.R).dot(tdata[s.wavehash][i, :, :]) for s in spectra]
+    return np.vstack(Tb)
+
+
+def solve_matrices_pca(M, y):
+    """zscan.py:1131-1152 (PCA branch): LAPACK gesv via np.linalg.solve."""
+    return np.linalg.solve(M, y.reshape(y.shape + (1,))).reshape(y.shape)
+
+
+def calc_zchi2_batch(spectra, tdata, weights, flux, wflux, nz, nbasis, prior=None):
+    """CPU branch of zscan.py:626-781 (PCA solve)."""
+    zchi2 = np.zeros(nz)
+    if weights.sum() == 0:
+        zchi2[:] = HUGE_CHI2
+        return zchi2, np.zeros((nz, nbasis))
+    zcoeff = np.zeros((nz, nbasis))
+    Rcsr = [scipy.sparse.csr_matrix(s.R) for s in spectra]
+    for i in range(nz):
+        Tb = np.vstack([R.dot(tdata[s.wavehash][i, :, :]) for R, s in zip(Rcsr, spectra)])
+        M = Tb.T.dot(np.multiply(weights[:, None], Tb))
+        y = Tb.T.dot(wflux)
+        if prior is not None:
+            M += prior
+        try:
+            zcoeff[i, :] = solve_matrices_pca(M, y)
+        except np.linalg.LinAlgError:
+            zchi2[i] = HUGE_CHI2
+            continue
+        model = Tb.dot(zcoeff[i, :])
+        zchi2[i] = np.dot((flux - model) ** 2, weights)
+    return zchi2, zcoeff
+
+
+def calc_negOII_penalty(OIItemplate, coeff):
+    """zscan.py:783-800."""
+    OIIflux = np.sum(coeff @ OIItemplate, axis=1)
+    return -OIIflux * (OIIflux < 0)
+
+
+def calc_zchi2(targets, tdata, nbasis, OIItemplate=None):
+    """Serial form of zscan.py:807-889 over a list of targets."""
+    nt = len(targets)
+    nz = next(iter(tdata.values())).shape[0]
+    zchi2 = np.zeros((nt, nz))
+    zpen = np.zeros((nt, nz))
+    zcoeff = np.zeros((nt, nz, nbasis))
+    for j, tg in enumerate(targets):
+        w, f, wf = spectral_data(tg.spectra)
+        if np.sum(w) == 0:
+            zchi2[j, :] = HUGE_CHI2
+            continue
+        zchi2[j], zcoeff[j] = calc_zchi2_batch(tg.spectra, tdata, w, f, wf, nz, nbasis)
+        if OIItemplate is not None:
+            zpen[j] = calc_negOII_penalty(OIItemplate, zcoeff[j])
+    return zchi2, zcoeff, zpen
+
+
+# ---------------------------------------------------------------------------
+# minima refinement (fitz.py)
+# ---------------------------------------------------------------------------
+def get_dv(z, zref):
+    """fitz.py:28-43."""
+    return C_KMS * (z - zref) / (1.0 + zref)
+
+
+def find_minima(x):
+    """fitz.py:46-67."""
+    x = np.asarray(x)
+    ii = np.where(np.r_[True, x[1:] <= x[:-1]] & np.r_[x[:-1] <= x[1:], True])[0]
+    jj = np.argsort(x[ii])
+    return ii[jj]
+
+
+def minfit(x, y):
+    """fitz.py:70-111."""
+    if len(x) < 3:
+        return (-1, -1, -1, ZW_BAD_MINFIT)
+    try:
+        a, b, c = np.polyfit(x, y, 2)
+    except np.linalg.LinAlgError:
+        return (-1, -1, -1, ZW_BAD_MINFIT)
+    if a == 0.0:
+        return (-1, -1, -1, ZW_BAD_MINFIT)
+    x0 = -b / (2 * a)
+    y0 = -(b ** 2) / (4 * a) + c
+    zwarn = 0
+    if (x0 <= np.min(x)) or (np.max(x) <= x0):
+        zwarn |= ZW_BAD_MINFIT
+    if y0 <= 0.:
+        zwarn |= ZW_BAD_MINFIT
+    if a > 0.0:
+        xerr = 1 / np.sqrt(a)
+    else:
+        xerr = 1 / np.sqrt(-a)
+        zwarn |= ZW_BAD_MINFIT
+    return (x0, xerr, y0, zwarn)
+
+
+def fitz(zchi2, redshifts, target, template, nminima=3, zminfit_npoints=15, lines=None):
+    """Non-archetype branch of fitz.py:133-395 on the CPU."""
+    assert len(zchi2) == len(redshifts)
+    nbasis = template.nbasis
+    spectra = target.spectra
+    dwave = {s.wavehash: s.wave for s in spectra}
+    weights, flux, wflux = spectral_data(spectra)
+    results = list()
+    nz = 15 if zminfit_npoints is None else zminfit_npoints
+    max_velo_diff = MAX_VELO_DIFF_STAR if template.template_type == 'STAR' else MAX_VELO_DIFF
+    oii = getattr(template, 'OIItemplate', None)
+    for imin in find_minima(zchi2):
+        if len(results) == nminima:
+            break
+        zprev = np.array([tmp['z'] for tmp in results])
+        dv = get_dv(z=redshifts[imin], zref=zprev)
+        if np.any(np.abs(dv) < max_velo_diff):
+            continue
+        ilo = max(0, imin - 1)
+        ihi = min(imin + 1, len(zchi2) - 1)
+        zz = np.linspace(redshifts[ilo], redshifts[ihi], nz)
+        xmin = template.minwave * (1 + zz.max())
+        xmax = template.maxwave * (1 + zz.min())
+        binned = rebin_template(template, zz, dwave, xmin=xmin, xmax=xmax)
+        for k in list(dwave.keys()):
+            T = transmission_Lyman(zz, dwave[k], always_return_array=False,
+                                   model=template.igm_model, lines=lines)
+            if T is None:
+                continue
+            binned[k] *= T[:, :, None]
+        zzchi2, zzcoeff = calc_zchi2_batch(spectra, binned, weights, flux, wflux, nz, nbasis)
+        if oii is not None:
+            zzchi2 += calc_negOII_penalty(oii, zzcoeff)
+        i = min(max(np.argmin(zzchi2), 1), len(zz) - 2)
+        zmin, sigma, chi2min, zwarn = minfit(zz[i - 1:i + 2], zzchi2[i - 1:i + 2])
+        try:
+            xmin = template.minwave * (1 + zmin)
+            xmax = template.maxwave * (1 + zmin)
+            binned = rebin_template(template, np.array([zmin]), dwave, xmin=xmin, xmax=xmax)
+            for k in list(dwave.keys()):
+                T = transmission_Lyman(np.array([zmin]), dwave[k], always_return_array=False,
+                                       model=template.igm_model, lines=lines)
+                if T is None:
+                    continue
+                binned[k] *= T[:, :, None]
+            chi2, coeff = calc_zchi2_batch(spectra, binned, weights, flux, wflux, 1, nbasis)
+        except ValueError as err:
+            if zmin < redshifts[0] or redshifts[-1] < zmin:
+                coeff = np.zeros(template.nbasis)
+                zwarn |= ZW_Z_FITLIMIT
+                zwarn |= ZW_BAD_MINFIT
+            else:
+                raise err
+        zbest = zmin
+        zerr = sigma
+        if zbest < zz[0] or zbest > zz[-1]:
+            zwarn |= ZW_BAD_MINFIT
+            imin = np.where(zzchi2 == np.min(zzchi2))[0][0]
+            zbest = zz[imin]
+            chi2min = zzchi2[imin]
+        if zbest < redshifts[1] or zbest > redshifts[-2]:
+            zwarn |= ZW_Z_FITLIMIT
+        if zmin < redshifts[1] or zmin > redshifts[-2]:
+            zwarn |= ZW_Z_FITLIMIT
+        zprev = np.array([tmp['z'] for tmp in results])
+        dv = get_dv(z=zbest, zref=zprev)
+        if np.any(np.abs(dv) < max_velo_diff):
+            continue
+        results.append(dict(z=zbest, zerr=zerr, zwarn=zwarn, chi2=chi2min, zz=zz,
+                            zzchi2=zzchi2, coeff=coeff, fitmethod=template.method))
+    ii = np.argsort([tmp['chi2'] for tmp in results])
+    results = [results[i] for i in ii]
+    if len(results) == 0:
+        xfloat = np.zeros((0, 1), dtype=float)
+        xint = np.zeros((0, 1), dtype=int)
+        xstr = np.zeros((0, 1), dtype=str)
+        return dict(z=xfloat, zerr=xfloat, zwarn=xint, chi2=xfloat, zz=xfloat, zzchi2=xfloat,
+                    coeff=xfloat, fitmethod=xstr, npixels=xint)
+    for r in results:
+        r['npixels'] = 0
+        for s in spectra:
+            r['npixels'] += (s.ivar > 0.).sum()
+    tmp = dict()
+    for k in results[0].keys():
+        tmp[k] = np.vstack([r[k] for r in results])
+    return tmp
+
+
+# ---------------------------------------------------------------------------
+# zfind helpers (zfind.py)
+# ---------------------------------------------------------------------------
+def calc_deltachi2(chi2, z, zwarn, dvlimit=MAX_VELO_DIFF):
+    """zfind.py:103-155."""
+    nz = len(chi2)
+    deltachi2 = np.zeros(nz)
+    okfit = (zwarn & BADFIT_MASK) == 0
+    if np.isscalar(dvlimit):
+        dvlimit = np.full(nz, dvlimit)
+    for i in range(len(chi2) - 1):
+        dv = get_dv(z[i + 1:], z[i])
+        ii = (np.abs(dv) > np.minimum(dvlimit[i], dvlimit[i + 1:])) & okfit[i + 1:]
+        if np.any(ii):
+            dchi2 = chi2[i + 1:] - chi2[i]
+            deltachi2[i] = np.min(dchi2[ii])
+    noti = np.ones(nz, dtype=bool)
+    setzwarn = np.zeros(nz, dtype=bool)
+    for i in range(nz):
+        noti[:] = True
+        noti[i] = False
+        alldeltachi2 = np.absolute(chi2[noti] - chi2[i])
+        alldv = np.absolute(get_dv(z=z[noti], zref=z[i]))
+        alldvlimit = np.minimum(dvlimit[i], dvlimit[noti])
+        zw = np.any(okfit[noti] & (alldeltachi2 < MIN_DELTACHI2) & (alldv >= alldvlimit))
+        if zw:
+            setzwarn[i] = True
+    return deltachi2, setzwarn
+
+
+def sort_dict_by_cols(d, colnames, sort_first_column_first=True):
+    """zfind.py:53-81."""
+    vals = tuple(d[c] for c in colnames)
+    if sort_first_column_first:
+        vals = vals[::-1]
+    idx = np.lexsort(vals, axis=0).flatten()
+    for k in d.keys():
+        d[k] = d[k][idx]
+
+
+def sort_zfit_dict(zfit):
+    """zfind.py:206-218."""
+    zfit['__badfit__'] = (zfit['zwarn'] & BADFIT_MASK) != 0
+    sort_dict_by_cols(zfit, ('__badfit__', 'chi2'), sort_first_column_first=True)
+    zfit.pop('__badfit__')

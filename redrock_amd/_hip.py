@@ -1,0 +1,108 @@
+"""ctypes binding of libredrock_hip.so (declared in include/redrock_hip.h).
+
+The library is built in-tree by ``python -m redrock_amd.build``.  There is no
+fallback: if it is missing or cannot be loaded, every scan entry point raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, 'libredrock_hip.so')
+
+RR_ABI_VERSION = 2
+RR_F64, RR_F32, RR_F16, RR_BF16 = 0, 1, 2, 3
+RR_PREC_F64, RR_PREC_F16, RR_PREC_BF16 = 0, 1, 2
+RR_STATUS_OK, RR_STATUS_SINGULAR = 0, 1
+RR_MAX_NBASIS = 12
+RR_MAX_LYMAN_LINES = 8
+
+PRECISIONS = {'f64': RR_PREC_F64, 'f16': RR_PREC_F16, 'bf16': RR_PREC_BF16}
+
+EXPORTS = ('rr_abi_version', 'rr_last_error', 'rr_rebin', 'rr_zscan',
+           'rr_zscan_workspace_bytes', 'rr_pack_mixed')
+
+_vp = ctypes.c_void_p
+_i = ctypes.c_int
+_ll = ctypes.c_longlong
+_d = ctypes.c_double
+
+
+class LymanLine(ctypes.Structure):
+    _fields_ = [('line', _d), ('A', _d), ('B', _d)]
+
+
+class ZscanDesc(ctypes.Structure):
+    _fields_ = [
+        ('nitems', _i), ('npix', _i), ('npix_pad', _i), ('nb', _i), ('precision', _i),
+        ('halo', _i),
+        ('d_tcache', _vp), ('cache_rows', _ll), ('cache_ldz', _ll), ('cache_nz', _i),
+        ('tscale', _d),
+        ('d_rband', _vp), ('d_w', _vp), ('d_wf', _vp), ('d_f', _vp),
+        ('d_rtile', _vp), ('d_sfcol', _vp), ('d_scales', _vp), ('uniform_z', _i),
+        ('d_item_target', _vp), ('d_item_z0', _vp), ('d_item_nz', _vp), ('d_item_out', _vp),
+        ('d_prior', _vp), ('d_oiisum', _vp),
+        ('d_zchi2', _vp), ('d_zcoeff', _vp), ('d_penalty', _vp), ('d_status', _vp),
+        ('max_item_nz', _i), ('total_rows', _ll),
+        ('d_work', _vp), ('work_bytes', _ll),
+        ('stages', _i),
+    ]
+
+
+RR_STAGE_STATS, RR_STAGE_MAIN, RR_STAGE_FINISH = 1, 2, 4
+
+
+class HipError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def lib():
+    """Load the library (once).  Raises if it is absent or of another ABI."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise HipError(f'{LIB_PATH} is missing: build it with `python -m redrock_amd.build`')
+    L = ctypes.CDLL(LIB_PATH)
+    L.rr_abi_version.restype = _i
+    L.rr_abi_version.argtypes = []
+    L.rr_last_error.restype = ctypes.c_char_p
+    L.rr_last_error.argtypes = []
+    L.rr_rebin.restype = _i
+    L.rr_rebin.argtypes = [_vp, _i, _vp, _i, _ll, _vp, _i, _vp, _vp, _i,
+                           ctypes.POINTER(LymanLine), _i, _vp, _i, _ll, _ll, _ll, _d, _vp]
+    L.rr_zscan.restype = _i
+    L.rr_zscan.argtypes = [ctypes.POINTER(ZscanDesc), _vp]
+    L.rr_zscan_workspace_bytes.restype = _ll
+    L.rr_zscan_workspace_bytes.argtypes = [ctypes.POINTER(ZscanDesc)]
+    L.rr_pack_mixed.restype = _i
+    L.rr_pack_mixed.argtypes = [_vp, _vp, _vp, _i, _i, _i, _vp, _vp, _vp, _vp]
+    v = L.rr_abi_version()
+    if v != RR_ABI_VERSION:
+        raise HipError(f'libredrock_hip ABI {v}, expected {RR_ABI_VERSION}')
+    _lib = L
+    return L
+
+
+def check(rc, what):
+    if rc != 0:
+        msg = lib().rr_last_error().decode(errors='replace')
+        raise HipError(f'{what} failed ({rc}): {msg}')
+
+
+def stream_handle(stream=None):
+    """Raw hipStream_t of a torch stream (default: the current stream)."""
+    import torch
+    s = torch.cuda.current_stream() if stream is None else stream
+    return ctypes.c_void_p(s.cuda_stream)
+
+
+def ptr(t):
+    """Device pointer of a torch tensor (None -> NULL)."""
+    if t is None:
+        return None
+    return ctypes.c_void_p(t.data_ptr())

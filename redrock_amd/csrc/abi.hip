@@ -1,0 +1,85 @@
+// Exported C ABI: version, error text, descriptor validation and dispatch.
+#include "common.hpp"
+#include "zscan_args.hpp"
+
+#include <cstring>
+
+namespace rr {
+static thread_local char g_err[1024] = "";
+
+void set_error(const char *fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof(g_err), fmt, ap);
+    va_end(ap);
+}
+}  // namespace rr
+
+extern "C" int rr_abi_version(void) { return RR_ABI_VERSION; }
+
+extern "C" const char *rr_last_error(void) { return rr::g_err; }
+
+static bool is_mixed(int precision) { return precision == RR_PREC_F16 || precision == RR_PREC_BF16; }
+
+extern "C" long long rr_zscan_workspace_bytes(const rr_zscan_desc *d) {
+    if (!d || d->nitems < 0 || d->nb < 1 || d->nb > RR_MAX_NBASIS || d->total_rows < 0)
+        return rr::fail(RR_EINVAL, "rr_zscan_workspace_bytes: bad descriptor");
+    return rr::workspace_bytes(d->nitems, d->nb, is_mixed(d->precision), d->total_rows);
+}
+
+extern "C" int rr_zscan(const rr_zscan_desc *d, void *stream) {
+    if (!d) return rr::fail(RR_EINVAL, "rr_zscan: null descriptor");
+    if (d->nitems < 0 || d->nitems > 65535)
+        return rr::fail(RR_EINVAL, "rr_zscan: nitems=%d outside 0..65535", d->nitems);
+    if (d->npix < 1 || d->npix_pad < d->npix || (d->npix_pad % 32) != 0)
+        return rr::fail(RR_EINVAL, "rr_zscan: bad pixel sizes npix=%d npix_pad=%d", d->npix,
+                        d->npix_pad);
+    if (d->nb < 1 || d->nb > RR_MAX_NBASIS)
+        return rr::fail(RR_EUNSUPPORTED, "rr_zscan: nb=%d outside 1..%d", d->nb, RR_MAX_NBASIS);
+    if (d->halo < 8 || d->cache_rows < (long long)d->npix_pad + 2 * d->halo)
+        return rr::fail(RR_EINVAL, "rr_zscan: cache rows %lld too small for npix_pad %d halo %d",
+                        d->cache_rows, d->npix_pad, d->halo);
+    if (!d->d_tcache || !d->d_rband || !d->d_w || !d->d_wf || !d->d_f || !d->d_item_target ||
+        !d->d_item_z0 || !d->d_item_nz || !d->d_item_out || !d->d_zchi2 || !d->d_zcoeff)
+        return rr::fail(RR_EINVAL, "rr_zscan: null required pointer");
+    if (d->nitems == 0 || d->max_item_nz <= 0) return RR_OK;
+    const bool mixed = is_mixed(d->precision);
+    const long long need = rr::workspace_bytes(d->nitems, d->nb, mixed, d->total_rows);
+    if (!d->d_work || d->work_bytes < need)
+        return rr::fail(RR_EINVAL, "rr_zscan: workspace of %lld bytes needed, %lld given", need,
+                        d->work_bytes);
+    if (d->precision == RR_PREC_F64) {
+        if (d->cache_ldz < d->cache_nz) return rr::fail(RR_EINVAL, "rr_zscan: cache_ldz < cache_nz");
+    } else if (mixed) {
+        if (!d->d_rtile || !d->d_sfcol || !d->d_scales)
+            return rr::fail(RR_EINVAL, "rr_zscan: mixed precision needs rr_pack_mixed outputs");
+        if (d->halo != 8) return rr::fail(RR_EINVAL, "rr_zscan: mixed precision needs halo == 8");
+        if (d->nb + 2 > 16) return rr::fail(RR_EUNSUPPORTED, "rr_zscan: mixed needs nb <= 14");
+    } else {
+        return rr::fail(RR_EINVAL, "rr_zscan: unknown precision %d", d->precision);
+    }
+
+    rr::ZArgs a;
+    a.nitems = d->nitems; a.npix = d->npix; a.npix_pad = d->npix_pad; a.nb = d->nb;
+    a.precision = d->precision; a.halo = d->halo;
+    a.tcache = d->d_tcache; a.cache_rows = d->cache_rows; a.cache_ldz = d->cache_ldz;
+    a.cache_nz = d->cache_nz; a.tscale = d->tscale;
+    a.rband = d->d_rband; a.w = d->d_w; a.wf = d->d_wf; a.f = d->d_f;
+    a.rtile = d->d_rtile; a.sfcol = d->d_sfcol; a.scales = d->d_scales;
+    a.uniform_z = d->uniform_z;
+    a.item_target = d->d_item_target; a.item_z0 = d->d_item_z0; a.item_nz = d->d_item_nz;
+    a.item_out = d->d_item_out; a.prior = d->d_prior; a.oiisum = d->d_oiisum;
+    a.zchi2 = d->d_zchi2; a.zcoeff = d->d_zcoeff; a.penalty = d->d_penalty; a.status = d->d_status;
+    a.max_item_nz = d->max_item_nz;
+    a.work = d->d_work; a.total_rows = d->total_rows;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+
+    const int st = d->stages ? d->stages : (RR_STAGE_STATS | RR_STAGE_MAIN | RR_STAGE_FINISH);
+    int rc = RR_OK;
+    if (st & RR_STAGE_STATS) rc = rr::zscan_stats(a, s);
+    if (rc) return rc;
+    if (st & RR_STAGE_MAIN) rc = mixed ? rr::zscan_mfma(a, s) : rr::zscan_f64(a, s);
+    if (rc) return rc;
+    if (st & RR_STAGE_FINISH) rc = rr::zscan_finish(a, s);
+    return rc;
+}

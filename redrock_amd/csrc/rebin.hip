@@ -1,0 +1,155 @@
+// Trapezoidal rebin of a template basis to every trial redshift with the
+// Lyman-series IGM factor fused in (rr_rebin).
+//
+// One thread owns one (redshift, output bin) and loops over the basis, so the
+// sample search and the IGM factor are computed once per bin.  The bin mean
+// follows the serial sweep of py/redrock/rebin.py:126-176 expression by
+// expression, and this file is compiled with FP contraction off, so the FP64
+// result equals the sweep's double for double: the sample index j is the
+// first x[j]*(1+z) above the lower edge (the sweep's `while x[j] <= edges[i]`,
+// rebin.py:142), found here by binary search instead of a running pointer.
+#include "common.hpp"
+
+#pragma clang fp contract(off)
+
+namespace {
+
+struct RebinArgs {
+    const double *x;
+    const double *y;
+    const double *edges;
+    const double *wave;
+    const double *z;
+    void *out;
+    long long y_ld, s_z, s_pix, s_b;
+    double scale;
+    int nx, nb, nbin, nz, nlines, out_type;
+    rr_lyman_line lines[RR_MAX_LYMAN_LINES];
+};
+
+// first j in [lo, hi) with x[j]*opz > e (or hi when none)
+__device__ __forceinline__ int upper_bound_scaled(const double *x, int lo, int hi, double opz, double e) {
+    while (lo < hi) {
+        int mid = (lo + hi) >> 1;
+        if (x[mid] * opz <= e) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+
+template <int OUT>
+__device__ __forceinline__ void store(const RebinArgs &a, long long off, double v) {
+    v *= a.scale;
+    if (OUT == RR_F64) {
+        static_cast<double *>(a.out)[off] = v;
+    } else if (OUT == RR_F32) {
+        static_cast<float *>(a.out)[off] = (float)v;
+    } else if (OUT == RR_F16) {
+        static_cast<uint16_t *>(a.out)[off] = rr::f32_to_f16_bits((float)v);
+    } else {
+        static_cast<uint16_t *>(a.out)[off] = rr::f32_to_bf16_bits((float)v);
+    }
+}
+
+template <int OUT>
+__global__ __launch_bounds__(256) void rebin_kernel(RebinArgs a, int nbinblocks) {
+    const int iz = blockIdx.x / nbinblocks;
+    const int ibin = (blockIdx.x - iz * nbinblocks) * blockDim.x + threadIdx.x;
+    if (ibin >= a.nbin || iz >= a.nz) return;
+
+    const double *x = a.x;
+    const double opz = 1. + a.z[iz];
+    const double elo = a.edges[ibin];
+    const double ehi = a.edges[ibin + 1];
+    const int nx = a.nx;
+
+    // rebin.py:142 — first sample beyond the lower edge; clamped so that an
+    // out-of-range call stays in bounds (the host checks the range first).
+    int j = upper_bound_scaled(x, 0, nx, opz, elo);
+    if (j < 1) j = 1;
+    if (j > nx - 1) j = nx - 1;
+    const double xj = x[j] * opz;
+    const double xjm1 = x[j - 1] * opz;
+    const bool inside = xj < ehi;
+    // rebin.py:154 — last interior sample of this bin
+    int jend = j;
+    if (inside) {
+        while (jend + 1 < nx - 1 && x[jend + 1] * opz < ehi) ++jend;
+    }
+
+    // IGM factor (igm.py:346-363): product over lines with lRest < line.
+    double T = 1.0;
+    if (a.nlines > 0) {
+        const double lobs = a.wave[ibin];
+        const double lrest = (1.0 / opz) * lobs;
+        for (int l = 0; l < a.nlines; ++l) {
+            const double line = a.lines[l].line;
+            if (lrest < line) {
+                const double zpix = lobs / line - 1.;
+                const double tau = a.lines[l].A * pow(1. + zpix, a.lines[l].B);
+                T *= exp(-tau);
+            }
+        }
+    }
+
+    const double width = ehi - elo;
+    const long long obase = (long long)iz * a.s_z + (long long)ibin * a.s_pix;
+    for (int b = 0; b < a.nb; ++b) {
+        const double *y = a.y + (long long)b * a.y_ld;
+        double r = 0.0;
+        if (inside) {
+            const double yedge = y[j - 1] + (elo - xjm1) * (y[j] - y[j - 1]) / (xj - xjm1);
+            r += 0.5 * (y[j] + yedge) * (xj - elo);
+            double xprev = xj;
+            for (int jj = j + 1; jj <= jend; ++jj) {
+                const double xc = x[jj] * opz;
+                r += 0.5 * (y[jj] + y[jj - 1]) * (xc - xprev);
+                xprev = xc;
+            }
+            const double xe1 = x[jend + 1] * opz;
+            const double yedge2 = y[jend] + (ehi - xprev) * (y[jend + 1] - y[jend]) / (xe1 - xprev);
+            r += 0.5 * (yedge2 + y[jend]) * (ehi - xprev);
+        } else {
+            const double ylo = y[j] + (elo - xj) * (y[j] - y[j - 1]) / (xj - xjm1);
+            const double yhi = y[j] + (ehi - xj) * (y[j] - y[j - 1]) / (xj - xjm1);
+            r += 0.5 * (ylo + yhi) * width;
+        }
+        r /= width;
+        if (a.nlines > 0) r *= T;
+        store<OUT>(a, obase + (long long)b * a.s_b, r);
+    }
+}
+
+}  // namespace
+
+extern "C" int rr_rebin(const double *d_x, int nx, const double *d_y, int nb, long long y_ld,
+                        const double *d_edges, int nbin, const double *d_wave,
+                        const double *d_z, int nz, const rr_lyman_line *lines, int nlines,
+                        void *d_out, int out_type, long long s_z, long long s_pix, long long s_b,
+                        double scale, void *stream) {
+    if (nx < 2 || nb < 1 || nbin < 1 || nz < 0 || y_ld < nx)
+        return rr::fail(RR_EINVAL, "rr_rebin: bad sizes nx=%d nb=%d nbin=%d nz=%d y_ld=%lld",
+                        nx, nb, nbin, nz, y_ld);
+    if (nlines < 0 || nlines > RR_MAX_LYMAN_LINES || (nlines > 0 && (!lines || !d_wave)))
+        return rr::fail(RR_EINVAL, "rr_rebin: bad IGM table (nlines=%d)", nlines);
+    if (!d_x || !d_y || !d_edges || !d_z || !d_out)
+        return rr::fail(RR_EINVAL, "rr_rebin: null pointer");
+    if (nz == 0) return RR_OK;
+    RebinArgs a;
+    a.x = d_x; a.y = d_y; a.edges = d_edges; a.wave = d_wave; a.z = d_z; a.out = d_out;
+    a.y_ld = y_ld; a.s_z = s_z; a.s_pix = s_pix; a.s_b = s_b; a.scale = scale;
+    a.nx = nx; a.nb = nb; a.nbin = nbin; a.nz = nz; a.nlines = nlines; a.out_type = out_type;
+    for (int l = 0; l < nlines; ++l) a.lines[l] = lines[l];
+    const int nbb = (nbin + 255) / 256;
+    if ((long long)nbb * nz > 0x7fffffffLL) return rr::fail(RR_EINVAL, "rr_rebin: grid too large");
+    dim3 block(256), grid(nbb * nz);
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    switch (out_type) {
+        case RR_F64: hipLaunchKernelGGL(rebin_kernel<RR_F64>, grid, block, 0, s, a, nbb); break;
+        case RR_F32: hipLaunchKernelGGL(rebin_kernel<RR_F32>, grid, block, 0, s, a, nbb); break;
+        case RR_F16: hipLaunchKernelGGL(rebin_kernel<RR_F16>, grid, block, 0, s, a, nbb); break;
+        case RR_BF16: hipLaunchKernelGGL(rebin_kernel<RR_BF16>, grid, block, 0, s, a, nbb); break;
+        default: return rr::fail(RR_EINVAL, "rr_rebin: unknown out_type %d", out_type);
+    }
+    return rr::check_launch("rr_rebin");
+}

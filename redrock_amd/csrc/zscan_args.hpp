@@ -1,0 +1,51 @@
+// Kernel-argument form of rr_zscan_desc (passed by value to the kernels).
+#pragma once
+#include "common.hpp"
+
+namespace rr {
+
+struct ZArgs {
+    int nitems, npix, npix_pad, nb, precision, halo;
+    const void *tcache;
+    long long cache_rows, cache_ldz;
+    int cache_nz;
+    double tscale;
+    const double *rband, *w, *wf, *f;
+    const void *rtile;
+    const float *sfcol;
+    const double *scales;
+    int uniform_z;
+    const int *item_target, *item_z0, *item_nz;
+    const long long *item_out;
+    const double *prior, *oiisum;
+    double *zchi2, *zcoeff, *penalty;
+    unsigned char *status;
+    int max_item_nz;
+    void *work;            // workspace: item stats then normal equations
+    long long total_rows;  // rows of the output arrays (>= max item_out + nz)
+};
+
+// Workspace layout (all offsets 256-byte aligned):
+//   stats  double[nitems][2]            f'Wf, sum(w) of the item's target
+//   ne     F64  : double[ne_count][total_rows]   (k-major, coalesced)
+//          mixed: float [ne_count][total_rows]
+// ne entry k of row r is the (i, j) element, i <= j, i < nb, of the upper
+// triangle of [M | y] (F64: j <= nb; mixed: j <= nb+1, y split hi/lo in the
+// two last columns), k = ne_index(i, j).
+__host__ __device__ constexpr int ne_cols(int nb, bool mixed) { return mixed ? nb + 2 : nb + 1; }
+__host__ __device__ constexpr int ne_count(int nb, bool mixed) {
+    return nb * ne_cols(nb, mixed) - nb * (nb - 1) / 2;
+}
+__host__ __device__ constexpr int ne_index(int i, int j, int nb, bool mixed) {
+    return i * ne_cols(nb, mixed) - i * (i - 1) / 2 + (j - i);
+}
+long long workspace_bytes(int nitems, int nb, bool mixed, long long total_rows);
+inline long long align256(long long x) { return (x + 255) & ~255LL; }
+
+int zscan_stats(const ZArgs &a, hipStream_t s);
+int zscan_finish(const ZArgs &a, hipStream_t s);
+
+int zscan_f64(const ZArgs &a, hipStream_t s);
+int zscan_mfma(const ZArgs &a, hipStream_t s);
+
+}  // namespace rr

@@ -1,0 +1,139 @@
+// Plain FP64 chi2 scan (rr_zscan with RR_PREC_F64).
+//
+// Work decomposition: one wavefront = one item (a target scanned over a z
+// range) x 64 consecutive redshifts, one lane per redshift.  A workgroup's 4
+// waves are 4 items on the same 64-z block, so their template-cache reads hit
+// the same lines in L1/L2.  Per lane and per block of P pixels:
+//   A[p][b] = sum_k R[p][k] T[b][p+k-5][z]      (banded R-apply, 11 taps)
+//   M[b1][b2] += w_p A[p][b1] A[p][b2],  y[b] += A[p][b] (w f)_p
+// R, w and w*f are wave-uniform (the target is uniform per wave), so they come
+// through scalar loads; the cache is laid out [b][pixel][z] so every load of a
+// window row is one coalesced 512-byte line run.  The kernel writes the
+// per-redshift normal equations [M | y] to the workspace; the FP64 solve and
+// chi2 follow in zscan_finish.hip.
+//
+// Reference path replaced: zscan.py:670-737 (GPU) / :752-780 (CPU).
+#include "common.hpp"
+#include "zscan_args.hpp"
+
+namespace {
+
+template <int NB, int P>
+__global__ __launch_bounds__(256) void zscan_f64_kernel(rr::ZArgs a, long long ne_off) {
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int item = blockIdx.y * 4 + wave;
+    if (item >= a.nitems) return;
+    const int t = a.item_target[item];
+    const int z0 = a.item_z0[item];
+    const int nzi = a.item_nz[item];
+    const int zbase = blockIdx.x * 64;
+    if (zbase >= nzi) return;
+    const int iz = zbase + lane;
+    const int zc = z0 + (iz < nzi ? iz : nzi - 1);
+
+    const long long npp = a.npix_pad;
+    const double *__restrict__ Rt = a.rband + (long long)t * npp * 11;
+    const double *__restrict__ Wt = a.w + (long long)t * npp;
+    const double *__restrict__ WFt = a.wf + (long long)t * npp;
+    const double *__restrict__ T = static_cast<const double *>(a.tcache);
+    const long long ldz = a.cache_ldz;
+    const long long bstride = a.cache_rows * ldz;
+
+    constexpr int NM = NB * (NB + 1) / 2;
+    double m[NM];
+    double y[NB];
+#pragma unroll
+    for (int i = 0; i < NM; ++i) m[i] = 0.0;
+#pragma unroll
+    for (int i = 0; i < NB; ++i) y[i] = 0.0;
+
+    // row of pixel p in the cache = p + halo; window of block p0 starts at p0-5
+    const double *Tz = T + (long long)(a.halo - 5) * ldz + zc;
+
+    for (int p0 = 0; p0 < a.npix_pad; p0 += P) {
+        double r[P][11];
+#pragma unroll
+        for (int i = 0; i < P; ++i)
+#pragma unroll
+            for (int k = 0; k < 11; ++k) r[i][k] = Rt[(long long)(p0 + i) * 11 + k];
+        double A[P][NB];
+#pragma unroll
+        for (int b = 0; b < NB; ++b) {
+            const double *col = Tz + b * bstride + (long long)p0 * ldz;
+            double win[P + 10];
+#pragma unroll
+            for (int i = 0; i < P + 10; ++i) win[i] = col[i * ldz];
+#pragma unroll
+            for (int i = 0; i < P; ++i) {
+                double s = r[i][0] * win[i];
+#pragma unroll
+                for (int k = 1; k < 11; ++k) s = fma(r[i][k], win[i + k], s);
+                A[i][b] = s;
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < P; ++i) {
+            const double w = Wt[p0 + i];
+            const double wf = WFt[p0 + i];
+            int idx = 0;
+#pragma unroll
+            for (int b1 = 0; b1 < NB; ++b1) {
+                const double wa = w * A[i][b1];
+                y[b1] = fma(A[i][b1], wf, y[b1]);
+#pragma unroll
+                for (int b2 = b1; b2 < NB; ++b2) {
+                    m[idx] = fma(wa, A[i][b2], m[idx]);
+                    ++idx;
+                }
+            }
+        }
+    }
+
+    if (iz >= nzi) return;
+    // normal equations of this (item, z) -> workspace, k-major (coalesced)
+    const long long row = a.item_out[item] + iz;
+    double *ne = reinterpret_cast<double *>(static_cast<char *>(a.work) + ne_off);
+    const long long R = a.total_rows;
+    int idx = 0;
+#pragma unroll
+    for (int b1 = 0; b1 < NB; ++b1) {
+#pragma unroll
+        for (int b2 = b1; b2 < NB; ++b2) {
+            ne[rr::ne_index(b1, b2, NB, false) * R + row] = m[idx];
+            ++idx;
+        }
+        ne[rr::ne_index(b1, NB, NB, false) * R + row] = y[b1];
+    }
+}
+
+template <int NB>
+int launch_nb(const rr::ZArgs &a, hipStream_t s) {
+    dim3 block(256);
+    dim3 grid((a.max_item_nz + 63) / 64, (a.nitems + 3) / 4);
+    const long long ne_off = rr::align256((long long)a.nitems * 2 * 8);
+    hipLaunchKernelGGL((zscan_f64_kernel<NB, 2>), grid, block, 0, s, a, ne_off);
+    return rr::check_launch("rr_zscan(f64)");
+}
+
+}  // namespace
+
+namespace rr {
+int zscan_f64(const ZArgs &a, hipStream_t s) {
+    switch (a.nb) {
+        case 1: return launch_nb<1>(a, s);
+        case 2: return launch_nb<2>(a, s);
+        case 3: return launch_nb<3>(a, s);
+        case 4: return launch_nb<4>(a, s);
+        case 5: return launch_nb<5>(a, s);
+        case 6: return launch_nb<6>(a, s);
+        case 7: return launch_nb<7>(a, s);
+        case 8: return launch_nb<8>(a, s);
+        case 9: return launch_nb<9>(a, s);
+        case 10: return launch_nb<10>(a, s);
+        case 11: return launch_nb<11>(a, s);
+        case 12: return launch_nb<12>(a, s);
+        default: return fail(RR_EUNSUPPORTED, "rr_zscan(f64): nb=%d unsupported (1..12)", a.nb);
+    }
+}
+}  // namespace rr

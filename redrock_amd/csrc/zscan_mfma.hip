@@ -1,0 +1,313 @@
+// Mixed-precision chi2 scan on the matrix cores (rr_zscan with RR_PREC_F16 /
+// RR_PREC_BF16) and the per-target packing it needs (rr_pack_mixed).
+//
+// For one target, 16 output pixels [16q, 16q+16) and one redshift:
+//
+//   D = Rq x Tq + Cq                      v_mfma_f32_16x16x32_{f16,bf16}
+//
+//   Rq (16 x 32)  rows = output pixels, cols = input pixels [16q-8, 16q+24):
+//                 the 11-diagonal band scaled by sqrt(w_p) (a dense band
+//                 block, packed once per target, reused for every z)
+//   Tq (32 x 16)  template cache rows of those input pixels, cols = basis
+//                 b < nb (zero for b >= nb)
+//   Cq            zero except columns nb, nb+1 = sqrt(w) f split hi + lo
+//
+// so D[p][b<nb] = sqrt(w_p) (R T)[p][b] and D[p][nb..nb+1] = sqrt(w_p) f_p.
+// The f32 outputs of two consecutive tiles (32 pixels) are rounded to 16 bits
+// and fed back, unchanged in place, as both operands of a third MFMA:
+//
+//   G += D^T D                            (16 x 16, K = 32 pixels)
+//
+// With the 16x16 output layout (lane = column l%16, rows 4(l/16)+r) a lane's
+// four accumulators of tile q and of tile q+1 are exactly the 8 K-slots the
+// operand layout expects for column l%16, so no data moves between the two
+// stages.  G holds M = A'WA (b,b < nb) and y = A'Wf (b, nb / nb+1), scaled;
+// the kernel writes them to the workspace and zscan_finish.hip unscales them,
+// solves M c = y by FP64 Cholesky (LU fallback) and forms chi2 = f'Wf - c'y
+// with f'Wf summed exactly in FP64.
+//
+// Reference path replaced: zscan.py:670-737 (the GPU path) — the M/y
+// contractions run in 16-bit on MFMA with fp32 accumulation, everything after
+// them in FP64.
+#include "common.hpp"
+#include "zscan_args.hpp"
+
+namespace {
+
+typedef _Float16 half8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bfv8 __attribute__((ext_vector_type(8)));
+typedef float float4v __attribute__((ext_vector_type(4)));
+typedef unsigned short u16x8 __attribute__((ext_vector_type(8)));
+
+template <int TY>
+__device__ __forceinline__ float4v mfma(u16x8 a, u16x8 b, float4v c) {
+    if constexpr (TY == RR_PREC_F16) {
+        return __builtin_amdgcn_mfma_f32_16x16x32_f16(__builtin_bit_cast(half8, a),
+                                                      __builtin_bit_cast(half8, b), c, 0, 0, 0);
+    } else {
+        return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bfv8, a),
+                                                       __builtin_bit_cast(bfv8, b), c, 0, 0, 0);
+    }
+}
+
+template <int TY>
+__device__ __forceinline__ unsigned short to16(float f) {
+    if constexpr (TY == RR_PREC_F16) return rr::f32_to_f16_bits(f);
+    else return rr::f32_to_bf16_bits(f);
+}
+
+template <int TY>
+__device__ __forceinline__ float from16(unsigned short h) {
+    if constexpr (TY == RR_PREC_F16) return __half2float(__ushort_as_half(h));
+    else return __uint_as_float(((unsigned)h) << 16);
+}
+
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+// ---------------------------------------------------------------------------
+// Packing: one workgroup (one wave) per target.
+//   rtile  (ntarget, ntile, 64, 8) 16-bit: lane l of tile q holds row i=l%16,
+//          K slots v=0..7 of group g=l/16 -> input pixel 16q-8+8g+v; value
+//          sqrt(w_p) * rscale * R[p][k'] with p = 16q+i, k' = 8g+v-i-3.
+//   sfcol  (ntarget, npix_pad, 2) f32: hi/lo 16-bit split of
+//          sqrt(w_p) f_p * fscale, hi exactly representable in 16 bits.
+//   scales (ntarget, 2) f64: rscale, fscale (powers of two).
+// ---------------------------------------------------------------------------
+template <int TY>
+__global__ __launch_bounds__(64) void pack_mixed_kernel(const double *__restrict__ rband,
+                                                        const double *__restrict__ w,
+                                                        const double *__restrict__ f, int npix_pad,
+                                                        unsigned short *__restrict__ rtile,
+                                                        float *__restrict__ sfcol,
+                                                        double *__restrict__ scales) {
+    const int t = blockIdx.x;
+    const int lane = threadIdx.x;
+    const double *W = w + (long long)t * npix_pad;
+    const double *F = f + (long long)t * npix_pad;
+    const double *Rb = rband + (long long)t * npix_pad * 11;
+    double mw = 0.0, mf = 0.0;
+    for (int p = lane; p < npix_pad; p += 64) {
+        const double sw = sqrt(W[p]);
+        mw = fmax(mw, sw);
+        mf = fmax(mf, fabs(sw * F[p]));
+    }
+    mw = wave_max(mw);
+    mf = wave_max(mf);
+    // largest power of two with mw*rscale <= 1 and mf*fscale <= 256
+    const double rscale = (mw > 0.0) ? exp2(-ceil(log2(mw))) : 1.0;
+    const double fscale = (mf > 0.0) ? exp2(8.0 - ceil(log2(mf))) : 1.0;
+    if (lane == 0) {
+        scales[2 * t] = rscale;
+        scales[2 * t + 1] = fscale;
+    }
+    const int ntile = npix_pad / 16;
+    unsigned short *RT = rtile + (long long)t * ntile * 64 * 8;
+    const int i = lane & 15, g = lane >> 4;
+    for (int q = 0; q < ntile; ++q) {
+        const int p = 16 * q + i;
+        const double sw = sqrt(W[p]) * rscale;
+        u16x8 v8;
+#pragma unroll
+        for (int v = 0; v < 8; ++v) {
+            const int k = 8 * g + v - i - 3;
+            const double r = (k >= 0 && k <= 10) ? Rb[(long long)p * 11 + k] * sw : 0.0;
+            v8[v] = to16<TY>((float)r);
+        }
+        *reinterpret_cast<u16x8 *>(RT + ((long long)q * 64 + lane) * 8) = v8;
+    }
+    float *SF = sfcol + (long long)t * npix_pad * 2;
+    for (int p = lane; p < npix_pad; p += 64) {
+        const float s = (float)(sqrt(W[p]) * F[p] * fscale);
+        const float hi = from16<TY>(to16<TY>(s));
+        const float lo = from16<TY>(to16<TY>(s - hi));
+        SF[2 * p] = hi;
+        SF[2 * p + 1] = lo;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Scan kernel.  Workgroup = 4 waves; wave = TW items (targets sharing the
+// same cache z range) x ZW redshifts.  All 4 waves of a workgroup work on the
+// same redshift block, so their template tiles are shared through L1/L2.
+// ---------------------------------------------------------------------------
+template <int NB, int TY, int TW, int ZW>
+__global__ __launch_bounds__(256) void zscan_mfma_kernel(rr::ZArgs a, const float *__restrict__ sfcol,
+                                                         long long ne_off) {
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int item0 = (blockIdx.x * 4 + wave) * TW;
+    if (item0 >= a.nitems) return;
+    const int z0 = a.item_z0[item0];
+    const int nzi = a.item_nz[item0];
+    const int zbase = blockIdx.y * ZW;
+    if (zbase >= nzi) return;
+
+    const int j = lane & 15;   // column (basis) of operand B / output
+    const int g = lane >> 4;   // K group / output row group
+    const long long npp = a.npix_pad;
+    const int ntile = a.npix_pad / 16;
+    const long long rows = a.cache_rows;
+    const unsigned short *__restrict__ T = static_cast<const unsigned short *>(a.tcache);
+    const unsigned short *__restrict__ RT = static_cast<const unsigned short *>(a.rtile);
+
+    int tgt[TW];
+    bool live[TW];
+#pragma unroll
+    for (int u = 0; u < TW; ++u) {
+        live[u] = (item0 + u) < a.nitems;
+        tgt[u] = a.item_target[live[u] ? item0 + u : item0];
+    }
+    // cache z index of each of the wave's redshifts (clamped past the end)
+    long long zoff[ZW];
+#pragma unroll
+    for (int s = 0; s < ZW; ++s) {
+        const int iz = zbase + s;
+        const int zc = z0 + (iz < nzi ? iz : nzi - 1);
+        zoff[s] = ((long long)zc * NB + (j < NB ? j : 0)) * rows + 8 * g;
+    }
+
+    float4v G[TW][ZW];
+#pragma unroll
+    for (int u = 0; u < TW; ++u)
+#pragma unroll
+        for (int s = 0; s < ZW; ++s) G[u][s] = float4v{0.f, 0.f, 0.f, 0.f};
+
+    const u16x8 zero8 = {0, 0, 0, 0, 0, 0, 0, 0};
+    const bool fcolumn = (j == NB) || (j == NB + 1);
+    const int fslot = (j == NB + 1) ? 1 : 0;
+
+    for (int q = 0; q < ntile; q += 2) {
+        u16x8 ra[TW][2];
+        float4v cf[TW][2];
+#pragma unroll
+        for (int u = 0; u < TW; ++u) {
+            const unsigned short *rt = RT + ((long long)tgt[u] * ntile + q) * 64 * 8 + lane * 8;
+            ra[u][0] = *reinterpret_cast<const u16x8 *>(rt);
+            ra[u][1] = *reinterpret_cast<const u16x8 *>(rt + 64 * 8);
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                float4v c = {0.f, 0.f, 0.f, 0.f};
+                if (fcolumn) {
+                    const float *sf = sfcol + ((long long)tgt[u] * npp + 16 * (q + h) + 4 * g) * 2 + fslot;
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) c[r] = sf[2 * r];
+                }
+                cf[u][h] = c;
+            }
+        }
+#pragma unroll
+        for (int s = 0; s < ZW; ++s) {
+            u16x8 b0 = zero8, b1 = zero8;
+            if (j < NB) {
+                const unsigned short *tp = T + zoff[s] + 16 * q;
+                b0 = *reinterpret_cast<const u16x8 *>(tp);
+                b1 = *reinterpret_cast<const u16x8 *>(tp + 16);
+            }
+#pragma unroll
+            for (int u = 0; u < TW; ++u) {
+                const float4v d0 = mfma<TY>(ra[u][0], b0, cf[u][0]);
+                const float4v d1 = mfma<TY>(ra[u][1], b1, cf[u][1]);
+                u16x8 x;
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    x[r] = to16<TY>(d0[r]);
+                    x[4 + r] = to16<TY>(d1[r]);
+                }
+                G[u][s] = mfma<TY>(x, x, G[u][s]);
+            }
+        }
+    }
+
+    // ---- epilogue: scaled normal equations -> workspace ----
+    // lane (j, g) register r holds G[4g+r][j]; keep the upper triangle of the
+    // first nb rows over columns 0..nb+1 ([M | y_hi | y_lo]).
+    float *ne = reinterpret_cast<float *>(static_cast<char *>(a.work) + ne_off);
+    const long long R = a.total_rows;
+#pragma unroll
+    for (int u = 0; u < TW; ++u) {
+        if (!live[u]) continue;
+        const long long row0 = a.item_out[item0 + u];
+#pragma unroll
+        for (int s = 0; s < ZW; ++s) {
+            const int iz = zbase + s;
+            if (iz >= nzi) continue;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int i = 4 * g + r;
+                if (i < NB && j >= i && j < NB + 2)
+                    ne[rr::ne_index(i, j, NB, true) * R + row0 + iz] = G[u][s][r];
+            }
+        }
+    }
+}
+
+template <int NB, int TY, int TW>
+int launch3(const rr::ZArgs &a, const float *sf, const double *sc, hipStream_t s) {
+    constexpr int ZW = 8;
+    const int per_wg = 4 * TW;
+    dim3 block(256);
+    dim3 grid((a.nitems + per_wg - 1) / per_wg, (a.max_item_nz + ZW - 1) / ZW);
+    const long long ne_off = rr::align256((long long)a.nitems * 2 * 8);
+    hipLaunchKernelGGL((zscan_mfma_kernel<NB, TY, TW, ZW>), grid, block, 0, s, a, sf, ne_off);
+    return rr::check_launch("rr_zscan(mixed)");
+}
+
+template <int NB, int TY>
+int launch2(const rr::ZArgs &a, const float *sf, const double *sc, int tw, hipStream_t s) {
+    if (tw == 2) return launch3<NB, TY, 2>(a, sf, sc, s);
+    return launch3<NB, TY, 1>(a, sf, sc, s);
+}
+
+template <int TY>
+int launch1(const rr::ZArgs &a, const float *sf, const double *sc, int tw, hipStream_t s) {
+    switch (a.nb) {
+        case 1: return launch2<1, TY>(a, sf, sc, tw, s);
+        case 2: return launch2<2, TY>(a, sf, sc, tw, s);
+        case 3: return launch2<3, TY>(a, sf, sc, tw, s);
+        case 4: return launch2<4, TY>(a, sf, sc, tw, s);
+        case 5: return launch2<5, TY>(a, sf, sc, tw, s);
+        case 6: return launch2<6, TY>(a, sf, sc, tw, s);
+        case 7: return launch2<7, TY>(a, sf, sc, tw, s);
+        case 8: return launch2<8, TY>(a, sf, sc, tw, s);
+        case 9: return launch2<9, TY>(a, sf, sc, tw, s);
+        case 10: return launch2<10, TY>(a, sf, sc, tw, s);
+        case 11: return launch2<11, TY>(a, sf, sc, tw, s);
+        case 12: return launch2<12, TY>(a, sf, sc, tw, s);
+        default: return rr::fail(RR_EUNSUPPORTED, "rr_zscan(mixed): nb=%d unsupported", a.nb);
+    }
+}
+
+}  // namespace
+
+namespace rr {
+int zscan_mfma(const ZArgs &a, hipStream_t s) {
+    const int tw = a.uniform_z ? 2 : 1;
+    if (a.precision == RR_PREC_F16) return launch1<RR_PREC_F16>(a, a.sfcol, a.scales, tw, s);
+    return launch1<RR_PREC_BF16>(a, a.sfcol, a.scales, tw, s);
+}
+}  // namespace rr
+
+extern "C" int rr_pack_mixed(const double *d_rband, const double *d_w, const double *d_f,
+                             int ntarget, int npix_pad, int precision, void *d_rtile,
+                             float *d_sfcol, double *d_scales, void *stream) {
+    if (ntarget < 0 || npix_pad < 32 || (npix_pad % 32) != 0)
+        return rr::fail(RR_EINVAL, "rr_pack_mixed: bad sizes ntarget=%d npix_pad=%d", ntarget, npix_pad);
+    if (!d_rband || !d_w || !d_f || !d_rtile || !d_sfcol || !d_scales)
+        return rr::fail(RR_EINVAL, "rr_pack_mixed: null pointer");
+    if (ntarget == 0) return RR_OK;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    unsigned short *rt = static_cast<unsigned short *>(d_rtile);
+    if (precision == RR_PREC_F16)
+        hipLaunchKernelGGL(pack_mixed_kernel<RR_PREC_F16>, dim3(ntarget), dim3(64), 0, s, d_rband, d_w,
+                           d_f, npix_pad, rt, d_sfcol, d_scales);
+    else if (precision == RR_PREC_BF16)
+        hipLaunchKernelGGL(pack_mixed_kernel<RR_PREC_BF16>, dim3(ntarget), dim3(64), 0, s, d_rband, d_w,
+                           d_f, npix_pad, rt, d_sfcol, d_scales);
+    else
+        return rr::fail(RR_EINVAL, "rr_pack_mixed: precision must be RR_PREC_F16 or RR_PREC_BF16");
+    return rr::check_launch("rr_pack_mixed");
+}

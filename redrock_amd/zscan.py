@@ -1,0 +1,492 @@
+"""Redshift chi2 scan on the GPU (py/redrock/zscan.py).
+
+Drop-in functions with the reference signatures:
+
+* calc_zchi2(target_ids, target_data, dtemplate, progress=None, use_gpu=False)
+  (zscan.py:807-889) — all targets of a template in one rr_zscan launch.
+* calc_zchi2_batch(spectra, tdata, weights, flux, wflux, nz, nbasis, ...)
+  (zscan.py:626-781) — one target over the redshifts of `tdata`.
+* calc_zchi2_targets(targets, templates, mp_procs=1, use_gpu=False)
+  (zscan.py:910-1105), calc_negOII_penalty (:783-800), spectral_data
+  (:192-209), HUGE_CHI2 (:18).
+
+Every call runs the HIP kernels of libredrock_hip.so; there is no CPU path.
+The ``use_gpu`` and ``fullprecision`` arguments are accepted for signature
+compatibility (the scan is always on the GPU and always deterministic).
+``precision`` selects 'f64' (default, the reference's arithmetic) or the
+mixed-precision MFMA forms 'f16' / 'bf16'; the default can be changed with
+the environment variable RR_PRECISION.
+
+Device-side objects:
+
+* PixelLayout   — the concatenated pixel axis of a target (spectra order).
+* TemplateCache — a template rebinned (IGM included) to every redshift on
+  every wavelength grid, resident in HBM in the layout of each kernel.
+* TargetBatch   — per-target R bands, ivar, flux, ivar*flux in HBM.
+* ScanPlan      — a prepared rr_zscan launch (descriptor, outputs, workspace)
+  that can be replayed.
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+import os
+
+import numpy as np
+
+from . import _hip
+from .igm import igm_applies, lyman_table
+from .rebin import centers2edges, rebin_into
+
+HUGE_CHI2 = 9999999.0
+HALO = 8
+
+
+def _roundup(n, m):
+    return (n + m - 1) // m * m
+
+
+def default_precision():
+    return os.environ.get('RR_PRECISION', 'f64')
+
+
+def spectral_data(spectra):
+    """Concatenated (weights, flux, wflux) of a list of spectra (zscan.py:192-209)."""
+    weights = np.concatenate([s.ivar for s in spectra])
+    flux = np.concatenate([s.flux for s in spectra])
+    return weights, flux, weights * flux
+
+
+def calc_negOII_penalty(OIItemplate, coeff):
+    """Penalty for negative [OII] flux (zscan.py:783-800)."""
+    if hasattr(coeff, 'detach'):
+        import torch
+        oii = torch.as_tensor(OIItemplate, device=coeff.device, dtype=coeff.dtype)
+        OIIflux = (coeff @ oii).sum(dim=1)
+        return -OIIflux * (OIIflux < 0)
+    OIIflux = np.sum(coeff @ OIItemplate, axis=1)
+    return -OIIflux * (OIIflux < 0)
+
+
+class PixelLayout(object):
+    """Concatenated pixel axis of a target: segments (wavehash, length) in order."""
+
+    def __init__(self, keys, sizes):
+        self.keys = tuple(keys)
+        self.sizes = tuple(int(n) for n in sizes)
+        self.offsets = tuple(int(o) for o in np.concatenate([[0], np.cumsum(self.sizes)])[:-1])
+        self.npix = int(sum(self.sizes))
+        self.npix_pad = _roundup(self.npix, 32)
+        self.halo = HALO
+        self.cache_rows = self.npix_pad + 32
+
+    @property
+    def signature(self):
+        return (self.keys, self.sizes)
+
+    @staticmethod
+    def of(spectra):
+        return PixelLayout([s.wavehash for s in spectra], [s.nwave for s in spectra])
+
+
+class TemplateCache(object):
+    """A template rebinned to a redshift grid on every wavelength grid, in HBM.
+
+    Equivalent of DistTemplate's ``local.data`` (templates.py:584-597): the
+    rebin (rebin.py:453) and the IGM multiply (igm.py:369) are one rr_rebin
+    launch per wavelength grid.  Raises ValueError like the reference when a
+    wavelength grid leaves the redshifted template range.
+    """
+
+    def __init__(self, template, dwave, redshifts=None, check_range=True):
+        self.template = template
+        self.dwave = dict(dwave)
+        self.redshifts = np.ascontiguousarray(
+            template.redshifts if redshifts is None else redshifts, dtype=np.float64)
+        self.nz = len(self.redshifts)
+        self.nb = template.nbasis
+        self.edges = {k: centers2edges(w) for k, w in self.dwave.items()}
+        if check_range and self.nz > 0:
+            xmin = template.minwave * (1 + self.redshifts.max())
+            xmax = template.maxwave * (1 + self.redshifts.min())
+            for e in self.edges.values():
+                if e[0] < xmin or e[-1] > xmax:
+                    raise ValueError('edges must be within input x range')
+        fmax = float(np.max(np.abs(template.flux)))
+        self.tscale = 2.0 ** (8 - math.ceil(math.log2(fmax))) if fmax > 0 else 1.0
+        self._dev = None
+        self._bufs = dict()
+
+    def lyman(self, key):
+        """Line table for grid `key`, or None when transmission_Lyman returns None."""
+        model = self.template.igm_model
+        if not igm_applies(self.redshifts, self.dwave[key], model):
+            return None
+        return lyman_table(model)
+
+    def device(self):
+        import torch
+        if self._dev is None:
+            dev = torch.device('cuda')
+            x, y = self.template.device_arrays() if hasattr(self.template, 'device_arrays') else (
+                torch.as_tensor(self.template.wave, device=dev),
+                torch.as_tensor(np.ascontiguousarray(self.template.flux), device=dev))
+            self._dev = dict(
+                x=x, y=y, z=torch.as_tensor(self.redshifts, device=dev),
+                edges={k: torch.as_tensor(e, device=dev) for k, e in self.edges.items()},
+                wave={k: torch.as_tensor(np.asarray(w, dtype=np.float64), device=dev)
+                      for k, w in self.dwave.items()})
+        return self._dev
+
+    def layout(self, keys):
+        return PixelLayout(keys, [len(self.dwave[k]) for k in keys])
+
+    def get(self, layout, precision):
+        """Device cache for `layout` in the layout of `precision`'s kernel.
+
+        f64 : float64 [nb][cache_rows][ldz]   (z fastest), ldz = nz rounded to 64
+        f16/bf16: 16-bit [nz][nb][cache_rows] (pixel fastest), values * tscale
+        Returns (tensor, ldz).
+        """
+        import torch
+        key = (layout.signature, precision)
+        if key in self._bufs:
+            return self._bufs[key]
+        d = self.device()
+        nb, nz, rows = self.nb, self.nz, layout.cache_rows
+        if precision == 'f64':
+            ldz = _roundup(max(nz, 1), 64)
+            buf = torch.zeros(nb * rows * ldz, dtype=torch.float64, device='cuda')
+            for k, off in zip(layout.keys, layout.offsets):
+                rebin_into(buf[(layout.halo + off) * ldz:], d['x'], d['y'], d['edges'][k], d['z'],
+                           1, ldz, rows * ldz, _hip.RR_F64, 1.0, self.lyman(k), d['wave'][k])
+        else:
+            ldz = 0
+            buf = torch.zeros(nz * nb * rows, dtype=torch.int16, device='cuda')
+            ot = _hip.RR_F16 if precision == 'f16' else _hip.RR_BF16
+            for k, off in zip(layout.keys, layout.offsets):
+                rebin_into(buf[layout.halo + off:], d['x'], d['y'], d['edges'][k], d['z'],
+                           nb * rows, 1, rows, ot, self.tscale, self.lyman(k), d['wave'][k])
+        self._bufs[key] = (buf, ldz)
+        return self._bufs[key]
+
+    def release(self):
+        self._bufs.clear()
+
+    def reference_layout(self):
+        """{wavehash: (nz, nwave, nbasis) float64 device tensor} (DistTemplate.local.data)."""
+        import torch
+        d = self.device()
+        out = dict()
+        for k, w in self.dwave.items():
+            n = len(w)
+            t = torch.empty((self.nz, n, self.nb), dtype=torch.float64, device='cuda')
+            rebin_into(t, d['x'], d['y'], d['edges'][k], d['z'], n * self.nb, self.nb, 1,
+                       _hip.RR_F64, 1.0, self.lyman(k), d['wave'][k])
+            out[k] = t
+        return out
+
+
+class TargetBatch(object):
+    """Per-target scan inputs in HBM for targets that share one PixelLayout.
+
+    rband (nt, npix_pad, 11), w, f, wf (nt, npix_pad) float64; pixels past
+    npix have zero weight.  w and f are the concatenated ivar and flux
+    (spectral_data, zscan.py:192-209); wf = w*f.
+    """
+
+    def __init__(self, targets=None, layout=None, arrays=None):
+        import torch
+        if arrays is not None:
+            layout, rband, w, f, wf = arrays
+        else:
+            layout = layout or PixelLayout.of(targets[0].spectra)
+            nt, P = len(targets), layout.npix_pad
+            rband = np.zeros((nt, P, 11))
+            w = np.zeros((nt, P))
+            f = np.zeros((nt, P))
+            for i, tg in enumerate(targets):
+                if PixelLayout.of(tg.spectra).signature != layout.signature:
+                    raise ValueError('targets of one batch must share their wavelength grids')
+                for s, off in zip(tg.spectra, layout.offsets):
+                    n = s.nwave
+                    rband[i, off:off + n] = s.rband()
+                    w[i, off:off + n] = s.ivar
+                    f[i, off:off + n] = s.flux
+            wf = w * f
+        self.layout = layout
+        self.nt = rband.shape[0]
+        dev = torch.device('cuda')
+        self.rband = torch.as_tensor(np.ascontiguousarray(rband), dtype=torch.float64, device=dev)
+        self.w = torch.as_tensor(np.ascontiguousarray(w), dtype=torch.float64, device=dev)
+        self.f = torch.as_tensor(np.ascontiguousarray(f), dtype=torch.float64, device=dev)
+        self.wf = torch.as_tensor(np.ascontiguousarray(wf), dtype=torch.float64, device=dev)
+        self._mixed = dict()
+
+    def mixed(self, precision):
+        """(rtile, sfcol, scales) for a mixed precision (rr_pack_mixed), cached."""
+        import torch
+        if precision not in self._mixed:
+            P = self.layout.npix_pad
+            rtile = torch.empty(self.nt * (P // 16) * 64 * 8, dtype=torch.int16, device='cuda')
+            sfcol = torch.empty((self.nt, P, 2), dtype=torch.float32, device='cuda')
+            scales = torch.empty((self.nt, 2), dtype=torch.float64, device='cuda')
+            rc = _hip.lib().rr_pack_mixed(_hip.ptr(self.rband), _hip.ptr(self.w), _hip.ptr(self.f),
+                                          self.nt, P, _hip.PRECISIONS[precision], _hip.ptr(rtile),
+                                          _hip.ptr(sfcol), _hip.ptr(scales), _hip.stream_handle())
+            _hip.check(rc, 'rr_pack_mixed')
+            self._mixed[precision] = (rtile, sfcol, scales)
+        return self._mixed[precision]
+
+
+class ScanPlan(object):
+    """A prepared rr_zscan launch: descriptor, outputs and workspace in HBM.
+
+    items: (target_slot, cache_z0, nz) per item; output rows of item i are
+    item_out[i] .. item_out[i]+nz-1 in zchi2/zcoeff/penalty/status.
+    """
+
+    def __init__(self, cache, batch, item_target, item_z0, item_nz, precision='f64',
+                 prior=None, oiisum=None):
+        import torch
+        if precision not in _hip.PRECISIONS:
+            raise ValueError(f'unknown precision {precision!r}')
+        dev = torch.device('cuda')
+        nb = cache.nb
+        if nb > _hip.RR_MAX_NBASIS:
+            raise NotImplementedError(f'nbasis={nb} > {_hip.RR_MAX_NBASIS}')
+        layout = batch.layout
+        tbuf, ldz = cache.get(layout, precision)
+        item_target = np.asarray(item_target, dtype=np.int32)
+        item_z0 = np.asarray(item_z0, dtype=np.int32)
+        item_nz = np.asarray(item_nz, dtype=np.int32)
+        if np.any(item_z0 < 0) or np.any(item_z0 + item_nz > cache.nz):
+            raise ValueError('item redshift range outside the template cache')
+        if np.any(item_target < 0) or np.any(item_target >= batch.nt):
+            raise ValueError('item target outside the batch')
+        item_out = np.concatenate([[0], np.cumsum(item_nz.astype(np.int64))])[:-1]
+        rows = int(item_nz.sum())
+        self.nitems = len(item_nz)
+        self.rows = rows
+        self.nb = nb
+        self.item_out = item_out
+        self.item_nz = item_nz
+        self._keep = [tbuf, batch]
+        self.t_item_target = torch.as_tensor(item_target, device=dev)
+        self.t_item_z0 = torch.as_tensor(item_z0, device=dev)
+        self.t_item_nz = torch.as_tensor(item_nz, device=dev)
+        self.t_item_out = torch.as_tensor(item_out, dtype=torch.int64, device=dev)
+        self.zchi2 = torch.empty(rows, dtype=torch.float64, device=dev)
+        self.zcoeff = torch.empty((rows, nb), dtype=torch.float64, device=dev)
+        self.penalty = torch.empty(rows, dtype=torch.float64, device=dev)
+        self.status = torch.empty(rows, dtype=torch.uint8, device=dev)
+        self.prior = None if prior is None else torch.as_tensor(
+            np.ascontiguousarray(prior, dtype=np.float64), device=dev)
+        self.oiisum = None if oiisum is None else torch.as_tensor(
+            np.ascontiguousarray(oiisum, dtype=np.float64), device=dev)
+        d = _hip.ZscanDesc()
+        d.nitems = self.nitems
+        d.npix = layout.npix
+        d.npix_pad = layout.npix_pad
+        d.nb = nb
+        d.precision = _hip.PRECISIONS[precision]
+        d.halo = layout.halo
+        d.d_tcache = tbuf.data_ptr()
+        d.cache_rows = layout.cache_rows
+        d.cache_ldz = ldz
+        d.cache_nz = cache.nz
+        d.tscale = cache.tscale
+        d.d_rband = batch.rband.data_ptr()
+        d.d_w = batch.w.data_ptr()
+        d.d_wf = batch.wf.data_ptr()
+        d.d_f = batch.f.data_ptr()
+        if precision != 'f64':
+            rtile, sfcol, scales = batch.mixed(precision)
+            d.d_rtile, d.d_sfcol, d.d_scales = rtile.data_ptr(), sfcol.data_ptr(), scales.data_ptr()
+            self._keep += [rtile, sfcol, scales]
+        d.uniform_z = int(self.nitems > 0 and np.all(item_z0 == item_z0[0])
+                          and np.all(item_nz == item_nz[0]))
+        d.d_item_target = self.t_item_target.data_ptr()
+        d.d_item_z0 = self.t_item_z0.data_ptr()
+        d.d_item_nz = self.t_item_nz.data_ptr()
+        d.d_item_out = self.t_item_out.data_ptr()
+        d.d_prior = None if self.prior is None else self.prior.data_ptr()
+        d.d_oiisum = None if self.oiisum is None else self.oiisum.data_ptr()
+        d.d_zchi2 = self.zchi2.data_ptr()
+        d.d_zcoeff = self.zcoeff.data_ptr()
+        d.d_penalty = self.penalty.data_ptr()
+        d.d_status = self.status.data_ptr()
+        d.max_item_nz = int(item_nz.max()) if self.nitems else 0
+        d.total_rows = rows
+        nbytes = _hip.lib().rr_zscan_workspace_bytes(ctypes.byref(d))
+        if nbytes < 0:
+            _hip.check(int(nbytes), 'rr_zscan_workspace_bytes')
+        self.work = torch.empty(max(int(nbytes), 256), dtype=torch.uint8, device=dev)
+        d.d_work = self.work.data_ptr()
+        d.work_bytes = self.work.numel()
+        self.desc = d
+
+    def run(self, stream=None, stages=0):
+        """Enqueue the scan on `stream` (default: torch's current stream).
+
+        stages: RR_STAGE_* bits (0 = stats, main and finish).
+        """
+        self.desc.stages = stages
+        rc = _hip.lib().rr_zscan(ctypes.byref(self.desc), _hip.stream_handle(stream))
+        _hip.check(rc, 'rr_zscan')
+        return self
+
+    def host(self):
+        """(zchi2, zcoeff, penalty, status) copied to numpy."""
+        return (self.zchi2.cpu().numpy(), self.zcoeff.cpu().numpy(),
+                self.penalty.cpu().numpy(), self.status.cpu().numpy())
+
+
+def _oiisum(template):
+    if hasattr(template, 'OIItemplate'):
+        return np.asarray(template.OIItemplate, dtype=np.float64).sum(axis=1)
+    return None
+
+
+def scan_targets(cache, targets, precision=None, oiisum=None, batches=None):
+    """Scan every target over every redshift of `cache`.
+
+    Targets are grouped by wavelength layout (one launch per layout).
+    Returns numpy (zchi2 (nt, nz), zcoeff (nt, nz, nb), penalty (nt, nz)).
+    """
+    precision = precision or default_precision()
+    nt, nz, nb = len(targets), cache.nz, cache.nb
+    zchi2 = np.zeros((nt, nz))
+    zcoeff = np.zeros((nt, nz, nb))
+    penalty = np.zeros((nt, nz))
+    groups = dict()
+    for i, tg in enumerate(targets):
+        groups.setdefault(PixelLayout.of(tg.spectra).signature, []).append(i)
+    for sig, idx in groups.items():
+        batch = None if batches is None else batches.get(sig)
+        if batch is None:
+            batch = TargetBatch([targets[i] for i in idx])
+        n = len(idx)
+        plan = ScanPlan(cache, batch, np.arange(n), np.zeros(n), np.full(n, nz), precision,
+                        oiisum=oiisum)
+        plan.run()
+        c2, cf, pen, _ = plan.host()
+        zchi2[idx] = c2.reshape(n, nz)
+        zcoeff[idx] = cf.reshape(n, nz, nb)
+        penalty[idx] = pen.reshape(n, nz)
+    return zchi2, zcoeff, penalty
+
+
+def calc_zchi2(target_ids, target_data, dtemplate, progress=None, use_gpu=False, precision=None):
+    """chi2 vs redshift of every target for one template (zscan.py:807-889).
+
+    Returns (zchi2[nt, nz], zcoeff[nt, nz, nbasis], zchi2penalty[nt, nz]).
+    """
+    cache = dtemplate.cache
+    oii = _oiisum(dtemplate.template)
+    zchi2, zcoeff, penalty = scan_targets(cache, list(target_data), precision, oii)
+    if progress is not None and dtemplate.comm is None:
+        for _ in range(len(target_data)):
+            progress.put(1)
+    return zchi2, zcoeff, penalty
+
+
+def calc_zchi2_batch(spectra, tdata, weights, flux, wflux, nz, nbasis, solve_matrices_algorithm=None,
+                     solver_args=None, use_gpu=False, fullprecision=True, prior=None,
+                     precision='f64'):
+    """chi2 and coefficients of one target at every redshift of `tdata` (zscan.py:626-781).
+
+    tdata: {wavehash: (nz, nwave, nbasis)} numpy arrays or device tensors.
+    Returns numpy (zchi2 (nz,), zcoeff (nz, nbasis)).
+    """
+    import torch
+    w = _host(weights)
+    zchi2 = np.zeros(nz)
+    if w.sum() == 0:
+        zchi2[:] = HUGE_CHI2
+        return zchi2, np.zeros((nz, nbasis))
+    algo = 'PCA' if solve_matrices_algorithm is None else solve_matrices_algorithm.upper()
+    if algo in ('NMF', 'NNLS', 'BVLS'):
+        raise NotImplementedError(f'{algo} solves are outside the chi2-scan path')
+    if algo != 'PCA':
+        # solve_matrices raises NotImplementedError, caught by the caller (zscan.py:721-724)
+        zchi2[:] = HUGE_CHI2
+        return zchi2, np.zeros((nz, nbasis))
+    layout = PixelLayout.of(spectra)
+    P = layout.npix_pad
+    rband = np.zeros((1, P, 11))
+    for s, off in zip(spectra, layout.offsets):
+        rband[0, off:off + s.nwave] = s.rband()
+    wa = np.zeros((1, P))
+    fa = np.zeros((1, P))
+    wfa = np.zeros((1, P))
+    wa[0, :layout.npix] = w
+    fa[0, :layout.npix] = _host(flux)
+    wfa[0, :layout.npix] = _host(wflux)
+    batch = TargetBatch(arrays=(layout, rband, wa, fa, wfa))
+    cache = _ExplicitCache(tdata, layout, nz, nbasis)
+    plan = ScanPlan(cache, batch, [0], [0], [nz], precision, prior=prior)
+    plan.run()
+    c2, cf, _, _ = plan.host()
+    return c2, cf
+
+
+def _host(a):
+    if hasattr(a, 'detach'):
+        return a.detach().cpu().numpy()
+    return np.asarray(a)
+
+
+class _ExplicitCache(object):
+    """Template cache assembled from an explicit tdata dict (calc_zchi2_batch)."""
+
+    def __init__(self, tdata, layout, nz, nb):
+        self.tdata = tdata
+        self.nz = nz
+        self.nb = nb
+        vals = [_host(v) for v in tdata.values()]
+        fmax = max(float(np.max(np.abs(v))) if v.size else 0.0 for v in vals)
+        self.tscale = 2.0 ** (8 - math.ceil(math.log2(fmax))) if fmax > 0 else 1.0
+        self._bufs = dict()
+
+    def get(self, layout, precision):
+        import torch
+        key = (layout.signature, precision)
+        if key in self._bufs:
+            return self._bufs[key]
+        nb, nz, rows = self.nb, self.nz, layout.cache_rows
+        if precision == 'f64':
+            ldz = _roundup(max(nz, 1), 64)
+            buf = torch.zeros((nb, rows, ldz), dtype=torch.float64, device='cuda')
+            for k, off, n in zip(layout.keys, layout.offsets, layout.sizes):
+                t = torch.as_tensor(self.tdata[k], dtype=torch.float64, device='cuda')
+                buf[:, layout.halo + off:layout.halo + off + n, :nz] = t.permute(2, 1, 0)
+            buf = buf.reshape(-1)
+        else:
+            ldz = 0
+            buf = torch.zeros((nz, nb, rows), dtype=torch.float32, device='cuda')
+            for k, off, n in zip(layout.keys, layout.offsets, layout.sizes):
+                t = torch.as_tensor(self.tdata[k], dtype=torch.float64, device='cuda')
+                buf[:, :, layout.halo + off:layout.halo + off + n] = (
+                    t.permute(0, 2, 1) * self.tscale).float()
+            dt = torch.float16 if precision == 'f16' else torch.bfloat16
+            buf = buf.to(dt).view(torch.int16).reshape(-1)
+        self._bufs[key] = (buf, ldz)
+        return self._bufs[key]
+
+
+def calc_zchi2_targets(targets, templates, mp_procs=1, use_gpu=False, precision=None):
+    """Coarse chi2 scan of every local target for every template (zscan.py:910-1105).
+
+    Returns results[targetid][fulltype] = dict(redshifts, zchi2, penalty, zcoeff).
+    """
+    results = {tid: dict() for tid in targets.local_target_ids()}
+    local = targets.local()
+    for t in templates:
+        ft = t.template.full_type
+        zchi2, zcoeff, penalty = calc_zchi2(targets.local_target_ids(), local, t,
+                                            precision=precision)
+        for i, tg in enumerate(local):
+            results[tg.id][ft] = dict(redshifts=t.template.redshifts, zchi2=zchi2[i],
+                                      penalty=penalty[i], zcoeff=zcoeff[i])
+    return results

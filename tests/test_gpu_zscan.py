@@ -1,0 +1,142 @@
+"""GPU chi2 scan (rr_zscan) against the float64 oracle.
+
+Plain FP64: |delta chi2|/chi2 <= 1e-9 and identical per-target best-z index.
+Mixed precision (fp16 / bf16 MFMA): tolerances stated per test.
+"""
+import numpy as np
+import pytest
+
+from oracle import rr_oracle as orc
+
+pytestmark = pytest.mark.gpu
+
+F64_RTOL = 1e-9
+
+
+@pytest.fixture(scope='module')
+def galaxy(gpu):
+    from redrock_amd import synth
+    from redrock_amd.templates import DistTemplate
+    tm = synth.galaxy_template()
+    targets, zt = synth.make_targets(tm, 3, seed=11)
+    dwave = {s.wavehash: s.wave for s in targets[0].spectra}
+    dt = DistTemplate(tm, dwave)
+    tdata = orc.template_cache(tm, dwave)
+    ref = orc.calc_zchi2(targets, tdata, tm.nbasis, tm.OIItemplate)
+    return tm, targets, dt, tdata, ref
+
+
+def test_f64_matches_oracle(galaxy):
+    from redrock_amd import zscan
+    tm, targets, dt, tdata, (rchi2, rcoeff, rpen) = galaxy
+    print('case: galaxy f64, 3 targets x', tm.redshifts.size, 'z', flush=True)
+    chi2, coeff, pen = zscan.calc_zchi2([t.id for t in targets], targets, dt, precision='f64')
+    rel = np.abs(chi2 - rchi2) / np.abs(rchi2)
+    print('max rel chi2 err', rel.max())
+    assert rel.max() <= F64_RTOL
+    assert np.array_equal(chi2.argmin(axis=1), rchi2.argmin(axis=1))
+    assert np.allclose(coeff, rcoeff, rtol=1e-7, atol=1e-9 * np.abs(rcoeff).max())
+    assert np.allclose(pen, rpen, rtol=1e-7, atol=1e-9 * max(np.abs(rpen).max(), 1.0))
+
+
+def test_f64_deterministic(galaxy):
+    """Run-to-run bit identity (the reference checks mp_procs=1 vs 2, test_zscan.py:113-140)."""
+    from redrock_amd import zscan
+    tm, targets, dt, *_ = galaxy
+    a = zscan.calc_zchi2([0], targets[:2], dt, precision='f64')
+    b = zscan.calc_zchi2([0], targets[:2], dt, precision='f64')
+    for x, y in zip(a, b):
+        assert np.array_equal(x, y)
+
+
+@pytest.mark.parametrize('precision,tol', [('f16', 1e-4), ('bf16', 2e-3)])
+def test_mixed_matches_oracle(galaxy, precision, tol):
+    from redrock_amd import zscan
+    tm, targets, dt, tdata, (rchi2, rcoeff, rpen) = galaxy
+    print(f'case: galaxy {precision}', flush=True)
+    chi2, coeff, pen = zscan.calc_zchi2([t.id for t in targets], targets, dt, precision=precision)
+    rel = np.abs(chi2 - rchi2) / np.abs(rchi2)
+    ibest = rchi2.argmin(axis=1)
+    print(f'{precision}: max rel err {rel.max():.3e}, at best z {rel[np.arange(3), ibest].max():.3e}')
+    assert rel.max() <= tol
+    assert np.array_equal(chi2.argmin(axis=1), ibest)
+
+
+def test_calc_zchi2_batch_with_prior(gpu):
+    """Explicit tdata + prior (zscan.py:711-712,761-762) on a multi-exposure layout."""
+    from redrock_amd import synth, zscan
+    from redrock_amd.targets import Spectrum, Target
+    tm = synth.galaxy_template()
+    tg, _ = synth.make_targets(tm, 1, seed=5)
+    # two exposures per camera: repeated wavehash in the pixel axis
+    spectra = []
+    for s in tg[0].spectra:
+        for e in range(2):
+            spectra.append(Spectrum(s.wave, s.flux + 0.1 * e, s.ivar.copy(), s.R, band=s.band))
+    dwave = {s.wavehash: s.wave for s in spectra}
+    zsel = tm.redshifts[300:360]
+    tdata = orc.template_cache(tm, dwave, myz=zsel)
+    w, f, wf = orc.spectral_data(spectra)
+    prior = np.diag(np.linspace(0.0, 2.0, tm.nbasis))
+    rchi2, rcoeff = orc.calc_zchi2_batch(spectra, tdata, w, f, wf, len(zsel), tm.nbasis, prior=prior)
+    chi2, coeff = zscan.calc_zchi2_batch(spectra, tdata, w, f, wf, len(zsel), tm.nbasis, 'PCA',
+                                         prior=prior)
+    assert np.max(np.abs(chi2 - rchi2) / rchi2) <= F64_RTOL
+    assert np.allclose(coeff, rcoeff, rtol=1e-7, atol=1e-9 * np.abs(rcoeff).max())
+
+
+def test_huge_chi2_conventions(gpu):
+    """Zero weights -> HUGE everywhere; a vanishing basis -> singular -> HUGE (zscan.py:657-660,769-771)."""
+    import torch
+    from redrock_amd import synth, zscan
+    from redrock_amd.templates import Template, DistTemplate
+    base = synth.galaxy_template()
+    flux = base.flux.copy()
+    flux[3, :] = 0.0   # basis 3 identically zero -> M singular at every z
+    tm = Template(spectype='GALAXY', redshifts=base.redshifts[500:530], wave=base.wave,
+                  flux=flux, igm_model='None')
+    targets, _ = synth.make_targets(base, 2, seed=3)
+    for s in targets[1].spectra:
+        s.ivar[:] = 0.0
+    dwave = {s.wavehash: s.wave for s in targets[0].spectra}
+    dt = DistTemplate(tm, dwave)
+    for precision in ('f64', 'f16'):
+        chi2, coeff, pen = zscan.calc_zchi2([0, 1], targets, dt, precision=precision)
+        assert np.all(chi2 == zscan.HUGE_CHI2)
+        assert np.all(coeff == 0) and np.all(pen == 0)
+    tdata = orc.template_cache(tm, dwave)
+    rchi2, _, _ = orc.calc_zchi2(targets, tdata, tm.nbasis)
+    assert np.all(rchi2 == orc.HUGE_CHI2)
+
+
+def test_qso_igm_f64(gpu):
+    """QSO with the fused Lyman-alpha IGM (nb=4) against the oracle on a z subset."""
+    from redrock_amd import synth, zscan
+    from redrock_amd.templates import Template, DistTemplate
+    q = synth.qso_template()
+    tm = Template(spectype='QSO', redshifts=q.redshifts[2000:2400], wave=q.wave, flux=q.flux,
+                  igm_model=q.igm_model)
+    targets, _ = synth.make_targets(q, 2, seed=21, zrange=(2.3, 2.9))
+    dwave = {s.wavehash: s.wave for s in targets[0].spectra}
+    dt = DistTemplate(tm, dwave)
+    chi2, coeff, _ = zscan.calc_zchi2([0, 1], targets, dt, precision='f64')
+    tdata = orc.template_cache(tm, dwave, lines=[(1215.67, 0.0036, 3.46)])
+    rchi2, rcoeff, _ = orc.calc_zchi2(targets, tdata, tm.nbasis)
+    assert np.max(np.abs(chi2 - rchi2) / rchi2) <= F64_RTOL
+    assert np.array_equal(chi2.argmin(axis=1), rchi2.argmin(axis=1))
+    c16, _, _ = zscan.calc_zchi2([0, 1], targets, dt, precision='f16')
+    assert np.max(np.abs(c16 - rchi2) / rchi2) <= 1e-4
+
+
+def test_star_f64(gpu):
+    from redrock_amd import synth, zscan
+    from redrock_amd.templates import DistTemplate
+    tm = synth.star_template()
+    targets, _ = synth.make_targets(tm, 2, seed=8)
+    dwave = {s.wavehash: s.wave for s in targets[0].spectra}
+    dt = DistTemplate(tm, dwave)
+    chi2, _, _ = zscan.calc_zchi2([0, 1], targets, dt, precision='f64')
+    tdata = orc.template_cache(tm, dwave)
+    rchi2, _, _ = orc.calc_zchi2(targets, tdata, tm.nbasis)
+    assert np.max(np.abs(chi2 - rchi2) / rchi2) <= F64_RTOL
+    assert np.array_equal(chi2.argmin(axis=1), rchi2.argmin(axis=1))
