@@ -186,8 +186,9 @@ long long rr_zscan_workspace_bytes(const rr_zscan_desc *desc);
  * (:135-162,:502-541) and calc_negOII_penalty (:783-800); with many items it
  * also replaces the per-target loop of zscan.calc_zchi2 (:853-887).
  *
- * chi2 = f'Wf - c'y (compensated FP64 sum), equal to the reference's
- * explicit residual sum (:780) at the solution.  Singular M (exactly zero LU
+ * chi2 = f'Wf - 2 c'y + c'Mc with the unregularised M (compensated FP64
+ * sum): the reference's explicit residual sum(w (f - A c)^2) (:777-780)
+ * expanded, exact with a prior too.  Singular M (exactly zero LU
  * pivot, np.linalg.solve's LinAlgError) gives chi2 = HUGE_CHI2, zero coeffs
  * and status RR_STATUS_SINGULAR (zscan.py:769-771).  Zero total weight gives
  * HUGE_CHI2 and zero coeffs with status OK (zscan.py:657-660).

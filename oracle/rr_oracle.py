@@ -141,7 +141,7 @@ def trapz_rebin(x, y, xnew=None, edges=None, myz=None, xmin=None, xmax=None,
         myz = np.array([myz], dtype=np.float64)
         scalar_z = True
     else:
-        myz = np.asarray(myz, dtype=np.float64)
+        myz = np.ascontiguousarray(myz, dtype=np.float64)
     nz = len(myz)
     if nz == 0:
         return np.zeros((0, nbins, 1), dtype=np.float64)

@@ -102,7 +102,15 @@ def stream_handle(stream=None):
 
 
 def ptr(t):
-    """Device pointer of a torch tensor (None -> NULL)."""
+    """Device pointer of a contiguous device tensor (None -> NULL).
+
+    The kernels index flat memory, so strided views are rejected rather than
+    silently misread.
+    """
     if t is None:
         return None
+    if not t.is_contiguous():
+        raise ValueError('device arrays passed to libredrock_hip must be contiguous')
+    if not t.is_cuda:
+        raise ValueError('libredrock_hip needs device (cuda) tensors')
     return ctypes.c_void_p(t.data_ptr())

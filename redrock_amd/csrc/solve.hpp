@@ -103,18 +103,24 @@ __device__ __forceinline__ void two_sum(double a, double b, double &s, double &e
     e = (a - (s - bb)) + (b - bb);
 }
 
-// Returns fwf - sum_b c[b]*y[b] with the products and the running sum
-// carried in double-double (Ogita-Rump-Oishi Dot2).
+// chi2(c) = fwf - 2 c'y + c'Mc for the unregularised M (no prior): the
+// reference's explicit residual sum(w (f - A c)^2) expanded, stationary in c
+// so that solve errors enter only at second order.  The final sum
+// fwf + sum_b c_b (t_b - 2 y_b), t = M c, is carried in double-double
+// (Ogita-Rump-Oishi Dot2) because fwf and c'y cancel at high S/N.
 template <int NB>
-__device__ __forceinline__ double compensated_chi2(double fwf, const double (&c)[NB], const double (&y)[NB]) {
+__device__ __forceinline__ double compensated_chi2(double fwf, const double (&c)[NB],
+                                                   const double (&y)[NB],
+                                                   const double (&t)[NB]) {
     double s = fwf, comp = 0.0;
 #pragma unroll
     for (int b = 0; b < NB; ++b) {
-        const double p = -c[b] * y[b];
-        const double pe = fma(-c[b], y[b], -p);
-        double t, e;
-        two_sum(s, p, t, e);
-        s = t;
+        const double g = t[b] - 2.0 * y[b];
+        const double p = c[b] * g;
+        const double pe = fma(c[b], g, -p);
+        double u, e;
+        two_sum(s, p, u, e);
+        s = u;
         comp += e + pe;
     }
     return s + comp;

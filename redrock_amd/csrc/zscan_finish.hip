@@ -10,8 +10,9 @@
 //              zero pivot -> HUGE_CHI2 and zero coefficients (:769-771).
 //   mixed    : Cholesky (M is a Gram matrix), LU when Cholesky meets a
 //              non-positive pivot.
-// chi2 = f'Wf - c'y evaluated with a compensated dot product; it equals the
-// reference's explicit residual sum(w (f - A c)^2) (:777-780) at the solution.
+// chi2 = f'Wf - 2 c'y + c'Mc with the unregularised M (compensated sum): the
+// reference's explicit residual sum(w (f - A c)^2) (:777-780) expanded, which
+// stays exact with a prior and is stationary in c.
 #include "common.hpp"
 #include "solve.hpp"
 #include "zscan_args.hpp"
@@ -59,37 +60,37 @@ __global__ __launch_bounds__(64) void finish_kernel(rr::ZArgs a, long long ne_of
     const double sumw = st[2 * item + 1];
     const char *wbase = static_cast<const char *>(a.work) + ne_off;
 
-    double M[NB][NB], y[NB], c[NB];
+    // element (i, j), i <= j, of the unregularised [M | y] in FP64
+    double ms = 1.0, ys = 1.0;
     if constexpr (MIXED) {
-        const float *ne = reinterpret_cast<const float *>(wbase);
         const int t = a.item_target[item];
         const double rscale = a.scales[2 * t], fscale = a.scales[2 * t + 1];
         // cache tiles hold tscale*T, R tiles sqrt(w)*rscale*R, f columns
         // sqrt(w)*f*fscale:  G_bb = (tscale rscale)^2 M, G_bf = tscale rscale fscale y
-        const double ms = 1.0 / (a.tscale * a.tscale * rscale * rscale);
-        const double ys = 1.0 / (a.tscale * rscale * fscale);
+        ms = 1.0 / (a.tscale * a.tscale * rscale * rscale);
+        ys = 1.0 / (a.tscale * rscale * fscale);
+    }
+    auto m_at = [&](int i, int j) -> double {
+        if constexpr (MIXED)
+            return (double)reinterpret_cast<const float *>(wbase)[rr::ne_index(i, j, NB, true) * R + row] * ms;
+        else
+            return reinterpret_cast<const double *>(wbase)[rr::ne_index(i, j, NB, false) * R + row];
+    };
+    double M[NB][NB], y[NB], c[NB];
 #pragma unroll
-        for (int i = 0; i < NB; ++i) {
+    for (int i = 0; i < NB; ++i) {
 #pragma unroll
-            for (int j = i; j < NB; ++j) {
-                const double v = (double)ne[rr::ne_index(i, j, NB, true) * R + row] * ms;
-                M[i][j] = v;
-                M[j][i] = v;
-            }
+        for (int j = i; j < NB; ++j) {
+            const double v = m_at(i, j);
+            M[i][j] = v;
+            M[j][i] = v;
+        }
+        if constexpr (MIXED) {
+            const float *ne = reinterpret_cast<const float *>(wbase);
             y[i] = ((double)ne[rr::ne_index(i, NB, NB, true) * R + row] +
                     (double)ne[rr::ne_index(i, NB + 1, NB, true) * R + row]) * ys;
-        }
-    } else {
-        const double *ne = reinterpret_cast<const double *>(wbase);
-#pragma unroll
-        for (int i = 0; i < NB; ++i) {
-#pragma unroll
-            for (int j = i; j < NB; ++j) {
-                const double v = ne[rr::ne_index(i, j, NB, false) * R + row];
-                M[i][j] = v;
-                M[j][i] = v;
-            }
-            y[i] = ne[rr::ne_index(i, NB, NB, false) * R + row];
+        } else {
+            y[i] = reinterpret_cast<const double *>(wbase)[rr::ne_index(i, NB, NB, false) * R + row];
         }
     }
     if (a.prior) {
@@ -128,7 +129,16 @@ __global__ __launch_bounds__(64) void finish_kernel(rr::ZArgs a, long long ne_of
 #pragma unroll
         for (int b = 0; b < NB; ++b) c[b] = 0.0;
     } else {
-        chi2 = rr::compensated_chi2<NB>(fwf, c, y);
+        // t = M c with the unregularised M re-read from the workspace
+        double tv[NB];
+#pragma unroll
+        for (int i = 0; i < NB; ++i) {
+            double s = 0.0;
+#pragma unroll
+            for (int j = 0; j < NB; ++j) s = fma(i <= j ? m_at(i, j) : m_at(j, i), c[j], s);
+            tv[i] = s;
+        }
+        chi2 = rr::compensated_chi2<NB>(fwf, c, y, tv);
     }
     a.zchi2[row] = chi2;
 #pragma unroll

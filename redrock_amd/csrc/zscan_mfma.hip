@@ -23,8 +23,8 @@
 // operand layout expects for column l%16, so no data moves between the two
 // stages.  G holds M = A'WA (b,b < nb) and y = A'Wf (b, nb / nb+1), scaled;
 // the kernel writes them to the workspace and zscan_finish.hip unscales them,
-// solves M c = y by FP64 Cholesky (LU fallback) and forms chi2 = f'Wf - c'y
-// with f'Wf summed exactly in FP64.
+// solves M c = y by FP64 Cholesky (LU fallback) and forms
+// chi2 = f'Wf - 2 c'y + c'Mc with f'Wf summed exactly in FP64.
 //
 // Reference path replaced: zscan.py:670-737 (the GPU path) — the M/y
 // contractions run in 16-bit on MFMA with fp32 accumulation, everything after

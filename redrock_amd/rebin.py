@@ -45,12 +45,15 @@ def rebin_into(out, x, y, edges, z, s_z, s_pix, s_b, out_type=_hip.RR_F64, scale
                lyman=None, wave=None, stream=None):
     """Launch rr_rebin writing element (iz, ibin, ib) at out + iz*s_z + ibin*s_pix + ib*s_b.
 
-    x, y, edges, z, wave are float64 device tensors; out is a device tensor
-    (or a view) of the element type `out_type`.  No range check here.
+    x (nx), y (nb, nx), edges, z, wave are contiguous float64 device tensors;
+    out is a contiguous device tensor (or a flat view) of the element type
+    `out_type`.  No range check here.
     """
     lines, nlines = lyman_array(lyman)
-    nb = y.shape[0]
-    rc = _hip.lib().rr_rebin(_hip.ptr(x), x.numel(), _hip.ptr(y), nb, y.stride(0),
+    nb, nx = y.shape
+    if x.numel() != nx:
+        raise ValueError('x and y have different lengths')
+    rc = _hip.lib().rr_rebin(_hip.ptr(x), nx, _hip.ptr(y), nb, nx,
                              _hip.ptr(edges), edges.numel() - 1,
                              _hip.ptr(wave) if nlines else None,
                              _hip.ptr(z), z.numel(), lines, nlines, _hip.ptr(out), out_type,
@@ -105,10 +108,10 @@ def trapz_rebin(x, y, xnew=None, edges=None, myz=None, use_gpu=False, xmin=None,
     if not scalar_z:
         result_shape = (nz,) + result_shape
     dev = torch.device('cuda')
-    xt = torch.as_tensor(xh, device=dev)
-    yt = torch.as_tensor(y2, device=dev)
-    et = torch.as_tensor(edges, device=dev)
-    zt = torch.as_tensor(myz, device=dev)
+    xt = torch.as_tensor(np.ascontiguousarray(xh), device=dev).contiguous()
+    yt = torch.as_tensor(y2, device=dev).contiguous()
+    et = torch.as_tensor(np.ascontiguousarray(edges), device=dev).contiguous()
+    zt = torch.as_tensor(np.ascontiguousarray(myz), device=dev).contiguous()
     out = torch.empty((nz, nbins, nbasis), dtype=torch.float64, device=dev)
     rebin_into(out, xt, yt, et, zt, nbins * nbasis, nbasis, 1)
     out = out.reshape(result_shape)

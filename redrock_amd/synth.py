@@ -101,11 +101,34 @@ def _gauss(w, mu, sig):
     return np.exp(-0.5 * ((w - mu) / sig) ** 2)
 
 
+def _texture(wave, rng, corr=15.0):
+    """Unit-rms smooth random field (Gaussian correlation length `corr` A).
+
+    Stands in for the stellar absorption structure of real templates: it keeps
+    every basis vector independent inside any observed window, so the normal
+    matrix stays full rank at every trial redshift (lines alone leave the
+    window at high z and make M exactly singular).
+    """
+    from scipy.ndimage import gaussian_filter1d
+    step = wave[1] - wave[0]
+    t = gaussian_filter1d(rng.normal(size=wave.size), corr / step, mode='nearest')
+    return t / np.sqrt(np.mean(t * t))
+
+
+def _with_texture(comps, wave, rng, amp):
+    out = []
+    for c in comps:
+        c = np.asarray(c, dtype=np.float64)
+        out.append(c + amp * np.sqrt(np.mean(c * c)) * _texture(wave, rng))
+    return out
+
+
 def galaxy_template(seed=1):
     rng = np.random.default_rng(seed)
     wave = np.linspace(1200.0, 11000.0, 19601)
     comps = [(wave / 5000.0) ** a for a in (-2.0, -1.0, 0.0, 1.0, 2.0)]
     comps += [_gauss(wave, mu, 2.0) for mu in (3727.0, 4861.0, 4959.0, 5007.0, 6563.0)]
+    comps = _with_texture(comps, wave, rng, 0.3)
     flux = _orthonormalise(comps, rng)
     z = np.exp(np.arange(np.log(1.005), np.log(2.7), 5e-4)) - 1
     return Template(spectype='GALAXY', redshifts=z, wave=wave, flux=flux,
@@ -120,6 +143,7 @@ def qso_template(seed=2):
              _gauss(wave, 1216.0, 20.0) + 0.3 * _gauss(wave, 1549.0, 20.0),
              _gauss(wave, 1549.0, 20.0) + 0.5 * _gauss(wave, 1909.0, 20.0),
              _gauss(wave, 1909.0, 20.0) + _gauss(wave, 2799.0, 20.0)]
+    comps = _with_texture(comps, wave, rng, 0.3)
     flux = _orthonormalise(comps, rng, mix=False)
     z = np.exp(np.arange(np.log(1.05), np.log(7.0), 5e-4)) - 1
     return Template(spectype='QSO', redshifts=z, wave=wave, flux=flux,
@@ -136,6 +160,7 @@ def star_template(seed=3):
     for mu in (3934.8, 3969.6, 4102.9, 4341.7, 4862.7, 6564.6, 8500.4, 8544.4, 8664.5):
         lines -= _gauss(wave, mu, 3.0)
     comps.append(lines)
+    comps = _with_texture(comps, wave, rng, 0.1)
     flux = _orthonormalise(comps, rng)
     z = np.linspace(-0.005, 0.005, 501)
     return Template(spectype='STAR', redshifts=z, wave=wave, flux=flux,

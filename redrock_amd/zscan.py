@@ -128,14 +128,13 @@ class TemplateCache(object):
         import torch
         if self._dev is None:
             dev = torch.device('cuda')
-            x, y = self.template.device_arrays() if hasattr(self.template, 'device_arrays') else (
-                torch.as_tensor(self.template.wave, device=dev),
-                torch.as_tensor(np.ascontiguousarray(self.template.flux), device=dev))
+            def up(a):
+                return torch.as_tensor(np.ascontiguousarray(a, dtype=np.float64),
+                                       device=dev).contiguous()
             self._dev = dict(
-                x=x, y=y, z=torch.as_tensor(self.redshifts, device=dev),
-                edges={k: torch.as_tensor(e, device=dev) for k, e in self.edges.items()},
-                wave={k: torch.as_tensor(np.asarray(w, dtype=np.float64), device=dev)
-                      for k, w in self.dwave.items()})
+                x=up(self.template.wave), y=up(self.template.flux), z=up(self.redshifts),
+                edges={k: up(e) for k, e in self.edges.items()},
+                wave={k: up(w) for k, w in self.dwave.items()})
         return self._dev
 
     def layout(self, keys):

@@ -104,6 +104,16 @@ def test_trapz_matches_exact_integral(seed):
             assert np.allclose(r[:, b], _exact_bin_means(xs, y[b], edges), rtol=1e-11, atol=1e-12)
 
 
+def test_trapz_strided_redshifts():
+    """A strided z view gives the same result as its contiguous copy (regression)."""
+    x = np.linspace(1000, 2000, 501)
+    y = np.sin(x / 37.0)
+    z = np.linspace(0, 1, 40)
+    e = np.linspace(2100, 2600, 30)
+    assert np.array_equal(orc.trapz_rebin(x, y, edges=e, myz=z[::7]),
+                          orc.trapz_rebin(x, y, edges=e, myz=z[::7].copy()))
+
+
 def _constant_density_ok(rebin):
     x = np.arange(10) * 1.1
     return all(np.all(rebin(x, np.ones(10), edges=np.linspace(0.5, 8.3, n)) == 1.0)
