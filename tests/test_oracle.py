@@ -108,8 +108,8 @@ def test_trapz_strided_redshifts():
     """A strided z view gives the same result as its contiguous copy (regression)."""
     x = np.linspace(1000, 2000, 501)
     y = np.sin(x / 37.0)
-    z = np.linspace(0, 1, 40)
-    e = np.linspace(2100, 2600, 30)
+    z = np.linspace(0.2, 0.3, 40)
+    e = np.linspace(1400, 2300, 30)
     assert np.array_equal(orc.trapz_rebin(x, y, edges=e, myz=z[::7]),
                           orc.trapz_rebin(x, y, edges=e, myz=z[::7].copy()))
 
