@@ -118,7 +118,8 @@ typedef struct {
      *  F64  : d_tcache is double, element (b, row, zc) at
      *         (b*cache_rows + row)*cache_ldz + zc           ([b][row][z])
      *  F16/BF16: d_tcache is 16-bit, element (zc, b, row) at
-     *         (zc*nb + b)*cache_rows + row                  ([z][b][row])
+     *         (zc*nb + b)*cache_rows + row                  ([z][b][row]),
+     *         followed by at least 64 zero elements
      * row = pixel + halo.  cache_nz is the extent of the z axis.            */
     const void *d_tcache;
     long long cache_rows;
@@ -134,10 +135,13 @@ typedef struct {
     /* Mixed precision modes only (NULL otherwise), written by rr_pack_mixed: */
     const void *d_rtile;     /* (ntarget, npix_pad/16, 64, 8) 16-bit MFMA
                                 A-operand tiles of sqrt(w) R                    */
-    const float *d_sfcol;    /* (ntarget, npix_pad, 2) hi/lo of sqrt(w) f       */
+    const void *d_sfcol;     /* (ntarget, npix_pad/32, 4, 2, 8) 16-bit hi/lo
+                                of sqrt(w) f in MFMA operand order              */
     const double *d_scales;  /* (ntarget, 2) power-of-two scales of R and f     */
     int uniform_z;           /* 1 if every item has the same z0 and nz: lets a
-                                wavefront share template tiles across 2 items   */
+                                wavefront share template tiles across items     */
+    int tile;                /* mixed: tile configuration, 0 = default (see
+                                rr_zscan_tiles)                                  */
 
     /* Work list, one entry per item. */
     const int *d_item_target;  /* target slot                                   */
@@ -207,8 +211,17 @@ int rr_zscan(const rr_zscan_desc *desc, void *stream);
  * precision is RR_PREC_F16 or RR_PREC_BF16.  Replaces the per-call CSR upload
  * of zscan.py:428-430 (done once per target here).
  */
+/*
+ * rr_zscan_tiles — the tile configurations of the mixed-precision scan
+ * kernel: entry i (i >= 1) processes tw[i] items x zw[i] redshifts per
+ * wavefront; entry 0 is the default choice.  Host arrays of `max` entries;
+ * returns the number of configurations.  Used by the autotuner to pick
+ * rr_zscan_desc.tile per (nbasis, pixel count, precision).
+ */
+int rr_zscan_tiles(int *tw, int *zw, int max);
+
 int rr_pack_mixed(const double *d_rband, const double *d_w, const double *d_f, int ntarget,
-                  int npix_pad, int precision, void *d_rtile, float *d_sfcol,
+                  int npix_pad, int precision, void *d_rtile, void *d_sfcol,
                   double *d_scales, void *stream);
 
 #ifdef __cplusplus

@@ -21,7 +21,7 @@ RR_MAX_LYMAN_LINES = 8
 PRECISIONS = {'f64': RR_PREC_F64, 'f16': RR_PREC_F16, 'bf16': RR_PREC_BF16}
 
 EXPORTS = ('rr_abi_version', 'rr_last_error', 'rr_rebin', 'rr_zscan',
-           'rr_zscan_workspace_bytes', 'rr_pack_mixed')
+           'rr_zscan_workspace_bytes', 'rr_pack_mixed', 'rr_zscan_tiles')
 
 _vp = ctypes.c_void_p
 _i = ctypes.c_int
@@ -40,7 +40,7 @@ class ZscanDesc(ctypes.Structure):
         ('d_tcache', _vp), ('cache_rows', _ll), ('cache_ldz', _ll), ('cache_nz', _i),
         ('tscale', _d),
         ('d_rband', _vp), ('d_w', _vp), ('d_wf', _vp), ('d_f', _vp),
-        ('d_rtile', _vp), ('d_sfcol', _vp), ('d_scales', _vp), ('uniform_z', _i),
+        ('d_rtile', _vp), ('d_sfcol', _vp), ('d_scales', _vp), ('uniform_z', _i), ('tile', _i),
         ('d_item_target', _vp), ('d_item_z0', _vp), ('d_item_nz', _vp), ('d_item_out', _vp),
         ('d_prior', _vp), ('d_oiisum', _vp),
         ('d_zchi2', _vp), ('d_zcoeff', _vp), ('d_penalty', _vp), ('d_status', _vp),
@@ -81,6 +81,8 @@ def lib():
     L.rr_zscan_workspace_bytes.argtypes = [ctypes.POINTER(ZscanDesc)]
     L.rr_pack_mixed.restype = _i
     L.rr_pack_mixed.argtypes = [_vp, _vp, _vp, _i, _i, _i, _vp, _vp, _vp, _vp]
+    L.rr_zscan_tiles.restype = _i
+    L.rr_zscan_tiles.argtypes = [ctypes.POINTER(_i), ctypes.POINTER(_i), _i]
     v = L.rr_abi_version()
     if v != RR_ABI_VERSION:
         raise HipError(f'libredrock_hip ABI {v}, expected {RR_ABI_VERSION}')

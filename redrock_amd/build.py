@@ -18,9 +18,12 @@ BUILD = os.path.join(HERE, '_build')
 LIB = os.path.join(HERE, 'libredrock_hip.so')
 HIPCC = os.environ.get('HIPCC', '/opt/rocm/bin/hipcc')
 
-# Packed FP32 instructions are disabled for all device code.
+# Packed FP32 instructions are disabled for all device code.  MFMA results are
+# kept in VGPRs (the scan kernel converts every R-apply result with VALU ops,
+# and the AGPR form would copy each one out and back).
 COMMON = ['--offload-arch=gfx950', '-std=c++17', '-fPIC',
           '-Xclang', '-target-feature', '-Xclang', '-packed-fp32-ops',
+          '-mllvm', '-amdgpu-mfma-vgpr-form',
           '-I', os.path.join(os.path.dirname(HERE), 'include'),
           '-Wno-unused-command-line-argument']
 

@@ -67,6 +67,7 @@ extern "C" int rr_zscan(const rr_zscan_desc *d, void *stream) {
     a.rband = d->d_rband; a.w = d->d_w; a.wf = d->d_wf; a.f = d->d_f;
     a.rtile = d->d_rtile; a.sfcol = d->d_sfcol; a.scales = d->d_scales;
     a.uniform_z = d->uniform_z;
+    a.tile = d->tile;
     a.item_target = d->d_item_target; a.item_z0 = d->d_item_z0; a.item_nz = d->d_item_nz;
     a.item_out = d->d_item_out; a.prior = d->d_prior; a.oiisum = d->d_oiisum;
     a.zchi2 = d->d_zchi2; a.zcoeff = d->d_zcoeff; a.penalty = d->d_penalty; a.status = d->d_status;

@@ -12,9 +12,10 @@ struct ZArgs {
     double tscale;
     const double *rband, *w, *wf, *f;
     const void *rtile;
-    const float *sfcol;
+    const void *sfcol;
     const double *scales;
     int uniform_z;
+    int tile;
     const int *item_target, *item_z0, *item_nz;
     const long long *item_out;
     const double *prior, *oiisum;

@@ -73,8 +73,9 @@ __device__ __forceinline__ double wave_max(double v) {
 //   rtile  (ntarget, ntile, 64, 8) 16-bit: lane l of tile q holds row i=l%16,
 //          K slots v=0..7 of group g=l/16 -> input pixel 16q-8+8g+v; value
 //          sqrt(w_p) * rscale * R[p][k'] with p = 16q+i, k' = 8g+v-i-3.
-//   sfcol  (ntarget, npix_pad, 2) f32: hi/lo 16-bit split of
-//          sqrt(w_p) f_p * fscale, hi exactly representable in 16 bits.
+//   sfcol  (ntarget, npix_pad/32, 4, 2, 8) 16-bit: hi/lo split of
+//          sqrt(w_p) f_p * fscale in the Gram-operand order of the lanes of
+//          columns nb (hi) and nb+1 (lo).
 //   scales (ntarget, 2) f64: rscale, fscale (powers of two).
 // ---------------------------------------------------------------------------
 template <int TY>
@@ -82,7 +83,7 @@ __global__ __launch_bounds__(64) void pack_mixed_kernel(const double *__restrict
                                                         const double *__restrict__ w,
                                                         const double *__restrict__ f, int npix_pad,
                                                         unsigned short *__restrict__ rtile,
-                                                        float *__restrict__ sfcol,
+                                                        unsigned short *__restrict__ sfcol,
                                                         double *__restrict__ scales) {
     const int t = blockIdx.x;
     const int lane = threadIdx.x;
@@ -119,13 +120,23 @@ __global__ __launch_bounds__(64) void pack_mixed_kernel(const double *__restrict
         }
         *reinterpret_cast<u16x8 *>(RT + ((long long)q * 64 + lane) * 8) = v8;
     }
-    float *SF = sfcol + (long long)t * npix_pad * 2;
-    for (int p = lane; p < npix_pad; p += 64) {
-        const float s = (float)(sqrt(W[p]) * F[p] * fscale);
-        const float hi = from16<TY>(to16<TY>(s));
-        const float lo = from16<TY>(to16<TY>(s - hi));
-        SF[2 * p] = hi;
-        SF[2 * p + 1] = lo;
+    // sqrt(w) f as the Gram operand of lanes (j = nb + h, g) of tile pair qp:
+    // slot v < 4 is pixel 32qp + 4g + v, slot v >= 4 is pixel 32qp + 16 + 4g + v-4.
+    const int npair = npix_pad / 32;
+    unsigned short *SF = sfcol + (long long)t * npair * 64;
+    for (int e = lane; e < npair * 4; e += 64) {
+        const int qp = e >> 2, gg = e & 3;
+        u16x8 hi8, lo8;
+#pragma unroll
+        for (int v = 0; v < 8; ++v) {
+            const int p = 32 * qp + (v < 4 ? 4 * gg + v : 16 + 4 * gg + (v - 4));
+            const float s = (float)(sqrt(W[p]) * F[p] * fscale);
+            const unsigned short h = to16<TY>(s);
+            hi8[v] = h;
+            lo8[v] = to16<TY>(s - from16<TY>(h));
+        }
+        *reinterpret_cast<u16x8 *>(SF + ((long long)e * 2) * 8) = hi8;
+        *reinterpret_cast<u16x8 *>(SF + ((long long)e * 2 + 1) * 8) = lo8;
     }
 }
 
@@ -135,8 +146,7 @@ __global__ __launch_bounds__(64) void pack_mixed_kernel(const double *__restrict
 // same redshift block, so their template tiles are shared through L1/L2.
 // ---------------------------------------------------------------------------
 template <int NB, int TY, int TW, int ZW>
-__global__ __launch_bounds__(256) void zscan_mfma_kernel(rr::ZArgs a, const float *__restrict__ sfcol,
-                                                         long long ne_off) {
+__global__ __launch_bounds__(256) void zscan_mfma_kernel(rr::ZArgs a, long long ne_off) {
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int item0 = (blockIdx.x * 4 + wave) * TW;
@@ -148,11 +158,12 @@ __global__ __launch_bounds__(256) void zscan_mfma_kernel(rr::ZArgs a, const floa
 
     const int j = lane & 15;   // column (basis) of operand B / output
     const int g = lane >> 4;   // K group / output row group
-    const long long npp = a.npix_pad;
     const int ntile = a.npix_pad / 16;
+    const int npair = ntile / 2;
     const long long rows = a.cache_rows;
     const unsigned short *__restrict__ T = static_cast<const unsigned short *>(a.tcache);
     const unsigned short *__restrict__ RT = static_cast<const unsigned short *>(a.rtile);
+    const unsigned short *__restrict__ SF = static_cast<const unsigned short *>(a.sfcol);
 
     int tgt[TW];
     bool live[TW];
@@ -161,13 +172,18 @@ __global__ __launch_bounds__(256) void zscan_mfma_kernel(rr::ZArgs a, const floa
         live[u] = (item0 + u) < a.nitems;
         tgt[u] = a.item_target[live[u] ? item0 + u : item0];
     }
-    // cache z index of each of the wave's redshifts (clamped past the end)
-    long long zoff[ZW];
+    // Template operand of lane (j, g): rows 16q+8g.. of basis j.  Lanes with
+    // j >= nb read the zero block that follows the cache (no branch, no
+    // per-iteration zeroing), so their output columns stay exactly zero.
+    const bool real = j < NB;
+    const long long qstep = real ? 16 : 0;
+    const unsigned short *tb[ZW];
 #pragma unroll
     for (int s = 0; s < ZW; ++s) {
         const int iz = zbase + s;
         const int zc = z0 + (iz < nzi ? iz : nzi - 1);
-        zoff[s] = ((long long)zc * NB + (j < NB ? j : 0)) * rows + 8 * g;
+        tb[s] = real ? T + ((long long)zc * NB + j) * rows + 8 * g
+                     : T + (long long)a.cache_nz * NB * rows;
     }
 
     float4v G[TW][ZW];
@@ -176,47 +192,38 @@ __global__ __launch_bounds__(256) void zscan_mfma_kernel(rr::ZArgs a, const floa
 #pragma unroll
         for (int s = 0; s < ZW; ++s) G[u][s] = float4v{0.f, 0.f, 0.f, 0.f};
 
-    const u16x8 zero8 = {0, 0, 0, 0, 0, 0, 0, 0};
+    // lanes of columns nb, nb+1 feed sqrt(w) f (hi, lo) into the Gram instead
+    // of their (zero) R-apply output
     const bool fcolumn = (j == NB) || (j == NB + 1);
-    const int fslot = (j == NB + 1) ? 1 : 0;
+    const int fsel = fcolumn ? j - NB : 0;
+    const float4v zero4 = {0.f, 0.f, 0.f, 0.f};
 
     for (int q = 0; q < ntile; q += 2) {
-        u16x8 ra[TW][2];
-        float4v cf[TW][2];
+        u16x8 ra[TW][2], fx[TW];
 #pragma unroll
         for (int u = 0; u < TW; ++u) {
             const unsigned short *rt = RT + ((long long)tgt[u] * ntile + q) * 64 * 8 + lane * 8;
             ra[u][0] = *reinterpret_cast<const u16x8 *>(rt);
             ra[u][1] = *reinterpret_cast<const u16x8 *>(rt + 64 * 8);
-#pragma unroll
-            for (int h = 0; h < 2; ++h) {
-                float4v c = {0.f, 0.f, 0.f, 0.f};
-                if (fcolumn) {
-                    const float *sf = sfcol + ((long long)tgt[u] * npp + 16 * (q + h) + 4 * g) * 2 + fslot;
-#pragma unroll
-                    for (int r = 0; r < 4; ++r) c[r] = sf[2 * r];
-                }
-                cf[u][h] = c;
-            }
+            fx[u] = *reinterpret_cast<const u16x8 *>(
+                SF + ((((long long)tgt[u] * npair + (q >> 1)) * 4 + g) * 2 + fsel) * 8);
         }
 #pragma unroll
         for (int s = 0; s < ZW; ++s) {
-            u16x8 b0 = zero8, b1 = zero8;
-            if (j < NB) {
-                const unsigned short *tp = T + zoff[s] + 16 * q;
-                b0 = *reinterpret_cast<const u16x8 *>(tp);
-                b1 = *reinterpret_cast<const u16x8 *>(tp + 16);
-            }
+            const unsigned short *tp = tb[s] + q * qstep;
+            const u16x8 b0 = *reinterpret_cast<const u16x8 *>(tp);
+            const u16x8 b1 = *reinterpret_cast<const u16x8 *>(tp + qstep);
 #pragma unroll
             for (int u = 0; u < TW; ++u) {
-                const float4v d0 = mfma<TY>(ra[u][0], b0, cf[u][0]);
-                const float4v d1 = mfma<TY>(ra[u][1], b1, cf[u][1]);
+                const float4v d0 = mfma<TY>(ra[u][0], b0, zero4);
+                const float4v d1 = mfma<TY>(ra[u][1], b1, zero4);
                 u16x8 x;
 #pragma unroll
                 for (int r = 0; r < 4; ++r) {
                     x[r] = to16<TY>(d0[r]);
                     x[4 + r] = to16<TY>(d1[r]);
                 }
+                x = fcolumn ? fx[u] : x;
                 G[u][s] = mfma<TY>(x, x, G[u][s]);
             }
         }
@@ -245,38 +252,52 @@ __global__ __launch_bounds__(256) void zscan_mfma_kernel(rr::ZArgs a, const floa
     }
 }
 
-template <int NB, int TY, int TW>
-int launch3(const rr::ZArgs &a, const float *sf, const double *sc, hipStream_t s) {
-    constexpr int ZW = 8;
+// Tile configurations (items per wavefront TW, redshifts per wavefront ZW),
+// selected by rr_zscan_desc.tile (0 = default); TW > 1 needs uniform_z.
+struct TileCfg {
+    int tw, zw;
+};
+constexpr TileCfg kTiles[] = {{2, 8}, {1, 8}, {2, 4}, {2, 8}, {4, 4}, {4, 8}, {1, 16}};
+constexpr int kNumTiles = sizeof(kTiles) / sizeof(kTiles[0]);
+
+template <int NB, int TY, int TW, int ZW>
+int launch_tile(const rr::ZArgs &a, hipStream_t s) {
     const int per_wg = 4 * TW;
     dim3 block(256);
     dim3 grid((a.nitems + per_wg - 1) / per_wg, (a.max_item_nz + ZW - 1) / ZW);
     const long long ne_off = rr::align256((long long)a.nitems * 2 * 8);
-    hipLaunchKernelGGL((zscan_mfma_kernel<NB, TY, TW, ZW>), grid, block, 0, s, a, sf, ne_off);
+    hipLaunchKernelGGL((zscan_mfma_kernel<NB, TY, TW, ZW>), grid, block, 0, s, a, ne_off);
     return rr::check_launch("rr_zscan(mixed)");
 }
 
 template <int NB, int TY>
-int launch2(const rr::ZArgs &a, const float *sf, const double *sc, int tw, hipStream_t s) {
-    if (tw == 2) return launch3<NB, TY, 2>(a, sf, sc, s);
-    return launch3<NB, TY, 1>(a, sf, sc, s);
+int launch_nb(const rr::ZArgs &a, int tile, hipStream_t s) {
+    switch (tile) {
+        case 1: return launch_tile<NB, TY, 1, 8>(a, s);
+        case 2: return launch_tile<NB, TY, 2, 4>(a, s);
+        case 3: return launch_tile<NB, TY, 2, 8>(a, s);
+        case 4: return launch_tile<NB, TY, 4, 4>(a, s);
+        case 5: return launch_tile<NB, TY, 4, 8>(a, s);
+        case 6: return launch_tile<NB, TY, 1, 16>(a, s);
+        default: return rr::fail(RR_EINVAL, "rr_zscan(mixed): unknown tile %d", tile);
+    }
 }
 
 template <int TY>
-int launch1(const rr::ZArgs &a, const float *sf, const double *sc, int tw, hipStream_t s) {
+int launch_ty(const rr::ZArgs &a, int tile, hipStream_t s) {
     switch (a.nb) {
-        case 1: return launch2<1, TY>(a, sf, sc, tw, s);
-        case 2: return launch2<2, TY>(a, sf, sc, tw, s);
-        case 3: return launch2<3, TY>(a, sf, sc, tw, s);
-        case 4: return launch2<4, TY>(a, sf, sc, tw, s);
-        case 5: return launch2<5, TY>(a, sf, sc, tw, s);
-        case 6: return launch2<6, TY>(a, sf, sc, tw, s);
-        case 7: return launch2<7, TY>(a, sf, sc, tw, s);
-        case 8: return launch2<8, TY>(a, sf, sc, tw, s);
-        case 9: return launch2<9, TY>(a, sf, sc, tw, s);
-        case 10: return launch2<10, TY>(a, sf, sc, tw, s);
-        case 11: return launch2<11, TY>(a, sf, sc, tw, s);
-        case 12: return launch2<12, TY>(a, sf, sc, tw, s);
+        case 1: return launch_nb<1, TY>(a, tile, s);
+        case 2: return launch_nb<2, TY>(a, tile, s);
+        case 3: return launch_nb<3, TY>(a, tile, s);
+        case 4: return launch_nb<4, TY>(a, tile, s);
+        case 5: return launch_nb<5, TY>(a, tile, s);
+        case 6: return launch_nb<6, TY>(a, tile, s);
+        case 7: return launch_nb<7, TY>(a, tile, s);
+        case 8: return launch_nb<8, TY>(a, tile, s);
+        case 9: return launch_nb<9, TY>(a, tile, s);
+        case 10: return launch_nb<10, TY>(a, tile, s);
+        case 11: return launch_nb<11, TY>(a, tile, s);
+        case 12: return launch_nb<12, TY>(a, tile, s);
         default: return rr::fail(RR_EUNSUPPORTED, "rr_zscan(mixed): nb=%d unsupported", a.nb);
     }
 }
@@ -285,15 +306,27 @@ int launch1(const rr::ZArgs &a, const float *sf, const double *sc, int tw, hipSt
 
 namespace rr {
 int zscan_mfma(const ZArgs &a, hipStream_t s) {
-    const int tw = a.uniform_z ? 2 : 1;
-    if (a.precision == RR_PREC_F16) return launch1<RR_PREC_F16>(a, a.sfcol, a.scales, tw, s);
-    return launch1<RR_PREC_BF16>(a, a.sfcol, a.scales, tw, s);
+    int tile = a.tile;
+    if (tile < 0 || tile >= kNumTiles) return fail(RR_EINVAL, "rr_zscan: tile %d outside 0..%d", tile, kNumTiles - 1);
+    if (tile == 0) tile = a.uniform_z ? 3 : 1;
+    if (!a.uniform_z && kTiles[tile].tw > 1) tile = (kTiles[tile].zw == 16) ? 6 : 1;
+    if (a.precision == RR_PREC_F16) return launch_ty<RR_PREC_F16>(a, tile, s);
+    return launch_ty<RR_PREC_BF16>(a, tile, s);
 }
 }  // namespace rr
 
+extern "C" int rr_zscan_tiles(int *tw, int *zw, int max) {
+    const int n = kNumTiles < max ? kNumTiles : max;
+    for (int i = 0; i < n; ++i) {
+        tw[i] = kTiles[i].tw;
+        zw[i] = kTiles[i].zw;
+    }
+    return kNumTiles;
+}
+
 extern "C" int rr_pack_mixed(const double *d_rband, const double *d_w, const double *d_f,
                              int ntarget, int npix_pad, int precision, void *d_rtile,
-                             float *d_sfcol, double *d_scales, void *stream) {
+                             void *d_sfcol, double *d_scales, void *stream) {
     if (ntarget < 0 || npix_pad < 32 || (npix_pad % 32) != 0)
         return rr::fail(RR_EINVAL, "rr_pack_mixed: bad sizes ntarget=%d npix_pad=%d", ntarget, npix_pad);
     if (!d_rband || !d_w || !d_f || !d_rtile || !d_sfcol || !d_scales)
@@ -301,12 +334,13 @@ extern "C" int rr_pack_mixed(const double *d_rband, const double *d_w, const dou
     if (ntarget == 0) return RR_OK;
     hipStream_t s = static_cast<hipStream_t>(stream);
     unsigned short *rt = static_cast<unsigned short *>(d_rtile);
+    unsigned short *sf = static_cast<unsigned short *>(d_sfcol);
     if (precision == RR_PREC_F16)
         hipLaunchKernelGGL(pack_mixed_kernel<RR_PREC_F16>, dim3(ntarget), dim3(64), 0, s, d_rband, d_w,
-                           d_f, npix_pad, rt, d_sfcol, d_scales);
+                           d_f, npix_pad, rt, sf, d_scales);
     else if (precision == RR_PREC_BF16)
         hipLaunchKernelGGL(pack_mixed_kernel<RR_PREC_BF16>, dim3(ntarget), dim3(64), 0, s, d_rband, d_w,
-                           d_f, npix_pad, rt, d_sfcol, d_scales);
+                           d_f, npix_pad, rt, sf, d_scales);
     else
         return rr::fail(RR_EINVAL, "rr_pack_mixed: precision must be RR_PREC_F16 or RR_PREC_BF16");
     return rr::check_launch("rr_pack_mixed");

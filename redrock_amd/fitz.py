@@ -1,0 +1,274 @@
+"""Refinement of chi2 minima (py/redrock/fitz.py) on the GPU, batched over targets.
+
+fitz(zchi2, redshifts, target, template, nminima=3, ...) keeps the reference
+signature (fitz.py:133-395, non-archetype branch).  fitz_targets() refines all
+targets of one template at once:
+
+1. every target's candidate minima come from find_minima (fitz.py:46-67);
+2. the first few candidates of every target are refined speculatively in one
+   pass: one rr_rebin launch per camera builds the template at every
+   candidate's 15-point fine grid (IGM fused), one rr_zscan launch computes
+   their chi2 and coefficients (FP64), a second rebin + scan gives the
+   coefficients at each parabola minimum (fitz.py:265-293);
+3. the reference's sequential control flow — skipping minima within
+   max_velo_diff of accepted ones, the parabola fit, zwarn bits, dedup and
+   sort — is replayed per target on the host from those results.  If a
+   target skips more candidates than were precomputed, another pass
+   refines the next ones, so the outcome is the reference's.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from . import constants
+from .zwarning import ZWarningMask as ZW
+
+
+def get_dv(z, zref):
+    """Velocity difference in km/s (fitz.py:28-43)."""
+    return constants.c_kms * (z - zref) / (1.0 + zref)
+
+
+def find_minima(x):
+    """Indices of local minima of x including edges, sorted by x (fitz.py:46-67)."""
+    x = np.asarray(x)
+    ii = np.where(np.r_[True, x[1:] <= x[:-1]] & np.r_[x[:-1] <= x[1:], True])[0]
+    jj = np.argsort(x[ii])
+    return ii[jj]
+
+
+def minfit(x, y):
+    """Fit y = y0 + ((x-x0)/xerr)**2 on 3 points (fitz.py:70-111)."""
+    if len(x) < 3:
+        return (-1, -1, -1, ZW.BAD_MINFIT)
+    try:
+        a, b, c = np.polyfit(x, y, 2)
+    except np.linalg.LinAlgError:
+        return (-1, -1, -1, ZW.BAD_MINFIT)
+    if a == 0.0:
+        return (-1, -1, -1, ZW.BAD_MINFIT)
+    x0 = -b / (2 * a)
+    y0 = -(b ** 2) / (4 * a) + c
+    zwarn = 0
+    if (x0 <= np.min(x)) or (np.max(x) <= x0):
+        zwarn |= ZW.BAD_MINFIT
+    if y0 <= 0.:
+        zwarn |= ZW.BAD_MINFIT
+    if a > 0.0:
+        xerr = 1 / np.sqrt(a)
+    else:
+        xerr = 1 / np.sqrt(-a)
+        zwarn |= ZW.BAD_MINFIT
+    return (x0, xerr, y0, zwarn)
+
+
+def _edges_ok(template, zlo, zhi, edges):
+    """rebin.py:332-341 as used by rebin_template with xmin/xmax passed by fitz."""
+    xmin = template.minwave * (1 + zhi)
+    xmax = template.maxwave * (1 + zlo)
+    return all(not (e[0] < xmin or e[-1] > xmax) for e in edges)
+
+
+class _Refiner(object):
+    """GPU refinement of (target, candidate-minimum) pairs for one template."""
+
+    def __init__(self, template, targets, redshifts, nz_fine, precision='f64'):
+        from .zscan import TargetBatch, PixelLayout, _oiisum
+        self.template = template
+        self.targets = targets
+        self.redshifts = np.asarray(redshifts, dtype=np.float64)
+        self.nz_fine = nz_fine
+        self.precision = precision
+        self.dwave = {s.wavehash: s.wave for s in targets[0].spectra}
+        from .rebin import centers2edges
+        self.edges = [centers2edges(w) for w in self.dwave.values()]
+        self.oii = _oiisum(template)
+        self.batches = dict()
+        self.slot = dict()
+        for i, tg in enumerate(targets):
+            sig = PixelLayout.of(tg.spectra).signature
+            self.batches.setdefault(sig, []).append(i)
+        self._tb = {sig: TargetBatch([targets[i] for i in idx]) for sig, idx in self.batches.items()}
+        for sig, idx in self.batches.items():
+            for k, i in enumerate(idx):
+                self.slot[i] = (sig, k)
+
+    def _scan(self, zlist, owners, per_item):
+        """chi2/coeff/penalty at zlist[j*per_item:(j+1)*per_item] for target owners[j]."""
+        from .zscan import TemplateCache, ScanPlan
+        nitem = len(owners)
+        nb = self.template.nbasis
+        chi2 = np.zeros((nitem, per_item))
+        coeff = np.zeros((nitem, per_item, nb))
+        pen = np.zeros((nitem, per_item))
+        if nitem == 0:
+            return chi2, coeff, pen
+        # candidates come from many targets: one cache for all their redshifts
+        dw = dict()
+        for tg in self.targets:
+            for s in tg.spectra:
+                dw.setdefault(s.wavehash, s.wave)
+        cache = TemplateCache(self.template, dw, redshifts=np.asarray(zlist), check_range=False)
+        for sig, idx in self.batches.items():
+            sel = [j for j, t in enumerate(owners) if self.slot[t][0] == sig]
+            if not sel:
+                continue
+            item_t = [self.slot[owners[j]][1] for j in sel]
+            z0 = [j * per_item for j in sel]
+            plan = ScanPlan(cache, self._tb[sig], item_t, z0, [per_item] * len(sel),
+                            self.precision, oiisum=self.oii)
+            plan.run()
+            c2, cf, pn, _ = plan.host()
+            chi2[sel] = c2.reshape(len(sel), per_item)
+            coeff[sel] = cf.reshape(len(sel), per_item, nb)
+            pen[sel] = pn.reshape(len(sel), per_item)
+        return chi2, coeff, pen
+
+    def refine(self, pairs, zchi2):
+        """Refine candidates pairs = [(target, imin)]: returns per-pair dicts.
+
+        A candidate whose fine grid leaves the template range gets
+        {'imin', 'error'}: the reference raises ValueError there
+        (fitz.py:226-231), but only if its control flow reaches the candidate.
+        """
+        nz = self.nz_fine
+        zs = self.redshifts
+        out = [None] * len(pairs)
+        good, zz_all = [], []
+        for j, (t, imin) in enumerate(pairs):
+            ilo = max(0, imin - 1)
+            ihi = min(imin + 1, len(zs) - 1)
+            zz = np.linspace(zs[ilo], zs[ihi], nz)
+            if not _edges_ok(self.template, zz.min(), zz.max(), self.edges):
+                out[j] = dict(imin=imin, error=ValueError('edges must be within input x range'))
+                continue
+            good.append(j)
+            zz_all.append(zz)
+        zzchi2, _, zzpen = self._scan(np.concatenate(zz_all) if zz_all else np.zeros(0),
+                                      [pairs[j][0] for j in good], nz)
+        for k, j in enumerate(good):
+            zz = zz_all[k]
+            c2 = zzchi2[k].copy()
+            if self.oii is not None:
+                c2 += zzpen[k]
+            i = min(max(np.argmin(c2), 1), len(zz) - 2)
+            zmin, sigma, chi2min, zwarn = minfit(zz[i - 1:i + 2], c2[i - 1:i + 2])
+            out[j] = dict(imin=pairs[j][1], zz=zz, zzchi2=c2, zmin=zmin, sigma=sigma,
+                          chi2min=chi2min, zwarn=zwarn)
+        # coefficients at each parabola minimum (fitz.py:265-304)
+        sel = [j for j in good if _edges_ok(self.template, out[j]['zmin'], out[j]['zmin'],
+                                            self.edges)]
+        _, cf, _ = self._scan(np.array([out[j]['zmin'] for j in sel], dtype=np.float64),
+                              [pairs[j][0] for j in sel], 1)
+        for k, j in enumerate(sel):
+            out[j]['coeff'] = cf[k, 0]
+        for j in good:
+            if 'coeff' in out[j]:
+                continue
+            zmin = out[j]['zmin']
+            if zmin < zs[0] or zs[-1] < zmin:
+                # beyond the redshift range: zero coefficients (fitz.py:295-301)
+                out[j]['coeff'] = np.zeros(self.template.nbasis)
+                out[j]['zwarn'] |= ZW.Z_FITLIMIT
+                out[j]['zwarn'] |= ZW.BAD_MINFIT
+            else:
+                out[j]['error'] = ValueError('edges must be within input x range')
+        return out
+
+
+def _finalize(template, target, redshifts, nminima, cands, get):
+    """Sequential control flow of fitz.py:198-395 over precomputed candidates.
+
+    get(k) returns the refined dict of the k-th candidate (computing it on
+    demand).  Returns the reference's dict of stacked arrays.
+    """
+    max_velo_diff = (constants.max_velo_diff_star if template.template_type == 'STAR'
+                     else constants.max_velo_diff)
+    results = list()
+    for k, imin in enumerate(cands):
+        if len(results) == nminima:
+            break
+        zprev = np.array([tmp['z'] for tmp in results])
+        dv = get_dv(z=redshifts[imin], zref=zprev)
+        if np.any(np.abs(dv) < max_velo_diff):
+            continue
+        r = get(k)
+        if 'error' in r:
+            raise r['error']
+        zz, zzchi2 = r['zz'], r['zzchi2']
+        zmin, zwarn, chi2min = r['zmin'], r['zwarn'], r['chi2min']
+        zbest = zmin
+        zerr = r['sigma']
+        if zbest < zz[0] or zbest > zz[-1]:
+            zwarn |= ZW.BAD_MINFIT
+            i = np.where(zzchi2 == np.min(zzchi2))[0][0]
+            zbest = zz[i]
+            chi2min = zzchi2[i]
+        if zbest < redshifts[1] or zbest > redshifts[-2]:
+            zwarn |= ZW.Z_FITLIMIT
+        if zmin < redshifts[1] or zmin > redshifts[-2]:
+            zwarn |= ZW.Z_FITLIMIT
+        zprev = np.array([tmp['z'] for tmp in results])
+        dv = get_dv(z=zbest, zref=zprev)
+        if np.any(np.abs(dv) < max_velo_diff):
+            continue
+        results.append(dict(z=zbest, zerr=zerr, zwarn=zwarn, chi2=chi2min, zz=zz,
+                            zzchi2=zzchi2, coeff=r['coeff'], fitmethod=template.method))
+    ii = np.argsort([tmp['chi2'] for tmp in results])
+    results = [results[i] for i in ii]
+    if len(results) == 0:
+        xfloat = np.zeros((0, 1), dtype=float)
+        xint = np.zeros((0, 1), dtype=int)
+        xstr = np.zeros((0, 1), dtype=str)
+        return dict(z=xfloat, zerr=xfloat, zwarn=xint, chi2=xfloat, zz=xfloat, zzchi2=xfloat,
+                    coeff=xfloat, fitmethod=xstr, npixels=xint)
+    npix = sum(int((s.ivar > 0.).sum()) for s in target.spectra)
+    for r in results:
+        r['npixels'] = npix
+    return {k: np.vstack([r[k] for r in results]) for k in results[0].keys()}
+
+
+def fitz_targets(zchi2, redshifts, targets, template, nminima=3, zminfit_npoints=15,
+                 precision='f64', speculate=2):
+    """fitz for every target of one template (zchi2: (nt, nz) incl. penalty).
+
+    Returns a list with the reference's per-target result dict.
+    """
+    zchi2 = np.atleast_2d(np.asarray(zchi2, dtype=np.float64))
+    assert zchi2.shape == (len(targets), len(redshifts))
+    nz = 15 if zminfit_npoints is None else zminfit_npoints
+    ref = _Refiner(template, targets, redshifts, nz, precision)
+    cands = [find_minima(zchi2[t]) for t in range(len(targets))]
+    done = [dict() for _ in targets]
+
+    def run(pairs):
+        if not pairs:
+            return
+        for (t, imin), r in zip(pairs, ref.refine(pairs, zchi2)):
+            done[t][imin] = r
+
+    first = [(t, int(c)) for t in range(len(targets))
+             for c in cands[t][:nminima + speculate]]
+    run(first)
+    out = []
+    for t, tg in enumerate(targets):
+        def get(k, t=t):
+            imin = int(cands[t][k])
+            if imin not in done[t]:
+                # the sequential logic needs a candidate beyond the speculative set
+                run([(t, int(c)) for c in cands[t][k:k + nminima + speculate]
+                     if int(c) not in done[t]])
+            return done[t][imin]
+        out.append(_finalize(template, tg, redshifts, nminima, cands[t], get))
+    return out
+
+
+def fitz(zchi2, redshifts, target, template, nminima=3, archetype=None, use_gpu=False,
+         deg_legendre=None, zminfit_npoints=15, per_camera=False, n_nearest=None,
+         prior_sigma=None):
+    """Refine the redshift around up to nminima minima (fitz.py:133-395)."""
+    assert len(zchi2) == len(redshifts)
+    if archetype is not None:
+        raise NotImplementedError('archetype fits are outside the chi2-scan path')
+    return fitz_targets(np.asarray(zchi2)[None, :], redshifts, [target], template,
+                        nminima=nminima, zminfit_npoints=zminfit_npoints)[0]

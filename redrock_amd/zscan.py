@@ -161,7 +161,8 @@ class TemplateCache(object):
                            1, ldz, rows * ldz, _hip.RR_F64, 1.0, self.lyman(k), d['wave'][k])
         else:
             ldz = 0
-            buf = torch.zeros(nz * nb * rows, dtype=torch.int16, device='cuda')
+            # + 64 zero elements read by the MFMA lanes of unused basis columns
+            buf = torch.zeros(nz * nb * rows + 64, dtype=torch.int16, device='cuda')
             ot = _hip.RR_F16 if precision == 'f16' else _hip.RR_BF16
             for k, off in zip(layout.keys, layout.offsets):
                 rebin_into(buf[layout.halo + off:], d['x'], d['y'], d['edges'][k], d['z'],
@@ -228,7 +229,7 @@ class TargetBatch(object):
         if precision not in self._mixed:
             P = self.layout.npix_pad
             rtile = torch.empty(self.nt * (P // 16) * 64 * 8, dtype=torch.int16, device='cuda')
-            sfcol = torch.empty((self.nt, P, 2), dtype=torch.float32, device='cuda')
+            sfcol = torch.empty(self.nt * (P // 32) * 64, dtype=torch.int16, device='cuda')
             scales = torch.empty((self.nt, 2), dtype=torch.float64, device='cuda')
             rc = _hip.lib().rr_pack_mixed(_hip.ptr(self.rband), _hip.ptr(self.w), _hip.ptr(self.f),
                                           self.nt, P, _hip.PRECISIONS[precision], _hip.ptr(rtile),
@@ -246,10 +247,11 @@ class ScanPlan(object):
     """
 
     def __init__(self, cache, batch, item_target, item_z0, item_nz, precision='f64',
-                 prior=None, oiisum=None):
+                 prior=None, oiisum=None, tile=0):
         import torch
         if precision not in _hip.PRECISIONS:
             raise ValueError(f'unknown precision {precision!r}')
+        self.tile = tile
         dev = torch.device('cuda')
         nb = cache.nb
         if nb > _hip.RR_MAX_NBASIS:
@@ -303,6 +305,7 @@ class ScanPlan(object):
             rtile, sfcol, scales = batch.mixed(precision)
             d.d_rtile, d.d_sfcol, d.d_scales = rtile.data_ptr(), sfcol.data_ptr(), scales.data_ptr()
             self._keep += [rtile, sfcol, scales]
+        d.tile = int(tile)
         d.uniform_z = int(self.nitems > 0 and np.all(item_z0 == item_z0[0])
                           and np.all(item_nz == item_nz[0]))
         d.d_item_target = self.t_item_target.data_ptr()
@@ -469,7 +472,8 @@ class _ExplicitCache(object):
                 buf[:, :, layout.halo + off:layout.halo + off + n] = (
                     t.permute(0, 2, 1) * self.tscale).float()
             dt = torch.float16 if precision == 'f16' else torch.bfloat16
-            buf = buf.to(dt).view(torch.int16).reshape(-1)
+            buf = torch.cat([buf.to(dt).view(torch.int16).reshape(-1),
+                             torch.zeros(64, dtype=torch.int16, device='cuda')])
         self._bufs[key] = (buf, ldz)
         return self._bufs[key]
 
