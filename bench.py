@@ -141,11 +141,16 @@ def main():
         torch.cuda.synchronize()
         return sum(e0.elapsed_time(e1) for e0, e1 in evs) / k
 
+    from redrock_amd import autotune
+    mix = plans[args.precision]
+    tile, tile_ms = autotune.choose(mix, use_cache=False)
+    table = autotune.tile_table()
+    log(f'autotune: tile {tile} {table[tile]} ; ' +
+        ', '.join(f'{table[t]}: {ms:.2f} ms' for t, ms in sorted(tile_ms.items())))
     for p, plan in plans.items():
         for _ in range(args.warmup):
             plan.run()
     torch.cuda.synchronize()
-    mix = plans[args.precision]
     t_mix = timed(mix, args.steps)
     k_mix = kernel_ms(mix, args.steps)
     f64 = plans['f64']
@@ -187,7 +192,9 @@ def main():
         'config': {'workload': f'galaxy petal: {nt} GALAXY targets x {nz} z x {nb} basis, '
                                f'3 cameras 7958 px, 11-diagonal R',
                    'targets': nt, 'redshifts': nz, 'nbasis': nb, 'snr_per_pixel': args.snr,
-                   'solve': 'FP64 Cholesky', 'accumulate': 'fp32'},
+                   'solve': 'FP64 Cholesky', 'accumulate': 'fp32 per 256 px, then FP64',
+                   'tile': {'items_per_wave': table[tile][0], 'z_per_wave': table[tile][1]}},
+        'autotune_ms': {f'{table[t][0]}x{table[t][1]}': ms for t, ms in tile_ms.items()},
         'fp64': {'value': evals / t_f64, 'ms_per_step': t_f64 * 1e3, 'kernel_ms': k_f64,
                  'roofline_frac': FLOP_PER_EVAL['GALAXY'] * evals / (k_f64 * 1e-3) / PEAK_FLOPS['f64']},
         'speedup_vs_fp64': t_f64 / t_mix,

@@ -1,0 +1,93 @@
+"""Tile-shape autotuner for the mixed-precision scan kernel.
+
+The kernel has several tile configurations (items x redshifts per
+wavefront, rr_zscan_tiles).  tune() times every configuration on a prepared
+ScanPlan — the main stage only, HIP events on the launch stream — and returns
+the fastest; choose() caches the winner per (nbasis, padded pixels,
+precision, item count bucket, uniform z) in a JSON file so later plans of the
+same shape reuse it without timing.
+
+Cache file: $RR_TUNE_CACHE, default ~/.cache/redrock_amd/tiles.json.
+"""
+from __future__ import annotations
+
+import ctypes
+import json
+import os
+
+from . import _hip
+
+
+def tile_table():
+    """[(tw, zw), ...] indexed by rr_zscan_desc.tile (entry 0 = default)."""
+    n = 16
+    tw = (ctypes.c_int * n)()
+    zw = (ctypes.c_int * n)()
+    k = _hip.lib().rr_zscan_tiles(tw, zw, n)
+    return [(tw[i], zw[i]) for i in range(min(k, n))]
+
+
+def _cache_path():
+    return os.environ.get('RR_TUNE_CACHE',
+                          os.path.join(os.path.expanduser('~'), '.cache', 'redrock_amd', 'tiles.json'))
+
+
+def _key(plan):
+    d = plan.desc
+    bucket = 1
+    while bucket < d.nitems:
+        bucket *= 2
+    return f'nb{d.nb}-p{d.npix_pad}-prec{d.precision}-items{bucket}-u{d.uniform_z}'
+
+
+def tune(plan, reps=3):
+    """Time every tile configuration on `plan`; returns (best_tile, {tile: ms})."""
+    import torch
+    table = tile_table()
+    saved_tile = plan.desc.tile
+    timings = dict()
+    plan.run()   # stats + a full pass so the workspace holds valid statistics
+    for t in range(1, len(table)):
+        tw, zw = table[t]
+        if tw > 1 and not plan.desc.uniform_z:
+            continue
+        plan.desc.tile = t
+        plan.run(stages=_hip.RR_STAGE_MAIN)   # warm-up (code load, caches)
+        evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+               for _ in range(reps)]
+        for e0, e1 in evs:
+            e0.record()
+            plan.run(stages=_hip.RR_STAGE_MAIN)
+            e1.record()
+        torch.cuda.synchronize()
+        timings[t] = min(e0.elapsed_time(e1) for e0, e1 in evs)
+    plan.desc.tile = saved_tile
+    best = min(timings, key=timings.get)
+    return best, timings
+
+
+def choose(plan, reps=3, use_cache=True):
+    """Best tile for plan's shape, from the cache or by timing; sets plan.desc.tile."""
+    path = _cache_path()
+    key = _key(plan)
+    cache = dict()
+    if use_cache and os.path.exists(path):
+        try:
+            with open(path) as fh:
+                cache = json.load(fh)
+        except (OSError, ValueError):
+            cache = dict()
+    if key in cache:
+        plan.desc.tile = int(cache[key]['tile'])
+        return plan.desc.tile, cache[key].get('ms', {})
+    best, timings = tune(plan, reps)
+    plan.desc.tile = best
+    cache[key] = dict(tile=best, ms={str(k): v for k, v in timings.items()})
+    try:
+        os.makedirs(os.path.dirname(path), exist_ok=True)
+        with open(path + '.tmp', 'w') as fh:
+            json.dump(cache, fh, indent=1)
+        os.replace(path + '.tmp', path)
+    except OSError:
+        pass
+    return best, timings

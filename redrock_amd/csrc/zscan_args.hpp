@@ -28,8 +28,8 @@ struct ZArgs {
 
 // Workspace layout (all offsets 256-byte aligned):
 //   stats  double[nitems][2]            f'Wf, sum(w) of the item's target
-//   ne     F64  : double[ne_count][total_rows]   (k-major, coalesced)
-//          mixed: float [ne_count][total_rows]
+//   ne     double[ne_count][total_rows]   (k-major, coalesced); the mixed
+//          modes store their scaled MFMA sums (unscaled by zscan_finish)
 // ne entry k of row r is the (i, j) element, i <= j, i < nb, of the upper
 // triangle of [M | y] (F64: j <= nb; mixed: j <= nb+1, y split hi/lo in the
 // two last columns), k = ne_index(i, j).

@@ -72,7 +72,7 @@ __global__ __launch_bounds__(64) void finish_kernel(rr::ZArgs a, long long ne_of
     }
     auto m_at = [&](int i, int j) -> double {
         if constexpr (MIXED)
-            return (double)reinterpret_cast<const float *>(wbase)[rr::ne_index(i, j, NB, true) * R + row] * ms;
+            return reinterpret_cast<const double *>(wbase)[rr::ne_index(i, j, NB, true) * R + row] * ms;
         else
             return reinterpret_cast<const double *>(wbase)[rr::ne_index(i, j, NB, false) * R + row];
     };
@@ -86,9 +86,9 @@ __global__ __launch_bounds__(64) void finish_kernel(rr::ZArgs a, long long ne_of
             M[j][i] = v;
         }
         if constexpr (MIXED) {
-            const float *ne = reinterpret_cast<const float *>(wbase);
-            y[i] = ((double)ne[rr::ne_index(i, NB, NB, true) * R + row] +
-                    (double)ne[rr::ne_index(i, NB + 1, NB, true) * R + row]) * ys;
+            const double *ne = reinterpret_cast<const double *>(wbase);
+            y[i] = (ne[rr::ne_index(i, NB, NB, true) * R + row] +
+                    ne[rr::ne_index(i, NB + 1, NB, true) * R + row]) * ys;
         } else {
             y[i] = reinterpret_cast<const double *>(wbase)[rr::ne_index(i, NB, NB, false) * R + row];
         }
@@ -172,7 +172,7 @@ namespace rr {
 
 long long workspace_bytes(int nitems, int nb, bool mixed, long long total_rows) {
     return align256((long long)nitems * 2 * 8) +
-           align256((long long)ne_count(nb, mixed) * total_rows * (mixed ? 4 : 8));
+           align256((long long)ne_count(nb, mixed) * total_rows * 8);
 }
 
 int zscan_stats(const ZArgs &a, hipStream_t s) {

@@ -186,11 +186,19 @@ __global__ __launch_bounds__(256) void zscan_mfma_kernel(rr::ZArgs a, long long 
                      : T + (long long)a.cache_nz * NB * rows;
     }
 
+    // The Gram is accumulated by the MFMAs in fp32 over FLUSH tile pairs
+    // (256 pixels) and then added into FP64 accumulators: one fp32 sum over
+    // all ~250 pairs would carry ~1e-6 relative error into M and y, which the
+    // cancellation in chi2 = f'Wf - 2c'y + c'Mc multiplies by the model power.
+    constexpr int FLUSH = 8;
     float4v G[TW][ZW];
+    double Gd[TW][ZW][4];
 #pragma unroll
     for (int u = 0; u < TW; ++u)
 #pragma unroll
-        for (int s = 0; s < ZW; ++s) G[u][s] = float4v{0.f, 0.f, 0.f, 0.f};
+        for (int s = 0; s < ZW; ++s)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) Gd[u][s][r] = 0.0;
 
     // lanes of columns nb, nb+1 feed sqrt(w) f (hi, lo) into the Gram instead
     // of their (zero) R-apply output
@@ -198,7 +206,13 @@ __global__ __launch_bounds__(256) void zscan_mfma_kernel(rr::ZArgs a, long long 
     const int fsel = fcolumn ? j - NB : 0;
     const float4v zero4 = {0.f, 0.f, 0.f, 0.f};
 
-    for (int q = 0; q < ntile; q += 2) {
+    for (int qb = 0; qb < ntile; qb += 2 * FLUSH) {
+#pragma unroll
+    for (int u = 0; u < TW; ++u)
+#pragma unroll
+        for (int s = 0; s < ZW; ++s) G[u][s] = zero4;
+    const int qe = (qb + 2 * FLUSH < ntile) ? qb + 2 * FLUSH : ntile;
+    for (int q = qb; q < qe; q += 2) {
         u16x8 ra[TW][2], fx[TW];
 #pragma unroll
         for (int u = 0; u < TW; ++u) {
@@ -228,11 +242,18 @@ __global__ __launch_bounds__(256) void zscan_mfma_kernel(rr::ZArgs a, long long 
             }
         }
     }
+#pragma unroll
+    for (int u = 0; u < TW; ++u)
+#pragma unroll
+        for (int s = 0; s < ZW; ++s)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) Gd[u][s][r] += (double)G[u][s][r];
+    }
 
     // ---- epilogue: scaled normal equations -> workspace ----
     // lane (j, g) register r holds G[4g+r][j]; keep the upper triangle of the
     // first nb rows over columns 0..nb+1 ([M | y_hi | y_lo]).
-    float *ne = reinterpret_cast<float *>(static_cast<char *>(a.work) + ne_off);
+    double *ne = reinterpret_cast<double *>(static_cast<char *>(a.work) + ne_off);
     const long long R = a.total_rows;
 #pragma unroll
     for (int u = 0; u < TW; ++u) {
@@ -246,7 +267,7 @@ __global__ __launch_bounds__(256) void zscan_mfma_kernel(rr::ZArgs a, long long 
             for (int r = 0; r < 4; ++r) {
                 const int i = 4 * g + r;
                 if (i < NB && j >= i && j < NB + 2)
-                    ne[rr::ne_index(i, j, NB, true) * R + row0 + iz] = G[u][s][r];
+                    ne[rr::ne_index(i, j, NB, true) * R + row0 + iz] = Gd[u][s][r];
             }
         }
     }
@@ -257,7 +278,7 @@ __global__ __launch_bounds__(256) void zscan_mfma_kernel(rr::ZArgs a, long long 
 struct TileCfg {
     int tw, zw;
 };
-constexpr TileCfg kTiles[] = {{2, 8}, {1, 8}, {2, 4}, {2, 8}, {4, 4}, {4, 8}, {1, 16}};
+constexpr TileCfg kTiles[] = {{2, 4}, {1, 8}, {2, 4}, {2, 8}, {4, 4}, {4, 8}, {1, 16}};
 constexpr int kNumTiles = sizeof(kTiles) / sizeof(kTiles[0]);
 
 template <int NB, int TY, int TW, int ZW>
@@ -308,7 +329,7 @@ namespace rr {
 int zscan_mfma(const ZArgs &a, hipStream_t s) {
     int tile = a.tile;
     if (tile < 0 || tile >= kNumTiles) return fail(RR_EINVAL, "rr_zscan: tile %d outside 0..%d", tile, kNumTiles - 1);
-    if (tile == 0) tile = a.uniform_z ? 3 : 1;
+    if (tile == 0) tile = a.uniform_z ? 2 : 1;
     if (!a.uniform_z && kTiles[tile].tw > 1) tile = (kTiles[tile].zw == 16) ? 6 : 1;
     if (a.precision == RR_PREC_F16) return launch_ty<RR_PREC_F16>(a, tile, s);
     return launch_ty<RR_PREC_BF16>(a, tile, s);

@@ -77,4 +77,4 @@ def test_workspace_size_formula():
     assert lib.rr_zscan_workspace_bytes(ctypes.byref(d)) == 256 + ((ne * 1000 * 8 + 255) // 256) * 256
     d.precision = _hip.RR_PREC_F16
     ne = 10 * 12 - 45
-    assert lib.rr_zscan_workspace_bytes(ctypes.byref(d)) == 256 + ((ne * 1000 * 4 + 255) // 256) * 256
+    assert lib.rr_zscan_workspace_bytes(ctypes.byref(d)) == 256 + ((ne * 1000 * 8 + 255) // 256) * 256

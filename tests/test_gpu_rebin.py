@@ -107,6 +107,8 @@ def test_cache_layouts_agree(gpu):
     t64, ldz = cache.get(lay, 'f64')
     t16, _ = cache.get(lay, 'f16')
     a = t64.reshape(tm.nbasis, lay.cache_rows, ldz)[:, :, :len(zsel)].permute(2, 0, 1)
-    b = t16.view(torch.float16).reshape(len(zsel), tm.nbasis, lay.cache_rows).double() / cache.tscale
+    n16 = len(zsel) * tm.nbasis * lay.cache_rows
+    assert torch.all(t16[n16:] == 0) and t16.numel() >= n16 + 64   # zero block after the cache
+    b = t16[:n16].view(torch.float16).reshape(len(zsel), tm.nbasis, lay.cache_rows).double() / cache.tscale
     assert torch.all(a[:, :, :lay.halo] == 0) and torch.all(b[:, :, :lay.halo] == 0)
     assert torch.allclose(a, b, rtol=2 ** -10, atol=1e-6 * float(a.abs().max()))
