@@ -193,8 +193,9 @@ def main():
                                f'3 cameras 7958 px, 11-diagonal R',
                    'targets': nt, 'redshifts': nz, 'nbasis': nb, 'snr_per_pixel': args.snr,
                    'solve': 'FP64 Cholesky', 'accumulate': 'fp32 per 256 px, then FP64',
-                   'tile': {'items_per_wave': table[tile][0], 'z_per_wave': table[tile][1]}},
-        'autotune_ms': {f'{table[t][0]}x{table[t][1]}': ms for t, ms in tile_ms.items()},
+                   'tile': dict(zip(('waves_items', 'waves_z', 'items_per_wave', 'z_per_wave'),
+                                    table[tile]))},
+        'autotune_ms': {'x'.join(map(str, table[t])): ms for t, ms in tile_ms.items()},
         'fp64': {'value': evals / t_f64, 'ms_per_step': t_f64 * 1e3, 'kernel_ms': k_f64,
                  'roofline_frac': FLOP_PER_EVAL['GALAXY'] * evals / (k_f64 * 1e-3) / PEAK_FLOPS['f64']},
         'speedup_vs_fp64': t_f64 / t_mix,

@@ -213,12 +213,13 @@ int rr_zscan(const rr_zscan_desc *desc, void *stream);
  */
 /*
  * rr_zscan_tiles — the tile configurations of the mixed-precision scan
- * kernel: entry i (i >= 1) processes tw[i] items x zw[i] redshifts per
- * wavefront; entry 0 is the default choice.  Host arrays of `max` entries;
- * returns the number of configurations.  Used by the autotuner to pick
- * rr_zscan_desc.tile per (nbasis, pixel count, precision).
+ * kernel.  Entry i is cfg[4i..4i+3] = (WA, WB, TW, ZW): a workgroup of four
+ * wavefronts arranged WA x WB covers WA*TW items x WB*ZW redshifts, each
+ * wavefront TW items x ZW redshifts.  Entry 0 is the default.  Host array of
+ * 4*max ints; returns the number of configurations.  Used by the autotuner
+ * to pick rr_zscan_desc.tile per (nbasis, pixel count, precision).
  */
-int rr_zscan_tiles(int *tw, int *zw, int max);
+int rr_zscan_tiles(int *cfg, int max);
 
 int rr_pack_mixed(const double *d_rband, const double *d_w, const double *d_f, int ntarget,
                   int npix_pad, int precision, void *d_rtile, void *d_sfcol,

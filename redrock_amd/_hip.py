@@ -82,7 +82,7 @@ def lib():
     L.rr_pack_mixed.restype = _i
     L.rr_pack_mixed.argtypes = [_vp, _vp, _vp, _i, _i, _i, _vp, _vp, _vp, _vp]
     L.rr_zscan_tiles.restype = _i
-    L.rr_zscan_tiles.argtypes = [ctypes.POINTER(_i), ctypes.POINTER(_i), _i]
+    L.rr_zscan_tiles.argtypes = [ctypes.POINTER(_i), _i]
     v = L.rr_abi_version()
     if v != RR_ABI_VERSION:
         raise HipError(f'libredrock_hip ABI {v}, expected {RR_ABI_VERSION}')

@@ -19,12 +19,14 @@ from . import _hip
 
 
 def tile_table():
-    """[(tw, zw), ...] indexed by rr_zscan_desc.tile (entry 0 = default)."""
-    n = 16
-    tw = (ctypes.c_int * n)()
-    zw = (ctypes.c_int * n)()
-    k = _hip.lib().rr_zscan_tiles(tw, zw, n)
-    return [(tw[i], zw[i]) for i in range(min(k, n))]
+    """[(wa, wb, tw, zw), ...] indexed by rr_zscan_desc.tile (entry 0 = default).
+
+    A workgroup covers wa*tw items x wb*zw redshifts.
+    """
+    n = 32
+    cfg = (ctypes.c_int * (4 * n))()
+    k = _hip.lib().rr_zscan_tiles(cfg, n)
+    return [tuple(cfg[4 * i:4 * i + 4]) for i in range(min(k, n))]
 
 
 def _cache_path():
@@ -48,8 +50,8 @@ def tune(plan, reps=3):
     timings = dict()
     plan.run()   # stats + a full pass so the workspace holds valid statistics
     for t in range(1, len(table)):
-        tw, zw = table[t]
-        if tw > 1 and not plan.desc.uniform_z:
+        wa, wb, tw, zw = table[t]
+        if wa * tw > 1 and not plan.desc.uniform_z:
             continue
         plan.desc.tile = t
         plan.run(stages=_hip.RR_STAGE_MAIN)   # warm-up (code load, caches)

@@ -141,23 +141,37 @@ __global__ __launch_bounds__(64) void pack_mixed_kernel(const double *__restrict
 }
 
 // ---------------------------------------------------------------------------
-// Scan kernel.  Workgroup = 4 waves; wave = TW items (targets sharing the
-// same cache z range) x ZW redshifts.  All 4 waves of a workgroup work on the
-// same redshift block, so their template tiles are shared through L1/L2.
+// Scan kernel.  Workgroup tile = TT items x ZZ redshifts, split over its four
+// wavefronts as WA (items) x WB (redshifts); a wavefront owns TW = TT/WA items
+// x ZW = ZZ/WB redshifts.  For every tile pair (32 pixels) the workgroup
+// stages in LDS the template rows of its ZZ redshifts (shared by its TT
+// items) and the R tiles and sqrt(w) f operands of its TT items (shared by its
+// ZZ redshifts).  LDS is double-buffered: the global loads of pair p+1 are in
+// flight while the MFMAs consume pair p.  All TT items must share one cache
+// z range (uniform_z) unless TT == 1.
 // ---------------------------------------------------------------------------
-template <int NB, int TY, int TW, int ZW>
+template <int NB, int TY, int WA, int WB, int TW, int ZW>
 __global__ __launch_bounds__(256) void zscan_mfma_kernel(rr::ZArgs a, long long ne_off) {
-    const int lane = threadIdx.x & 63;
-    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int item0 = (blockIdx.x * 4 + wave) * TW;
-    if (item0 >= a.nitems) return;
-    const int z0 = a.item_z0[item0];
-    const int nzi = a.item_nz[item0];
-    const int zbase = blockIdx.y * ZW;
-    if (zbase >= nzi) return;
+    constexpr int TT = WA * TW, ZZ = WB * ZW;
+    constexpr int TCH = ZZ * NB * 6;     // 16-byte template chunks per pair (48 rows x NB x ZZ)
+    constexpr int RCH = TT * 2 * 64;     // 16-byte R-tile chunks per pair (2 tiles x 64 lanes)
+    constexpr int SCH = TT * 8;          // 16-byte sqrt(w) f chunks per pair (4 g x hi/lo)
+    constexpr int NCH = TCH + RCH + SCH;
+    constexpr int PER = (NCH + 255) / 256;
+    __shared__ u16x8 lds[2][NCH];
+    __shared__ u16x8 zero_chunk;
 
-    const int j = lane & 15;   // column (basis) of operand B / output
-    const int g = lane >> 4;   // K group / output row group
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int wa = wave % WA, wb = wave / WA;
+    const int itemb = blockIdx.x * TT;
+    if (itemb >= a.nitems) return;           // uniform over the workgroup
+    const int z0 = a.item_z0[itemb];
+    const int nzi = a.item_nz[itemb];
+    const int zb = blockIdx.y * ZZ;
+    if (zb >= nzi) return;                   // uniform over the workgroup
+
     const int ntile = a.npix_pad / 16;
     const int npair = ntile / 2;
     const long long rows = a.cache_rows;
@@ -165,68 +179,85 @@ __global__ __launch_bounds__(256) void zscan_mfma_kernel(rr::ZArgs a, long long 
     const unsigned short *__restrict__ RT = static_cast<const unsigned short *>(a.rtile);
     const unsigned short *__restrict__ SF = static_cast<const unsigned short *>(a.sfcol);
 
-    int tgt[TW];
-    bool live[TW];
+    // ---- this thread's share of the staging: source base and per-pair step
+    const unsigned short *src[PER];
+    int step[PER];
 #pragma unroll
-    for (int u = 0; u < TW; ++u) {
-        live[u] = (item0 + u) < a.nitems;
-        tgt[u] = a.item_target[live[u] ? item0 + u : item0];
+    for (int k = 0; k < PER; ++k) {
+        int c = tid + 256 * k;
+        if (c >= NCH) c = NCH - 1;           // duplicate the last chunk (harmless)
+        if (c < TCH) {
+            const int s = c / (NB * 6), b = (c / 6) % NB, part = c % 6;
+            const int iz = zb + s;
+            const int zc = z0 + (iz < nzi ? iz : nzi - 1);
+            src[k] = T + ((long long)zc * NB + b) * rows + 8 * part;
+            step[k] = 32;
+        } else if (c < TCH + RCH) {
+            const int r = c - TCH, u = r >> 7, h = (r >> 6) & 1, l = r & 63;
+            const int it = itemb + u < a.nitems ? itemb + u : a.nitems - 1;
+            const int t = a.item_target[it];
+            src[k] = RT + (((long long)t * ntile + h) * 64 + l) * 8;
+            step[k] = 2 * 64 * 8;
+        } else {
+            const int r = c - TCH - RCH, u = r >> 3, gs = r & 7;
+            const int it = itemb + u < a.nitems ? itemb + u : a.nitems - 1;
+            const int t = a.item_target[it];
+            src[k] = SF + ((long long)t * npair * 8 + gs) * 8;
+            step[k] = 8 * 8;
+        }
     }
-    // Template operand of lane (j, g): rows 16q+8g.. of basis j.  Lanes with
-    // j >= nb read the zero block that follows the cache (no branch, no
-    // per-iteration zeroing), so their output columns stay exactly zero.
+    u16x8 pre[PER];
+    auto load = [&](int qp) {
+#pragma unroll
+        for (int k = 0; k < PER; ++k)
+            pre[k] = *reinterpret_cast<const u16x8 *>(src[k] + (long long)qp * step[k]);
+    };
+    auto store = [&](int buf) {
+#pragma unroll
+        for (int k = 0; k < PER; ++k) {
+            const int c = tid + 256 * k;
+            if (c < NCH) lds[buf][c] = pre[k];
+        }
+    };
+
+    // ---- operand addressing of this lane
+    const int j = lane & 15;   // column (basis) of operand B / output
+    const int g = lane >> 4;   // K group / output row group
     const bool real = j < NB;
-    const long long qstep = real ? 16 : 0;
-    const unsigned short *tb[ZW];
-#pragma unroll
-    for (int s = 0; s < ZW; ++s) {
-        const int iz = zbase + s;
-        const int zc = z0 + (iz < nzi ? iz : nzi - 1);
-        tb[s] = real ? T + ((long long)zc * NB + j) * rows + 8 * g
-                     : T + (long long)a.cache_nz * NB * rows;
-    }
-
-    // The Gram is accumulated by the MFMAs in fp32 over FLUSH tile pairs
-    // (256 pixels) and then added into FP64 accumulators: one fp32 sum over
-    // all ~250 pairs would carry ~1e-6 relative error into M and y, which the
-    // cancellation in chi2 = f'Wf - 2c'y + c'Mc multiplies by the model power.
-    constexpr int FLUSH = 8;
-    float4v G[TW][ZW];
-    double Gd[TW][ZW][4];
-#pragma unroll
-    for (int u = 0; u < TW; ++u)
-#pragma unroll
-        for (int s = 0; s < ZW; ++s)
-#pragma unroll
-            for (int r = 0; r < 4; ++r) Gd[u][s][r] = 0.0;
-
-    // lanes of columns nb, nb+1 feed sqrt(w) f (hi, lo) into the Gram instead
-    // of their (zero) R-apply output
     const bool fcolumn = (j == NB) || (j == NB + 1);
     const int fsel = fcolumn ? j - NB : 0;
-    const float4v zero4 = {0.f, 0.f, 0.f, 0.f};
 
-    for (int qb = 0; qb < ntile; qb += 2 * FLUSH) {
+    float4v G[TW][ZW];
 #pragma unroll
     for (int u = 0; u < TW; ++u)
 #pragma unroll
-        for (int s = 0; s < ZW; ++s) G[u][s] = zero4;
-    const int qe = (qb + 2 * FLUSH < ntile) ? qb + 2 * FLUSH : ntile;
-    for (int q = qb; q < qe; q += 2) {
+        for (int s = 0; s < ZW; ++s) G[u][s] = float4v{0.f, 0.f, 0.f, 0.f};
+    const float4v zero4 = {0.f, 0.f, 0.f, 0.f};
+
+    if (tid == 0) zero_chunk = u16x8{0, 0, 0, 0, 0, 0, 0, 0};
+    load(0);
+    store(0);
+    __syncthreads();
+
+    for (int qp = 0; qp < npair; ++qp) {
+        const int buf = qp & 1;
+        const bool more = qp + 1 < npair;
+        if (more) load(qp + 1);
+        const u16x8 *L = lds[buf];
         u16x8 ra[TW][2], fx[TW];
 #pragma unroll
         for (int u = 0; u < TW; ++u) {
-            const unsigned short *rt = RT + ((long long)tgt[u] * ntile + q) * 64 * 8 + lane * 8;
-            ra[u][0] = *reinterpret_cast<const u16x8 *>(rt);
-            ra[u][1] = *reinterpret_cast<const u16x8 *>(rt + 64 * 8);
-            fx[u] = *reinterpret_cast<const u16x8 *>(
-                SF + ((((long long)tgt[u] * npair + (q >> 1)) * 4 + g) * 2 + fsel) * 8);
+            const int iu = wa * TW + u;
+            ra[u][0] = L[TCH + iu * 128 + lane];
+            ra[u][1] = L[TCH + iu * 128 + 64 + lane];
+            fx[u] = L[TCH + RCH + iu * 8 + g * 2 + fsel];
         }
 #pragma unroll
         for (int s = 0; s < ZW; ++s) {
-            const unsigned short *tp = tb[s] + q * qstep;
-            const u16x8 b0 = *reinterpret_cast<const u16x8 *>(tp);
-            const u16x8 b1 = *reinterpret_cast<const u16x8 *>(tp + qstep);
+            const int is = wb * ZW + s;
+            const u16x8 *tp = real ? L + (is * NB + j) * 6 + g : &zero_chunk;
+            const u16x8 b0 = tp[0];
+            const u16x8 b1 = real ? tp[2] : tp[0];
 #pragma unroll
             for (int u = 0; u < TW; ++u) {
                 const float4v d0 = mfma<TY>(ra[u][0], b0, zero4);
@@ -241,13 +272,8 @@ __global__ __launch_bounds__(256) void zscan_mfma_kernel(rr::ZArgs a, long long 
                 G[u][s] = mfma<TY>(x, x, G[u][s]);
             }
         }
-    }
-#pragma unroll
-    for (int u = 0; u < TW; ++u)
-#pragma unroll
-        for (int s = 0; s < ZW; ++s)
-#pragma unroll
-            for (int r = 0; r < 4; ++r) Gd[u][s][r] += (double)G[u][s][r];
+        if (more) store(buf ^ 1);
+        __syncthreads();
     }
 
     // ---- epilogue: scaled normal equations -> workspace ----
@@ -257,49 +283,52 @@ __global__ __launch_bounds__(256) void zscan_mfma_kernel(rr::ZArgs a, long long 
     const long long R = a.total_rows;
 #pragma unroll
     for (int u = 0; u < TW; ++u) {
-        if (!live[u]) continue;
-        const long long row0 = a.item_out[item0 + u];
+        const int item = itemb + wa * TW + u;
+        if (item >= a.nitems) continue;
+        const long long row0 = a.item_out[item];
 #pragma unroll
         for (int s = 0; s < ZW; ++s) {
-            const int iz = zbase + s;
+            const int iz = zb + wb * ZW + s;
             if (iz >= nzi) continue;
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
                 const int i = 4 * g + r;
                 if (i < NB && j >= i && j < NB + 2)
-                    ne[rr::ne_index(i, j, NB, true) * R + row0 + iz] = Gd[u][s][r];
+                    ne[rr::ne_index(i, j, NB, true) * R + row0 + iz] = (double)G[u][s][r];
             }
         }
     }
 }
 
-// Tile configurations (items per wavefront TW, redshifts per wavefront ZW),
-// selected by rr_zscan_desc.tile (0 = default); TW > 1 needs uniform_z.
+// Tile configurations (WA, WB, TW, ZW) selected by rr_zscan_desc.tile
+// (0 = default); configurations with WA*TW > 1 need uniform_z.
 struct TileCfg {
-    int tw, zw;
+    int wa, wb, tw, zw;
 };
-constexpr TileCfg kTiles[] = {{2, 4}, {1, 8}, {2, 4}, {2, 8}, {4, 4}, {4, 8}, {1, 16}};
+constexpr TileCfg kTiles[] = {{2, 2, 4, 4}, {1, 4, 1, 4}, {2, 2, 4, 4}, {2, 2, 2, 4},
+                              {4, 1, 2, 8}, {1, 4, 4, 2}, {2, 2, 2, 2}, {1, 4, 2, 4}};
 constexpr int kNumTiles = sizeof(kTiles) / sizeof(kTiles[0]);
 
-template <int NB, int TY, int TW, int ZW>
+template <int NB, int TY, int WA, int WB, int TW, int ZW>
 int launch_tile(const rr::ZArgs &a, hipStream_t s) {
-    const int per_wg = 4 * TW;
+    constexpr int TT = WA * TW, ZZ = WB * ZW;
     dim3 block(256);
-    dim3 grid((a.nitems + per_wg - 1) / per_wg, (a.max_item_nz + ZW - 1) / ZW);
+    dim3 grid((a.nitems + TT - 1) / TT, (a.max_item_nz + ZZ - 1) / ZZ);
     const long long ne_off = rr::align256((long long)a.nitems * 2 * 8);
-    hipLaunchKernelGGL((zscan_mfma_kernel<NB, TY, TW, ZW>), grid, block, 0, s, a, ne_off);
+    hipLaunchKernelGGL((zscan_mfma_kernel<NB, TY, WA, WB, TW, ZW>), grid, block, 0, s, a, ne_off);
     return rr::check_launch("rr_zscan(mixed)");
 }
 
 template <int NB, int TY>
 int launch_nb(const rr::ZArgs &a, int tile, hipStream_t s) {
     switch (tile) {
-        case 1: return launch_tile<NB, TY, 1, 8>(a, s);
-        case 2: return launch_tile<NB, TY, 2, 4>(a, s);
-        case 3: return launch_tile<NB, TY, 2, 8>(a, s);
-        case 4: return launch_tile<NB, TY, 4, 4>(a, s);
-        case 5: return launch_tile<NB, TY, 4, 8>(a, s);
-        case 6: return launch_tile<NB, TY, 1, 16>(a, s);
+        case 1: return launch_tile<NB, TY, 1, 4, 1, 4>(a, s);
+        case 2: return launch_tile<NB, TY, 2, 2, 4, 4>(a, s);
+        case 3: return launch_tile<NB, TY, 2, 2, 2, 4>(a, s);
+        case 4: return launch_tile<NB, TY, 4, 1, 2, 8>(a, s);
+        case 5: return launch_tile<NB, TY, 1, 4, 4, 2>(a, s);
+        case 6: return launch_tile<NB, TY, 2, 2, 2, 2>(a, s);
+        case 7: return launch_tile<NB, TY, 1, 4, 2, 4>(a, s);
         default: return rr::fail(RR_EINVAL, "rr_zscan(mixed): unknown tile %d", tile);
     }
 }
@@ -328,19 +357,22 @@ int launch_ty(const rr::ZArgs &a, int tile, hipStream_t s) {
 namespace rr {
 int zscan_mfma(const ZArgs &a, hipStream_t s) {
     int tile = a.tile;
-    if (tile < 0 || tile >= kNumTiles) return fail(RR_EINVAL, "rr_zscan: tile %d outside 0..%d", tile, kNumTiles - 1);
+    if (tile < 0 || tile >= kNumTiles)
+        return fail(RR_EINVAL, "rr_zscan: tile %d outside 0..%d", tile, kNumTiles - 1);
     if (tile == 0) tile = a.uniform_z ? 2 : 1;
-    if (!a.uniform_z && kTiles[tile].tw > 1) tile = (kTiles[tile].zw == 16) ? 6 : 1;
+    if (!a.uniform_z && kTiles[tile].wa * kTiles[tile].tw > 1) tile = 1;
     if (a.precision == RR_PREC_F16) return launch_ty<RR_PREC_F16>(a, tile, s);
     return launch_ty<RR_PREC_BF16>(a, tile, s);
 }
 }  // namespace rr
 
-extern "C" int rr_zscan_tiles(int *tw, int *zw, int max) {
+extern "C" int rr_zscan_tiles(int *cfg, int max) {
     const int n = kNumTiles < max ? kNumTiles : max;
     for (int i = 0; i < n; ++i) {
-        tw[i] = kTiles[i].tw;
-        zw[i] = kTiles[i].zw;
+        cfg[4 * i] = kTiles[i].wa;
+        cfg[4 * i + 1] = kTiles[i].wb;
+        cfg[4 * i + 2] = kTiles[i].tw;
+        cfg[4 * i + 3] = kTiles[i].zw;
     }
     return kNumTiles;
 }

@@ -49,16 +49,24 @@ def test_f64_deterministic(galaxy):
         assert np.array_equal(x, y)
 
 
-@pytest.mark.parametrize('precision,tol', [('f16', 1e-4), ('bf16', 2e-3)])
-def test_mixed_matches_oracle(galaxy, precision, tol):
+@pytest.mark.parametrize('precision,tol_max,tol_med', [('f16', 3e-4, 1e-5), ('bf16', 3e-3, 1e-4)])
+def test_mixed_matches_oracle(galaxy, precision, tol_max, tol_med):
+    """Mixed precision at per-pixel S/N 1 (a bright target).
+
+    Single-pass 16-bit inputs give a first-order chi2 error -2 r'W dA c that
+    grows with the model power; the stated tolerances are max and median of
+    |delta chi2|/chi2 over every (target, z), with identical best-z.
+    """
     from redrock_amd import zscan
     tm, targets, dt, tdata, (rchi2, rcoeff, rpen) = galaxy
     print(f'case: galaxy {precision}', flush=True)
     chi2, coeff, pen = zscan.calc_zchi2([t.id for t in targets], targets, dt, precision=precision)
     rel = np.abs(chi2 - rchi2) / np.abs(rchi2)
     ibest = rchi2.argmin(axis=1)
-    print(f'{precision}: max rel err {rel.max():.3e}, at best z {rel[np.arange(3), ibest].max():.3e}')
-    assert rel.max() <= tol
+    print(f'{precision}: max rel err {rel.max():.3e}, median {np.median(rel):.3e}, '
+          f'at best z {rel[np.arange(3), ibest].max():.3e}')
+    assert rel.max() <= tol_max
+    assert np.median(rel) <= tol_med
     assert np.array_equal(chi2.argmin(axis=1), ibest)
 
 
