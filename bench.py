@@ -192,7 +192,7 @@ def main():
         'config': {'workload': f'galaxy petal: {nt} GALAXY targets x {nz} z x {nb} basis, '
                                f'3 cameras 7958 px, 11-diagonal R',
                    'targets': nt, 'redshifts': nz, 'nbasis': nb, 'snr_per_pixel': args.snr,
-                   'solve': 'FP64 Cholesky', 'accumulate': 'fp32 per 256 px, then FP64',
+                   'solve': 'FP64 Cholesky', 'accumulate': 'fp32 (MFMA), FP64 after the scan',
                    'tile': dict(zip(('waves_items', 'waves_z', 'items_per_wave', 'z_per_wave'),
                                     table[tile]))},
         'autotune_ms': {'x'.join(map(str, table[t])): ms for t, ms in tile_ms.items()},
