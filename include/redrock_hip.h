@@ -225,6 +225,46 @@ int rr_pack_mixed(const double *d_rband, const double *d_w, const double *d_f, i
                   int npix_pad, int precision, void *d_rtile, void *d_sfcol,
                   double *d_scales, void *stream);
 
+/*
+ * Adjoint problem descriptor for rr_zscan_grad.  One item = one target at one
+ * redshift with its best-fit coefficients (from rr_zscan).  The pixel axis is
+ * the one of rr_zscan_desc; each real pixel carries its observed-frame bin
+ * edges and centre.
+ */
+typedef struct {
+    int nitems;
+    int npix, npix_pad, nb;
+    const double *d_x;          /* (nx) rest-frame template wavelengths            */
+    int nx;
+    const double *d_y;          /* (nb, nx) template fluxes                         */
+    const double *d_pe_lo;      /* (npix) lower bin edge of every pixel             */
+    const double *d_pe_hi;      /* (npix) upper bin edge                            */
+    const double *d_pwave;      /* (npix) bin centre (IGM); NULL if nlines == 0     */
+    const rr_lyman_line *lines; /* host array, IGM lines of the template            */
+    int nlines;
+    const double *d_rband, *d_w, *d_f;   /* per target, as in rr_zscan_desc         */
+    const int *d_item_target;   /* (nitems)                                         */
+    const double *d_item_z;     /* (nitems) redshift                                */
+    const double *d_item_coeff; /* (nitems, nb) coefficients at that redshift       */
+    double *d_dchi2_dz;         /* (nitems) out                                     */
+    double *d_chi2;             /* (nitems) out, explicit FP64 residual, or NULL    */
+    double *d_dchi2_dy;         /* (nb, nx) accumulated (+=, FP64 atomics), or NULL */
+    const double *d_item_scale; /* (nitems) weight of each item's d_dchi2_dy term,
+                                   or NULL for 1 (a loss sum_i s_i chi2_i)          */
+} rr_grad_desc;
+
+/*
+ * rr_zscan_grad — gradient of chi2(z) = min_c |W^1/2 (f - R T(z) c)|^2 with
+ * respect to z and to the template fluxes, at the given coefficients (the
+ * envelope theorem makes the coefficient derivative vanish).  The rebin is
+ * recomputed from the template, not read from a cache.  No reference
+ * counterpart (redrock has no gradients); it serves the autograd Function and
+ * the Newton refinement that stand beside fitz's parabola fit
+ * (py/redrock/fitz.py:259-261).  The chi2 output is the explicit residual
+ * sum(w (f - R T c)^2) (zscan.py:777-780), a check on the forward pass.
+ */
+int rr_zscan_grad(const rr_grad_desc *desc, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

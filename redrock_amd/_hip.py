@@ -21,7 +21,7 @@ RR_MAX_LYMAN_LINES = 8
 PRECISIONS = {'f64': RR_PREC_F64, 'f16': RR_PREC_F16, 'bf16': RR_PREC_BF16}
 
 EXPORTS = ('rr_abi_version', 'rr_last_error', 'rr_rebin', 'rr_zscan',
-           'rr_zscan_workspace_bytes', 'rr_pack_mixed', 'rr_zscan_tiles')
+           'rr_zscan_workspace_bytes', 'rr_pack_mixed', 'rr_zscan_tiles', 'rr_zscan_grad')
 
 _vp = ctypes.c_void_p
 _i = ctypes.c_int

@@ -229,11 +229,45 @@ def _finalize(template, target, redshifts, nminima, cands, get):
 
 
 def fitz_targets(zchi2, redshifts, targets, template, nminima=3, zminfit_npoints=15,
-                 precision='f64', speculate=2):
+                 precision='f64', speculate=2, newton=False):
     """fitz for every target of one template (zchi2: (nt, nz) incl. penalty).
 
-    Returns a list with the reference's per-target result dict.
+    Returns a list with the reference's per-target result dict.  With
+    newton=True every minimum is also refined by Newton iterations on the
+    analytic d chi2/dz inside its fine grid (redrock_amd.grad.newton_refine)
+    and the dicts gain 'znewton' and 'chi2newton' beside the parabola's 'z'
+    and 'chi2'.
     """
+    out = _fitz_targets(zchi2, redshifts, targets, template, nminima, zminfit_npoints,
+                        precision, speculate)
+    if newton:
+        _add_newton(out, targets, template)
+    return out
+
+
+def _add_newton(out, targets, template):
+    from .grad import GradProblem, newton_refine
+    owner, z0, lo, hi = [], [], [], []
+    for t, r in enumerate(out):
+        for k in range(r['z'].shape[0]):
+            owner.append((t, k))
+            z0.append(r['z'][k, 0])
+            lo.append(r['zz'][k, 0])
+            hi.append(r['zz'][k, -1])
+    for r in out:
+        r['znewton'] = np.zeros_like(r['z'])
+        r['chi2newton'] = np.zeros_like(r['chi2'])
+    if not owner:
+        return
+    prob = GradProblem(template, targets, [t for t, _ in owner])
+    zn, c2, _ = newton_refine(prob, np.array(z0), np.array(lo), np.array(hi))
+    for (t, k), zv, cv in zip(owner, zn, c2):
+        out[t]['znewton'][k, 0] = zv
+        out[t]['chi2newton'][k, 0] = cv
+
+
+def _fitz_targets(zchi2, redshifts, targets, template, nminima, zminfit_npoints, precision,
+                  speculate):
     zchi2 = np.atleast_2d(np.asarray(zchi2, dtype=np.float64))
     assert zchi2.shape == (len(targets), len(redshifts))
     nz = 15 if zminfit_npoints is None else zminfit_npoints

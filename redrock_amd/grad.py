@@ -1,0 +1,189 @@
+"""Gradients of chi2(z) and the Newton refinement that stands beside fitz's parabola.
+
+chi2(z) = min_c |W^1/2 (f - R T(z) c)|^2 for one target and one template.
+rr_zscan_grad (csrc/zscan_grad.hip) returns d chi2/dz and d chi2/d(template
+flux) at the best-fit coefficients, recomputing the rebin instead of storing
+it.  Chi2AtRedshift wraps forward (rr_rebin + rr_zscan in FP64) and backward
+(rr_zscan_grad) as a torch.autograd.Function.  newton_refine() drives z to
+the stationary point of chi2 using the analytic gradient and a central
+difference of it for the curvature; fitz_targets(..., newton=True) reports its
+result beside the reference's parabola fit (fitz.py:259-261).
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import _hip
+from .igm import igm_applies, lyman_table
+from .rebin import centers2edges, lyman_array
+from .zscan import PixelLayout, ScanPlan, TargetBatch, TemplateCache
+
+_vp, _i, _d = ctypes.c_void_p, ctypes.c_int, ctypes.c_double
+
+
+class GradDesc(ctypes.Structure):
+    _fields_ = [
+        ('nitems', _i), ('npix', _i), ('npix_pad', _i), ('nb', _i),
+        ('d_x', _vp), ('nx', _i), ('d_y', _vp), ('d_pe_lo', _vp), ('d_pe_hi', _vp),
+        ('d_pwave', _vp), ('lines', ctypes.POINTER(_hip.LymanLine)), ('nlines', _i),
+        ('d_rband', _vp), ('d_w', _vp), ('d_f', _vp),
+        ('d_item_target', _vp), ('d_item_z', _vp), ('d_item_coeff', _vp),
+        ('d_dchi2_dz', _vp), ('d_chi2', _vp), ('d_dchi2_dy', _vp), ('d_item_scale', _vp),
+    ]
+
+
+def _lib():
+    L = _hip.lib()
+    if not hasattr(L, '_grad_ready'):
+        L.rr_zscan_grad.restype = _i
+        L.rr_zscan_grad.argtypes = [ctypes.POINTER(GradDesc), _vp]
+        L._grad_ready = True
+    return L
+
+
+class GradProblem(object):
+    """Targets x one template, evaluated at per-item redshifts.
+
+    items: item_target[i] is the slot (in `targets`) of item i.
+    """
+
+    def __init__(self, template, targets, item_target):
+        import torch
+        self.template = template
+        self.targets = list(targets)
+        self.item_target = np.asarray(item_target, dtype=np.int32)
+        self.layout = PixelLayout.of(self.targets[0].spectra)
+        self.batch = TargetBatch(self.targets, self.layout)
+        self.dwave = {s.wavehash: s.wave for s in self.targets[0].spectra}
+        dev = torch.device('cuda')
+        lo, hi, wc = [], [], []
+        for k in self.layout.keys:
+            e = centers2edges(self.dwave[k])
+            lo.append(e[:-1])
+            hi.append(e[1:])
+            wc.append(np.asarray(self.dwave[k], dtype=np.float64))
+        self.pe_lo = torch.as_tensor(np.concatenate(lo), device=dev).contiguous()
+        self.pe_hi = torch.as_tensor(np.concatenate(hi), device=dev).contiguous()
+        self.pwave = torch.as_tensor(np.concatenate(wc), device=dev).contiguous()
+        self.x = torch.as_tensor(np.ascontiguousarray(template.wave, dtype=np.float64),
+                                 device=dev).contiguous()
+        self.t_item_target = torch.as_tensor(self.item_target, device=dev)
+        model = template.igm_model
+        self.lines = None if model is None or model == 'None' else lyman_table(model)
+
+    def forward(self, z, flux=None):
+        """FP64 chi2 and coefficients of every item at z (host or device array).
+
+        Returns device tensors (chi2 (n,), coeff (n, nb)).
+        """
+        zh = z.detach().cpu().numpy() if hasattr(z, 'detach') else np.asarray(z, dtype=np.float64)
+        n = len(self.item_target)
+        cache = TemplateCache(self.template, self.dwave, redshifts=zh, flux=flux)
+        plan = ScanPlan(cache, self.batch, self.item_target, np.arange(n), np.ones(n), 'f64')
+        plan.run()
+        return plan.zchi2, plan.zcoeff
+
+    def grad(self, z, coeff, flux=None, want_dy=False, scale=None):
+        """(dchi2/dz (n,), explicit chi2 (n,), dchi2/dflux (nb, nx) or None) as device tensors.
+
+        scale (n,) weights the per-item template-flux gradients before summing.
+        """
+        import torch
+        dev = torch.device('cuda')
+        n = len(self.item_target)
+        zt = torch.as_tensor(z, dtype=torch.float64, device=dev).contiguous()
+        y = (flux.detach() if flux is not None else
+             torch.as_tensor(np.ascontiguousarray(self.template.flux), device=dev)).contiguous()
+        nb, nx = y.shape
+        dz = torch.empty(n, dtype=torch.float64, device=dev)
+        chi2 = torch.empty(n, dtype=torch.float64, device=dev)
+        dy = torch.zeros((nb, nx), dtype=torch.float64, device=dev) if want_dy else None
+        sc = None if scale is None else torch.as_tensor(scale, dtype=torch.float64,
+                                                        device=dev).contiguous()
+        coeff = coeff.contiguous()
+        lines, nlines = lyman_array(self.lines)
+        d = GradDesc()
+        d.nitems, d.npix, d.npix_pad, d.nb = n, self.layout.npix, self.layout.npix_pad, nb
+        d.d_x, d.nx, d.d_y = self.x.data_ptr(), nx, y.data_ptr()
+        d.d_pe_lo, d.d_pe_hi, d.d_pwave = self.pe_lo.data_ptr(), self.pe_hi.data_ptr(), self.pwave.data_ptr()
+        d.lines, d.nlines = lines, nlines
+        d.d_rband, d.d_w, d.d_f = (self.batch.rband.data_ptr(), self.batch.w.data_ptr(),
+                                   self.batch.f.data_ptr())
+        d.d_item_target = self.t_item_target.data_ptr()
+        d.d_item_z, d.d_item_coeff = zt.data_ptr(), coeff.data_ptr()
+        d.d_dchi2_dz, d.d_chi2 = dz.data_ptr(), chi2.data_ptr()
+        d.d_dchi2_dy = None if dy is None else dy.data_ptr()
+        d.d_item_scale = None if sc is None else sc.data_ptr()
+        rc = _lib().rr_zscan_grad(ctypes.byref(d), _hip.stream_handle())
+        _hip.check(rc, 'rr_zscan_grad')
+        return dz, chi2, dy
+
+
+def _autograd_function():
+    import torch
+
+    class Chi2AtRedshift(torch.autograd.Function):
+        """chi2 of every item at its redshift; differentiable in z and template flux."""
+
+        @staticmethod
+        def forward(ctx, z, flux, problem):
+            chi2, coeff = problem.forward(z, flux)
+            ctx.problem = problem
+            ctx.save_for_backward(z, flux, coeff)
+            return chi2.clone()
+
+        @staticmethod
+        def backward(ctx, gout):
+            z, flux, coeff = ctx.saved_tensors
+            want_dy = ctx.needs_input_grad[1]
+            dz, _, dy = ctx.problem.grad(z, coeff, flux, want_dy=want_dy, scale=gout)
+            gz = gout * dz if ctx.needs_input_grad[0] else None
+            return gz, dy, None
+
+    return Chi2AtRedshift
+
+
+_FN = None
+
+
+def chi2_at_redshift(z, flux, problem):
+    """Differentiable chi2(z) (torch float64 tensors z (n,), flux (nb, nx))."""
+    global _FN
+    if _FN is None:
+        _FN = _autograd_function()
+    return _FN.apply(z, flux, problem)
+
+
+def newton_refine(problem, z0, zlo, zhi, iters=4, dz_fd=1e-6):
+    """Newton iterations on dchi2/dz = 0 for every item, inside [zlo, zhi].
+
+    The curvature is the central difference of the analytic gradient.  A step
+    that would leave the bracket or meets non-positive curvature is halved
+    toward the lower-chi2 side.  Returns (z, chi2, dchi2/dz) as numpy arrays.
+    """
+    import torch
+    z = np.array(z0, dtype=np.float64)
+    zlo = np.asarray(zlo, dtype=np.float64)
+    zhi = np.asarray(zhi, dtype=np.float64)
+    n = len(z)
+    for _ in range(iters):
+        zz = np.concatenate([z, z - dz_fd, z + dz_fd])
+        sub = GradProblem.__new__(GradProblem)
+        sub.__dict__.update(problem.__dict__)
+        sub.item_target = np.tile(problem.item_target, 3)
+        sub.t_item_target = torch.as_tensor(sub.item_target, device='cuda')
+        chi2, coeff = sub.forward(zz)
+        g, _, _ = sub.grad(zz, coeff)
+        g = g.cpu().numpy()
+        g0, gm, gp = g[:n], g[n:2 * n], g[2 * n:]
+        h = (gp - gm) / (2 * dz_fd)
+        step = np.where(h > 0, -g0 / np.where(h > 0, h, 1.0), -np.sign(g0) * 0.25 * (zhi - zlo))
+        znew = z + step
+        bad = (znew < zlo) | (znew > zhi)
+        znew = np.where(bad, np.clip(z + 0.5 * step, zlo, zhi), znew)
+        z = znew
+    chi2, coeff = problem.forward(z)
+    g, _, _ = problem.grad(z, coeff)
+    return z, chi2.cpu().numpy(), g.cpu().numpy()

@@ -98,8 +98,10 @@ class TemplateCache(object):
     wavelength grid leaves the redshifted template range.
     """
 
-    def __init__(self, template, dwave, redshifts=None, check_range=True):
+    def __init__(self, template, dwave, redshifts=None, check_range=True, flux=None):
+        """flux: optional (nbasis, nwave) float64 device tensor replacing template.flux."""
         self.template = template
+        self._flux = flux
         self.dwave = dict(dwave)
         self.redshifts = np.ascontiguousarray(
             template.redshifts if redshifts is None else redshifts, dtype=np.float64)
@@ -112,7 +114,8 @@ class TemplateCache(object):
             for e in self.edges.values():
                 if e[0] < xmin or e[-1] > xmax:
                     raise ValueError('edges must be within input x range')
-        fmax = float(np.max(np.abs(template.flux)))
+        fmax = (float(flux.abs().max()) if flux is not None
+                else float(np.max(np.abs(template.flux))))
         self.tscale = 2.0 ** (8 - math.ceil(math.log2(fmax))) if fmax > 0 else 1.0
         self._dev = None
         self._bufs = dict()
@@ -132,7 +135,10 @@ class TemplateCache(object):
                 return torch.as_tensor(np.ascontiguousarray(a, dtype=np.float64),
                                        device=dev).contiguous()
             self._dev = dict(
-                x=up(self.template.wave), y=up(self.template.flux), z=up(self.redshifts),
+                x=up(self.template.wave),
+                y=(self._flux.detach().contiguous() if self._flux is not None
+                   else up(self.template.flux)),
+                z=up(self.redshifts),
                 edges={k: up(e) for k, e in self.edges.items()},
                 wave={k: up(w) for k, w in self.dwave.items()})
         return self._dev
