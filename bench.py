@@ -83,6 +83,8 @@ def main():
     ap.add_argument('--seed', type=int, default=0)
     ap.add_argument('--cpu-seconds', type=float, default=12.0)
     ap.add_argument('--no-cpu', action='store_true')
+    ap.add_argument('--variants', default='bf16,f32',
+                    help='other mixed precisions to time and compare (comma separated)')
     args = ap.parse_args()
     if args.gpus != 1:
         print(f'bench.py measures one GPU; --gpus {args.gpus} is not supported '
@@ -174,6 +176,26 @@ def main():
     log(f'best-z agreement {agree:.4f}; max rel chi2 err {rel.max():.2e}, at best z {rel_best.max():.2e}; '
         f'f64 best z within 5e-3(1+z) of truth: {zerr_ok:.3f}')
 
+    # other precisions on the same inputs: speed, chi2 error vs FP64, best-z agreement
+    variants = {}
+    for p in [v for v in args.variants.split(',') if v and v != args.precision]:
+        plan = zscan.ScanPlan(dt.cache, batch, *items, precision=p, oiisum=oii)
+        plan.run()
+        tv = timed(plan, max(2, args.steps // 2))
+        plan.run()
+        torch.cuda.synchronize()
+        cv = plan.zchi2.reshape(nt, nz).cpu().numpy()
+        rv = np.abs(cv - c_f64) / np.abs(c_f64)
+        variants[p] = {'value': evals / tv, 'ms_per_step': tv * 1e3,
+                       'speedup_vs_fp64': t_f64 / tv,
+                       'best_z_agreement': float(np.mean(cv.argmin(axis=1) == bz_f64)),
+                       'max_rel_chi2_err': float(rv.max()),
+                       'median_rel_chi2_err': float(np.median(rv)),
+                       'max_rel_chi2_err_at_best_z': float(rv[np.arange(nt), bz_f64].max())}
+        log(f'{p}: {tv*1e3:.2f} ms/step, max rel err {rv.max():.2e}, median {np.median(rv):.2e}, '
+            f'best-z agreement {variants[p]["best_z_agreement"]:.4f}')
+        del plan
+
     flops = FLOP_PER_EVAL['GALAXY'] * evals
     ach = flops / (k_mix * 1e-3)
     out = {
@@ -207,6 +229,8 @@ def main():
                      'unit': 'TFLOP/s', 'frac': ach / PEAK_FLOPS[args.precision], 'traffic': None,
                      'kernel': 'zscan_mfma_kernel', 'kernel_ms': k_mix,
                      'flop_per_eval': FLOP_PER_EVAL['GALAXY'], 'evals_per_launch': evals},
+        'median_rel_chi2_err': float(np.median(rel)),
+        'variants': variants,
         'cpu_baseline': cpu,
     }
     print(json.dumps(out), flush=True)

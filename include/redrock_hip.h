@@ -38,6 +38,7 @@ extern "C" {
 #define RR_PREC_F64 0   /* plain FP64: vector FP64 FMA, FP64 LU solve           */
 #define RR_PREC_F16 1   /* MFMA fp16 inputs, fp32 accumulate, FP64 Cholesky     */
 #define RR_PREC_BF16 2  /* MFMA bf16 inputs, fp32 accumulate, FP64 Cholesky     */
+#define RR_PREC_F32 3   /* MFMA fp32 inputs, fp32 accumulate, FP64 Cholesky     */
 
 /* Resolution band: half width 5 (11 diagonals), the DESI layout. */
 #define RR_BAND_HALF 5
@@ -208,7 +209,10 @@ int rr_zscan(const rr_zscan_desc *desc, void *stream);
  * pixels [16q, 16q+16) and input pixels [16q-8, 16q+24)), splits sqrt(w) f
  * into 16-bit hi/lo columns and chooses power-of-two scales that keep both in
  * range.  Inputs are the (ntarget, npix_pad[, 11]) arrays of rr_zscan_desc;
- * precision is RR_PREC_F16 or RR_PREC_BF16.  Replaces the per-call CSR upload
+ * precision is RR_PREC_F16, RR_PREC_BF16 (16-bit tiles, sqrt(w) f split hi/lo)
+ * or RR_PREC_F32 (fp32 tiles of twice the size, no split: rtile (ntarget,
+ * npix_pad/16, 64, 8) and d_sfcol (ntarget, npix_pad/32, 4, 8) floats).
+ * Replaces the per-call CSR upload
  * of zscan.py:428-430 (done once per target here).
  */
 /*

@@ -13,12 +13,12 @@ LIB_PATH = os.path.join(HERE, 'libredrock_hip.so')
 
 RR_ABI_VERSION = 2
 RR_F64, RR_F32, RR_F16, RR_BF16 = 0, 1, 2, 3
-RR_PREC_F64, RR_PREC_F16, RR_PREC_BF16 = 0, 1, 2
+RR_PREC_F64, RR_PREC_F16, RR_PREC_BF16, RR_PREC_F32 = 0, 1, 2, 3
 RR_STATUS_OK, RR_STATUS_SINGULAR = 0, 1
 RR_MAX_NBASIS = 12
 RR_MAX_LYMAN_LINES = 8
 
-PRECISIONS = {'f64': RR_PREC_F64, 'f16': RR_PREC_F16, 'bf16': RR_PREC_BF16}
+PRECISIONS = {'f64': RR_PREC_F64, 'f16': RR_PREC_F16, 'bf16': RR_PREC_BF16, 'f32': RR_PREC_F32}
 
 EXPORTS = ('rr_abi_version', 'rr_last_error', 'rr_rebin', 'rr_zscan',
            'rr_zscan_workspace_bytes', 'rr_pack_mixed', 'rr_zscan_tiles', 'rr_zscan_grad')

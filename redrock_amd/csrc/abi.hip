@@ -19,7 +19,9 @@ extern "C" int rr_abi_version(void) { return RR_ABI_VERSION; }
 
 extern "C" const char *rr_last_error(void) { return rr::g_err; }
 
-static bool is_mixed(int precision) { return precision == RR_PREC_F16 || precision == RR_PREC_BF16; }
+static bool is_mixed(int precision) {
+    return precision == RR_PREC_F16 || precision == RR_PREC_BF16 || precision == RR_PREC_F32;
+}
 
 extern "C" long long rr_zscan_workspace_bytes(const rr_zscan_desc *d) {
     if (!d || d->nitems < 0 || d->nb < 1 || d->nb > RR_MAX_NBASIS || d->total_rows < 0)
@@ -79,7 +81,10 @@ extern "C" int rr_zscan(const rr_zscan_desc *d, void *stream) {
     int rc = RR_OK;
     if (st & RR_STAGE_STATS) rc = rr::zscan_stats(a, s);
     if (rc) return rc;
-    if (st & RR_STAGE_MAIN) rc = mixed ? rr::zscan_mfma(a, s) : rr::zscan_f64(a, s);
+    if (st & RR_STAGE_MAIN)
+        rc = d->precision == RR_PREC_F32 ? rr::zscan_mfma32(a, s)
+             : mixed                     ? rr::zscan_mfma(a, s)
+                                         : rr::zscan_f64(a, s);
     if (rc) return rc;
     if (st & RR_STAGE_FINISH) rc = rr::zscan_finish(a, s);
     return rc;

@@ -394,7 +394,9 @@ extern "C" int rr_pack_mixed(const double *d_rband, const double *d_w, const dou
     else if (precision == RR_PREC_BF16)
         hipLaunchKernelGGL(pack_mixed_kernel<RR_PREC_BF16>, dim3(ntarget), dim3(64), 0, s, d_rband, d_w,
                            d_f, npix_pad, rt, sf, d_scales);
+    else if (precision == RR_PREC_F32)
+        return rr::pack_mixed32(d_rband, d_w, d_f, ntarget, npix_pad, d_rtile, d_sfcol, d_scales, s);
     else
-        return rr::fail(RR_EINVAL, "rr_pack_mixed: precision must be RR_PREC_F16 or RR_PREC_BF16");
+        return rr::fail(RR_EINVAL, "rr_pack_mixed: precision must be RR_PREC_F16, _BF16 or _F32");
     return rr::check_launch("rr_pack_mixed");
 }

@@ -168,8 +168,9 @@ class TemplateCache(object):
         else:
             ldz = 0
             # + 64 zero elements read by the MFMA lanes of unused basis columns
-            buf = torch.zeros(nz * nb * rows + 64, dtype=torch.int16, device='cuda')
-            ot = _hip.RR_F16 if precision == 'f16' else _hip.RR_BF16
+            dt = torch.float32 if precision == 'f32' else torch.int16
+            buf = torch.zeros(nz * nb * rows + 64, dtype=dt, device='cuda')
+            ot = {'f16': _hip.RR_F16, 'bf16': _hip.RR_BF16, 'f32': _hip.RR_F32}[precision]
             for k, off in zip(layout.keys, layout.offsets):
                 rebin_into(buf[layout.halo + off:], d['x'], d['y'], d['edges'][k], d['z'],
                            nb * rows, 1, rows, ot, self.tscale, self.lyman(k), d['wave'][k])
@@ -234,8 +235,11 @@ class TargetBatch(object):
         import torch
         if precision not in self._mixed:
             P = self.layout.npix_pad
-            rtile = torch.empty(self.nt * (P // 16) * 64 * 8, dtype=torch.int16, device='cuda')
-            sfcol = torch.empty(self.nt * (P // 32) * 64, dtype=torch.int16, device='cuda')
+            dt = torch.float32 if precision == 'f32' else torch.int16
+            rtile = torch.empty(self.nt * (P // 16) * 64 * 8, dtype=dt, device='cuda')
+            # 16-bit: hi and lo halves per lane group; fp32: one value, same count
+            sfcol = torch.empty(self.nt * (P // 32) * (32 if precision == 'f32' else 64),
+                                dtype=dt, device='cuda')
             scales = torch.empty((self.nt, 2), dtype=torch.float64, device='cuda')
             rc = _hip.lib().rr_pack_mixed(_hip.ptr(self.rband), _hip.ptr(self.w), _hip.ptr(self.f),
                                           self.nt, P, _hip.PRECISIONS[precision], _hip.ptr(rtile),
@@ -477,9 +481,12 @@ class _ExplicitCache(object):
                 t = torch.as_tensor(self.tdata[k], dtype=torch.float64, device='cuda')
                 buf[:, :, layout.halo + off:layout.halo + off + n] = (
                     t.permute(0, 2, 1) * self.tscale).float()
-            dt = torch.float16 if precision == 'f16' else torch.bfloat16
-            buf = torch.cat([buf.to(dt).view(torch.int16).reshape(-1),
-                             torch.zeros(64, dtype=torch.int16, device='cuda')])
+            if precision == 'f32':
+                buf = torch.cat([buf.reshape(-1), torch.zeros(64, dtype=torch.float32, device='cuda')])
+            else:
+                dt = torch.float16 if precision == 'f16' else torch.bfloat16
+                buf = torch.cat([buf.to(dt).view(torch.int16).reshape(-1),
+                                 torch.zeros(64, dtype=torch.int16, device='cuda')])
         self._bufs[key] = (buf, ldz)
         return self._bufs[key]
 
