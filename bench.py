@@ -72,6 +72,18 @@ def cpu_baseline(tm, targets, dwave, seconds):
                        f'one process per core, {wall:.1f} s wall')
 
 
+def calibrate_copy(mib):
+    """A plain device-to-device copy of known size, for scaling HBM byte counters."""
+    import torch
+    n = mib << 20
+    x = torch.ones(n, dtype=torch.uint8, device='cuda')
+    y = torch.empty_like(x)
+    for _ in range(3):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({'copy_bytes': n, 'read_bytes': n, 'write_bytes': n}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
@@ -85,7 +97,14 @@ def main():
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--variants', default='bf16,f32',
                     help='other mixed precisions to time and compare (comma separated)')
+    ap.add_argument('--tile', type=int, default=-1,
+                    help='tile configuration of the mixed kernel (-1: autotune)')
+    ap.add_argument('--calibrate-copy-mb', type=int, default=0,
+                    help='only run a plain device copy of this many MiB (HBM counter calibration)')
     args = ap.parse_args()
+    if args.calibrate_copy_mb:
+        calibrate_copy(args.calibrate_copy_mb)
+        return
     if args.gpus != 1:
         print(f'bench.py measures one GPU; --gpus {args.gpus} is not supported '
               f'(targets shard across GPUs with no communication; see the design notes)',
@@ -145,10 +164,14 @@ def main():
 
     from redrock_amd import autotune
     mix = plans[args.precision]
-    tile, tile_ms = autotune.choose(mix, use_cache=False)
     table = autotune.tile_table()
-    log(f'autotune: tile {tile} {table[tile]} ; ' +
-        ', '.join(f'{table[t]}: {ms:.2f} ms' for t, ms in sorted(tile_ms.items())))
+    if args.tile < 0:
+        tile, tile_ms = autotune.choose(mix, use_cache=False)
+        log(f'autotune: tile {tile} {table[tile]} ; ' +
+            ', '.join(f'{table[t]}: {ms:.2f} ms' for t, ms in sorted(tile_ms.items())))
+    else:
+        tile, tile_ms = args.tile, {}
+        mix.desc.tile = tile
     for p, plan in plans.items():
         for _ in range(args.warmup):
             plan.run()

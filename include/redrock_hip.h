@@ -134,10 +134,12 @@ typedef struct {
     const double *d_wf;      /* (ntarget, npix_pad) ivar*flux                   */
     const double *d_f;       /* (ntarget, npix_pad) flux                        */
     /* Mixed precision modes only (NULL otherwise), written by rr_pack_mixed: */
-    const void *d_rtile;     /* (ntarget, npix_pad/16, 64, 8) 16-bit MFMA
-                                A-operand tiles of sqrt(w) R                    */
-    const void *d_sfcol;     /* (ntarget, npix_pad/32, 4, 2, 8) 16-bit hi/lo
-                                of sqrt(w) f in MFMA operand order              */
+    const void *d_rtile;     /* (ntarget, npix_pad/16, 64, 8) MFMA A-operand
+                                tiles of sqrt(w) R (16-bit: K slots 0, 1 of
+                                each row hold sqrt(w) f split hi + lo)          */
+    const void *d_sfcol;     /* RR_PREC_F32 only (NULL allowed otherwise):
+                                (ntarget, npix_pad/32, 4, 8) fp32 sqrt(w) f in
+                                MFMA operand order                              */
     const double *d_scales;  /* (ntarget, 2) power-of-two scales of R and f     */
     int uniform_z;           /* 1 if every item has the same z0 and nz: lets a
                                 wavefront share template tiles across items     */
@@ -205,13 +207,15 @@ int rr_zscan(const rr_zscan_desc *desc, void *stream);
 
 /*
  * rr_pack_mixed — per-target preparation for the mixed precision modes:
- * packs sqrt(w) R into 16-bit MFMA A-operand tiles (tile q covers output
- * pixels [16q, 16q+16) and input pixels [16q-8, 16q+24)), splits sqrt(w) f
- * into 16-bit hi/lo columns and chooses power-of-two scales that keep both in
- * range.  Inputs are the (ntarget, npix_pad[, 11]) arrays of rr_zscan_desc;
- * precision is RR_PREC_F16, RR_PREC_BF16 (16-bit tiles, sqrt(w) f split hi/lo)
- * or RR_PREC_F32 (fp32 tiles of twice the size, no split: rtile (ntarget,
- * npix_pad/16, 64, 8) and d_sfcol (ntarget, npix_pad/32, 4, 8) floats).
+ * packs sqrt(w) R into MFMA A-operand tiles (tile q covers output pixels
+ * [16q, 16q+16) and input pixels [16q-8, 16q+24)) and chooses power-of-two
+ * scales that keep R and sqrt(w) f in range.  Inputs are the (ntarget,
+ * npix_pad[, 11]) arrays of rr_zscan_desc; precision is RR_PREC_F16 or
+ * RR_PREC_BF16 (16-bit tiles; sqrt(w) f split hi + lo into the two K slots of
+ * each tile row that lie outside the band; d_sfcol is not used and may be
+ * NULL) or RR_PREC_F32 (fp32 tiles of twice the size, rtile (ntarget,
+ * npix_pad/16, 64, 8) floats, and sqrt(w) f in d_sfcol (ntarget, npix_pad/32,
+ * 4, 8) floats).
  * Replaces the per-call CSR upload
  * of zscan.py:428-430 (done once per target here).
  */

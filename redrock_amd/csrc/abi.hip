@@ -53,7 +53,7 @@ extern "C" int rr_zscan(const rr_zscan_desc *d, void *stream) {
     if (d->precision == RR_PREC_F64) {
         if (d->cache_ldz < d->cache_nz) return rr::fail(RR_EINVAL, "rr_zscan: cache_ldz < cache_nz");
     } else if (mixed) {
-        if (!d->d_rtile || !d->d_sfcol || !d->d_scales)
+        if (!d->d_rtile || !d->d_scales || (d->precision == RR_PREC_F32 && !d->d_sfcol))
             return rr::fail(RR_EINVAL, "rr_zscan: mixed precision needs rr_pack_mixed outputs");
         if (d->halo != 8) return rr::fail(RR_EINVAL, "rr_zscan: mixed precision needs halo == 8");
         if (d->nb + 2 > 16) return rr::fail(RR_EUNSUPPORTED, "rr_zscan: mixed needs nb <= 14");

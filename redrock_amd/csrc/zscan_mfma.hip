@@ -9,8 +9,11 @@
 //                 the 11-diagonal band scaled by sqrt(w_p) (a dense band
 //                 block, packed once per target, reused for every z)
 //   Tq (32 x 16)  template cache rows of those input pixels, cols = basis
-//                 b < nb (zero for b >= nb)
-//   Cq            zero except columns nb, nb+1 = sqrt(w) f split hi + lo
+//                 b < nb; columns nb and nb+1 are unit vectors on K slots 0
+//                 and 1, and K slots 0 and 1 of the basis columns are zero
+//   Rq K slots 0, 1 (outside the band: input pixels 16q-8, 16q-7 are at
+//                 least 8 pixels from every output row) hold sqrt(w) f split
+//                 hi + lo
 //
 // so D[p][b<nb] = sqrt(w_p) (R T)[p][b] and D[p][nb..nb+1] = sqrt(w_p) f_p.
 // The f32 outputs of two consecutive tiles (32 pixels) are rounded to 16 bits
@@ -31,6 +34,7 @@
 // them in FP64.
 #include "common.hpp"
 #include "zscan_args.hpp"
+#include "xcd_map.hpp"
 
 namespace {
 
@@ -56,6 +60,25 @@ __device__ __forceinline__ unsigned short to16(float f) {
     else return rr::f32_to_bf16_bits(f);
 }
 
+typedef _Float16 half4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bfv4 __attribute__((ext_vector_type(4)));
+typedef unsigned short u16x4 __attribute__((ext_vector_type(4)));
+
+// two MFMA outputs -> one 8-wide 16-bit operand, round to nearest even
+// (v_cvt_pk_{f16,bf16}_f32: one instruction per pair)
+template <int TY>
+__device__ __forceinline__ u16x8 to16x8(float4v a, float4v b) {
+    u16x4 lo, hi;
+    if constexpr (TY == RR_PREC_F16) {
+        lo = __builtin_bit_cast(u16x4, __builtin_convertvector(a, half4));
+        hi = __builtin_bit_cast(u16x4, __builtin_convertvector(b, half4));
+    } else {
+        lo = __builtin_bit_cast(u16x4, __builtin_convertvector(a, bfv4));
+        hi = __builtin_bit_cast(u16x4, __builtin_convertvector(b, bfv4));
+    }
+    return u16x8{lo[0], lo[1], lo[2], lo[3], hi[0], hi[1], hi[2], hi[3]};
+}
+
 template <int TY>
 __device__ __forceinline__ float from16(unsigned short h) {
     if constexpr (TY == RR_PREC_F16) return __half2float(__ushort_as_half(h));
@@ -73,9 +96,8 @@ __device__ __forceinline__ double wave_max(double v) {
 //   rtile  (ntarget, ntile, 64, 8) 16-bit: lane l of tile q holds row i=l%16,
 //          K slots v=0..7 of group g=l/16 -> input pixel 16q-8+8g+v; value
 //          sqrt(w_p) * rscale * R[p][k'] with p = 16q+i, k' = 8g+v-i-3.
-//   sfcol  (ntarget, npix_pad/32, 4, 2, 8) 16-bit: hi/lo split of
-//          sqrt(w_p) f_p * fscale in the Gram-operand order of the lanes of
-//          columns nb (hi) and nb+1 (lo).
+//          K slots 0 and 1 of group g = 0 (always outside the band) hold
+//          sqrt(w_p) f_p * fscale split hi + lo.
 //   scales (ntarget, 2) f64: rscale, fscale (powers of two).
 // ---------------------------------------------------------------------------
 template <int TY>
@@ -83,7 +105,6 @@ __global__ __launch_bounds__(64) void pack_mixed_kernel(const double *__restrict
                                                         const double *__restrict__ w,
                                                         const double *__restrict__ f, int npix_pad,
                                                         unsigned short *__restrict__ rtile,
-                                                        unsigned short *__restrict__ sfcol,
                                                         double *__restrict__ scales) {
     const int t = blockIdx.x;
     const int lane = threadIdx.x;
@@ -110,33 +131,21 @@ __global__ __launch_bounds__(64) void pack_mixed_kernel(const double *__restrict
     const int i = lane & 15, g = lane >> 4;
     for (int q = 0; q < ntile; ++q) {
         const int p = 16 * q + i;
-        const double sw = sqrt(W[p]) * rscale;
+        const double sw = sqrt(W[p]);
         u16x8 v8;
 #pragma unroll
         for (int v = 0; v < 8; ++v) {
-            const int k = 8 * g + v - i - 3;
-            const double r = (k >= 0 && k <= 10) ? Rb[(long long)p * 11 + k] * sw : 0.0;
+            const int k = 8 * g + v - i - 3;     // < 0 for g = 0, v <= 1
+            const double r = (k >= 0 && k <= 10) ? Rb[(long long)p * 11 + k] * sw * rscale : 0.0;
             v8[v] = to16<TY>((float)r);
         }
-        *reinterpret_cast<u16x8 *>(RT + ((long long)q * 64 + lane) * 8) = v8;
-    }
-    // sqrt(w) f as the Gram operand of lanes (j = nb + h, g) of tile pair qp:
-    // slot v < 4 is pixel 32qp + 4g + v, slot v >= 4 is pixel 32qp + 16 + 4g + v-4.
-    const int npair = npix_pad / 32;
-    unsigned short *SF = sfcol + (long long)t * npair * 64;
-    for (int e = lane; e < npair * 4; e += 64) {
-        const int qp = e >> 2, gg = e & 3;
-        u16x8 hi8, lo8;
-#pragma unroll
-        for (int v = 0; v < 8; ++v) {
-            const int p = 32 * qp + (v < 4 ? 4 * gg + v : 16 + 4 * gg + (v - 4));
-            const float s = (float)(sqrt(W[p]) * F[p] * fscale);
-            const unsigned short h = to16<TY>(s);
-            hi8[v] = h;
-            lo8[v] = to16<TY>(s - from16<TY>(h));
+        if (g == 0) {
+            const float sf = (float)(sw * F[p] * fscale);
+            const unsigned short h = to16<TY>(sf);
+            v8[0] = h;
+            v8[1] = to16<TY>(sf - from16<TY>(h));
         }
-        *reinterpret_cast<u16x8 *>(SF + ((long long)e * 2) * 8) = hi8;
-        *reinterpret_cast<u16x8 *>(SF + ((long long)e * 2 + 1) * 8) = lo8;
+        *reinterpret_cast<u16x8 *>(RT + ((long long)q * 64 + lane) * 8) = v8;
     }
 }
 
@@ -145,31 +154,33 @@ __global__ __launch_bounds__(64) void pack_mixed_kernel(const double *__restrict
 // wavefronts as WA (items) x WB (redshifts); a wavefront owns TW = TT/WA items
 // x ZW = ZZ/WB redshifts.  For every tile pair (32 pixels) the workgroup
 // stages in LDS the template rows of its ZZ redshifts (shared by its TT
-// items) and the R tiles and sqrt(w) f operands of its TT items (shared by its
-// ZZ redshifts).  LDS is double-buffered: the global loads of pair p+1 are in
-// flight while the MFMAs consume pair p.  All TT items must share one cache
-// z range (uniform_z) unless TT == 1.
+// items) and the R tiles of its TT items (shared by its ZZ redshifts), with
+// direct global->LDS loads (global_load_lds_dwordx4: lane l of a wave writes
+// 16 bytes at the wave's LDS base + 16 l).  LDS is double-buffered: the loads
+// of pair p+1 are in flight while the MFMAs consume pair p.  All TT items must
+// share one cache z range (uniform_z) unless TT == 1.
 // ---------------------------------------------------------------------------
 template <int NB, int TY, int WA, int WB, int TW, int ZW>
-__global__ __launch_bounds__(256) void zscan_mfma_kernel(rr::ZArgs a, long long ne_off) {
+__global__ __launch_bounds__(256) void zscan_mfma_kernel(rr::ZArgs a, long long ne_off, rr::TileMap map) {
     constexpr int TT = WA * TW, ZZ = WB * ZW;
     constexpr int TCH = ZZ * NB * 6;     // 16-byte template chunks per pair (48 rows x NB x ZZ)
     constexpr int RCH = TT * 2 * 64;     // 16-byte R-tile chunks per pair (2 tiles x 64 lanes)
-    constexpr int SCH = TT * 8;          // 16-byte sqrt(w) f chunks per pair (4 g x hi/lo)
-    constexpr int NCH = TCH + RCH + SCH;
+    constexpr int NCH = TCH + RCH;
     constexpr int PER = (NCH + 255) / 256;
-    __shared__ u16x8 lds[2][NCH];
-    __shared__ u16x8 zero_chunk;
+    // every thread loads PER chunks; the tail of the last round lands in padding
+    __shared__ u16x8 lds[2][PER * 256];
+    __shared__ u16x8 zero_chunk[3];
 
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int wa = wave % WA, wb = wave / WA;
-    const int itemb = blockIdx.x * TT;
-    if (itemb >= a.nitems) return;           // uniform over the workgroup
+    int ti, tz;
+    if (!rr::map_tile(map, blockIdx.x, ti, tz)) return;   // uniform over the workgroup
+    const int itemb = ti * TT;
     const int z0 = a.item_z0[itemb];
     const int nzi = a.item_nz[itemb];
-    const int zb = blockIdx.y * ZZ;
+    const int zb = tz * ZZ;
     if (zb >= nzi) return;                   // uniform over the workgroup
 
     const int ntile = a.npix_pad / 16;
@@ -177,7 +188,6 @@ __global__ __launch_bounds__(256) void zscan_mfma_kernel(rr::ZArgs a, long long 
     const long long rows = a.cache_rows;
     const unsigned short *__restrict__ T = static_cast<const unsigned short *>(a.tcache);
     const unsigned short *__restrict__ RT = static_cast<const unsigned short *>(a.rtile);
-    const unsigned short *__restrict__ SF = static_cast<const unsigned short *>(a.sfcol);
 
     // ---- this thread's share of the staging: source base and per-pair step
     const unsigned short *src[PER];
@@ -185,47 +195,45 @@ __global__ __launch_bounds__(256) void zscan_mfma_kernel(rr::ZArgs a, long long 
 #pragma unroll
     for (int k = 0; k < PER; ++k) {
         int c = tid + 256 * k;
-        if (c >= NCH) c = NCH - 1;           // duplicate the last chunk (harmless)
+        if (c >= NCH) c = NCH - 1;           // duplicate the last chunk into the padding
         if (c < TCH) {
             const int s = c / (NB * 6), b = (c / 6) % NB, part = c % 6;
             const int iz = zb + s;
             const int zc = z0 + (iz < nzi ? iz : nzi - 1);
             src[k] = T + ((long long)zc * NB + b) * rows + 8 * part;
             step[k] = 32;
-        } else if (c < TCH + RCH) {
+        } else {
             const int r = c - TCH, u = r >> 7, h = (r >> 6) & 1, l = r & 63;
             const int it = itemb + u < a.nitems ? itemb + u : a.nitems - 1;
             const int t = a.item_target[it];
             src[k] = RT + (((long long)t * ntile + h) * 64 + l) * 8;
             step[k] = 2 * 64 * 8;
-        } else {
-            const int r = c - TCH - RCH, u = r >> 3, gs = r & 7;
-            const int it = itemb + u < a.nitems ? itemb + u : a.nitems - 1;
-            const int t = a.item_target[it];
-            src[k] = SF + ((long long)t * npair * 8 + gs) * 8;
-            step[k] = 8 * 8;
         }
     }
-    u16x8 pre[PER];
-    auto load = [&](int qp) {
+    auto load = [&](int qp, int buf) {
 #pragma unroll
         for (int k = 0; k < PER; ++k)
-            pre[k] = *reinterpret_cast<const u16x8 *>(src[k] + (long long)qp * step[k]);
-    };
-    auto store = [&](int buf) {
-#pragma unroll
-        for (int k = 0; k < PER; ++k) {
-            const int c = tid + 256 * k;
-            if (c < NCH) lds[buf][c] = pre[k];
-        }
+            __builtin_amdgcn_global_load_lds(
+                (const void *)(src[k] + (long long)qp * step[k]),
+                (void __attribute__((address_space(3))) *)(&lds[buf][256 * k + 64 * wave]), 16, 0, 0);
     };
 
     // ---- operand addressing of this lane
     const int j = lane & 15;   // column (basis) of operand B / output
     const int g = lane >> 4;   // K group / output row group
     const bool real = j < NB;
-    const bool fcolumn = (j == NB) || (j == NB + 1);
-    const int fsel = fcolumn ? j - NB : 0;
+    // B operand, first dword (K slots 0, 1 of group g = 0): zero for the basis
+    // columns (those slots carry sqrt(w) f in the R tile), a unit on slot 0 /
+    // slot 1 for columns nb / nb+1; other lanes pass their template rows.
+    constexpr unsigned one16 = TY == RR_PREC_F16 ? 0x3C00u : 0x3F80u;
+    const unsigned keep0 = g == 0 ? 0u : 0xffffffffu;
+    const unsigned set0 = (g != 0) ? 0u : (j == NB) ? one16 : (j == NB + 1) ? (one16 << 16) : 0u;
+    typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+    auto bop = [&](u16x8 v) {
+        u32x4 d = __builtin_bit_cast(u32x4, v);
+        d[0] = (d[0] & keep0) | set0;
+        return __builtin_bit_cast(u16x8, d);
+    };
 
     float4v G[TW][ZW];
 #pragma unroll
@@ -234,45 +242,44 @@ __global__ __launch_bounds__(256) void zscan_mfma_kernel(rr::ZArgs a, long long 
         for (int s = 0; s < ZW; ++s) G[u][s] = float4v{0.f, 0.f, 0.f, 0.f};
     const float4v zero4 = {0.f, 0.f, 0.f, 0.f};
 
-    if (tid == 0) zero_chunk = u16x8{0, 0, 0, 0, 0, 0, 0, 0};
-    load(0);
-    store(0);
+    if (tid < 3) zero_chunk[tid] = u16x8{0, 0, 0, 0, 0, 0, 0, 0};
+    load(0, 0);
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
     __syncthreads();
 
     for (int qp = 0; qp < npair; ++qp) {
         const int buf = qp & 1;
-        const bool more = qp + 1 < npair;
-        if (more) load(qp + 1);
+        // all operands of this pair into registers first, then the loads of
+        // the next pair: they complete under this pair's MFMAs
         const u16x8 *L = lds[buf];
-        u16x8 ra[TW][2], fx[TW];
+        u16x8 ra[TW][2], bz[ZW][2];
 #pragma unroll
         for (int u = 0; u < TW; ++u) {
             const int iu = wa * TW + u;
             ra[u][0] = L[TCH + iu * 128 + lane];
             ra[u][1] = L[TCH + iu * 128 + 64 + lane];
-            fx[u] = L[TCH + RCH + iu * 8 + g * 2 + fsel];
         }
 #pragma unroll
         for (int s = 0; s < ZW; ++s) {
             const int is = wb * ZW + s;
-            const u16x8 *tp = real ? L + (is * NB + j) * 6 + g : &zero_chunk;
-            const u16x8 b0 = tp[0];
-            const u16x8 b1 = real ? tp[2] : tp[0];
+            const u16x8 *tp = real ? L + (is * NB + j) * 6 + g : zero_chunk;
+            bz[s][0] = bop(tp[0]);
+            bz[s][1] = bop(tp[2]);
+        }
+        if (qp + 1 < npair) load(qp + 1, buf ^ 1);
+#pragma unroll
+        for (int s = 0; s < ZW; ++s) {
 #pragma unroll
             for (int u = 0; u < TW; ++u) {
-                const float4v d0 = mfma<TY>(ra[u][0], b0, zero4);
-                const float4v d1 = mfma<TY>(ra[u][1], b1, zero4);
-                u16x8 x;
-#pragma unroll
-                for (int r = 0; r < 4; ++r) {
-                    x[r] = to16<TY>(d0[r]);
-                    x[4 + r] = to16<TY>(d1[r]);
-                }
-                x = fcolumn ? fx[u] : x;
+                const float4v d0 = mfma<TY>(ra[u][0], bz[s][0], zero4);
+                const float4v d1 = mfma<TY>(ra[u][1], bz[s][1], zero4);
+                const u16x8 x = to16x8<TY>(d0, d1);
                 G[u][s] = mfma<TY>(x, x, G[u][s]);
             }
         }
-        if (more) store(buf ^ 1);
+        // the wait for the next pair's loads stays after every MFMA of this one
+        __builtin_amdgcn_sched_barrier(0);
+        asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
         __syncthreads();
     }
 
@@ -312,10 +319,12 @@ constexpr int kNumTiles = sizeof(kTiles) / sizeof(kTiles[0]);
 template <int NB, int TY, int WA, int WB, int TW, int ZW>
 int launch_tile(const rr::ZArgs &a, hipStream_t s) {
     constexpr int TT = WA * TW, ZZ = WB * ZW;
-    dim3 block(256);
-    dim3 grid((a.nitems + TT - 1) / TT, (a.max_item_nz + ZZ - 1) / ZZ);
+    auto kernel = zscan_mfma_kernel<NB, TY, WA, WB, TW, ZW>;
+    const rr::TileMap map = rr::make_tile_map((a.nitems + TT - 1) / TT, (a.max_item_nz + ZZ - 1) / ZZ,
+                                              rr::resident_per_xcd(kernel, 256),
+                                              TT * 2.0 * 64 * 16, ZZ * NB * 6.0 * 16);
     const long long ne_off = rr::align256((long long)a.nitems * 2 * 8);
-    hipLaunchKernelGGL((zscan_mfma_kernel<NB, TY, WA, WB, TW, ZW>), grid, block, 0, s, a, ne_off);
+    hipLaunchKernelGGL(kernel, dim3((unsigned)map.blocks), dim3(256), 0, s, a, ne_off, map);
     return rr::check_launch("rr_zscan(mixed)");
 }
 
@@ -382,18 +391,17 @@ extern "C" int rr_pack_mixed(const double *d_rband, const double *d_w, const dou
                              void *d_sfcol, double *d_scales, void *stream) {
     if (ntarget < 0 || npix_pad < 32 || (npix_pad % 32) != 0)
         return rr::fail(RR_EINVAL, "rr_pack_mixed: bad sizes ntarget=%d npix_pad=%d", ntarget, npix_pad);
-    if (!d_rband || !d_w || !d_f || !d_rtile || !d_sfcol || !d_scales)
+    if (!d_rband || !d_w || !d_f || !d_rtile || !d_scales || (precision == RR_PREC_F32 && !d_sfcol))
         return rr::fail(RR_EINVAL, "rr_pack_mixed: null pointer");
     if (ntarget == 0) return RR_OK;
     hipStream_t s = static_cast<hipStream_t>(stream);
     unsigned short *rt = static_cast<unsigned short *>(d_rtile);
-    unsigned short *sf = static_cast<unsigned short *>(d_sfcol);
     if (precision == RR_PREC_F16)
         hipLaunchKernelGGL(pack_mixed_kernel<RR_PREC_F16>, dim3(ntarget), dim3(64), 0, s, d_rband, d_w,
-                           d_f, npix_pad, rt, sf, d_scales);
+                           d_f, npix_pad, rt, d_scales);
     else if (precision == RR_PREC_BF16)
         hipLaunchKernelGGL(pack_mixed_kernel<RR_PREC_BF16>, dim3(ntarget), dim3(64), 0, s, d_rband, d_w,
-                           d_f, npix_pad, rt, sf, d_scales);
+                           d_f, npix_pad, rt, d_scales);
     else if (precision == RR_PREC_F32)
         return rr::pack_mixed32(d_rband, d_w, d_f, ntarget, npix_pad, d_rtile, d_sfcol, d_scales, s);
     else

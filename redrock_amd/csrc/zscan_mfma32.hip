@@ -16,6 +16,7 @@
 // 24 MFMAs (16 + 8).  LDS staging and tile shapes follow the 16-bit kernel.
 #include "common.hpp"
 #include "zscan_args.hpp"
+#include "xcd_map.hpp"
 
 namespace {
 
@@ -89,7 +90,7 @@ __global__ __launch_bounds__(64) void pack32_kernel(const double *__restrict__ r
 }
 
 template <int NB, int WA, int WB, int TW, int ZW>
-__global__ __launch_bounds__(256) void zscan_mfma32_kernel(rr::ZArgs a, long long ne_off) {
+__global__ __launch_bounds__(256) void zscan_mfma32_kernel(rr::ZArgs a, long long ne_off, rr::TileMap map) {
     constexpr int TT = WA * TW, ZZ = WB * ZW;
     constexpr int TCH = ZZ * NB * 12;    // 16-byte chunks: 48 rows x 4 B per basis per z
     constexpr int RCH = TT * 2 * 128;    // 2 tiles x 64 lanes x 32 B
@@ -103,11 +104,12 @@ __global__ __launch_bounds__(256) void zscan_mfma32_kernel(rr::ZArgs a, long lon
     const int lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int wa = wave % WA, wb = wave / WA;
-    const int itemb = blockIdx.x * TT;
-    if (itemb >= a.nitems) return;
+    int ti, tz;
+    if (!rr::map_tile(map, blockIdx.x, ti, tz)) return;   // uniform over the workgroup
+    const int itemb = ti * TT;
     const int z0 = a.item_z0[itemb];
     const int nzi = a.item_nz[itemb];
-    const int zb = blockIdx.y * ZZ;
+    const int zb = tz * ZZ;
     if (zb >= nzi) return;
 
     const int ntile = a.npix_pad / 16;
@@ -255,9 +257,12 @@ __global__ __launch_bounds__(256) void zscan_mfma32_kernel(rr::ZArgs a, long lon
 template <int NB, int WA, int WB, int TW, int ZW>
 int launch_tile(const rr::ZArgs &a, hipStream_t s) {
     constexpr int TT = WA * TW, ZZ = WB * ZW;
-    dim3 grid((a.nitems + TT - 1) / TT, (a.max_item_nz + ZZ - 1) / ZZ);
+    auto kernel = zscan_mfma32_kernel<NB, WA, WB, TW, ZW>;
+    const rr::TileMap map = rr::make_tile_map((a.nitems + TT - 1) / TT, (a.max_item_nz + ZZ - 1) / ZZ,
+                                              rr::resident_per_xcd(kernel, 256),
+                                              TT * 2.0 * 128 * 16, ZZ * NB * 12.0 * 16);
     const long long ne_off = rr::align256((long long)a.nitems * 2 * 8);
-    hipLaunchKernelGGL((zscan_mfma32_kernel<NB, WA, WB, TW, ZW>), grid, dim3(256), 0, s, a, ne_off);
+    hipLaunchKernelGGL(kernel, dim3((unsigned)map.blocks), dim3(256), 0, s, a, ne_off, map);
     return rr::check_launch("rr_zscan(f32)");
 }
 

@@ -237,9 +237,9 @@ class TargetBatch(object):
             P = self.layout.npix_pad
             dt = torch.float32 if precision == 'f32' else torch.int16
             rtile = torch.empty(self.nt * (P // 16) * 64 * 8, dtype=dt, device='cuda')
-            # 16-bit: hi and lo halves per lane group; fp32: one value, same count
-            sfcol = torch.empty(self.nt * (P // 32) * (32 if precision == 'f32' else 64),
-                                dtype=dt, device='cuda')
+            # fp32 only: sqrt(w) f in operand order (16-bit tiles carry it themselves)
+            sfcol = (torch.empty(self.nt * (P // 32) * 32, dtype=dt, device='cuda')
+                     if precision == 'f32' else None)
             scales = torch.empty((self.nt, 2), dtype=torch.float64, device='cuda')
             rc = _hip.lib().rr_pack_mixed(_hip.ptr(self.rband), _hip.ptr(self.w), _hip.ptr(self.f),
                                           self.nt, P, _hip.PRECISIONS[precision], _hip.ptr(rtile),
@@ -313,7 +313,8 @@ class ScanPlan(object):
         d.d_f = batch.f.data_ptr()
         if precision != 'f64':
             rtile, sfcol, scales = batch.mixed(precision)
-            d.d_rtile, d.d_sfcol, d.d_scales = rtile.data_ptr(), sfcol.data_ptr(), scales.data_ptr()
+            d.d_rtile, d.d_scales = rtile.data_ptr(), scales.data_ptr()
+            d.d_sfcol = None if sfcol is None else sfcol.data_ptr()
             self._keep += [rtile, sfcol, scales]
         d.tile = int(tile)
         d.uniform_z = int(self.nitems > 0 and np.all(item_z0 == item_z0[0])
