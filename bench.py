@@ -31,7 +31,7 @@ sys.path.insert(0, ROOT)
 METRIC = 'zchi2 evals/sec on 1 GPU: mixed-precision vs FP64 plain at same best-z'
 # reference-form FLOPs per eval (R-apply, weights, Gram, y, solve, model, chi2)
 FLOP_PER_EVAL = {'GALAXY': 3763200, 'QSO': 1137348, 'STAR': 1495337}
-PEAK_FLOPS = {'f16': 2.5e15, 'bf16': 2.5e15, 'f64': 78.6e12}
+PEAK_FLOPS = {'f16': 2.5e15, 'bf16': 2.5e15, 'f16x3': 2.5e15, 'f64': 78.6e12}
 
 
 def log(*a):
@@ -90,12 +90,12 @@ def main():
     ap.add_argument('--steps', type=int, default=10)
     ap.add_argument('--warmup', type=int, default=2)
     ap.add_argument('--ntargets', type=int, default=500)
-    ap.add_argument('--precision', default='f16', choices=('f16', 'bf16'))
+    ap.add_argument('--precision', default='f16', choices=('f16', 'bf16', 'f16x3'))
     ap.add_argument('--snr', type=float, default=1.0)
     ap.add_argument('--seed', type=int, default=0)
     ap.add_argument('--cpu-seconds', type=float, default=12.0)
     ap.add_argument('--no-cpu', action='store_true')
-    ap.add_argument('--variants', default='bf16,f32',
+    ap.add_argument('--variants', default='bf16,f32,f16x3',
                     help='other mixed precisions to time and compare (comma separated)')
     ap.add_argument('--tile', type=int, default=-1,
                     help='tile configuration of the mixed kernel (-1: autotune)')
@@ -164,7 +164,7 @@ def main():
 
     from redrock_amd import autotune
     mix = plans[args.precision]
-    table = autotune.tile_table()
+    table = autotune.tile_table(args.precision)
     if args.tile < 0:
         tile, tile_ms = autotune.choose(mix, use_cache=False)
         log(f'autotune: tile {tile} {table[tile]} ; ' +
@@ -250,7 +250,8 @@ def main():
         'setup_s': setup_s,
         'roofline': {'bound': 'mfma', 'achieved': ach / 1e12, 'peak': PEAK_FLOPS[args.precision] / 1e12,
                      'unit': 'TFLOP/s', 'frac': ach / PEAK_FLOPS[args.precision], 'traffic': None,
-                     'kernel': 'zscan_mfma_kernel', 'kernel_ms': k_mix,
+                     'kernel': 'zscan_mfma3_kernel' if args.precision == 'f16x3' else 'zscan_mfma_kernel',
+                     'kernel_ms': k_mix,
                      'flop_per_eval': FLOP_PER_EVAL['GALAXY'], 'evals_per_launch': evals},
         'median_rel_chi2_err': float(np.median(rel)),
         'variants': variants,

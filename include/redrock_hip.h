@@ -39,6 +39,9 @@ extern "C" {
 #define RR_PREC_F16 1   /* MFMA fp16 inputs, fp32 accumulate, FP64 Cholesky     */
 #define RR_PREC_BF16 2  /* MFMA bf16 inputs, fp32 accumulate, FP64 Cholesky     */
 #define RR_PREC_F32 3   /* MFMA fp32 inputs, fp32 accumulate, FP64 Cholesky     */
+#define RR_PREC_F16X3 4 /* MFMA fp16 hi+lo split inputs (three products per
+                           term), fp32 partial sums added in FP64, FP64
+                           Cholesky                                           */
 
 /* Resolution band: half width 5 (11 diagonals), the DESI layout. */
 #define RR_BAND_HALF 5
@@ -121,6 +124,11 @@ typedef struct {
      *  F16/BF16: d_tcache is 16-bit, element (zc, b, row) at
      *         (zc*nb + b)*cache_rows + row                  ([z][b][row]),
      *         followed by at least 64 zero elements
+     *  F32  : as F16 with fp32 elements
+     *  F16X3: fp16, element (zc, h, b, row) at
+     *         ((zc*2 + h)*nb + b)*cache_rows + row  ([z][hi,lo][b][row]),
+     *         h = 0: fp16(v), h = 1: fp16(v - hi), followed by at least 64
+     *         zero elements
      * row = pixel + halo.  cache_nz is the extent of the z axis.            */
     const void *d_tcache;
     long long cache_rows;
@@ -136,7 +144,9 @@ typedef struct {
     /* Mixed precision modes only (NULL otherwise), written by rr_pack_mixed: */
     const void *d_rtile;     /* (ntarget, npix_pad/16, 64, 8) MFMA A-operand
                                 tiles of sqrt(w) R (16-bit: K slots 0, 1 of
-                                each row hold sqrt(w) f split hi + lo)          */
+                                each row hold sqrt(w) f split hi + lo; F16X3:
+                                (ntarget, npix_pad/16, 2, 64, 8), hi and lo
+                                tiles)                                          */
     const void *d_sfcol;     /* RR_PREC_F32 only (NULL allowed otherwise):
                                 (ntarget, npix_pad/32, 4, 8) fp32 sqrt(w) f in
                                 MFMA operand order                              */
@@ -213,9 +223,10 @@ int rr_zscan(const rr_zscan_desc *desc, void *stream);
  * npix_pad[, 11]) arrays of rr_zscan_desc; precision is RR_PREC_F16 or
  * RR_PREC_BF16 (16-bit tiles; sqrt(w) f split hi + lo into the two K slots of
  * each tile row that lie outside the band; d_sfcol is not used and may be
- * NULL) or RR_PREC_F32 (fp32 tiles of twice the size, rtile (ntarget,
- * npix_pad/16, 64, 8) floats, and sqrt(w) f in d_sfcol (ntarget, npix_pad/32,
- * 4, 8) floats).
+ * NULL), RR_PREC_F16X3 (hi and lo fp16 tiles, twice the 16-bit size;
+ * d_sfcol not used) or RR_PREC_F32 (fp32 tiles of twice the 16-bit size,
+ * rtile (ntarget, npix_pad/16, 64, 8) floats, and sqrt(w) f in d_sfcol
+ * (ntarget, npix_pad/32, 4, 8) floats).
  * Replaces the per-call CSR upload
  * of zscan.py:428-430 (done once per target here).
  */
@@ -228,6 +239,13 @@ int rr_zscan(const rr_zscan_desc *desc, void *stream);
  * to pick rr_zscan_desc.tile per (nbasis, pixel count, precision).
  */
 int rr_zscan_tiles(int *cfg, int max);
+
+/*
+ * rr_zscan_tile_table — as rr_zscan_tiles for the kernel of `precision`
+ * (RR_PREC_F16 / _BF16: the table of rr_zscan_tiles; RR_PREC_F16X3: its own
+ * table; other modes have a single configuration and return 0).
+ */
+int rr_zscan_tile_table(int precision, int *cfg, int max);
 
 int rr_pack_mixed(const double *d_rband, const double *d_w, const double *d_f, int ntarget,
                   int npix_pad, int precision, void *d_rtile, void *d_sfcol,

@@ -13,15 +13,17 @@ LIB_PATH = os.path.join(HERE, 'libredrock_hip.so')
 
 RR_ABI_VERSION = 2
 RR_F64, RR_F32, RR_F16, RR_BF16 = 0, 1, 2, 3
-RR_PREC_F64, RR_PREC_F16, RR_PREC_BF16, RR_PREC_F32 = 0, 1, 2, 3
+RR_PREC_F64, RR_PREC_F16, RR_PREC_BF16, RR_PREC_F32, RR_PREC_F16X3 = 0, 1, 2, 3, 4
 RR_STATUS_OK, RR_STATUS_SINGULAR = 0, 1
 RR_MAX_NBASIS = 12
 RR_MAX_LYMAN_LINES = 8
 
-PRECISIONS = {'f64': RR_PREC_F64, 'f16': RR_PREC_F16, 'bf16': RR_PREC_BF16, 'f32': RR_PREC_F32}
+PRECISIONS = {'f64': RR_PREC_F64, 'f16': RR_PREC_F16, 'bf16': RR_PREC_BF16, 'f32': RR_PREC_F32,
+              'f16x3': RR_PREC_F16X3}
 
 EXPORTS = ('rr_abi_version', 'rr_last_error', 'rr_rebin', 'rr_zscan',
-           'rr_zscan_workspace_bytes', 'rr_pack_mixed', 'rr_zscan_tiles', 'rr_zscan_grad')
+           'rr_zscan_workspace_bytes', 'rr_pack_mixed', 'rr_zscan_tiles', 'rr_zscan_tile_table',
+           'rr_zscan_grad')
 
 _vp = ctypes.c_void_p
 _i = ctypes.c_int
@@ -83,6 +85,8 @@ def lib():
     L.rr_pack_mixed.argtypes = [_vp, _vp, _vp, _i, _i, _i, _vp, _vp, _vp, _vp]
     L.rr_zscan_tiles.restype = _i
     L.rr_zscan_tiles.argtypes = [ctypes.POINTER(_i), _i]
+    L.rr_zscan_tile_table.restype = _i
+    L.rr_zscan_tile_table.argtypes = [_i, ctypes.POINTER(_i), _i]
     v = L.rr_abi_version()
     if v != RR_ABI_VERSION:
         raise HipError(f'libredrock_hip ABI {v}, expected {RR_ABI_VERSION}')

@@ -18,14 +18,20 @@ import os
 from . import _hip
 
 
-def tile_table():
-    """[(wa, wb, tw, zw), ...] indexed by rr_zscan_desc.tile (entry 0 = default).
+def tile_table(precision=_hip.RR_PREC_F16):
+    """[(wa, wb, tw, zw), ...] indexed by rr_zscan_desc.tile (entry 0 = default)
+    for the kernel of `precision` (an RR_PREC_* code or a PRECISIONS name);
+    empty for modes with a single configuration.
 
     A workgroup covers wa*tw items x wb*zw redshifts.
     """
+    if isinstance(precision, str):
+        precision = _hip.PRECISIONS[precision]
     n = 32
     cfg = (ctypes.c_int * (4 * n))()
-    k = _hip.lib().rr_zscan_tiles(cfg, n)
+    k = _hip.lib().rr_zscan_tile_table(int(precision), cfg, n)
+    if k < 0:
+        _hip.check(k, 'rr_zscan_tile_table')
     return [tuple(cfg[4 * i:4 * i + 4]) for i in range(min(k, n))]
 
 
@@ -45,7 +51,9 @@ def _key(plan):
 def tune(plan, reps=3):
     """Time every tile configuration on `plan`; returns (best_tile, {tile: ms})."""
     import torch
-    table = tile_table()
+    table = tile_table(plan.desc.precision)
+    if len(table) < 2:
+        return 0, dict()
     saved_tile = plan.desc.tile
     timings = dict()
     plan.run()   # stats + a full pass so the workspace holds valid statistics

@@ -20,7 +20,8 @@ extern "C" int rr_abi_version(void) { return RR_ABI_VERSION; }
 extern "C" const char *rr_last_error(void) { return rr::g_err; }
 
 static bool is_mixed(int precision) {
-    return precision == RR_PREC_F16 || precision == RR_PREC_BF16 || precision == RR_PREC_F32;
+    return precision == RR_PREC_F16 || precision == RR_PREC_BF16 || precision == RR_PREC_F32 ||
+           precision == RR_PREC_F16X3;
 }
 
 extern "C" long long rr_zscan_workspace_bytes(const rr_zscan_desc *d) {
@@ -82,10 +83,18 @@ extern "C" int rr_zscan(const rr_zscan_desc *d, void *stream) {
     if (st & RR_STAGE_STATS) rc = rr::zscan_stats(a, s);
     if (rc) return rc;
     if (st & RR_STAGE_MAIN)
-        rc = d->precision == RR_PREC_F32 ? rr::zscan_mfma32(a, s)
-             : mixed                     ? rr::zscan_mfma(a, s)
-                                         : rr::zscan_f64(a, s);
+        rc = d->precision == RR_PREC_F32     ? rr::zscan_mfma32(a, s)
+             : d->precision == RR_PREC_F16X3 ? rr::zscan_mfma3(a, s)
+             : mixed                         ? rr::zscan_mfma(a, s)
+                                             : rr::zscan_f64(a, s);
     if (rc) return rc;
     if (st & RR_STAGE_FINISH) rc = rr::zscan_finish(a, s);
     return rc;
+}
+
+extern "C" int rr_zscan_tile_table(int precision, int *cfg, int max) {
+    if (!cfg || max < 0) return rr::fail(RR_EINVAL, "rr_zscan_tile_table: bad arguments");
+    if (precision == RR_PREC_F16 || precision == RR_PREC_BF16) return rr::zscan_tiles16(cfg, max);
+    if (precision == RR_PREC_F16X3) return rr::zscan_tiles3(cfg, max);
+    return 0;
 }

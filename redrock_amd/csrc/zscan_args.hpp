@@ -49,6 +49,11 @@ int zscan_finish(const ZArgs &a, hipStream_t s);
 int zscan_f64(const ZArgs &a, hipStream_t s);
 int zscan_mfma(const ZArgs &a, hipStream_t s);
 int zscan_mfma32(const ZArgs &a, hipStream_t s);
+int zscan_mfma3(const ZArgs &a, hipStream_t s);
+int pack_mixed3(const double *d_rband, const double *d_w, const double *d_f, int ntarget, int npix_pad,
+                void *d_rtile, double *d_scales, hipStream_t s);
+int zscan_tiles3(int *cfg, int max);
+int zscan_tiles16(int *cfg, int max);
 int pack_mixed32(const double *d_rband, const double *d_w, const double *d_f, int ntarget,
                  int npix_pad, void *d_rtile, void *d_sfcol, double *d_scales, hipStream_t s);
 

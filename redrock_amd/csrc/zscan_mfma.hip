@@ -35,61 +35,9 @@
 #include "common.hpp"
 #include "zscan_args.hpp"
 #include "xcd_map.hpp"
+#include "mfma16.hpp"
 
 namespace {
-
-typedef _Float16 half8 __attribute__((ext_vector_type(8)));
-typedef __bf16 bfv8 __attribute__((ext_vector_type(8)));
-typedef float float4v __attribute__((ext_vector_type(4)));
-typedef unsigned short u16x8 __attribute__((ext_vector_type(8)));
-
-template <int TY>
-__device__ __forceinline__ float4v mfma(u16x8 a, u16x8 b, float4v c) {
-    if constexpr (TY == RR_PREC_F16) {
-        return __builtin_amdgcn_mfma_f32_16x16x32_f16(__builtin_bit_cast(half8, a),
-                                                      __builtin_bit_cast(half8, b), c, 0, 0, 0);
-    } else {
-        return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bfv8, a),
-                                                       __builtin_bit_cast(bfv8, b), c, 0, 0, 0);
-    }
-}
-
-template <int TY>
-__device__ __forceinline__ unsigned short to16(float f) {
-    if constexpr (TY == RR_PREC_F16) return rr::f32_to_f16_bits(f);
-    else return rr::f32_to_bf16_bits(f);
-}
-
-typedef _Float16 half4 __attribute__((ext_vector_type(4)));
-typedef __bf16 bfv4 __attribute__((ext_vector_type(4)));
-typedef unsigned short u16x4 __attribute__((ext_vector_type(4)));
-
-// two MFMA outputs -> one 8-wide 16-bit operand, round to nearest even
-// (v_cvt_pk_{f16,bf16}_f32: one instruction per pair)
-template <int TY>
-__device__ __forceinline__ u16x8 to16x8(float4v a, float4v b) {
-    u16x4 lo, hi;
-    if constexpr (TY == RR_PREC_F16) {
-        lo = __builtin_bit_cast(u16x4, __builtin_convertvector(a, half4));
-        hi = __builtin_bit_cast(u16x4, __builtin_convertvector(b, half4));
-    } else {
-        lo = __builtin_bit_cast(u16x4, __builtin_convertvector(a, bfv4));
-        hi = __builtin_bit_cast(u16x4, __builtin_convertvector(b, bfv4));
-    }
-    return u16x8{lo[0], lo[1], lo[2], lo[3], hi[0], hi[1], hi[2], hi[3]};
-}
-
-template <int TY>
-__device__ __forceinline__ float from16(unsigned short h) {
-    if constexpr (TY == RR_PREC_F16) return __half2float(__ushort_as_half(h));
-    else return __uint_as_float(((unsigned)h) << 16);
-}
-
-__device__ __forceinline__ double wave_max(double v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
-    return v;
-}
 
 // ---------------------------------------------------------------------------
 // Packing: one workgroup (one wave) per target.
@@ -313,7 +261,8 @@ struct TileCfg {
     int wa, wb, tw, zw;
 };
 constexpr TileCfg kTiles[] = {{2, 2, 4, 4}, {1, 4, 1, 4}, {2, 2, 4, 4}, {2, 2, 2, 4},
-                              {4, 1, 2, 8}, {1, 4, 4, 2}, {2, 2, 2, 2}, {1, 4, 2, 4}};
+                              {4, 1, 2, 8}, {1, 4, 4, 2}, {2, 2, 4, 8}, {1, 4, 4, 4},
+                              {4, 1, 4, 4}, {2, 2, 8, 2}};
 constexpr int kNumTiles = sizeof(kTiles) / sizeof(kTiles[0]);
 
 template <int NB, int TY, int WA, int WB, int TW, int ZW>
@@ -336,8 +285,10 @@ int launch_nb(const rr::ZArgs &a, int tile, hipStream_t s) {
         case 3: return launch_tile<NB, TY, 2, 2, 2, 4>(a, s);
         case 4: return launch_tile<NB, TY, 4, 1, 2, 8>(a, s);
         case 5: return launch_tile<NB, TY, 1, 4, 4, 2>(a, s);
-        case 6: return launch_tile<NB, TY, 2, 2, 2, 2>(a, s);
-        case 7: return launch_tile<NB, TY, 1, 4, 2, 4>(a, s);
+        case 6: return launch_tile<NB, TY, 2, 2, 4, 8>(a, s);
+        case 7: return launch_tile<NB, TY, 1, 4, 4, 4>(a, s);
+        case 8: return launch_tile<NB, TY, 4, 1, 4, 4>(a, s);
+        case 9: return launch_tile<NB, TY, 2, 2, 8, 2>(a, s);
         default: return rr::fail(RR_EINVAL, "rr_zscan(mixed): unknown tile %d", tile);
     }
 }
@@ -375,7 +326,8 @@ int zscan_mfma(const ZArgs &a, hipStream_t s) {
 }
 }  // namespace rr
 
-extern "C" int rr_zscan_tiles(int *cfg, int max) {
+namespace rr {
+int zscan_tiles16(int *cfg, int max) {
     const int n = kNumTiles < max ? kNumTiles : max;
     for (int i = 0; i < n; ++i) {
         cfg[4 * i] = kTiles[i].wa;
@@ -384,6 +336,12 @@ extern "C" int rr_zscan_tiles(int *cfg, int max) {
         cfg[4 * i + 3] = kTiles[i].zw;
     }
     return kNumTiles;
+}
+}  // namespace rr
+
+extern "C" int rr_zscan_tiles(int *cfg, int max) {
+    if (!cfg || max < 0) return rr::fail(RR_EINVAL, "rr_zscan_tiles: bad arguments");
+    return rr::zscan_tiles16(cfg, max);
 }
 
 extern "C" int rr_pack_mixed(const double *d_rband, const double *d_w, const double *d_f,
@@ -402,9 +360,11 @@ extern "C" int rr_pack_mixed(const double *d_rband, const double *d_w, const dou
     else if (precision == RR_PREC_BF16)
         hipLaunchKernelGGL(pack_mixed_kernel<RR_PREC_BF16>, dim3(ntarget), dim3(64), 0, s, d_rband, d_w,
                            d_f, npix_pad, rt, d_scales);
+    else if (precision == RR_PREC_F16X3)
+        return rr::pack_mixed3(d_rband, d_w, d_f, ntarget, npix_pad, d_rtile, d_scales, s);
     else if (precision == RR_PREC_F32)
         return rr::pack_mixed32(d_rband, d_w, d_f, ntarget, npix_pad, d_rtile, d_sfcol, d_scales, s);
     else
-        return rr::fail(RR_EINVAL, "rr_pack_mixed: precision must be RR_PREC_F16, _BF16 or _F32");
+        return rr::fail(RR_EINVAL, "rr_pack_mixed: precision must be RR_PREC_F16, _BF16, _F16X3 or _F32");
     return rr::check_launch("rr_pack_mixed");
 }

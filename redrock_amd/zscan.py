@@ -150,7 +150,8 @@ class TemplateCache(object):
         """Device cache for `layout` in the layout of `precision`'s kernel.
 
         f64 : float64 [nb][cache_rows][ldz]   (z fastest), ldz = nz rounded to 64
-        f16/bf16: 16-bit [nz][nb][cache_rows] (pixel fastest), values * tscale
+        f16/bf16/f32: [nz][nb][cache_rows] (pixel fastest), values * tscale
+        f16x3: fp16 [nz][hi, lo][nb][cache_rows], hi + lo = values * tscale
         Returns (tensor, ldz).
         """
         import torch
@@ -165,6 +166,13 @@ class TemplateCache(object):
             for k, off in zip(layout.keys, layout.offsets):
                 rebin_into(buf[(layout.halo + off) * ldz:], d['x'], d['y'], d['edges'][k], d['z'],
                            1, ldz, rows * ldz, _hip.RR_F64, 1.0, self.lyman(k), d['wave'][k])
+        elif precision == 'f16x3':
+            ldz = 0
+            full = torch.zeros(nz * nb * rows, dtype=torch.float32, device='cuda')
+            for k, off in zip(layout.keys, layout.offsets):
+                rebin_into(full[layout.halo + off:], d['x'], d['y'], d['edges'][k], d['z'],
+                           nb * rows, 1, rows, _hip.RR_F32, self.tscale, self.lyman(k), d['wave'][k])
+            buf = _split_planes(full.reshape(nz, nb, rows))
         else:
             ldz = 0
             # + 64 zero elements read by the MFMA lanes of unused basis columns
@@ -236,7 +244,8 @@ class TargetBatch(object):
         if precision not in self._mixed:
             P = self.layout.npix_pad
             dt = torch.float32 if precision == 'f32' else torch.int16
-            rtile = torch.empty(self.nt * (P // 16) * 64 * 8, dtype=dt, device='cuda')
+            planes = 2 if precision == 'f16x3' else 1      # hi and lo tiles
+            rtile = torch.empty(self.nt * (P // 16) * planes * 64 * 8, dtype=dt, device='cuda')
             # fp32 only: sqrt(w) f in operand order (16-bit tiles carry it themselves)
             sfcol = (torch.empty(self.nt * (P // 32) * 32, dtype=dt, device='cuda')
                      if precision == 'f32' else None)
@@ -450,6 +459,15 @@ def _host(a):
     return np.asarray(a)
 
 
+def _split_planes(full):
+    """(nz, nb, rows) float32 -> fp16 [nz][hi, lo][nb][rows] as int16, + 64 zeros."""
+    import torch
+    hi = full.half()
+    lo = (full - hi.float()).half()
+    planes = torch.stack([hi, lo], dim=1).view(torch.int16).reshape(-1)
+    return torch.cat([planes, torch.zeros(64, dtype=torch.int16, device=full.device)])
+
+
 class _ExplicitCache(object):
     """Template cache assembled from an explicit tdata dict (calc_zchi2_batch)."""
 
@@ -484,6 +502,8 @@ class _ExplicitCache(object):
                     t.permute(0, 2, 1) * self.tscale).float()
             if precision == 'f32':
                 buf = torch.cat([buf.reshape(-1), torch.zeros(64, dtype=torch.float32, device='cuda')])
+            elif precision == 'f16x3':
+                buf = _split_planes(buf)
             else:
                 dt = torch.float16 if precision == 'f16' else torch.bfloat16
                 buf = torch.cat([buf.to(dt).view(torch.int16).reshape(-1),

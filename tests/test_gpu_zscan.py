@@ -50,7 +50,7 @@ def test_f64_deterministic(galaxy):
 
 
 @pytest.mark.parametrize('precision,tol_max,tol_med', [('f16', 3e-4, 1e-5), ('bf16', 3e-3, 1e-4),
-                                                       ('f32', 1e-6, 1e-7)])
+                                                       ('f32', 1e-6, 1e-7), ('f16x3', 1e-6, 1e-7)])
 def test_mixed_matches_oracle(galaxy, precision, tol_max, tol_med):
     """Mixed precision at per-pixel S/N 1 (a bright target).
 
