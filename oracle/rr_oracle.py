@@ -28,6 +28,12 @@ fitz                       restatement only for the control flow; its pieces
                            are pinned above.  Control flow: parity unpinned.
 calc_deltachi2             reference vectors (test_zfind.py:141-177,
 sort_zfit_dict             test_zscan.py:101-111, test_zfind.py:86-139)
+zfind_assemble             restatement of zfind.py:467-714; its pieces
+                           (sort, delta chi2) pinned above, the nminima trim
+                           and NODATA / LITTLE_COVERAGE bits by hand-built
+                           cases with known answers.  Column stacking:
+                           parity unpinned (no reference fixture; astropy
+                           absent).
 =========================  ====================================================
 """
 from __future__ import annotations
@@ -494,3 +500,86 @@ def sort_zfit_dict(zfit):
     zfit['__badfit__'] = (zfit['zwarn'] & BADFIT_MASK) != 0
     sort_dict_by_cols(zfit, ('__badfit__', 'chi2'), sort_first_column_first=True)
     zfit.pop('__badfit__')
+
+
+ZW_LITTLE_COVERAGE = 2 ** 1
+ZW_NODATA = 2 ** 9
+
+
+def zfind_assemble(allresults, target_ids, maxcoeff, nminima=3):
+    """Per-target table assembly of zfind.py:467-714 (no archetypes).
+
+    allresults[tid][fulltype]['zfit'] holds fitz's dict; fulltype is
+    'SPECTYPE' or 'SPECTYPE:::SUBTYPE'.  maxcoeff is the largest template
+    nbasis.  Returns one dict of columns, one row per (target, minimum).
+    """
+    allzfit = []
+    for tid in target_ids:
+        tzfit = []
+        for fulltype in allresults[tid]:
+            if fulltype == 'meta':
+                continue
+            tmp = dict(allresults[tid][fulltype]['zfit'])
+            if ':::' in fulltype:
+                spectype, subtype = fulltype.split(':::')
+            else:
+                spectype, subtype = fulltype, ''
+            nmin = len(tmp['chi2'])
+            if nmin == 0:
+                continue
+            tmp['spectype'] = np.full((nmin, 1), spectype)
+            tmp['subtype'] = np.full((nmin, 1), subtype)
+            tmp['ncoeff'] = np.array([tmp['coeff'].shape[1]] * nmin).reshape((nmin, 1))
+            if spectype.upper() == 'STAR':
+                mvd = MAX_VELO_DIFF_STAR
+            else:
+                mvd = MAX_VELO_DIFF
+            tmp['max_velo_diff'] = np.full((nmin, 1), mvd)
+            tzfit.append(tmp)
+        if not tzfit:
+            continue
+        maxnc = max([tmp['coeff'].shape[1] for tmp in tzfit])
+        for tmp in tzfit:
+            if tmp['coeff'].shape[1] < maxnc:
+                n = maxnc - tmp['coeff'].shape[1]
+                tmp['coeff'] = np.append(tmp['coeff'], np.zeros((tmp['coeff'].shape[0], n)), axis=1)
+        tz = dict()
+        for k in tzfit[0].keys():
+            tz[k] = np.vstack([tzfit[i][k] for i in range(len(tzfit))])
+            if k != 'coeff' and tz[k].shape[1] == 1:
+                tz[k] = tz[k].flatten()
+        tz['targetid'] = np.array([tid] * len(tz['chi2']))
+        tz['zwarn'][tz['npixels'] == 0] |= ZW_NODATA
+        tz['zwarn'][(tz['npixels'] < 10 * tz['ncoeff'])] |= ZW_LITTLE_COVERAGE
+        sort_zfit_dict(tz)
+        iikeep = list()
+        for spectype in np.unique(tz['spectype']):
+            ii = np.where(tz['spectype'] == spectype)[0]
+            iikeep.extend(ii[0:nminima])
+        if len(iikeep) < len(tz['chi2']):
+            for k in tz.keys():
+                tz[k] = tz[k][iikeep]
+            sort_zfit_dict(tz)
+        tz['znum'] = np.arange(len(tz['chi2']))
+        deltachi2, setzwarn = calc_deltachi2(tz['chi2'], tz['z'], tz['zwarn'], dvlimit=tz['max_velo_diff'])
+        tz['deltachi2'] = deltachi2
+        tz['zwarn'][setzwarn] |= ZW_SMALL_DELTA_CHI2
+        del tz['max_velo_diff']
+        allzfit.append(tz)
+    out = dict()
+    if not allzfit:
+        return out
+    widest = max(t['coeff'].shape[1] for t in allzfit)
+    for t in allzfit:
+        if t['coeff'].shape[1] < widest:
+            t['coeff'] = np.append(t['coeff'], np.zeros((t['coeff'].shape[0], widest - t['coeff'].shape[1])), axis=1)
+    for k in allzfit[0].keys():
+        out[k] = np.concatenate([t[k] for t in allzfit])
+    ntarg, ncoeff = out['coeff'].shape
+    if ncoeff != maxcoeff:
+        coeff = np.zeros((ntarg, maxcoeff))
+        coeff[:, 0:ncoeff] = out['coeff']
+        out['coeff'] = coeff
+    if len(np.unique(out['ncoeff'])) == 1:
+        out['coeff'] = out['coeff'][:, 0:int(np.unique(out['ncoeff'])[0])]
+    return out

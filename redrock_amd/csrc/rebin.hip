@@ -2,7 +2,10 @@
 // Lyman-series IGM factor fused in (rr_rebin).
 //
 // One thread owns one (redshift, output bin) and loops over the basis, so the
-// sample search and the IGM factor are computed once per bin.  The bin mean
+// sample search and the IGM factor are computed once per bin.  Threads are
+// laid out so that a wavefront's stores are contiguous: 256 bins of one
+// redshift per workgroup when bins are the fast output axis, 64 redshifts x 4
+// bins per workgroup when the output is z-fastest (the FP64 scan cache).  The bin mean
 // follows the serial sweep of py/redrock/rebin.py:126-176 expression by
 // expression, and this file is compiled with FP contraction off, so the FP64
 // result equals the sweep's double for double: the sample index j is the
@@ -51,10 +54,17 @@ __device__ __forceinline__ void store(const RebinArgs &a, long long off, double 
     }
 }
 
-template <int OUT>
-__global__ __launch_bounds__(256) void rebin_kernel(RebinArgs a, int nbinblocks) {
-    const int iz = blockIdx.x / nbinblocks;
-    const int ibin = (blockIdx.x - iz * nbinblocks) * blockDim.x + threadIdx.x;
+template <int OUT, bool ZFAST>
+__global__ __launch_bounds__(256) void rebin_kernel(RebinArgs a, int nfast) {
+    int iz, ibin;
+    if (ZFAST) {   // lane -> redshift, wavefront -> bin; nfast = redshift blocks of 64
+        const int zb = blockIdx.x % nfast, bb = blockIdx.x / nfast;
+        iz = zb * 64 + (threadIdx.x & 63);
+        ibin = bb * 4 + (threadIdx.x >> 6);
+    } else {       // thread -> bin; nfast = bin blocks of 256
+        iz = blockIdx.x / nfast;
+        ibin = (blockIdx.x - iz * nfast) * blockDim.x + threadIdx.x;
+    }
     if (ibin >= a.nbin || iz >= a.nz) return;
 
     const double *x = a.x;
@@ -120,6 +130,12 @@ __global__ __launch_bounds__(256) void rebin_kernel(RebinArgs a, int nbinblocks)
     }
 }
 
+template <int OUT>
+void launch(bool zfast, dim3 grid, dim3 block, hipStream_t s, const RebinArgs &a, int nfast) {
+    if (zfast) hipLaunchKernelGGL((rebin_kernel<OUT, true>), grid, block, 0, s, a, nfast);
+    else hipLaunchKernelGGL((rebin_kernel<OUT, false>), grid, block, 0, s, a, nfast);
+}
+
 }  // namespace
 
 extern "C" int rr_rebin(const double *d_x, int nx, const double *d_y, int nb, long long y_ld,
@@ -140,15 +156,17 @@ extern "C" int rr_rebin(const double *d_x, int nx, const double *d_y, int nb, lo
     a.y_ld = y_ld; a.s_z = s_z; a.s_pix = s_pix; a.s_b = s_b; a.scale = scale;
     a.nx = nx; a.nb = nb; a.nbin = nbin; a.nz = nz; a.nlines = nlines; a.out_type = out_type;
     for (int l = 0; l < nlines; ++l) a.lines[l] = lines[l];
-    const int nbb = (nbin + 255) / 256;
-    if ((long long)nbb * nz > 0x7fffffffLL) return rr::fail(RR_EINVAL, "rr_rebin: grid too large");
-    dim3 block(256), grid(nbb * nz);
+    const bool zfast = s_z == 1 && s_pix != 1;
+    const int nfast = zfast ? (nz + 63) / 64 : (nbin + 255) / 256;
+    const long long nblocks = zfast ? (long long)nfast * ((nbin + 3) / 4) : (long long)nfast * nz;
+    if (nblocks > 0x7fffffffLL) return rr::fail(RR_EINVAL, "rr_rebin: grid too large");
+    dim3 block(256), grid((unsigned)nblocks);
     hipStream_t s = static_cast<hipStream_t>(stream);
     switch (out_type) {
-        case RR_F64: hipLaunchKernelGGL(rebin_kernel<RR_F64>, grid, block, 0, s, a, nbb); break;
-        case RR_F32: hipLaunchKernelGGL(rebin_kernel<RR_F32>, grid, block, 0, s, a, nbb); break;
-        case RR_F16: hipLaunchKernelGGL(rebin_kernel<RR_F16>, grid, block, 0, s, a, nbb); break;
-        case RR_BF16: hipLaunchKernelGGL(rebin_kernel<RR_BF16>, grid, block, 0, s, a, nbb); break;
+        case RR_F64: launch<RR_F64>(zfast, grid, block, s, a, nfast); break;
+        case RR_F32: launch<RR_F32>(zfast, grid, block, s, a, nfast); break;
+        case RR_F16: launch<RR_F16>(zfast, grid, block, s, a, nfast); break;
+        case RR_BF16: launch<RR_BF16>(zfast, grid, block, s, a, nfast); break;
         default: return rr::fail(RR_EINVAL, "rr_rebin: unknown out_type %d", out_type);
     }
     return rr::check_launch("rr_rebin");

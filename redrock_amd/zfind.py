@@ -1,0 +1,301 @@
+"""Redshift finding: coarse scan, fine fits and the per-target result table.
+
+Mirror of py/redrock/zfind.py (zfind.py:220-714) over the HIP scan
+(redrock_amd.zscan) and the batched fitz (redrock_amd.fitz): the chi2 scan of
+every target x template runs on the GPU, fitz refines the minima of every
+target of a template in batched GPU rebin + scan launches, and the small
+per-target assembly (priors, penalty, candidate sorting, nminima trim,
+delta chi2 and zwarn bits) runs on the host, as in the reference.
+
+The reference returns an astropy Table; astropy is not a dependency here, so
+``allzfit`` is a dict of numpy columns in the reference's column order
+(``targetid`` first), one row per (target, minimum).
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from . import constants
+from .fitz import fitz_targets, get_dv
+from .templates import parse_fulltype
+from .zscan import calc_zchi2_targets
+from .zwarning import ZWarningMask as ZW
+from .zwarning import badfit_mask
+
+
+def sort_dict_by_col(d, colname):
+    """Sort a dict of np.ndarrays by one key (zfind.py:35-48)."""
+    if colname not in d:
+        raise KeyError('Key ' + str(colname) + ' is not in dictionary')
+    for k in d.keys():
+        if type(d[k]) is not np.ndarray:
+            raise ValueError('Column ' + str(k) + ' is not an np.array')
+    idx = d[colname].argsort(0).flatten()
+    for k in d.keys():
+        d[k] = d[k][idx]
+
+
+def sort_dict_by_cols(d, colnames, sort_first_column_first=True):
+    """Sort a dict of np.ndarrays by several keys (zfind.py:50-81).
+
+    With sort_first_column_first the first name is the primary key (the
+    astropy Table.sort order); np.lexsort takes its last key as primary.
+    """
+    for c in colnames:
+        if c not in d:
+            raise KeyError('Key ' + str(c) + ' is not in dictionary')
+    for k in d.keys():
+        if type(d[k]) is not np.ndarray:
+            raise ValueError('Column ' + str(k) + ' is not an np.array')
+    keys = tuple(d[c] for c in colnames)
+    if sort_first_column_first:
+        keys = keys[::-1]
+    idx = np.lexsort(keys, axis=0).flatten()
+    for k in d.keys():
+        d[k] = d[k][idx]
+
+
+def sort_zfit_dict(zfit):
+    """Sort candidate rows by (bad-fit zwarn bits, chi2), in place (zfind.py:206-218)."""
+    zfit['__badfit__'] = (zfit['zwarn'] & badfit_mask) != 0
+    sort_dict_by_cols(zfit, ('__badfit__', 'chi2'), sort_first_column_first=True)
+    zfit.pop('__badfit__')
+
+
+def calc_deltachi2(chi2, z, zwarn, dvlimit=constants.max_velo_diff):
+    """chi2 difference of every candidate to the next good fit (zfind.py:103-155).
+
+    deltachi2[i] = min over later candidates k > i that are good fits and lie
+    further than min(dvlimit[i], dvlimit[k]) in velocity of chi2[k] - chi2[i]
+    (0 when there is none).  setzwarn[i] is True when some other good fit, at
+    such a velocity separation, is within constants.min_deltachi2 in chi2
+    (either side).  Returns (deltachi2, setzwarn).
+    """
+    chi2 = np.asarray(chi2, dtype=np.float64)
+    z = np.asarray(z, dtype=np.float64)
+    n = len(chi2)
+    okfit = (np.asarray(zwarn) & badfit_mask) == 0
+    dvlimit = np.full(n, dvlimit, dtype=np.float64) if np.isscalar(dvlimit) \
+        else np.asarray(dvlimit, dtype=np.float64)
+    deltachi2 = np.zeros(n)
+    setzwarn = np.zeros(n, dtype=bool)
+    if n == 0:
+        return deltachi2, setzwarn
+    # pairwise velocity separations dv[i, k] = c (z_k - z_i) / (1 + z_i)
+    dv = get_dv(z[None, :], z[:, None])
+    lim = np.minimum(dvlimit[:, None], dvlimit[None, :])
+    far = np.abs(dv) > lim
+    dchi2 = chi2[None, :] - chi2[:, None]
+    later = np.triu(np.ones((n, n), dtype=bool), k=1)
+    cand = later & far & okfit[None, :]
+    has = cand.any(axis=1)
+    deltachi2[has] = np.where(cand, dchi2, np.inf).min(axis=1)[has]
+    # SMALL_DELTA_CHI2: any other good fit at >= the velocity limit within min_deltachi2
+    other = ~np.eye(n, dtype=bool)
+    close = other & okfit[None, :] & (np.abs(dchi2) < constants.min_deltachi2) & (np.abs(dv) >= lim)
+    setzwarn[:] = close.any(axis=1)
+    return deltachi2, setzwarn
+
+
+class Priors(object):
+    """Redshift priors added to the coarse chi2 (priors.py:10-126).
+
+    The reference reads TARGETID, Z, SIGMA and FUNCTION from the PRIORS HDU of
+    a FITS file; here the same columns are given as arrays.  function is one
+    of 'gaussian', 'lorentzien', 'tophat'.
+    """
+
+    def __init__(self, targetid, z, sigma, function='gaussian'):
+        self._param = {int(t): (float(a), float(s)) for t, a, s in zip(targetid, z, sigma)}
+        if function not in ('gaussian', 'lorentzien', 'tophat'):
+            raise ValueError(f'unknown prior function {function!r}')
+        self._type = function
+        self._func = getattr(self, function)
+
+    def eval(self, targetid, z):
+        """Prior chi2 contribution on the grid z (0. for unknown targets)."""
+        if int(targetid) not in self._param:
+            return 0.
+        z0, s0 = self._param[int(targetid)]
+        return self._func(np.asarray(z, dtype=np.float64), z0, s0)
+
+    @staticmethod
+    def gaussian(z, z0, s0):
+        return ((z - z0) / s0) ** 2
+
+    @staticmethod
+    def lorentzien(z, z0, s0):
+        return -2 * np.log(1 / (1. + ((z - z0) / s0) ** 2))
+
+    @staticmethod
+    def tophat(z, z0, s0):
+        """0 inside |z - z0| < s0/2, +inf on the grid points bordering it, NaN outside."""
+        prior = np.where(np.abs(z - z0) < s0 / 2, 0., np.nan)
+        inside = np.flatnonzero(prior >= 0.0)
+        if inside.size == 0:
+            return prior
+        left, right = inside[0], inside[-1]
+        prior[left if left == 0 else left - 1] = np.inf
+        prior[right if right == prior.size - 1 else right + 1] = np.inf
+        return prior
+
+
+def _local_targets(targets):
+    return targets.local() if hasattr(targets, 'local') else list(targets)
+
+
+def _all_ids(targets):
+    if hasattr(targets, 'all_target_ids'):
+        ids = targets.all_target_ids
+        return list(ids() if callable(ids) else ids)
+    return [t.id for t in targets]
+
+
+class _Local(object):
+    """DistTargets-like view of a plain list of targets."""
+
+    def __init__(self, targets):
+        self._t = list(targets)
+
+    def local(self):
+        return self._t
+
+    def local_target_ids(self):
+        return [t.id for t in self._t]
+
+
+def zfind(targets, templates, mp_procs=1, nminima=3, archetypes=None, priors=None,
+          chi2_scan=None, use_gpu=False, zminfit_npoints=15, per_camera=None,
+          deg_legendre=None, n_nearest=None, prior_sigma=None, ncamera=None,
+          precision=None, newton=False):
+    """All redshift fits of the targets (zfind.py:220-714).
+
+    targets: DistTargets (redrock_amd.targets) or a list of Target;
+    templates: DistTemplate list.  priors: a Priors object (or None).
+    precision: scan precision ('f64', 'f16x3', ...; default
+    zscan.default_precision()); the fine fits always run in FP64.
+    newton=True adds 'znewton'/'chi2newton' columns (Newton refinement on the
+    analytic d chi2 / dz beside the parabola fit).
+
+    Returns (allresults, allzfit): allresults[targetid][fulltype] =
+    dict(redshifts, zchi2, penalty, zcoeff); allzfit the dict of columns.
+    """
+    if archetypes:
+        raise NotImplementedError('archetype fits are outside the chi2-scan path')
+    if chi2_scan is not None:
+        raise NotImplementedError('reading a saved chi2 scan (rrdetails HDF5) is not supported')
+    if priors is not None and not hasattr(priors, 'eval'):
+        raise NotImplementedError('priors must be a redrock_amd.zfind.Priors '
+                                  '(reading the FITS prior file needs astropy)')
+    local = _local_targets(targets)
+    dist = targets if hasattr(targets, 'local_target_ids') else _Local(local)
+    results = calc_zchi2_targets(dist, templates, mp_procs=mp_procs, use_gpu=use_gpu,
+                                 precision=precision)
+
+    if priors is not None:                                   # zfind.py:318-322
+        for tid in results:
+            for ft in results[tid]:
+                results[tid][ft]['zchi2'] = (results[tid][ft]['zchi2']
+                                             + priors.eval(tid, results[tid][ft]['redshifts']))
+
+    order = np.array([t.template.full_type for t in templates]).argsort()
+    for t in np.array(list(templates), dtype=object)[order]:      # zfind.py:330-367
+        ft = t.template.full_type
+        if not local:
+            continue
+        chi2 = np.stack([results[tg.id][ft]['zchi2'] + results[tg.id][ft]['penalty']
+                         for tg in local])
+        fits = fitz_targets(chi2, t.template.redshifts, local, t.template, nminima=nminima,
+                            zminfit_npoints=zminfit_npoints, newton=newton)
+        for tg, zfit in zip(local, fits):
+            results[tg.id][ft]['zfit'] = zfit
+
+    for tg in local:
+        results[tg.id]['meta'] = getattr(tg, 'meta', dict())
+
+    allzfit = _assemble(results, _all_ids(targets), templates, nminima)
+    for tid in results:
+        results[tid].pop('meta', None)
+    return results, allzfit
+
+
+def _assemble(allresults, target_ids, templates, nminima):
+    """Per-target candidate tables -> one table (zfind.py:467-714, no archetypes)."""
+    rows = []
+    for tid in target_ids:
+        tzfit = []
+        for fulltype in allresults[tid]:
+            if fulltype == 'meta':
+                continue
+            tmp = allresults[tid][fulltype].pop('zfit')
+            spectype, subtype = parse_fulltype(fulltype)
+            nmin = len(tmp['chi2'])
+            if nmin == 0:
+                print(f'WARNING: no {fulltype} chi2 vs. z minima for targetid {tid}')
+                continue
+            tmp['spectype'] = np.full((nmin, 1), spectype)
+            tmp['subtype'] = np.full((nmin, 1), subtype)
+            tmp['ncoeff'] = np.full((nmin, 1), tmp['coeff'].shape[1])
+            mv = constants.max_velo_diff_star if spectype.upper() == 'STAR' else constants.max_velo_diff
+            tmp['max_velo_diff'] = np.full((nmin, 1), mv)
+            tzfit.append(tmp)
+        if not tzfit:
+            continue
+        maxnc = max(tmp['coeff'].shape[1] for tmp in tzfit)
+        for tmp in tzfit:
+            n = maxnc - tmp['coeff'].shape[1]
+            if n > 0:
+                tmp['coeff'] = np.append(tmp['coeff'], np.zeros((tmp['coeff'].shape[0], n)), axis=1)
+        tz = dict()
+        for k in tzfit[0].keys():
+            tz[k] = np.vstack([tmp[k] for tmp in tzfit])
+            if k != 'coeff' and tz[k].shape[1] == 1:
+                tz[k] = tz[k].flatten()
+        tz['targetid'] = np.full(len(tz['chi2']), tid)
+        tz['zwarn'][tz['npixels'] == 0] |= ZW.NODATA
+        tz['zwarn'][tz['npixels'] < 10 * tz['ncoeff']] |= ZW.LITTLE_COVERAGE
+        sort_zfit_dict(tz)
+        keep = []
+        for spectype in np.unique(tz['spectype']):            # nminima per spectype
+            keep.extend(np.where(tz['spectype'] == spectype)[0][:nminima])
+        if len(keep) < len(tz['chi2']):
+            for k in tz:
+                tz[k] = tz[k][keep]
+            sort_zfit_dict(tz)
+        tz['znum'] = np.arange(len(tz['chi2']))
+        dchi2, setzwarn = calc_deltachi2(tz['chi2'], tz['z'], tz['zwarn'], dvlimit=tz['max_velo_diff'])
+        tz['deltachi2'] = dchi2
+        tz['zwarn'][setzwarn] |= ZW.SMALL_DELTA_CHI2
+        del tz['max_velo_diff']
+        rows.append(tz)
+    if not rows:
+        return dict()
+    # stack the targets (astropy vstack: coeff padded to the widest)
+    maxnc = max(r['coeff'].shape[1] for r in rows)
+    for r in rows:
+        n = maxnc - r['coeff'].shape[1]
+        if n > 0:
+            r['coeff'] = np.append(r['coeff'], np.zeros((r['coeff'].shape[0], n)), axis=1)
+    names = ['targetid'] + [k for k in rows[0].keys() if k != 'targetid']
+    allzfit = {k: np.concatenate([r[k] for r in rows]) for k in names}
+    # target metadata columns (zfind.py:598-636): '<key>_datatype' entries give dtypes
+    meta0 = allresults[allzfit['targetid'][0]].get('meta', dict())
+    metakeys = sorted(k for k in meta0 if not k.endswith('_datatype'))
+    for mk in metakeys:
+        dt = meta0.get(mk + '_datatype')
+        vals = [allresults[t]['meta'][mk] for t in allzfit['targetid']]
+        allzfit[mk] = np.array(vals, dtype=dt) if dt is not None else np.array(vals)
+    allzfit['subtype'] = allzfit['subtype'].astype(
+        f"<U{max(20, max((len(s) for s in allzfit['subtype']), default=0))}")
+    allzfit['spectype'] = allzfit['spectype'].astype('<U6')
+    maxcoeff = max(t.template.nbasis for t in templates)
+    coeff = allzfit['coeff']
+    if coeff.shape[1] != maxcoeff:
+        c = np.zeros((coeff.shape[0], maxcoeff), dtype=coeff.dtype)
+        c[:, :coeff.shape[1]] = coeff
+        coeff = c
+    nc = np.unique(allzfit['ncoeff'])
+    if len(nc) == 1:
+        coeff = coeff[:, :int(nc[0])]
+    allzfit['coeff'] = coeff
+    return allzfit

@@ -1,0 +1,177 @@
+"""zfind assembly (redrock_amd.zfind) on the host: delta chi2, sorting, priors,
+and the per-target result table against the oracle's restatement."""
+import numpy as np
+import pytest
+
+from oracle import rr_oracle as orc
+from redrock_amd import zfind as zf
+from redrock_amd.zwarning import ZWarningMask as ZW
+
+
+def test_calc_deltachi2_reference_vectors():
+    # test_zfind.py:141-170 and test_zscan.py:101-111
+    chi2 = np.array([1.0, 2.0, 20.0, 40.0])
+    d, s = zf.calc_deltachi2(chi2, np.array([3.0, 3.1, 3.2, 3.3]), np.array([0, 0, 0, 0]))
+    assert np.all(d == [1, 18, 20, 0.0]) and np.all(s == [True, True, False, False])
+    d, s = zf.calc_deltachi2(chi2, np.array([3.0, 3.0001, 4.0, 4.0001]), np.array([0, 0, 0, 0]))
+    assert np.all(d == [19, 18, 0.0, 0.0]) and not np.any(s)
+    d, s = zf.calc_deltachi2(chi2, np.array([3.0, 3.1, 3.2, 3.3]), np.array([0, 32, 0, 0]))
+    assert np.all(d == [19, 18, 20, 0.0]) and np.all(s == [False, True, False, False])
+    chi2 = np.array([1.0, 2.0, 4.0, 8.0])
+    d, _ = zf.calc_deltachi2(chi2, np.array([3.0, 3.1, 3.2, 3.3]), np.zeros(4, int))
+    assert np.all(d == [1, 2, 4, 0.0])
+    d, _ = zf.calc_deltachi2(chi2, np.array([3.0, 3.0, 4.0, 4.0]), np.zeros(4, int))
+    assert np.all(d == [3, 2, 0.0, 0.0])
+
+
+@pytest.mark.parametrize('seed', range(20))
+def test_calc_deltachi2_matches_oracle(seed):
+    rng = np.random.default_rng(seed)
+    n = int(rng.integers(1, 12))
+    chi2 = np.sort(rng.uniform(0, 40, n)).round(rng.integers(0, 3))   # ties included
+    z = rng.choice([0.5, 0.5001, 0.51, 1.2, 1.2003, 2.0], n) + rng.normal(0, 1e-5, n) * rng.integers(0, 2)
+    zwarn = rng.choice([0, 0, 0, 32, 1024, 256, 4], n)
+    dv = rng.choice([100.0, 1000.0], n)
+    d1, s1 = zf.calc_deltachi2(chi2, z, zwarn, dvlimit=dv)
+    d0, s0 = orc.calc_deltachi2(chi2, z, zwarn, dvlimit=dv)
+    assert np.array_equal(d1, d0) and np.array_equal(s1, s0)
+
+
+def test_sort_zfit_dict_reference_vectors():
+    # test_zfind.py:86-139
+    cases = [([0, 0, 0], [2, 1, 0]), ([0, 0, 32], [1, 0, 2]), ([0, 0, 1024], [1, 0, 2]),
+             ([0, 0, 256], [2, 1, 0]), ([0, 256, 288], [1, 0, 2])]
+    for zwarn, rank in cases:
+        z = dict(id=np.array([1, 2, 3]), zwarn=np.array(zwarn), chi2=np.array([30, 20, 10]),
+                 rank=np.array(rank))
+        zf.sort_zfit_dict(z)
+        assert list(z['rank']) == [0, 1, 2]
+        assert list(z.keys()) == ['id', 'zwarn', 'chi2', 'rank']
+
+
+def test_sort_dict_by_cols_order():
+    # test_zfind.py:180-206: zwarn first, then chi2
+    d = dict(chi2=np.array([10.0, 3.0, 1.0, 5.0, 20.0]), zwarn=np.array([0, 4, 0, 1, 0]),
+             z=np.array([2.0, 5.0, 1.0, 4.0, 3.0]))
+    zf.sort_dict_by_cols(d, ('zwarn', 'chi2'))
+    assert list(d['z']) == [1.0, 2.0, 3.0, 4.0, 5.0]
+    d = dict(chi2=np.array([10.0, 3.0, 1.0, 5.0, 20.0]), zwarn=np.array([0, 4, 0, 1, 0]))
+    with pytest.raises(KeyError):
+        zf.sort_dict_by_cols(d, ('zwarn', 'nope'))
+
+
+def test_priors_closed_forms():
+    p = zf.Priors([7, 8], [1.0, 2.0], [0.1, 0.5], 'gaussian')
+    z = np.array([0.9, 1.0, 1.1])
+    assert np.allclose(p.eval(7, z), [1.0, 0.0, 1.0])
+    assert p.eval(99, z) == 0.
+    p = zf.Priors([7], [1.0], [0.1], 'lorentzien')
+    assert np.allclose(p.eval(7, z), [2 * np.log(2), 0.0, 2 * np.log(2)])
+    p = zf.Priors([7], [1.0], [0.25], 'tophat')
+    z = np.linspace(0.7, 1.3, 13)             # step 0.05: |z - 1| < 0.125 -> 5 points
+    v = p.eval(7, z)
+    inside = np.abs(z - 1.0) < 0.125
+    assert np.all(v[inside] == 0.0)
+    assert np.isinf(v[3]) and np.isinf(v[9])
+    assert np.all(np.isnan(np.delete(v, [3, 4, 5, 6, 7, 8, 9])))
+    # tophat touching the grid edge: the edge point itself becomes inf
+    v = p.eval(7, np.linspace(0.95, 1.5, 12))
+    assert np.isinf(v[0])
+    with pytest.raises(ValueError):
+        zf.Priors([1], [1.0], [1.0], 'boxcar')
+
+
+class _T(object):
+    def __init__(self, nbasis):
+        self.nbasis = nbasis
+
+
+class _DT(object):
+    def __init__(self, nbasis):
+        self.template = _T(nbasis)
+
+
+def _fake_zfit(rng, nmin, nb, npix, badbits=(0,)):
+    z = np.sort(rng.uniform(0.1, 2.0, nmin))[:, None]
+    return dict(z=z, zerr=rng.uniform(1e-5, 1e-4, (nmin, 1)),
+                zwarn=rng.choice(badbits, (nmin, 1)).astype(np.int64),
+                chi2=rng.uniform(7000, 7100, (nmin, 1)),
+                zz=rng.uniform(0, 2, (nmin, 15)), zzchi2=rng.uniform(7000, 8000, (nmin, 15)),
+                coeff=rng.normal(size=(nmin, nb)),
+                fitmethod=np.full((nmin, 1), 'PCA'),
+                npixels=np.full((nmin, 1), npix, dtype=np.int64))
+
+
+def _fake_results(seed):
+    rng = np.random.default_rng(seed)
+    types = [('GALAXY', 10), ('QSO', 4), ('STAR:::K', 5), ('STAR:::M', 5)]
+    res = dict()
+    tids = [11, 12, 13, 14]
+    for i, tid in enumerate(tids):
+        res[tid] = dict()
+        npix = [7958, 0, 30, 7958][i]           # NODATA, LITTLE_COVERAGE cases
+        for ft, nb in types:
+            nmin = int(rng.integers(0 if i == 3 and ft == 'QSO' else 1, 4))
+            res[tid][ft] = dict(zfit=_fake_zfit(rng, nmin, nb, npix, badbits=(0, 0, 32, 1024, 256)))
+    return res, tids, [_DT(nb) for _, nb in types]
+
+
+@pytest.mark.parametrize('seed', range(6))
+def test_assemble_matches_oracle(seed):
+    res, tids, templates = _fake_results(seed)
+    import copy
+    want = orc.zfind_assemble(copy.deepcopy(res), tids, maxcoeff=10, nminima=3)
+    got = zf._assemble(copy.deepcopy(res), tids, templates, nminima=3)
+    assert list(got.keys())[0] == 'targetid'
+    assert set(got.keys()) == set(want.keys())
+    for k in want:
+        assert got[k].shape == want[k].shape, k
+        if got[k].dtype.kind in 'fc':
+            assert np.array_equal(got[k], want[k]), k
+        else:
+            assert np.array_equal(got[k].astype(str), want[k].astype(str)), k
+
+
+def test_assemble_known_answer():
+    """Hand-computed: nminima trim per spectype, bad fits last, flags."""
+    def zfit(zs, chi2s, zwarn, nb, npix):
+        n = len(zs)
+        return dict(z=np.array(zs, float)[:, None], zerr=np.full((n, 1), 1e-4),
+                    zwarn=np.array(zwarn, np.int64)[:, None], chi2=np.array(chi2s, float)[:, None],
+                    zz=np.zeros((n, 15)), zzchi2=np.zeros((n, 15)), coeff=np.ones((n, nb)),
+                    fitmethod=np.full((n, 1), 'PCA'), npixels=np.full((n, 1), npix, np.int64))
+    def fresh():
+        return {5: {'GALAXY': dict(zfit=zfit([0.5, 1.0], [100.0, 105.0], [0, 0], 3, 100)),
+                    'STAR:::K': dict(zfit=zfit([0.0, 0.001], [90.0, 200.0], [0, 0], 2, 100)),
+                    'STAR:::M': dict(zfit=zfit([0.0002, 0.0005], [95.0, 50.0], [0, 32], 2, 100))}}
+    res = fresh()
+    got = zf._assemble(res, [5], [_DT(3), _DT(2), _DT(2)], nminima=2)
+    # STAR keeps its 2 best good fits (90, 95); the Z_FITLIMIT one (50) sorts
+    # after every good fit and is trimmed; GALAXY keeps both
+    assert list(got['chi2']) == [90.0, 95.0, 100.0, 105.0]
+    assert list(got['spectype']) == ['STAR', 'STAR', 'GALAXY', 'GALAXY']
+    assert list(got['subtype']) == ['K', 'M', '', '']
+    assert list(got['znum']) == [0, 1, 2, 3]
+    # STAR rows: 0.0 vs 0.0002 is 60 km/s apart (< 100): same solution
+    assert got['deltachi2'][0] == 10.0          # next good fit far enough: 100 (GALAXY z=0.5)
+    assert got['deltachi2'][2] == 5.0 and got['deltachi2'][3] == 0.0
+    # |100 - 105| < 9 and 1000 km/s apart: SMALL_DELTA_CHI2 on both GALAXY rows
+    assert got['zwarn'][2] & ZW.SMALL_DELTA_CHI2 and got['zwarn'][3] & ZW.SMALL_DELTA_CHI2
+    assert not (got['zwarn'][0] & ZW.SMALL_DELTA_CHI2)
+    assert got['coeff'].shape == (4, 3)         # padded to the widest template
+    assert np.array_equal(got['ncoeff'], [2, 2, 3, 3])
+    assert got['zwarn'][1] & ZW.SMALL_DELTA_CHI2    # 95 vs GALAXY 100, far apart
+    want = orc.zfind_assemble(fresh(), [5], maxcoeff=3, nminima=2)
+    for k in want:
+        assert np.array_equal(np.asarray(got[k]).astype(str), np.asarray(want[k]).astype(str)), k
+
+
+def test_assemble_nodata_and_coverage_bits():
+    res, tids, templates = _fake_results(1)
+    got = zf._assemble(res, tids, templates, nminima=3)
+    t12 = got['targetid'] == 12
+    assert np.all(got['zwarn'][t12] & ZW.NODATA)
+    t13 = got['targetid'] == 13
+    assert np.all(got['zwarn'][t13] & ZW.LITTLE_COVERAGE)
+    t11 = got['targetid'] == 11
+    assert not np.any(got['zwarn'][t11] & (ZW.NODATA | ZW.LITTLE_COVERAGE))
