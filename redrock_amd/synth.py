@@ -241,3 +241,30 @@ def workload(name, ntarget=None, seed=0, snr=1.0):
         raise ValueError(name)
     targets, zt = make_targets(t, 10 if ntarget is None else ntarget, seed=seed, snr=snr)
     return t, targets, zt
+
+
+def mixed_targets(ntarget, seed=0, snr=1.0, mask_frac=(0.0, 0.3), templates=None):
+    """A shuffled mix of GALAXY / QSO / STAR targets with masked pixel runs.
+
+    Thirds of the targets are drawn from each template (templates =
+    (galaxy, qso, star) Template objects, built here when None); every target
+    gets a masked fraction U(mask_frac) in contiguous runs of 50-500 pixels.
+    Returns (targets, spectype per target, z_true); target ids are 0..n-1 in
+    the returned order.
+    """
+    if templates is None:
+        templates = (galaxy_template(), qso_template(), star_template())
+    counts = [ntarget // 3 + (1 if i < ntarget % 3 else 0) for i in range(3)]
+    tg, types, zt = [], [], []
+    for k, (t, n) in enumerate(zip(templates, counts)):
+        if n == 0:
+            continue
+        ts, z = make_targets(t, n, seed=seed + 101 * k, snr=snr, mask_frac=mask_frac)
+        tg += ts
+        types += [t.template_type] * n
+        zt += list(z)
+    order = np.random.default_rng(seed).permutation(len(tg))
+    targets = []
+    for new_id, i in enumerate(order):
+        targets.append(Target(new_id, tg[i].spectra))
+    return targets, [types[i] for i in order], np.array([zt[i] for i in order])

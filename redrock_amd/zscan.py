@@ -399,6 +399,73 @@ def scan_targets(cache, targets, precision=None, oiisum=None, batches=None):
     return zchi2, zcoeff, penalty
 
 
+class ScanGraph(object):
+    """Every target x every redshift x every template class in one graph launch.
+
+    One ScanPlan per template (its own nbasis, redshift grid and tile
+    configuration; the kernels are specialised per nbasis) over one
+    TargetBatch.  capture() records the stats / main / finish stages of all
+    plans into a single HIP graph; replay() re-runs the whole batched scan with
+    one launch and no host work.  Targets with masked pixels need nothing
+    special: a zero weight removes the pixel from every sum.
+
+    This is the batched form of calc_zchi2_targets (zscan.py:910-1105) for
+    targets that share one wavelength layout.
+    """
+
+    def __init__(self, caches, batch, precision=None, tune=False):
+        """caches: list of (TemplateCache, oiisum or None); tune: autotune each plan's tile."""
+        precision = precision or default_precision()
+        self.precision = precision
+        self.batch = batch
+        nt = batch.nt
+        self.plans = []
+        for cache, oii in caches:
+            plan = ScanPlan(cache, batch, np.arange(nt), np.zeros(nt), np.full(nt, cache.nz),
+                            precision, oiisum=oii)
+            if tune:
+                from . import autotune
+                autotune.choose(plan)
+            self.plans.append(plan)
+        self.graph = None
+        self.stream = None
+
+    def run(self):
+        """Eager: enqueue every plan on the current stream."""
+        for p in self.plans:
+            p.run()
+        return self
+
+    def capture(self):
+        """Record all plans into one HIP graph (after one eager warm-up run)."""
+        import torch
+        self.run()
+        torch.cuda.synchronize()
+        self.stream = torch.cuda.Stream()
+        self.graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(self.graph, stream=self.stream):
+            for p in self.plans:
+                p.run()
+        torch.cuda.synchronize()
+        return self
+
+    def replay(self):
+        if self.graph is None:
+            raise RuntimeError('ScanGraph.replay() before capture()')
+        self.graph.replay()
+        return self
+
+    def results(self):
+        """[(zchi2 (nt, nz), zcoeff (nt, nz, nb), penalty (nt, nz)), ...] per template, numpy."""
+        nt = self.batch.nt
+        out = []
+        for p in self.plans:
+            c2, cf, pen, _ = p.host()
+            nz = len(c2) // nt if nt else 0
+            out.append((c2.reshape(nt, nz), cf.reshape(nt, nz, p.nb), pen.reshape(nt, nz)))
+        return out
+
+
 def calc_zchi2(target_ids, target_data, dtemplate, progress=None, use_gpu=False, precision=None):
     """chi2 vs redshift of every target for one template (zscan.py:807-889).
 
