@@ -46,7 +46,7 @@ _CPU = {}
 def _cpu_one(i):
     from oracle import rr_oracle as orc
     tm, targets, tdata = _CPU['tm'], _CPU['targets'], _CPU['tdata']
-    orc.calc_zchi2([targets[i]], tdata, tm.nbasis, tm.OIItemplate)
+    orc.calc_zchi2([targets[i]], tdata, tm.nbasis, getattr(tm, 'OIItemplate', None))
     return i
 
 
@@ -55,7 +55,9 @@ def cpu_baseline(tm, targets, dwave, seconds):
     import multiprocessing as mp
     from oracle import rr_oracle as orc
     ncores = min(16, len(os.sched_getaffinity(0)))
-    tdata = orc.template_cache(tm, dwave)
+    # the QSO workload's IGM: Lyman-alpha only, tau = 0.0036 (1+z_abs)^3.46
+    lines = [(1215.67, 0.0036, 3.46)] if tm.template_type == 'QSO' else None
+    tdata = orc.template_cache(tm, dwave, lines=lines)
     _CPU.update(tm=tm, targets=targets, tdata=tdata)
     t0 = time.perf_counter()
     _cpu_one(0)
@@ -157,6 +159,90 @@ def bench_mixed(args):
     }), flush=True)
 
 
+def bench_grad(args):
+    """GALAXY targets: the 3 lowest minima of each coarse scan refined by Newton
+    iterations on the analytic d chi2/dz (adjoint kernel, rebin recomputed)
+    inside +-5 grid steps, beside the reference's 15-point parabola fit."""
+    import numpy as np
+    import torch
+    from redrock_amd import synth, zscan
+    from redrock_amd import fitz as fz
+    from redrock_amd.templates import DistTemplate
+    from redrock_amd.grad import GradProblem, newton_refine
+    tm = synth.galaxy_template()
+    targets, ztrue = synth.make_targets(tm, args.grad_targets, seed=args.seed, snr=args.snr)
+    dwave = {s.wavehash: s.wave for s in targets[0].spectra}
+    assert torch.cuda.is_available(), 'bench.py needs a GPU'
+    nt, zg = len(targets), tm.redshifts
+    nz = len(zg)
+    dt = DistTemplate(tm, dwave)
+    batch = zscan.TargetBatch(targets)
+    oii = zscan._oiisum(tm)
+    plan = zscan.ScanPlan(dt.cache, batch, np.arange(nt), np.zeros(nt), np.full(nt, nz),
+                          precision=args.precision, oiisum=oii)
+    plan.run()
+    c2 = (plan.zchi2 + plan.penalty).reshape(nt, nz).cpu().numpy()
+    del plan
+    owner, imin = [], []
+    for t in range(nt):
+        for i in fz.find_minima(c2[t])[:3]:
+            if 5 <= i < nz - 5:
+                owner.append(t)
+                imin.append(int(i))
+    owner, imin = np.array(owner), np.array(imin)
+    n = len(owner)
+    log(f'grad workload: {nt} targets, {n} minima (3 lowest per target, +-5 grid steps)')
+    z0, zlo, zhi = zg[imin], zg[imin - 5], zg[imin + 5]
+    prob = GradProblem(tm, targets, owner, batch=batch)
+
+    def timed(fn, k):
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(k):
+            r = fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t) / k, r
+
+    chi2, coeff = prob.forward(z0)
+    prob.grad(z0, coeff, want_dy=True)             # warm-up
+    k = max(2, args.steps // 2)
+    t_fwd, _ = timed(lambda: prob.forward(z0), k)
+    t_grad, _ = timed(lambda: prob.grad(z0, coeff), k)
+    t_grad_dy, _ = timed(lambda: prob.grad(z0, coeff, want_dy=True), k)
+    t_newton, (zn, c2n, gn) = timed(lambda: newton_refine(prob, z0, zlo, zhi), 1)
+    log(f'forward (rebin + FP64 solve at {n} z) {t_fwd * 1e3:.1f} ms; adjoint dchi2/dz '
+        f'{t_grad * 1e3:.1f} ms, with dchi2/dflux {t_grad_dy * 1e3:.1f} ms; Newton (4 it.) '
+        f'{t_newton * 1e3:.0f} ms')
+    # reference method: 15-point fine grid over z[i-1..i+1] and a parabola through the minimum
+    ref = fz._Refiner(tm, targets, zg, 15, 'f64', batch=batch)
+    t_par, rr = timed(lambda: ref.refine(list(zip(owner.tolist(), imin.tolist())), c2), 1)
+    zpar = np.array([r.get('zmin', np.nan) for r in rr])
+    # compare where the parabola vertex lies inside the Newton bracket
+    ok = np.isfinite(zpar) & (zpar >= zlo) & (zpar <= zhi)
+    c2par = prob.forward(np.where(ok, zpar, z0))[0].cpu().numpy()
+    dv = 299792.458 * (zn - zpar) / (1 + zpar)
+    better = np.mean(c2n[ok] <= c2par[ok] + 1e-9 * np.abs(c2par[ok]))
+    log(f'parabola fit {t_par * 1e3:.0f} ms; |Newton - parabola| median {np.median(np.abs(dv[ok])):.3f} '
+        f'km/s, p99 {np.quantile(np.abs(dv[ok]), 0.99):.3f} km/s; chi2(Newton) <= chi2(parabola) '
+        f'for {better:.4f} of minima')
+    print(json.dumps({
+        'metric': 'adjoint d chi2/dz evaluations/sec on 1 GPU (GALAXY, rebin recomputed)',
+        'value': n / t_grad, 'unit': 'gradients/s', 'n_gpus': 1, 'steps': k,
+        'warmup': 1, 'ms_per_step': t_grad * 1e3, 'higher_is_better': True, 'scaling': 'weak',
+        'vs_baseline': None, 'dtype': 'f64', 'data': 'synthetic',
+        'config': {'workload': f'{nt} GALAXY targets, 3 lowest minima each, Newton inside +-5 '
+                               f'grid steps (coarse scan in {args.precision})',
+                   'targets': nt, 'minima': n},
+        'forward_ms': t_fwd * 1e3, 'grad_ms': t_grad * 1e3, 'grad_with_dflux_ms': t_grad_dy * 1e3,
+        'newton_ms': t_newton * 1e3, 'parabola_fit_ms': t_par * 1e3,
+        'newton_minus_parabola_kms': {'median_abs': float(np.median(np.abs(dv[ok]))),
+                                      'p99_abs': float(np.quantile(np.abs(dv[ok]), 0.99))},
+        'newton_chi2_not_above_parabola': float(better),
+        'minima_compared': int(ok.sum()),
+        'max_abs_dchi2dz_at_newton': float(np.max(np.abs(gn))),
+    }), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
@@ -174,10 +260,13 @@ def main():
                     help='tile configuration of the mixed kernel (-1: autotune)')
     ap.add_argument('--calibrate-copy-mb', type=int, default=0,
                     help='only run a plain device copy of this many MiB (HBM counter calibration)')
-    ap.add_argument('--workload', default='galaxy', choices=('galaxy', 'mixed'),
-                    help="galaxy: the 500-target galaxy petal (headline); mixed: GALAXY/QSO/STAR "
-                         "targets with masked runs over all three templates in one HIP graph")
+    ap.add_argument('--workload', default='galaxy', choices=('galaxy', 'qso', 'mixed', 'grad'),
+                    help="galaxy: the 500-target galaxy petal (headline); qso: 500 QSO targets "
+                         "with the fused Lyman-alpha IGM; mixed: GALAXY/QSO/STAR targets with "
+                         "masked runs over all three templates in one HIP graph; grad: adjoint "
+                         "gradient and Newton refinement of the 3 lowest minima per target")
     ap.add_argument('--mixed-targets', type=int, default=5000)
+    ap.add_argument('--grad-targets', type=int, default=5000)
     args = ap.parse_args()
     if args.calibrate_copy_mb:
         calibrate_copy(args.calibrate_copy_mb)
@@ -190,11 +279,14 @@ def main():
 
     if args.workload == 'mixed':
         return bench_mixed(args)
+    if args.workload == 'grad':
+        return bench_grad(args)
 
     import numpy as np
     from redrock_amd import synth
 
-    tm = synth.galaxy_template()
+    tm = synth.galaxy_template() if args.workload == 'galaxy' else synth.qso_template()
+    kind = tm.template_type
     targets, ztrue = synth.make_targets(tm, args.ntargets, seed=args.seed, snr=args.snr)
     dwave = {s.wavehash: s.wave for s in targets[0].spectra}
     nt, nz, nb = len(targets), len(tm.redshifts), tm.nbasis
@@ -215,7 +307,7 @@ def main():
     t0 = time.perf_counter()
     dt = DistTemplate(tm, dwave)
     batch = zscan.TargetBatch(targets)
-    oii = tm.OIItemplate.sum(axis=1)
+    oii = tm.OIItemplate.sum(axis=1) if hasattr(tm, 'OIItemplate') else None
     items = (np.arange(nt), np.zeros(nt), np.full(nt, nz))
     plans = {p: zscan.ScanPlan(dt.cache, batch, *items, precision=p, oiisum=oii)
              for p in ('f64', args.precision)}
@@ -299,7 +391,7 @@ def main():
             f'best-z agreement {variants[p]["best_z_agreement"]:.4f}')
         del plan
 
-    flops = FLOP_PER_EVAL['GALAXY'] * evals
+    flops = FLOP_PER_EVAL[kind] * evals
     ach = flops / (k_mix * 1e-3)
     out = {
         'metric': METRIC,
@@ -314,17 +406,19 @@ def main():
         'vs_baseline': None,
         'dtype': args.precision,
         'data': 'synthetic',
-        'config': {'workload': f'galaxy petal: {nt} GALAXY targets x {nz} z x {nb} basis, '
-                               f'3 cameras 7958 px, 11-diagonal R',
+        'config': {'workload': (f'galaxy petal: {nt} GALAXY targets x {nz} z x {nb} basis, '
+                                if kind == 'GALAXY' else
+                                f'{nt} QSO targets x {nz} z x {nb} basis, Lyman-alpha IGM fused, ')
+                               + '3 cameras 7958 px, 11-diagonal R',
                    'targets': nt, 'redshifts': nz, 'nbasis': nb, 'snr_per_pixel': args.snr,
                    'solve': 'FP64 Cholesky',
-                   'accumulate': ('fp32 MFMA partial sums over 128 pixels, added in FP64'
+                   'accumulate': ('fp32 MFMA partial sums over 64 pixels, added in FP64'
                                   if args.precision == 'f16x3' else 'fp32 (MFMA), FP64 after the scan'),
                    'tile': dict(zip(('waves_items', 'waves_z', 'items_per_wave', 'z_per_wave'),
                                     table[tile]))},
         'autotune_ms': {'x'.join(map(str, table[t])): ms for t, ms in tile_ms.items()},
         'fp64': {'value': evals / t_f64, 'ms_per_step': t_f64 * 1e3, 'kernel_ms': k_f64,
-                 'roofline_frac': FLOP_PER_EVAL['GALAXY'] * evals / (k_f64 * 1e-3) / PEAK_FLOPS['f64']},
+                 'roofline_frac': FLOP_PER_EVAL[kind] * evals / (k_f64 * 1e-3) / PEAK_FLOPS['f64']},
         'speedup_vs_fp64': t_f64 / t_mix,
         'best_z_agreement': agree,
         'max_rel_chi2_err': float(rel.max()),
@@ -334,7 +428,7 @@ def main():
                      'unit': 'TFLOP/s', 'frac': ach / PEAK_FLOPS[args.precision], 'traffic': None,
                      'kernel': 'zscan_mfma3_kernel' if args.precision == 'f16x3' else 'zscan_mfma_kernel',
                      'kernel_ms': k_mix,
-                     'flop_per_eval': FLOP_PER_EVAL['GALAXY'], 'evals_per_launch': evals},
+                     'flop_per_eval': FLOP_PER_EVAL[kind], 'evals_per_launch': evals},
         'median_rel_chi2_err': float(np.median(rel)),
         'variants': variants,
         'cpu_baseline': cpu,

@@ -14,7 +14,7 @@
 //   G += Xh' Xh + Xh' (2 Xl)           2 MFMAs; the symmetric part of G is
 //                                      Xh'Xh + Xh'Xl + Xl'Xh
 //
-// G is accumulated in fp32 over four tile pairs (128 pixels) and then added
+// G is accumulated in fp32 over two tile pairs (64 pixels) and then added
 // into FP64 registers, so the sum over the ~8000 pixels of a spectrum does
 // not collect fp32 rounding.  The epilogue writes the symmetric part of the
 // FP64 sums; zscan_finish.hip solves and forms chi2 exactly as for the other
@@ -29,7 +29,7 @@
 
 namespace {
 
-constexpr int kFlushPairs = 4;   // fp32 partial sums span at most this many tile pairs
+constexpr int kFlushPairs = 2;   // fp32 partial sums span at most this many tile pairs
 
 // rtile (nt, ntile, 2, 64, 8) fp16: [tile q][hi, lo][lane l][slot v]; lane
 // (i = l%16, g = l/16) slot v -> sqrt(w_p) rscale R[p][8g+v-i-3], p = 16q+i,

@@ -72,7 +72,7 @@ def _edges_ok(template, zlo, zhi, edges):
 class _Refiner(object):
     """GPU refinement of (target, candidate-minimum) pairs for one template."""
 
-    def __init__(self, template, targets, redshifts, nz_fine, precision='f64'):
+    def __init__(self, template, targets, redshifts, nz_fine, precision='f64', batch=None):
         from .zscan import TargetBatch, PixelLayout, _oiisum
         self.template = template
         self.targets = targets
@@ -88,7 +88,11 @@ class _Refiner(object):
         for i, tg in enumerate(targets):
             sig = PixelLayout.of(tg.spectra).signature
             self.batches.setdefault(sig, []).append(i)
-        self._tb = {sig: TargetBatch([targets[i] for i in idx]) for sig, idx in self.batches.items()}
+        if batch is not None and len(self.batches) == 1 and batch.nt == len(targets):
+            self._tb = {sig: batch for sig in self.batches}       # one layout: reuse it
+        else:
+            self._tb = {sig: TargetBatch([targets[i] for i in idx])
+                        for sig, idx in self.batches.items()}
         for sig, idx in self.batches.items():
             for k, i in enumerate(idx):
                 self.slot[i] = (sig, k)

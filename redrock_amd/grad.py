@@ -49,13 +49,17 @@ class GradProblem(object):
     items: item_target[i] is the slot (in `targets`) of item i.
     """
 
-    def __init__(self, template, targets, item_target):
+    def __init__(self, template, targets, item_target, batch=None):
+        """batch: an existing TargetBatch of exactly `targets` (reused, not rebuilt)."""
         import torch
         self.template = template
         self.targets = list(targets)
         self.item_target = np.asarray(item_target, dtype=np.int32)
         self.layout = PixelLayout.of(self.targets[0].spectra)
-        self.batch = TargetBatch(self.targets, self.layout)
+        if batch is not None and (batch.nt != len(self.targets)
+                                  or batch.layout.signature != self.layout.signature):
+            raise ValueError('batch does not hold these targets')
+        self.batch = batch if batch is not None else TargetBatch(self.targets, self.layout)
         self.dwave = {s.wavehash: s.wave for s in self.targets[0].spectra}
         dev = torch.device('cuda')
         lo, hi, wc = [], [], []
