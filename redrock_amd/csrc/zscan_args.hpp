@@ -43,6 +43,12 @@ __host__ __device__ constexpr int ne_index(int i, int j, int nb, bool mixed) {
 long long workspace_bytes(int nitems, int nb, bool mixed, long long total_rows);
 inline long long align256(long long x) { return (x + 255) & ~255LL; }
 
+// Redshifts packed side by side into the 16 columns of one 16-bit MFMA tile:
+// ZP*nb basis columns + 2 sqrt(w) f columns <= 16 (at most 4).
+__host__ __device__ constexpr int zpack(int nb) {
+    return nb <= 3 ? 4 : (14 / nb >= 2 ? 14 / nb : 1);
+}
+
 int zscan_stats(const ZArgs &a, hipStream_t s);
 int zscan_finish(const ZArgs &a, hipStream_t s);
 

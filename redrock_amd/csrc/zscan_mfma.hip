@@ -98,11 +98,14 @@ __global__ __launch_bounds__(64) void pack_mixed_kernel(const double *__restrict
 }
 
 // ---------------------------------------------------------------------------
-// Scan kernel.  Workgroup tile = TT items x ZZ redshifts, split over its four
-// wavefronts as WA (items) x WB (redshifts); a wavefront owns TW = TT/WA items
-// x ZW = ZZ/WB redshifts.  For every tile pair (32 pixels) the workgroup
-// stages in LDS the template rows of its ZZ redshifts (shared by its TT
-// items) and the R tiles of its TT items (shared by its ZZ redshifts), with
+// Scan kernel.  Workgroup tile = TT items x ZZ redshift groups, split over
+// its four wavefronts as WA (items) x WB (groups); a wavefront owns TW = TT/WA
+// items x ZW = ZZ/WB groups.  A group is ZP = zpack(NB) redshifts whose basis
+// columns sit side by side in one MFMA tile (ZP > 1 for NB <= 7: one R-apply
+// and one Gram MFMA then serve ZP redshifts; the off-diagonal blocks of the
+// Gram between them are not used).  For every tile pair (32 pixels) the
+// workgroup stages in LDS the template rows of its ZZ*ZP redshifts (shared by
+// its TT items) and the R tiles of its TT items (shared by its redshifts), with
 // direct global->LDS loads (global_load_lds_dwordx4: lane l of a wave writes
 // 16 bytes at the wave's LDS base + 16 l).  LDS is double-buffered: the loads
 // of pair p+1 are in flight while the MFMAs consume pair p.  All TT items must
@@ -110,8 +113,9 @@ __global__ __launch_bounds__(64) void pack_mixed_kernel(const double *__restrict
 // ---------------------------------------------------------------------------
 template <int NB, int TY, int WA, int WB, int TW, int ZW>
 __global__ __launch_bounds__(256) void zscan_mfma_kernel(rr::ZArgs a, long long ne_off, rr::TileMap map) {
-    constexpr int TT = WA * TW, ZZ = WB * ZW;
-    constexpr int TCH = ZZ * NB * 6;     // 16-byte template chunks per pair (48 rows x NB x ZZ)
+    constexpr int ZP = rr::zpack(NB);    // redshifts side by side in the 16 MFMA columns
+    constexpr int TT = WA * TW, ZZ = WB * ZW, ZR = ZZ * ZP;
+    constexpr int TCH = ZR * NB * 6;     // 16-byte template chunks per pair (48 rows x NB x ZR)
     constexpr int RCH = TT * 2 * 64;     // 16-byte R-tile chunks per pair (2 tiles x 64 lanes)
     constexpr int NCH = TCH + RCH;
     constexpr int PER = (NCH + 255) / 256;
@@ -128,7 +132,7 @@ __global__ __launch_bounds__(256) void zscan_mfma_kernel(rr::ZArgs a, long long 
     const int itemb = ti * TT;
     const int z0 = a.item_z0[itemb];
     const int nzi = a.item_nz[itemb];
-    const int zb = tz * ZZ;
+    const int zb = tz * ZR;
     if (zb >= nzi) return;                   // uniform over the workgroup
 
     const int ntile = a.npix_pad / 16;
@@ -167,15 +171,17 @@ __global__ __launch_bounds__(256) void zscan_mfma_kernel(rr::ZArgs a, long long 
     };
 
     // ---- operand addressing of this lane
-    const int j = lane & 15;   // column (basis) of operand B / output
+    // Column j < ZP*NB of operand B / output is basis j%NB of redshift j/NB
+    // of the lane's redshift group; columns ZP*NB, ZP*NB+1 carry sqrt(w) f.
+    const int j = lane & 15;
     const int g = lane >> 4;   // K group / output row group
-    const bool real = j < NB;
+    const bool real = j < ZP * NB;
     // B operand, first dword (K slots 0, 1 of group g = 0): zero for the basis
     // columns (those slots carry sqrt(w) f in the R tile), a unit on slot 0 /
-    // slot 1 for columns nb / nb+1; other lanes pass their template rows.
+    // slot 1 for the two f columns; other lanes pass their template rows.
     constexpr unsigned one16 = TY == RR_PREC_F16 ? 0x3C00u : 0x3F80u;
     const unsigned keep0 = g == 0 ? 0u : 0xffffffffu;
-    const unsigned set0 = (g != 0) ? 0u : (j == NB) ? one16 : (j == NB + 1) ? (one16 << 16) : 0u;
+    const unsigned set0 = (g != 0) ? 0u : (j == ZP * NB) ? one16 : (j == ZP * NB + 1) ? (one16 << 16) : 0u;
     typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
     auto bop = [&](u16x8 v) {
         u32x4 d = __builtin_bit_cast(u32x4, v);
@@ -210,7 +216,7 @@ __global__ __launch_bounds__(256) void zscan_mfma_kernel(rr::ZArgs a, long long 
 #pragma unroll
         for (int s = 0; s < ZW; ++s) {
             const int is = wb * ZW + s;
-            const u16x8 *tp = real ? L + (is * NB + j) * 6 + g : zero_chunk;
+            const u16x8 *tp = real ? L + (is * ZP * NB + j) * 6 + g : zero_chunk;
             bz[s][0] = bop(tp[0]);
             bz[s][1] = bop(tp[2]);
         }
@@ -232,8 +238,9 @@ __global__ __launch_bounds__(256) void zscan_mfma_kernel(rr::ZArgs a, long long 
     }
 
     // ---- epilogue: scaled normal equations -> workspace ----
-    // lane (j, g) register r holds G[4g+r][j]; keep the upper triangle of the
-    // first nb rows over columns 0..nb+1 ([M | y_hi | y_lo]).
+    // lane (j, g) register r holds G[4g+r][j]; per redshift q of the group
+    // keep the upper triangle of its NB x NB diagonal block and its rows of
+    // the two f columns ([M | y_hi | y_lo]).
     double *ne = reinterpret_cast<double *>(static_cast<char *>(a.work) + ne_off);
     const long long R = a.total_rows;
 #pragma unroll
@@ -243,13 +250,15 @@ __global__ __launch_bounds__(256) void zscan_mfma_kernel(rr::ZArgs a, long long 
         const long long row0 = a.item_out[item];
 #pragma unroll
         for (int s = 0; s < ZW; ++s) {
-            const int iz = zb + wb * ZW + s;
-            if (iz >= nzi) continue;
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
                 const int i = 4 * g + r;
-                if (i < NB && j >= i && j < NB + 2)
-                    ne[rr::ne_index(i, j, NB, true) * R + row0 + iz] = (double)G[u][s][r];
+                const int q = i / NB, ii = i - q * NB;
+                const int iz = zb + (wb * ZW + s) * ZP + q;
+                const int jj = (j >= q * NB + ii && j < (q + 1) * NB) ? j - q * NB
+                               : (j == ZP * NB || j == ZP * NB + 1) ? NB + j - ZP * NB : -1;
+                if (i < ZP * NB && jj >= 0 && iz < nzi)
+                    ne[rr::ne_index(ii, jj, NB, true) * R + row0 + iz] = (double)G[u][s][r];
             }
         }
     }
@@ -267,11 +276,11 @@ constexpr int kNumTiles = sizeof(kTiles) / sizeof(kTiles[0]);
 
 template <int NB, int TY, int WA, int WB, int TW, int ZW>
 int launch_tile(const rr::ZArgs &a, hipStream_t s) {
-    constexpr int TT = WA * TW, ZZ = WB * ZW;
+    constexpr int TT = WA * TW, ZR = WB * ZW * rr::zpack(NB);
     auto kernel = zscan_mfma_kernel<NB, TY, WA, WB, TW, ZW>;
-    const rr::TileMap map = rr::make_tile_map((a.nitems + TT - 1) / TT, (a.max_item_nz + ZZ - 1) / ZZ,
+    const rr::TileMap map = rr::make_tile_map((a.nitems + TT - 1) / TT, (a.max_item_nz + ZR - 1) / ZR,
                                               rr::resident_per_xcd(kernel, 256),
-                                              TT * 2.0 * 64 * 16, ZZ * NB * 6.0 * 16);
+                                              TT * 2.0 * 64 * 16, ZR * NB * 6.0 * 16);
     const long long ne_off = rr::align256((long long)a.nitems * 2 * 8);
     hipLaunchKernelGGL(kernel, dim3((unsigned)map.blocks), dim3(256), 0, s, a, ne_off, map);
     return rr::check_launch("rr_zscan(mixed)");

@@ -103,15 +103,18 @@ __device__ __forceinline__ void split8(float4v a, float4v b, float m1, u16x8 &xh
     xl2 = __builtin_bit_cast(u16x8, l);
 }
 
-// Workgroup tile TT items x ZZ redshifts over four wavefronts (WA x WB); per
-// tile pair the workgroup stages in LDS the hi and lo template rows of its ZZ
-// redshifts and the hi and lo R tiles of its TT items, with direct
+// Workgroup tile TT items x ZZ redshift groups over four wavefronts (WA x
+// WB); a group is ZP = zpack(NB) redshifts side by side in the MFMA columns
+// (as in zscan_mfma.hip).  Per tile pair the workgroup stages in LDS the hi
+// and lo template rows of its ZZ*ZP redshifts and the hi and lo R tiles of
+// its TT items, with direct
 // global->LDS loads (lane l of a wave writes 16 bytes at the wave base + 16 l),
 // double-buffered.
 template <int NB, int WA, int WB, int TW, int ZW>
 __global__ __launch_bounds__(256) void zscan_mfma3_kernel(rr::ZArgs a, long long ne_off, rr::TileMap map) {
-    constexpr int TT = WA * TW, ZZ = WB * ZW, NB2 = 2 * NB;
-    constexpr int TCH = ZZ * NB2 * 6;    // 16-byte template chunks per pair: 48 rows x (hi, lo) x NB x ZZ
+    constexpr int ZP = rr::zpack(NB);    // redshifts side by side in the 16 MFMA columns
+    constexpr int TT = WA * TW, ZZ = WB * ZW, ZR = ZZ * ZP, NB2 = 2 * NB;
+    constexpr int TCH = ZR * NB2 * 6;    // 16-byte template chunks per pair: 48 rows x (hi, lo) x NB x ZR
     constexpr int RCH = TT * 256;        // 16-byte R chunks per pair: 2 tiles x (hi, lo) x 64 lanes x TT
     constexpr int NCH = TCH + RCH;
     constexpr int PER = (NCH + 255) / 256;
@@ -127,7 +130,7 @@ __global__ __launch_bounds__(256) void zscan_mfma3_kernel(rr::ZArgs a, long long
     const int itemb = ti * TT;
     const int z0 = a.item_z0[itemb];
     const int nzi = a.item_nz[itemb];
-    const int zb = tz * ZZ;
+    const int zb = tz * ZR;
     if (zb >= nzi) return;                   // uniform over the workgroup
 
     const int ntile = a.npix_pad / 16;
@@ -164,14 +167,17 @@ __global__ __launch_bounds__(256) void zscan_mfma3_kernel(rr::ZArgs a, long long
                 (void __attribute__((address_space(3))) *)(&lds[buf][256 * k + 64 * wave]), 16, 0, 0);
     };
 
+    // column j < ZP*NB: basis j%NB of redshift j/NB of the group; columns
+    // ZP*NB, ZP*NB+1: sqrt(w) f
     const int j = lane & 15;
     const int g = lane >> 4;
-    const bool real = j < NB;
+    const bool real = j < ZP * NB;
+    const int jz = real ? j / NB : 0, jb = real ? j - jz * NB : 0;
     // first dword of a B operand (K slots 0, 1 of group 0): zero for the basis
-    // columns, unit vectors on slots 0 / 1 for columns nb / nb+1 (hi only)
+    // columns, unit vectors on slots 0 / 1 for the two f columns (hi only)
     constexpr unsigned one16 = 0x3C00u;
     const unsigned keep0 = g == 0 ? 0u : 0xffffffffu;
-    const unsigned set0 = (g != 0) ? 0u : (j == NB) ? one16 : (j == NB + 1) ? (one16 << 16) : 0u;
+    const unsigned set0 = (g != 0) ? 0u : (j == ZP * NB) ? one16 : (j == ZP * NB + 1) ? (one16 << 16) : 0u;
     typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
     auto bop = [&](u16x8 v, unsigned set) {
         u32x4 d = __builtin_bit_cast(u32x4, v);
@@ -215,7 +221,7 @@ __global__ __launch_bounds__(256) void zscan_mfma3_kernel(rr::ZArgs a, long long
             const int is = wb * ZW + s;
 #pragma unroll
             for (int h = 0; h < 2; ++h) {
-                const u16x8 *tp = real ? L + (is * NB2 + h * NB + j) * 6 + g : zero_chunk;
+                const u16x8 *tp = real ? L + ((is * ZP + jz) * NB2 + h * NB + jb) * 6 + g : zero_chunk;
                 bz[s][0][h] = bop(tp[0], h == 0 ? set0 : 0u);
                 bz[s][1][h] = bop(tp[2], h == 0 ? set0 : 0u);
             }
@@ -255,7 +261,8 @@ __global__ __launch_bounds__(256) void zscan_mfma3_kernel(rr::ZArgs a, long long
 
     // ---- epilogue: symmetric part of the FP64 sums -> workspace ----
     // lane (j, g) register r holds G[4g+r][j]; G[j][4g+r] sits in lane
-    // (4g+r) + 16 (j/4), register j%4.
+    // (4g+r) + 16 (j/4), register j%4.  Per redshift q of the group: the upper
+    // triangle of its NB x NB diagonal block and its rows of the f columns.
     double *ne = reinterpret_cast<double *>(static_cast<char *>(a.work) + ne_off);
     const long long R = a.total_rows;
 #pragma unroll
@@ -264,7 +271,6 @@ __global__ __launch_bounds__(256) void zscan_mfma3_kernel(rr::ZArgs a, long long
         const long long row0 = item < a.nitems ? a.item_out[item] : 0;
 #pragma unroll
         for (int s = 0; s < ZW; ++s) {
-            const int iz = zb + wb * ZW + s;
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
                 const int i = 4 * g + r;
@@ -273,8 +279,12 @@ __global__ __launch_bounds__(256) void zscan_mfma3_kernel(rr::ZArgs a, long long
 #pragma unroll
                 for (int k = 0; k < 4; ++k) tr[k] = __shfl(Gd[u][s][k], srcl, 64);
                 const double gji = (j & 3) == 0 ? tr[0] : (j & 3) == 1 ? tr[1] : (j & 3) == 2 ? tr[2] : tr[3];
-                if (item < a.nitems && iz < nzi && i < NB && j >= i && j < NB + 2)
-                    ne[rr::ne_index(i, j, NB, true) * R + row0 + iz] = 0.5 * (Gd[u][s][r] + gji);
+                const int q = i / NB, ii = i - q * NB;
+                const int iz = zb + (wb * ZW + s) * ZP + q;
+                const int jj = (j >= i && j < (q + 1) * NB) ? j - q * NB
+                               : (j == ZP * NB || j == ZP * NB + 1) ? NB + j - ZP * NB : -1;
+                if (item < a.nitems && iz < nzi && i < ZP * NB && jj >= 0)
+                    ne[rr::ne_index(ii, jj, NB, true) * R + row0 + iz] = 0.5 * (Gd[u][s][r] + gji);
             }
         }
     }
@@ -290,11 +300,11 @@ constexpr int kNumTiles3 = sizeof(kTiles3) / sizeof(kTiles3[0]);
 
 template <int NB, int WA, int WB, int TW, int ZW>
 int launch_tile(const rr::ZArgs &a, hipStream_t s) {
-    constexpr int TT = WA * TW, ZZ = WB * ZW;
+    constexpr int TT = WA * TW, ZR = WB * ZW * rr::zpack(NB);
     auto kernel = zscan_mfma3_kernel<NB, WA, WB, TW, ZW>;
-    const rr::TileMap map = rr::make_tile_map((a.nitems + TT - 1) / TT, (a.max_item_nz + ZZ - 1) / ZZ,
+    const rr::TileMap map = rr::make_tile_map((a.nitems + TT - 1) / TT, (a.max_item_nz + ZR - 1) / ZR,
                                               rr::resident_per_xcd(kernel, 256),
-                                              TT * 4.0 * 64 * 16, ZZ * NB * 12.0 * 16);
+                                              TT * 4.0 * 64 * 16, ZR * NB * 12.0 * 16);
     const long long ne_off = rr::align256((long long)a.nitems * 2 * 8);
     hipLaunchKernelGGL(kernel, dim3((unsigned)map.blocks), dim3(256), 0, s, a, ne_off, map);
     return rr::check_launch("rr_zscan(f16x3)");

@@ -149,3 +149,41 @@ def test_star_f64(gpu):
     rchi2, _, _ = orc.calc_zchi2(targets, tdata, tm.nbasis)
     assert np.max(np.abs(chi2 - rchi2) / rchi2) <= F64_RTOL
     assert np.array_equal(chi2.argmin(axis=1), rchi2.argmin(axis=1))
+
+
+@pytest.mark.parametrize('kind', ['qso4', 'star5', 'gal3', 'gal2'])
+@pytest.mark.parametrize('precision,tol_max', [('f16', 1e-3), ('f16x3', 1e-5)])
+def test_packed_small_basis_matches_oracle(gpu, kind, precision, tol_max):
+    """nbasis <= 7: several redshifts share one MFMA tile (4 for nb <= 3, 3 for
+    nb 4, 2 for nb 5-7); every redshift of a group must match the oracle, with
+    grids whose lengths are not multiples of the group size."""
+    from redrock_amd import synth, zscan
+    from redrock_amd.templates import Template, DistTemplate
+    lines = None
+    if kind == 'qso4':
+        q = synth.qso_template()
+        tm = Template(spectype='QSO', redshifts=q.redshifts[1500:1617], wave=q.wave, flux=q.flux,
+                      igm_model=q.igm_model)
+        targets, _ = synth.make_targets(q, 3, seed=41, zrange=(1.2, 1.4))
+        lines = [(1215.67, 0.0036, 3.46)]
+    elif kind == 'star5':
+        tm = synth.star_template()
+        targets, _ = synth.make_targets(tm, 3, seed=42)
+    else:
+        g = synth.galaxy_template()
+        nb = int(kind[-1])
+        tm = Template(spectype='GALAXY', redshifts=g.redshifts[700:803], wave=g.wave,
+                      flux=g.flux[:nb], igm_model=g.igm_model)
+        targets, _ = synth.make_targets(tm, 3, seed=43, zrange=(0.45, 0.6))
+    print(f'case: packed {kind} {precision}, nz={len(tm.redshifts)}', flush=True)
+    dwave = {s.wavehash: s.wave for s in targets[0].spectra}
+    dt = DistTemplate(tm, dwave)
+    chi2, coeff, pen = zscan.calc_zchi2([t.id for t in targets], targets, dt, precision=precision)
+    tdata = orc.template_cache(tm, dwave, lines=lines)
+    rchi2, rcoeff, rpen = orc.calc_zchi2(targets, tdata, tm.nbasis, getattr(tm, 'OIItemplate', None))
+    rel = np.abs(chi2 - rchi2) / np.abs(rchi2)
+    print(f'{kind} {precision}: max rel err {rel.max():.3e}')
+    assert rel.max() <= tol_max
+    assert np.array_equal(chi2.argmin(axis=1), rchi2.argmin(axis=1))
+    # every redshift of a group is filled (no stale rows)
+    assert np.all(np.isfinite(chi2)) and np.all(chi2 > 0)
