@@ -40,6 +40,30 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def kernel_signature(precision, nb, cfg):
+    """Demangled name of the mixed scan kernel instance for `cfg` = (wa, wb, tw, zw)."""
+    wa, wb, tw, zw = cfg
+    if precision == 'f16x3':
+        return f'void zscan_mfma3_kernel<{nb}, {wa}, {wb}, {tw}, {zw}>'
+    ty = {'f16': 1, 'bf16': 2}[precision]
+    return f'void zscan_mfma_kernel<{nb}, {ty}, {wa}, {wb}, {tw}, {zw}>'
+
+
+def measured_traffic(signature):
+    """HBM bytes per launch of `signature` on the galaxy petal, from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (calibrated against a plain copy);
+    None when that summary does not cover this kernel instance."""
+    path = os.path.join(ROOT, 'profiles', 'zscan_galaxy500_counters.md')
+    if not os.path.exists(path):
+        return None
+    key = f'traffic_bytes_per_launch {signature} = '
+    with open(path) as fh:
+        for line in fh:
+            if line.startswith(key):
+                return float(line[len(key):])
+    return None
+
+
 _CPU = {}
 
 
@@ -425,8 +449,10 @@ def main():
         'max_rel_chi2_err_at_best_z': float(rel_best.max()),
         'setup_s': setup_s,
         'roofline': {'bound': 'mfma', 'achieved': ach / 1e12, 'peak': PEAK_FLOPS[args.precision] / 1e12,
-                     'unit': 'TFLOP/s', 'frac': ach / PEAK_FLOPS[args.precision], 'traffic': None,
-                     'kernel': 'zscan_mfma3_kernel' if args.precision == 'f16x3' else 'zscan_mfma_kernel',
+                     'unit': 'TFLOP/s', 'frac': ach / PEAK_FLOPS[args.precision],
+                     'traffic': (measured_traffic(kernel_signature(args.precision, nb, table[tile]))
+                                 if kind == 'GALAXY' and nt == 500 else None),
+                     'kernel': kernel_signature(args.precision, nb, table[tile]),
                      'kernel_ms': k_mix,
                      'flop_per_eval': FLOP_PER_EVAL[kind], 'evals_per_launch': evals},
         'median_rel_chi2_err': float(np.median(rel)),

@@ -20,7 +20,7 @@
 extern "C" {
 #endif
 
-#define RR_ABI_VERSION 2
+#define RR_ABI_VERSION 3
 
 /* Error codes. */
 #define RR_OK 0

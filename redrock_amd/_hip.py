@@ -11,7 +11,7 @@ import os
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, 'libredrock_hip.so')
 
-RR_ABI_VERSION = 2
+RR_ABI_VERSION = 3
 RR_F64, RR_F32, RR_F16, RR_BF16 = 0, 1, 2, 3
 RR_PREC_F64, RR_PREC_F16, RR_PREC_BF16, RR_PREC_F32, RR_PREC_F16X3 = 0, 1, 2, 3, 4
 RR_STATUS_OK, RR_STATUS_SINGULAR = 0, 1
