@@ -97,16 +97,35 @@ class _Refiner(object):
             for k, i in enumerate(idx):
                 self.slot[i] = (sig, k)
 
+    @staticmethod
+    def _cache_budget():
+        """HBM bytes for one fine-grid template cache (nbasis x padded pixels x
+        redshifts doubles): an eighth of the free device memory, 1-32 GiB."""
+        import torch
+        free, _ = torch.cuda.mem_get_info()
+        return int(min(max(free // 8, 1 << 30), 32 << 30))
+
     def _scan(self, zlist, owners, per_item):
         """chi2/coeff/penalty at zlist[j*per_item:(j+1)*per_item] for target owners[j]."""
-        from .zscan import TemplateCache, ScanPlan
+        nitem = len(owners)
+        nb = self.template.nbasis
+        if nitem == 0:
+            return np.zeros((0, per_item)), np.zeros((0, per_item, nb)), np.zeros((0, per_item))
+        npix = sum(len(w) for w in self.dwave.values()) + 64
+        chunk = max(1, int(self._cache_budget() // (nb * npix * 8 * per_item)))
+        if nitem <= chunk:
+            return self._scan_chunk(zlist, owners, per_item)
+        parts = [self._scan_chunk(zlist[i * per_item:(i + chunk) * per_item], owners[i:i + chunk], per_item)
+                 for i in range(0, nitem, chunk)]
+        return tuple(np.concatenate([p[k] for p in parts]) for k in range(3))
+
+    def _scan_chunk(self, zlist, owners, per_item):
+        from .zscan import TemplateCache, ScanPlan, target_runs
         nitem = len(owners)
         nb = self.template.nbasis
         chi2 = np.zeros((nitem, per_item))
         coeff = np.zeros((nitem, per_item, nb))
         pen = np.zeros((nitem, per_item))
-        if nitem == 0:
-            return chi2, coeff, pen
         # candidates come from many targets: one cache for all their redshifts
         dw = dict()
         for tg in self.targets:
@@ -117,10 +136,15 @@ class _Refiner(object):
             sel = [j for j, t in enumerate(owners) if self.slot[t][0] == sig]
             if not sel:
                 continue
-            item_t = [self.slot[owners[j]][1] for j in sel]
-            z0 = [j * per_item for j in sel]
-            plan = ScanPlan(cache, self._tb[sig], item_t, z0, [per_item] * len(sel),
-                            self.precision, oiisum=self.oii)
+            slots = np.array([self.slot[owners[j]][1] for j in sel])
+            if len(sel) == nitem and self.precision == 'f64':
+                # consecutive candidates of one target share a scan item (their
+                # fine grids are consecutive in zlist): more redshifts per wavefront
+                tr, zr, nr = target_runs(np.repeat(slots, per_item))
+                plan = ScanPlan(cache, self._tb[sig], tr, zr, nr, self.precision, oiisum=self.oii)
+            else:
+                plan = ScanPlan(cache, self._tb[sig], slots, [j * per_item for j in sel],
+                                [per_item] * len(sel), self.precision, oiisum=self.oii)
             plan.run()
             c2, cf, pn, _ = plan.host()
             chi2[sel] = c2.reshape(len(sel), per_item)

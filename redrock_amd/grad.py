@@ -18,7 +18,7 @@ import numpy as np
 from . import _hip
 from .igm import igm_applies, lyman_table
 from .rebin import centers2edges, lyman_array
-from .zscan import PixelLayout, ScanPlan, TargetBatch, TemplateCache
+from .zscan import PixelLayout, ScanPlan, TargetBatch, TemplateCache, target_runs
 
 _vp, _i, _d = ctypes.c_void_p, ctypes.c_int, ctypes.c_double
 
@@ -83,9 +83,9 @@ class GradProblem(object):
         Returns device tensors (chi2 (n,), coeff (n, nb)).
         """
         zh = z.detach().cpu().numpy() if hasattr(z, 'detach') else np.asarray(z, dtype=np.float64)
-        n = len(self.item_target)
         cache = TemplateCache(self.template, self.dwave, redshifts=zh, flux=flux)
-        plan = ScanPlan(cache, self.batch, self.item_target, np.arange(n), np.ones(n), 'f64')
+        # consecutive entries of one target share a scan item (one z per lane)
+        plan = ScanPlan(cache, self.batch, *target_runs(self.item_target), 'f64')
         plan.run()
         return plan.zchi2, plan.zcoeff
 
@@ -172,16 +172,18 @@ def newton_refine(problem, z0, zlo, zhi, iters=4, dz_fd=1e-6):
     zlo = np.asarray(zlo, dtype=np.float64)
     zhi = np.asarray(zhi, dtype=np.float64)
     n = len(z)
+    # (z - h, z, z + h) of an item are consecutive, so a target's points form
+    # one multi-redshift scan item
+    sub = GradProblem.__new__(GradProblem)
+    sub.__dict__.update(problem.__dict__)
+    sub.item_target = np.repeat(problem.item_target, 3)
+    sub.t_item_target = torch.as_tensor(sub.item_target, device='cuda')
     for _ in range(iters):
-        zz = np.concatenate([z, z - dz_fd, z + dz_fd])
-        sub = GradProblem.__new__(GradProblem)
-        sub.__dict__.update(problem.__dict__)
-        sub.item_target = np.tile(problem.item_target, 3)
-        sub.t_item_target = torch.as_tensor(sub.item_target, device='cuda')
+        zz = np.stack([z - dz_fd, z, z + dz_fd], axis=1).ravel()
         chi2, coeff = sub.forward(zz)
         g, _, _ = sub.grad(zz, coeff)
-        g = g.cpu().numpy()
-        g0, gm, gp = g[:n], g[n:2 * n], g[2 * n:]
+        g = g.cpu().numpy().reshape(n, 3)
+        gm, g0, gp = g[:, 0], g[:, 1], g[:, 2]
         h = (gp - gm) / (2 * dz_fd)
         step = np.where(h > 0, -g0 / np.where(h > 0, h, 1.0), -np.sign(g0) * 0.25 * (zhi - zlo))
         znew = z + step

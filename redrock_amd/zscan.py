@@ -13,9 +13,10 @@ Drop-in functions with the reference signatures:
 Every call runs the HIP kernels of libredrock_hip.so; there is no CPU path.
 The ``use_gpu`` and ``fullprecision`` arguments are accepted for signature
 compatibility (the scan is always on the GPU and always deterministic).
-``precision`` selects 'f64' (default, the reference's arithmetic) or the
-mixed-precision MFMA forms 'f16' / 'bf16'; the default can be changed with
-the environment variable RR_PRECISION.
+``precision`` selects 'f64' (default, the reference's arithmetic) or a
+mixed-precision MFMA form: 'f16x3' (split fp16, chi2 within ~1e-5 of FP64),
+'f16', 'bf16' or 'f32'; the default can be changed with the environment
+variable RR_PRECISION.
 
 Device-side objects:
 
@@ -25,6 +26,8 @@ Device-side objects:
 * TargetBatch   — per-target R bands, ivar, flux, ivar*flux in HBM.
 * ScanPlan      — a prepared rr_zscan launch (descriptor, outputs, workspace)
   that can be replayed.
+* ScanGraph     — every template class over one TargetBatch captured in one
+  HIP graph.
 """
 from __future__ import annotations
 
@@ -256,6 +259,22 @@ class TargetBatch(object):
             _hip.check(rc, 'rr_pack_mixed')
             self._mixed[precision] = (rtile, sfcol, scales)
         return self._mixed[precision]
+
+
+def target_runs(item_target):
+    """Consecutive entries of one target as one scan item.
+
+    Returns (targets, z0, nz): entry k of a run is redshift z0 + k of the
+    cache, so per-entry redshift lists become multi-z items (the FP64 kernel
+    puts a target's redshifts on the lanes of one wavefront) and the output
+    rows keep the entry order.
+    """
+    t = np.asarray(item_target, dtype=np.int64)
+    if t.size == 0:
+        return t.astype(np.int32), np.zeros(0, np.int32), np.zeros(0, np.int32)
+    start = np.flatnonzero(np.r_[True, t[1:] != t[:-1]])
+    nz = np.diff(np.r_[start, t.size])
+    return t[start].astype(np.int32), start.astype(np.int32), nz.astype(np.int32)
 
 
 class ScanPlan(object):
