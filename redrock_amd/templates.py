@@ -1,7 +1,7 @@
 """Templates and the device-resident rebinned template cache.
 
-Template mirrors the in-memory constructor of py/redrock/templates.py:29-272
-(FITS reading and template discovery are outside the scan path).
+Template mirrors py/redrock/templates.py:29-272: the in-memory constructor
+and redrock-format template files (read with redrock_amd.fits).
 DistTemplate mirrors py/redrock/templates.py:525-663 for one process: it
 rebins the template to every redshift on every wavelength grid once
 (templates.py:584-597) — here on the GPU, with the IGM multiply fused — and
@@ -9,9 +9,90 @@ keeps the cache in HBM for the whole scan.
 """
 from __future__ import annotations
 
+import os
+
 import numpy as np
 
 valid_template_methods = ('PCA', 'NMF')
+
+
+def get_template_dir(template_path=None):
+    """Template directory: template_path, else $RR_TEMPLATE_DIR (templates.py:320-345)."""
+    if template_path is not None:
+        return template_path
+    d = os.environ.get('RR_TEMPLATE_DIR')
+    if d is None:
+        raise IOError('RR_TEMPLATE_DIR is not set and no template path was given')
+    return d
+
+
+def _read_template_file(filename):
+    """Fields of a redrock-format template file (templates.py:47-155).
+
+    Returns dict(version, wave, flux, method, redshifts or None, rrtype,
+    subtype, igm_model or None).
+    """
+    from . import fits
+    if not os.path.exists(filename):
+        alt = os.path.join(get_template_dir(), filename)
+        if not os.path.exists(alt):
+            raise IOError('unable to find ' + filename)
+        filename = alt
+    fx = fits.read(filename)
+    hdu = fx['BASIS_VECTORS']
+    hdr = hdu.header
+    out = dict()
+    out['version'] = hdr['VERSION'] if 'VERSION' in hdr else 'unknown'
+    wave = np.asarray(hdr['CRVAL1'] + hdr['CDELT1'] * np.arange(hdr['NAXIS1']), dtype=np.float64)
+    if 'LOGLAM' in hdr and hdr['LOGLAM'] != 0:
+        wave = 10 ** wave
+    out['wave'] = wave
+    out['flux'] = np.asarray(hdu.data, dtype=np.float64)
+    out['method'] = hdr['RRMETHOD'].upper() if 'RRMETHOD' in hdr else 'PCA'
+    red = None
+    if 'REDSHIFTS' in fx:
+        red = fx['REDSHIFTS'].data
+        if isinstance(red, dict):                     # a one-column table
+            red = next(iter(red.values()))
+        red = np.asarray(red, dtype=np.float64).ravel()
+    out['redshifts'] = red
+    out['rrtype'] = hdr['RRTYPE'].strip().upper()
+    out['subtype'] = hdr['RRSUBTYP'].strip().upper() if 'RRSUBTYP' in hdr else ''
+    out['igm_model'] = hdr['RRIGM'] if 'RRIGM' in hdr else None
+    return out
+
+
+def _default_redshifts(rrtype, zscan_galaxy=None, zscan_qso=None, zscan_star=None):
+    """Redshift grid of an old-style template file without REDSHIFTS (templates.py:89-118)."""
+    if rrtype == 'GALAXY':
+        if zscan_galaxy is not None:
+            zmin, zmax, dz = zscan_galaxy.split(',')
+            return 10 ** np.arange(np.log10(1 + float(zmin)), np.log10(1 + float(zmax)), float(dz)) - 1
+        return 10 ** np.arange(np.log10(1 - 0.005), np.log10(1 + 1.7), 3e-4) - 1
+    if rrtype == 'STAR':
+        if zscan_star is not None:
+            zmin, zmax, dz = zscan_star.split(',')
+            return np.arange(float(zmin), float(zmax), float(dz))
+        return np.arange(-0.002, 0.00201, 4e-5)
+    if rrtype == 'QSO':
+        if zscan_qso is not None:
+            zmin, zmax, dz = zscan_qso.split(',')
+            return 10 ** np.arange(np.log10(1 + float(zmin)), np.log10(1 + float(zmax)), float(dz)) - 1
+        return 10 ** np.arange(np.log10(1 + 0.05), np.log10(1 + 6.0), 5e-4) - 1
+    raise ValueError('Unknown redshift range to use for template type {}'.format(rrtype))
+
+
+def _default_igm(rrtype, version):
+    """IGM model of a template file without RRIGM (templates.py:127-143)."""
+    if rrtype == 'STAR':
+        return 'None'
+    if rrtype == 'GALAXY' and version == '2.6':
+        return 'Calura12'
+    if rrtype == 'QSO' and version in ('0.1', '1.0'):
+        return 'Calura12'
+    if rrtype == 'QSO' and version == '1.1':
+        return 'Kamble20'
+    raise ValueError('Missing keyword RRIGM specifying IGM model to use')
 
 
 def parse_fulltype(fulltype):
@@ -35,9 +116,12 @@ def make_fulltype(spectype, subtype):
 
 
 class Template(object):
-    """A spectral template basis (templates.py:29-272, in-memory form).
+    """A spectral template basis (templates.py:29-272).
 
     Args:
+        filename (str): a redrock-format template file (absolute, or relative
+            to $RR_TEMPLATE_DIR); then the arguments below except the zscan_*
+            ranges (for files without a REDSHIFTS HDU) are read from it.
         spectype (str): GALAXY, QSO, STAR, ...
         redshifts (array): trial redshifts of the coarse scan.
         wave (array): rest-frame wavelengths, increasing.
@@ -49,10 +133,15 @@ class Template(object):
     def __init__(self, filename=None, spectype=None, redshifts=None, wave=None,
                  flux=None, subtype=None, method='PCA', igm_model='Inoue14',
                  zscan_galaxy=None, zscan_qso=None, zscan_star=None, version=None):
+        self._filename = filename
         if filename is not None:
-            raise NotImplementedError('template files are outside the chi2-scan path; '
-                                      'construct Template from arrays')
-        self._filename = None
+            f = _read_template_file(filename)
+            spectype, subtype, version = f['rrtype'], f['subtype'], f['version']
+            wave, flux, method = f['wave'], f['flux'], f['method']
+            redshifts = f['redshifts']
+            if redshifts is None:
+                redshifts = _default_redshifts(spectype, zscan_galaxy, zscan_qso, zscan_star)
+            igm_model = f['igm_model'] if f['igm_model'] is not None else _default_igm(spectype, version)
         self._rrtype = spectype
         self._redshifts = np.asarray(redshifts, dtype=np.float64)
         self.wave = np.asarray(wave, dtype=np.float64)
@@ -195,3 +284,34 @@ class _LazyPiece(object):
         if self._data is None:
             self._data = self._dt.cache.reference_layout()
         return self._data
+
+
+def write_template(path, template, with_redshifts=True, loglam=False):
+    """Write `template` as a redrock-format file (BASIS_VECTORS image with
+    CRVAL1/CDELT1/RRTYPE/RRSUBTYP/RRIGM/RRMETHOD/VERSION, REDSHIFTS image).
+
+    The wavelength grid must be uniform in lambda (or in log10 lambda with
+    loglam=True), as the format stores only its start and step.
+    """
+    from . import fits
+    w = np.asarray(template.wave, dtype=np.float64)
+    x = np.log10(w) if loglam else w
+    step = (x[-1] - x[0]) / (len(x) - 1)
+    if not np.allclose(np.diff(x), step, rtol=1e-9, atol=0):
+        raise ValueError('template wavelengths are not uniformly spaced')
+    hdr = {'CRVAL1': float(x[0]), 'CDELT1': float(step), 'LOGLAM': 1 if loglam else 0,
+           'RRTYPE': template.template_type, 'RRSUBTYP': template.sub_type or '',
+           'RRMETHOD': template.solve_matrices_algorithm,
+           'VERSION': str(template.version if template.version is not None else 'unknown')}
+    if template.igm_model is not None:
+        hdr['RRIGM'] = str(template.igm_model)
+    hdus = [fits.image_hdu(template.flux, 'BASIS_VECTORS', hdr)]
+    if with_redshifts:
+        hdus.append(fits.image_hdu(template.redshifts, 'REDSHIFTS'))
+    fits.write(path, hdus)
+
+
+def load_templates(filenames, zscan_galaxy=None, zscan_qso=None, zscan_star=None):
+    """Template objects for a list of template files (templates.py:394-444)."""
+    return [Template(f, zscan_galaxy=zscan_galaxy, zscan_qso=zscan_qso, zscan_star=zscan_star)
+            for f in filenames]
