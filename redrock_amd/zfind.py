@@ -100,12 +100,19 @@ def calc_deltachi2(chi2, z, zwarn, dvlimit=constants.max_velo_diff):
 class Priors(object):
     """Redshift priors added to the coarse chi2 (priors.py:10-126).
 
-    The reference reads TARGETID, Z, SIGMA and FUNCTION from the PRIORS HDU of
-    a FITS file; here the same columns are given as arrays.  function is one
-    of 'gaussian', 'lorentzien', 'tophat'.
+    Priors(filename) reads TARGETID, Z, SIGMA and FUNCTION from the PRIORS HDU
+    of a FITS file, as the reference does; Priors(targetid, z, sigma,
+    function) takes the same columns as arrays.  function is one of
+    'gaussian', 'lorentzien', 'tophat'.
     """
 
-    def __init__(self, targetid, z, sigma, function='gaussian'):
+    def __init__(self, targetid, z=None, sigma=None, function='gaussian'):
+        if isinstance(targetid, str):
+            import os
+            from . import fits
+            t = fits.read(os.path.expandvars(targetid))['PRIORS'].data
+            targetid, z, sigma = t['TARGETID'], t['Z'], t['SIGMA']
+            function = str(t['FUNCTION'][0]).strip()
         self._param = {int(t): (float(a), float(s)) for t, a, s in zip(targetid, z, sigma)}
         if function not in ('gaussian', 'lorentzien', 'tophat'):
             raise ValueError(f'unknown prior function {function!r}')
@@ -184,9 +191,8 @@ def zfind(targets, templates, mp_procs=1, nminima=3, archetypes=None, priors=Non
         raise NotImplementedError('archetype fits are outside the chi2-scan path')
     if chi2_scan is not None:
         raise NotImplementedError('reading a saved chi2 scan (rrdetails HDF5) is not supported')
-    if priors is not None and not hasattr(priors, 'eval'):
-        raise NotImplementedError('priors must be a redrock_amd.zfind.Priors '
-                                  '(reading the FITS prior file needs astropy)')
+    if isinstance(priors, str):
+        priors = Priors(priors)
     local = _local_targets(targets)
     dist = targets if hasattr(targets, 'local_target_ids') else _Local(local)
     results = calc_zchi2_targets(dist, templates, mp_procs=mp_procs, use_gpu=use_gpu,
@@ -299,3 +305,20 @@ def _assemble(allresults, target_ids, templates, nminima):
         coeff = coeff[:, :int(nc[0])]
     allzfit['coeff'] = coeff
     return allzfit
+
+
+ZBEST_COLUMNS = ('TARGETID', 'Z', 'ZERR', 'ZWARN', 'CHI2', 'COEFF', 'NPIXELS', 'SPECTYPE',
+                 'SUBTYPE', 'NCOEFF', 'DELTACHI2')
+
+
+def zbest_table(allzfit):
+    """The best fit of every target (znum == 0) as the REDSHIFTS columns of
+    redrock's output file (the rrdesi table; upper-case names)."""
+    best = allzfit['znum'] == 0
+    return {name: np.asarray(allzfit[name.lower()])[best] for name in ZBEST_COLUMNS}
+
+
+def write_zbest(path, allzfit, header=None):
+    """Write zbest_table(allzfit) as the REDSHIFTS HDU of a FITS file."""
+    from . import fits
+    fits.write(path, [fits.table_hdu(zbest_table(allzfit), 'REDSHIFTS')], primary_header=header)

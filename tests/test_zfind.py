@@ -1,5 +1,7 @@
 """zfind assembly (redrock_amd.zfind) on the host: delta chi2, sorting, priors,
 and the per-target result table against the oracle's restatement."""
+import os
+
 import numpy as np
 import pytest
 
@@ -175,3 +177,34 @@ def test_assemble_nodata_and_coverage_bits():
     assert np.all(got['zwarn'][t13] & ZW.LITTLE_COVERAGE)
     t11 = got['targetid'] == 11
     assert not np.any(got['zwarn'][t11] & (ZW.NODATA | ZW.LITTLE_COVERAGE))
+
+
+def test_write_zbest_roundtrip(tmp_path):
+    """Best rows in the REDSHIFTS layout of the reference's output file."""
+    from redrock_amd import fits
+    res, tids, templates = _fake_results(2)
+    got = zf._assemble(res, tids, templates, nminima=3)
+    path = str(tmp_path / 'zbest.fits')
+    zf.write_zbest(path, got, header={'RRVER': 'redrock_amd'})
+    t = fits.read(path)['REDSHIFTS'].data
+    ref = fits.read(os.path.join(os.path.dirname(__file__), 'golden', 'redrock-test.fits'))['REDSHIFTS'].data
+    assert list(t.keys()) == list(ref.keys())          # same columns, same order
+    best = got['znum'] == 0
+    assert np.array_equal(t['TARGETID'], got['targetid'][best])
+    assert sorted(t['TARGETID']) == sorted(set(got['targetid']))
+    for k in ('Z', 'ZERR', 'CHI2', 'DELTACHI2', 'COEFF', 'ZWARN', 'NPIXELS', 'NCOEFF'):
+        assert np.array_equal(t[k], np.asarray(got[k.lower()])[best]), k
+    assert list(t['SPECTYPE']) == list(got['spectype'][best])
+
+
+def test_priors_from_fits(tmp_path):
+    """PRIORS HDU with TARGETID, Z, SIGMA, FUNCTION (priors.py:26-34)."""
+    from redrock_amd import fits
+    path = str(tmp_path / 'priors.fits')
+    fits.write(path, [fits.table_hdu({'TARGETID': np.array([10, 11]), 'Z': np.array([0.5, 1.0]),
+                                      'SIGMA': np.array([0.1, 0.2]),
+                                      'FUNCTION': np.array(['lorentzien', 'lorentzien'])}, 'PRIORS')])
+    p = zf.Priors(path)
+    z = np.array([0.4, 0.5, 0.6])
+    assert np.allclose(p.eval(10, z), [2 * np.log(2), 0.0, 2 * np.log(2)])
+    assert p.eval(12, z) == 0.
