@@ -49,14 +49,15 @@ def kernel_signature(precision, nb, cfg):
     return f'void zscan_mfma_kernel<{nb}, {ty}, {wa}, {wb}, {tw}, {zw}>'
 
 
-def measured_traffic(signature):
-    """HBM bytes per launch of `signature` on the galaxy petal, from the committed
-    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (calibrated against a plain copy);
-    None when that summary does not cover this kernel instance."""
+def measured_counter(name, signature):
+    """A per-launch figure of `signature` on the galaxy petal from the committed
+    rocprofv3 counter summary: 'traffic_bytes_per_launch' (FETCH_SIZE +
+    WRITE_SIZE, calibrated against a plain copy) or 'mfma_util_percent'
+    (MfmaUtil); None when the summary does not cover this kernel instance."""
     path = os.path.join(ROOT, 'profiles', 'zscan_galaxy500_counters.md')
     if not os.path.exists(path):
         return None
-    key = f'traffic_bytes_per_launch {signature} = '
+    key = f'{name} {signature} = '
     with open(path) as fh:
         for line in fh:
             if line.startswith(key):
@@ -416,6 +417,7 @@ def main():
         del plan
 
     flops = FLOP_PER_EVAL[kind] * evals
+    sig = kernel_signature(args.precision, nb, table[tile])
     ach = flops / (k_mix * 1e-3)
     out = {
         'metric': METRIC,
@@ -450,9 +452,12 @@ def main():
         'setup_s': setup_s,
         'roofline': {'bound': 'mfma', 'achieved': ach / 1e12, 'peak': PEAK_FLOPS[args.precision] / 1e12,
                      'unit': 'TFLOP/s', 'frac': ach / PEAK_FLOPS[args.precision],
-                     'traffic': (measured_traffic(kernel_signature(args.precision, nb, table[tile]))
+                     'traffic': (measured_counter('traffic_bytes_per_launch', sig)
                                  if kind == 'GALAXY' and nt == 500 else None),
-                     'kernel': kernel_signature(args.precision, nb, table[tile]),
+                     'mfma_busy': (measured_counter('mfma_util_percent', sig) / 100
+                                   if kind == 'GALAXY' and nt == 500
+                                   and measured_counter('mfma_util_percent', sig) is not None else None),
+                     'kernel': sig,
                      'kernel_ms': k_mix,
                      'flop_per_eval': FLOP_PER_EVAL[kind], 'evals_per_launch': evals},
         'median_rel_chi2_err': float(np.median(rel)),
