@@ -42,10 +42,12 @@ extern "C" int rr_zscan(const rr_zscan_desc *d, void *stream) {
     if (d->halo < 8 || d->cache_rows < (long long)d->npix_pad + 2 * d->halo)
         return rr::fail(RR_EINVAL, "rr_zscan: cache rows %lld too small for npix_pad %d halo %d",
                         d->cache_rows, d->npix_pad, d->halo);
+    // no items (or only empty ones): a valid call with nothing to do; zero-size
+    // device arrays may then have null addresses
+    if (d->nitems == 0 || d->max_item_nz <= 0) return RR_OK;
     if (!d->d_tcache || !d->d_rband || !d->d_w || !d->d_wf || !d->d_f || !d->d_item_target ||
         !d->d_item_z0 || !d->d_item_nz || !d->d_item_out || !d->d_zchi2 || !d->d_zcoeff)
         return rr::fail(RR_EINVAL, "rr_zscan: null required pointer");
-    if (d->nitems == 0 || d->max_item_nz <= 0) return RR_OK;
     const bool mixed = is_mixed(d->precision);
     const long long need = rr::workspace_bytes(d->nitems, d->nb, mixed, d->total_rows);
     if (!d->d_work || d->work_bytes < need)

@@ -47,7 +47,7 @@ inline TileMap make_tile_map(int nti, int ntz, int resident, double bytes_item, 
 }
 
 // Workgroup L -> (item tile, z tile); false for the padding workgroups.
-__device__ __forceinline__ bool map_tile(const TileMap &m, int L, int &ti, int &tz) {
+__host__ __device__ __forceinline__ bool map_tile(const TileMap &m, int L, int &ti, int &tz) {
     const int xcd = L % kXcds, slot = L / kXcds;
     const int per = m.si * m.sz;
     const int S = (slot / per) * kXcds + xcd;

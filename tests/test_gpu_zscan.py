@@ -187,3 +187,40 @@ def test_packed_small_basis_matches_oracle(gpu, kind, precision, tol_max):
     assert np.array_equal(chi2.argmin(axis=1), rchi2.argmin(axis=1))
     # every redshift of a group is filled (no stale rows)
     assert np.all(np.isfinite(chi2)) and np.all(chi2 > 0)
+
+
+@pytest.mark.parametrize('kind', ['galaxy', 'qso'])
+def test_ragged_items_every_precision(gpu, kind):
+    """Items with their own (target, z0, nz) — the layout fitz and zfind use —
+    in every precision: FP64 rows are bit-identical to the full scan's, mixed
+    rows within the mode's tolerance of them; nz not a multiple of the
+    redshift group (QSO packs 3 per MFMA tile) and single-z items included."""
+    from redrock_amd import synth, zscan
+    from redrock_amd.templates import DistTemplate
+    tm = synth.galaxy_template() if kind == 'galaxy' else synth.qso_template()
+    targets, _ = synth.make_targets(tm, 4, seed=51)
+    dwave = {s.wavehash: s.wave for s in targets[0].spectra}
+    dt = DistTemplate(tm, dwave)
+    nz = len(tm.redshifts)
+    full, _, _ = zscan.calc_zchi2([t.id for t in targets], targets, dt, precision='f64')
+    items = [(0, 0, 37), (1, 100, 1), (2, 500, 64), (0, 1000, 5), (3, nz - 7, 7), (2, 7, 2)]
+    it, z0, n = (np.array(c) for c in zip(*items))
+    batch = zscan.TargetBatch(targets)
+    want = np.concatenate([full[t, a:a + k] for t, a, k in items])
+    tol = {'f64': 0.0, 'f16x3': 1e-5, 'f32': 1e-5, 'f16': 2e-3, 'bf16': 1e-2}
+    for p, rtol in tol.items():
+        print(f'case: ragged {kind} {p}', flush=True)
+        plan = zscan.ScanPlan(dt.cache, batch, it, z0, n, precision=p,
+                              oiisum=zscan._oiisum(tm))
+        plan.run()
+        got = plan.zchi2.cpu().numpy()
+        rel = np.abs(got - want) / np.abs(want)
+        print(f'{kind} {p}: max rel err {rel.max():.2e}')
+        if p == 'f64':
+            assert np.array_equal(got, want)
+        else:
+            assert rel.max() <= rtol
+    # no items: nothing to do, no error
+    empty = zscan.ScanPlan(dt.cache, batch, np.zeros(0, int), np.zeros(0, int), np.zeros(0, int), 'f16x3')
+    empty.run()
+    assert empty.zchi2.numel() == 0
