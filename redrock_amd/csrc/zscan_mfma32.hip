@@ -91,8 +91,9 @@ __global__ __launch_bounds__(64) void pack32_kernel(const double *__restrict__ r
 
 template <int NB, int WA, int WB, int TW, int ZW>
 __global__ __launch_bounds__(256) void zscan_mfma32_kernel(rr::ZArgs a, long long ne_off, rr::TileMap map) {
-    constexpr int TT = WA * TW, ZZ = WB * ZW;
-    constexpr int TCH = ZZ * NB * 12;    // 16-byte chunks: 48 rows x 4 B per basis per z
+    constexpr int ZP = rr::zpack(NB);    // redshifts side by side in the 16 MFMA columns
+    constexpr int TT = WA * TW, ZZ = WB * ZW, ZR = ZZ * ZP;
+    constexpr int TCH = ZR * NB * 12;    // 16-byte chunks: 48 rows x 4 B per basis per z
     constexpr int RCH = TT * 2 * 128;    // 2 tiles x 64 lanes x 32 B
     constexpr int SCH = TT * 8;          // 4 g x 32 B
     constexpr int NCH = TCH + RCH + SCH;
@@ -109,7 +110,7 @@ __global__ __launch_bounds__(256) void zscan_mfma32_kernel(rr::ZArgs a, long lon
     const int itemb = ti * TT;
     const int z0 = a.item_z0[itemb];
     const int nzi = a.item_nz[itemb];
-    const int zb = tz * ZZ;
+    const int zb = tz * ZR;
     if (zb >= nzi) return;
 
     const int ntile = a.npix_pad / 16;
@@ -159,10 +160,13 @@ __global__ __launch_bounds__(256) void zscan_mfma32_kernel(rr::ZArgs a, long lon
         }
     };
 
+    // column j < ZP*NB: basis j%NB of redshift j/NB of the group; column
+    // ZP*NB: sqrt(w) f; the others stay zero
     const int j = lane & 15;
     const int g = lane >> 4;
-    const bool real = j < NB;
-    const bool fcolumn = (j == NB);
+    const bool real = j < ZP * NB;
+    const bool fcolumn = (j == ZP * NB);
+    const int jz = real ? j / NB : 0, jb = real ? j - jz * NB : 0;
 
     // fp32 MFMA partial sums of one pair (32 pixels) are flushed into FP64:
     // with fp32 operands the accumulation over ~8000 pixels would otherwise
@@ -204,8 +208,8 @@ __global__ __launch_bounds__(256) void zscan_mfma32_kernel(rr::ZArgs a, long lon
             float tb[2][8];
 #pragma unroll
             for (int v = 0; v < 8; ++v) {
-                tb[0][v] = real ? L[(is * NB + j) * 48 + 8 * g + v] : 0.f;
-                tb[1][v] = real ? L[(is * NB + j) * 48 + 16 + 8 * g + v] : 0.f;
+                tb[0][v] = real ? L[((is * ZP + jz) * NB + jb) * 48 + 8 * g + v] : 0.f;
+                tb[1][v] = real ? L[((is * ZP + jz) * NB + jb) * 48 + 16 + 8 * g + v] : 0.f;
             }
 #pragma unroll
             for (int u = 0; u < TW; ++u) {
@@ -242,13 +246,17 @@ __global__ __launch_bounds__(256) void zscan_mfma32_kernel(rr::ZArgs a, long lon
         const long long row0 = a.item_out[item];
 #pragma unroll
         for (int s = 0; s < ZW; ++s) {
-            const int iz = zb + wb * ZW + s;
-            if (iz >= nzi) continue;
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
+                // per redshift q of the group: its NB x NB block, the f column
+                // (y) and the always-zero column after it (y_lo of the 16-bit modes)
                 const int i = 4 * g + r;
-                if (i < NB && j >= i && j < NB + 2)
-                    ne[rr::ne_index(i, j, NB, true) * R + row0 + iz] = Gd[u][s][r];
+                const int q = i / NB, ii = i - q * NB;
+                const int iz = zb + (wb * ZW + s) * ZP + q;
+                const int jj = (j >= i && j < (q + 1) * NB) ? j - q * NB
+                               : (j == ZP * NB || j == ZP * NB + 1) ? NB + j - ZP * NB : -1;
+                if (i < ZP * NB && jj >= 0 && iz < nzi)
+                    ne[rr::ne_index(ii, jj, NB, true) * R + row0 + iz] = Gd[u][s][r];
             }
         }
     }
@@ -256,11 +264,11 @@ __global__ __launch_bounds__(256) void zscan_mfma32_kernel(rr::ZArgs a, long lon
 
 template <int NB, int WA, int WB, int TW, int ZW>
 int launch_tile(const rr::ZArgs &a, hipStream_t s) {
-    constexpr int TT = WA * TW, ZZ = WB * ZW;
+    constexpr int TT = WA * TW, ZR = WB * ZW * rr::zpack(NB);
     auto kernel = zscan_mfma32_kernel<NB, WA, WB, TW, ZW>;
-    const rr::TileMap map = rr::make_tile_map((a.nitems + TT - 1) / TT, (a.max_item_nz + ZZ - 1) / ZZ,
+    const rr::TileMap map = rr::make_tile_map((a.nitems + TT - 1) / TT, (a.max_item_nz + ZR - 1) / ZR,
                                               rr::resident_per_xcd(kernel, 256),
-                                              TT * 2.0 * 128 * 16, ZZ * NB * 12.0 * 16);
+                                              TT * 2.0 * 128 * 16, ZR * NB * 12.0 * 16);
     const long long ne_off = rr::align256((long long)a.nitems * 2 * 8);
     hipLaunchKernelGGL(kernel, dim3((unsigned)map.blocks), dim3(256), 0, s, a, ne_off, map);
     return rr::check_launch("rr_zscan(f32)");

@@ -152,7 +152,7 @@ def test_star_f64(gpu):
 
 
 @pytest.mark.parametrize('kind', ['qso4', 'star5', 'gal3', 'gal2'])
-@pytest.mark.parametrize('precision,tol_max', [('f16', 1e-3), ('f16x3', 1e-5)])
+@pytest.mark.parametrize('precision,tol_max', [('f16', 1e-3), ('f16x3', 1e-5), ('f32', 1e-5)])
 def test_packed_small_basis_matches_oracle(gpu, kind, precision, tol_max):
     """nbasis <= 7: several redshifts share one MFMA tile (4 for nb <= 3, 3 for
     nb 4, 2 for nb 5-7); every redshift of a group must match the oracle, with
