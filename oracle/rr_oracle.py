@@ -22,12 +22,23 @@ calc_zchi2_batch           weighted least squares by np.linalg.lstsq on
                            quadratic identity chi2 = f'Wf - y'c, HUGE_CHI2
                            conventions (zero weights, singular M)
 calc_negOII_penalty        closed form
-find_minima                reference docstring vector (fitz.py:52-53)
-minfit                     exact parabola recovery (textbook)
+find_minima                reference docstring vector (fitz.py:52-53); the
+                           reference's details file (tests/golden/
+                           rrdetails-test.h5, tests/test_results.py): the
+                           candidate choice replayed on its stored scans
+                           gives every stored fine grid
+minfit                     exact parabola recovery (textbook); the stored
+                           fits of the details file (z 1e-11, zerr 3e-8 =
+                           polyfit conditioning, chi2 1e-10, zwarn bits)
 fitz                       restatement only for the control flow; its pieces
-                           are pinned above.  Control flow: parity unpinned.
+                           are pinned above, and candidate selection and the
+                           BAD_MINFIT / Z_FITLIMIT rules by the details file.
+                           The fine-grid chi2 itself: parity unpinned beyond
+                           calc_zchi2_batch (the file's spectra are absent).
 calc_deltachi2             reference vectors (test_zfind.py:141-177,
-sort_zfit_dict             test_zscan.py:101-111, test_zfind.py:86-139)
+sort_zfit_dict             test_zscan.py:101-111, test_zfind.py:86-139), and
+                           the details file's deltachi2, SMALL_DELTA_CHI2
+                           bits and row order (exact)
 zfind_assemble             restatement of zfind.py:467-714; its pieces
                            (sort, delta chi2) pinned above, the nminima trim
                            and NODATA / LITTLE_COVERAGE bits by hand-built

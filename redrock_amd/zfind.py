@@ -181,6 +181,8 @@ def zfind(targets, templates, mp_procs=1, nminima=3, archetypes=None, priors=Non
     templates: DistTemplate list.  priors: a Priors object (or None).
     precision: scan precision ('f64', 'f16x3', ...; default
     zscan.default_precision()); the fine fits always run in FP64.
+    chi2_scan: a details file (results.write_zscan) whose scan is used
+    instead of scanning again; the fine fits still run.
     newton=True adds 'znewton'/'chi2newton' columns (Newton refinement on the
     analytic d chi2 / dz beside the parabola fit).
 
@@ -189,14 +191,22 @@ def zfind(targets, templates, mp_procs=1, nminima=3, archetypes=None, priors=Non
     """
     if archetypes:
         raise NotImplementedError('archetype fits are outside the chi2-scan path')
-    if chi2_scan is not None:
-        raise NotImplementedError('reading a saved chi2 scan (rrdetails HDF5) is not supported')
     if isinstance(priors, str):
         priors = Priors(priors)
     local = _local_targets(targets)
     dist = targets if hasattr(targets, 'local_target_ids') else _Local(local)
-    results = calc_zchi2_targets(dist, templates, mp_procs=mp_procs, use_gpu=use_gpu,
-                                 precision=precision)
+    if chi2_scan is None:                                    # zfind.py:289-292
+        results = calc_zchi2_targets(dist, templates, mp_procs=mp_procs, use_gpu=use_gpu,
+                                     precision=precision)
+    else:
+        from .results import read_zscan_redrock
+        saved = read_zscan_redrock(chi2_scan)
+        results = dict()
+        for tg in local:
+            if tg.id not in saved:
+                raise KeyError(f'{chi2_scan}: no scan for target {tg.id}')
+            results[tg.id] = {ft: {k: np.asarray(v, dtype=np.float64) for k, v in d.items()}
+                              for ft, d in saved[tg.id].items()}
 
     if priors is not None:                                   # zfind.py:318-322
         for tid in results:

@@ -94,3 +94,29 @@ def test_zfind_mixed_precision_best_z(gpu):
     assert best_a.keys() == best_b.keys()
     for t in best_a:
         assert abs(best_a[t] - best_b[t]) < 1e-9 * (1 + best_a[t])
+
+
+def test_zfind_resumes_from_details_file(gpu, tmp_path):
+    """zfind -> write_zscan -> zfind(chi2_scan=file) (zfind.py:289-292): the
+    stored float32 scan selects the same minima, and the fine fits, which run
+    again in FP64, give the same table."""
+    from redrock_amd import zfind as zf, results
+    targets, dts, dwave, dist = _setup()
+    print('case: zfind resume from details file, 3 targets', flush=True)
+    allresults, a = zf.zfind(dist, dts, nminima=3, precision='f64')
+    path = str(tmp_path / 'rrdetails.h5')
+    results.write_zscan(path, allresults, a)
+    saved = results.read_zscan_redrock(path)
+    for t in targets:
+        for dt in dts:
+            ft = dt.template.full_type
+            assert np.array_equal(saved[t.id][ft]['zchi2'],
+                                  allresults[t.id][ft]['zchi2'].astype(np.float32))
+    _, b = zf.zfind(dist, dts, nminima=3, chi2_scan=path)
+    for k in ('targetid', 'spectype', 'subtype', 'znum', 'zwarn', 'npixels'):
+        assert np.array_equal(a[k], b[k]), k
+    assert np.allclose(a['z'], b['z'], rtol=1e-12, atol=0)
+    assert np.allclose(a['chi2'], b['chi2'], rtol=1e-12, atol=0)
+    # read_zfit stacks the targets in the order of the zbest table: zfind's order
+    zfit = results.read_zfit(path)
+    assert np.array_equal(zfit['z'], a['z']) and np.array_equal(zfit['targetid'], a['targetid'])
