@@ -20,7 +20,7 @@
 extern "C" {
 #endif
 
-#define RR_ABI_VERSION 3
+#define RR_ABI_VERSION 4
 
 /* Error codes. */
 #define RR_OK 0
@@ -290,6 +290,72 @@ typedef struct {
  * sum(w (f - R T c)^2) (zscan.py:777-780), a check on the forward pass.
  */
 int rr_zscan_grad(const rr_grad_desc *desc, void *stream);
+
+/*
+ * Archetype fit (redrock's archetype mode, the final fit of every minimum).
+ *
+ * Replaces Archetype.get_best_archetype and nearest_neighbour_model
+ * (py/redrock/archetypes.py:241-400) over per_camera_coeff_with_least_square_batch
+ * (py/redrock/zscan.py:245-348) and calc_zchi2_batch with solve_matrices
+ * "BVLS" (py/redrock/zscan.py:626-781,1107-1152): for one target at one
+ * redshift the design matrix is [R a_1 .. R a_na | R L_1 .. R L_nleg] (a_j
+ * rebinned archetypes times the IGM factor, L_l Legendre columns on the
+ * pixel axis), M = A'WA + prior, y = A'Wf, the coefficients solve
+ * min |M c - y|^2 with c_j >= 0 for j < nbound (scipy lsq_linear 'bvls' on
+ * (M, y), zscan.py:1175-1201), and chi2 = sum w (f - A c)^2.
+ */
+#define RR_ARCH_MAX_NA 4     /* archetype columns per problem (n_nearest)      */
+#define RR_ARCH_MAX_NLEG 12  /* Legendre columns (cameras x degree)            */
+
+/*
+ * rr_arch_prep — the z-independent Legendre terms of every target:
+ *   d_wrl (ntarget, nleg, npix_pad)  w * (R L_l)
+ *   d_gll (ntarget, nleg, nleg)      (R L)' W (R L)
+ *   d_yl  (ntarget, nleg)            (R L)' W f
+ *   d_fwf (ntarget)                  f' W f
+ * d_leg (nleg, npix_pad) holds the Legendre columns on the pixel axis
+ * (zero outside the camera a column belongs to); d_rband, d_w, d_f as in
+ * rr_zscan_desc.  nleg may be 0 (then only d_fwf is written).
+ */
+int rr_arch_prep(const double *d_rband, const double *d_w, const double *d_f, int ntarget,
+                 int npix_pad, const double *d_leg, int nleg, double *d_wrl, double *d_gll,
+                 double *d_yl, double *d_fwf, void *stream);
+
+typedef struct {
+    int npair;                  /* (target, redshift) pairs                         */
+    int nprob;                  /* problems per pair                                */
+    int na;                     /* archetype columns per problem, 1..RR_ARCH_MAX_NA */
+    int nleg;                   /* Legendre columns, 0..RR_ARCH_MAX_NLEG            */
+    int nbound;                 /* coefficients 0..nbound-1 are bounded below by 0  */
+    int npix_pad, halo;         /* pixel axis of the targets; halo >= 5 zero rows   */
+    const int *d_pair_target;   /* (npair) target of each pair                      */
+    const double *d_acache;     /* (npair, npix_pad + 2 halo, ld) rebinned
+                                   archetypes: row halo + p is pixel p              */
+    long long ld;               /* leading dimension of d_acache, >= narch          */
+    int narch;                  /* archetypes in d_acache                           */
+    const int *d_columns;       /* (npair, nprob, na) archetype of every column, or
+                                   NULL: problem q uses archetype q (na == 1)       */
+    const double *d_rband, *d_w, *d_wf;             /* per target, rr_zscan_desc    */
+    const double *d_wrl, *d_gll, *d_yl, *d_fwf;     /* rr_arch_prep                 */
+    const double *d_prior;      /* (na+nleg)^2 added to M, or NULL                  */
+    const double *d_wave;       /* (npix_pad) pixel wavelengths; NULL if nlines 0  */
+    const double *d_zigm;       /* (npair) redshift of the IGM factor; NaN = none   */
+    const rr_lyman_line *lines; /* host array                                       */
+    int nlines;
+    double *d_coeff;            /* (npair, nprob, na + nleg) out                     */
+    double *d_chi2;             /* (npair, nprob) out                                */
+    void *d_work;               /* rr_arch_workspace_bytes bytes                     */
+    long long work_bytes;
+} rr_arch_desc;
+
+long long rr_arch_workspace_bytes(const rr_arch_desc *desc);
+
+/*
+ * rr_arch_fit — normal equations, bounded solve and chi2 of every problem.
+ * A basis column with zero weighted norm (and no prior) gets coefficient 0,
+ * the minimum-norm choice of the least-squares solve.
+ */
+int rr_arch_fit(const rr_arch_desc *desc, void *stream);
 
 #ifdef __cplusplus
 }

@@ -39,6 +39,18 @@ calc_deltachi2             reference vectors (test_zfind.py:141-177,
 sort_zfit_dict             test_zscan.py:101-111, test_zfind.py:86-139), and
                            the details file's deltachi2, SMALL_DELTA_CHI2
                            bits and row order (exact)
+solve_matrices_bvls        scipy lsq_linear 'bvls' (the reference's routine);
+                           pinned against brute-force active sets with the
+                           KKT conditions and against np.linalg.solve where
+                           no bound is active (tests/test_archetypes.py)
+get_best_archetype,        restatement of archetypes.py:241-400 and
+per_camera_coeff           zscan.py:245-348 over the pinned rebin, IGM and
+                           BVLS pieces; Legendre rows by scipy.special;
+                           the coefficient column layout pinned by the
+                           reference's split_archetype_coeff vectors
+                           (test_archetypes.py:31-57).  The nearest-
+                           neighbour form without per_camera: parity
+                           unpinned (the reference stops there).
 zfind_assemble             restatement of zfind.py:467-714; its pieces
                            (sort, delta chi2) pinned above, the nminima trim
                            and NODATA / LITTLE_COVERAGE bits by hand-built
@@ -289,8 +301,15 @@ def solve_matrices_pca(M, y):
     return np.linalg.solve(M, y.reshape(y.shape + (1,))).reshape(y.shape)
 
 
-def calc_zchi2_batch(spectra, tdata, weights, flux, wflux, nz, nbasis, prior=None):
-    """CPU branch of zscan.py:626-781 (PCA solve)."""
+def solve_matrices_bvls(M, y, bounds):
+    """zscan.py:1175-1201 (BVLS branch): scipy's lsq_linear(M, y, bounds,
+    method='bvls') -- the library routine the reference calls."""
+    from scipy.optimize import lsq_linear
+    return lsq_linear(M, y, bounds=bounds, method='bvls').x
+
+
+def calc_zchi2_batch(spectra, tdata, weights, flux, wflux, nz, nbasis, prior=None, bounds=None):
+    """CPU branch of zscan.py:626-781: PCA solve, or BVLS when bounds (2, nbasis) are given."""
     zchi2 = np.zeros(nz)
     if weights.sum() == 0:
         zchi2[:] = HUGE_CHI2
@@ -304,7 +323,10 @@ def calc_zchi2_batch(spectra, tdata, weights, flux, wflux, nz, nbasis, prior=Non
         if prior is not None:
             M += prior
         try:
-            zcoeff[i, :] = solve_matrices_pca(M, y)
+            if bounds is None:
+                zcoeff[i, :] = solve_matrices_pca(M, y)
+            else:
+                zcoeff[i, :] = solve_matrices_bvls(M, y, bounds)
         except np.linalg.LinAlgError:
             zchi2[i] = HUGE_CHI2
             continue
@@ -594,3 +616,127 @@ def zfind_assemble(allresults, target_ids, maxcoeff, nminima=3):
     if len(np.unique(out['ncoeff'])) == 1:
         out['coeff'] = out['coeff'][:, 0:int(np.unique(out['ncoeff'])[0])]
     return out
+
+
+# ---------------------------------------------------------------------------
+# archetype fit (archetypes.py, zscan.py:245-348)
+# ---------------------------------------------------------------------------
+def reduced_wavelength(wave):
+    """utils.py:260-271."""
+    return 2 * (wave - wave.min()) / (wave.max() - wave.min()) - 1
+
+
+def legendre(dwave, nleg):
+    """targets.py:237-254: scipy.special.legendre(i) of the reduced wavelength."""
+    import scipy.special
+    return {hs: np.array([scipy.special.legendre(i)(reduced_wavelength(w)) for i in range(nleg)])
+            for hs, w in dwave.items()}
+
+
+def prior_on_coeffs(n_nbh, deg_legendre, sigma, ncamera):
+    """fitz.py:113-130."""
+    nbasis = n_nbh + deg_legendre * ncamera
+    prior = np.zeros((nbasis, nbasis))
+    np.fill_diagonal(prior, 1 / sigma ** 2)
+    for i in range(n_nbh):
+        prior[i][i] = 0.
+    return prior
+
+
+def per_camera_coeff(spectra, bands, tdata, weights, flux, wflux, nleg, narch, n_nbh, prior):
+    """CPU branch of per_camera_coeff_with_least_square_batch (zscan.py:245-348),
+    method 'BVLS': Legendre block i of the i-th grid, blocks returned in band order."""
+    ncam = len(bands)
+    nbasis = n_nbh + nleg * ncam
+    bounds = np.zeros((2, nbasis))
+    bounds[0][n_nbh:] = -np.inf
+    bounds[1] = np.inf
+    zzchi2 = np.zeros(narch)
+    zzcoeff = np.zeros((narch, nbasis))
+    for j in range(narch):
+        tdata2 = dict()
+        for i, hs in enumerate(tdata):
+            t2 = np.zeros((tdata[hs].shape[1], nbasis))
+            for k in range(n_nbh):
+                t2[:, k] = tdata[hs][j, :, k]
+            for k in range(n_nbh, n_nbh + nleg):
+                t2[:, k + nleg * i] = tdata[hs][j, :, k]
+            tdata2[hs] = t2[None, :, :]
+        c2, cf = calc_zchi2_batch(spectra, tdata2, weights, flux, wflux, 1, nbasis, prior=prior,
+                                  bounds=bounds)
+        zzchi2[j] = c2[0]
+        zzcoeff[j] = cf[0]
+    alpha = zzcoeff[:, :n_nbh]
+    if nleg >= 1:
+        split = np.split(zzcoeff[:, n_nbh:], ncam, axis=1)
+        old = {band: split[i] for i, band in enumerate(bands)}
+        coeff = np.concatenate([alpha] + [old[b] for b in sorted(bands)], axis=1)
+    else:
+        coeff = alpha
+    return zzchi2, coeff
+
+
+def archetype_scan(spectra, bands, arch_wave, arch_flux, z, nleg, per_camera, prior=None, trans=None):
+    """Every archetype alone at redshift z (archetypes.py:342-389): (zzchi2,
+    zzcoeff, binned) with binned[wavehash] (nwave, narch) after the IGM factor."""
+    dwave = {s.wavehash: s.wave for s in spectra}
+    weights, flux, wflux = spectral_data(spectra)
+    leg = legendre(dwave, nleg)
+    narch = arch_flux.shape[0]
+    binned = {hs: trapz_rebin((1. + z) * arch_wave, arch_flux, w) for hs, w in dwave.items()}
+    for hs in dwave:
+        if trans is not None and trans.get(hs) is not None:
+            binned[hs] = binned[hs] * np.asarray(trans[hs])[:, None]
+    tdata = dict()
+    for hs in dwave:
+        tdata[hs] = binned[hs].T[:, :, None]
+        if nleg > 0:
+            tdata[hs] = np.append(tdata[hs], np.tile(leg[hs].T[None, :, :], (narch, 1, 1)), axis=2)
+    if per_camera:
+        zzchi2, zzcoeff = per_camera_coeff(spectra, bands, tdata, weights, flux, wflux, nleg, narch, 1, prior)
+    else:
+        nbasis = 1 + nleg
+        bounds = np.zeros((2, nbasis))
+        bounds[0][1:] = -np.inf
+        bounds[1] = np.inf
+        zzchi2, zzcoeff = calc_zchi2_batch(spectra, tdata, weights, flux, wflux, narch, nbasis, bounds=bounds)
+    return zzchi2, zzcoeff, binned
+
+
+def get_best_archetype(spectra, bands, arch_wave, arch_flux, fulltypes, z, nleg, per_camera,
+                       n_nearest=None, prior=None, trans=None):
+    """archetypes.py:316-400 with nearest_neighbour_model (archetypes.py:241-314).
+
+    trans: {wavehash: (nwave,) transmission or None}.  Without per_camera the
+    n_nearest combination is solved like the single archetypes (bounded, no
+    prior): the reference passes no solve method there and stops with an
+    AttributeError.
+    """
+    dwave = {s.wavehash: s.wave for s in spectra}
+    weights, flux, wflux = spectral_data(spectra)
+    leg = legendre(dwave, nleg)
+    nprior = prior
+    if n_nearest is not None and prior is not None:
+        prior = prior[n_nearest - 1:, n_nearest - 1:]
+    zzchi2, zzcoeff, binned = archetype_scan(spectra, bands, arch_wave, arch_flux, z, nleg, per_camera,
+                                             prior, trans)
+    if n_nearest is None:
+        i = np.argmin(zzchi2)
+        return zzchi2[i], zzcoeff[i], fulltypes[i]
+    ibest = np.argsort(zzchi2)[0:n_nearest]
+    td = dict()
+    for hs in dwave:
+        td[hs] = binned[hs][:, ibest][None, :, :]
+        if nleg > 0:
+            td[hs] = np.append(td[hs], leg[hs].T[None, :, :], axis=2)
+    if per_camera:
+        c2, cf = per_camera_coeff(spectra, bands, td, weights, flux, wflux, nleg, 1, n_nearest, nprior)
+    else:
+        nb = n_nearest + nleg
+        bounds = np.zeros((2, nb))
+        bounds[0][n_nearest:] = -np.inf
+        bounds[1] = np.inf
+        c2, cf = calc_zchi2_batch(spectra, td, weights, flux, wflux, 1, nb, bounds=bounds)
+    rrtype = fulltypes[0].split(':::')[0]
+    subs = [fulltypes[i].split(':::')[1] for i in ibest]
+    return c2[0], cf[0], rrtype + ':::' + ';'.join(subs)

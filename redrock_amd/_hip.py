@@ -11,7 +11,7 @@ import os
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, 'libredrock_hip.so')
 
-RR_ABI_VERSION = 3
+RR_ABI_VERSION = 4
 RR_F64, RR_F32, RR_F16, RR_BF16 = 0, 1, 2, 3
 RR_PREC_F64, RR_PREC_F16, RR_PREC_BF16, RR_PREC_F32, RR_PREC_F16X3 = 0, 1, 2, 3, 4
 RR_STATUS_OK, RR_STATUS_SINGULAR = 0, 1
@@ -23,7 +23,9 @@ PRECISIONS = {'f64': RR_PREC_F64, 'f16': RR_PREC_F16, 'bf16': RR_PREC_BF16, 'f32
 
 EXPORTS = ('rr_abi_version', 'rr_last_error', 'rr_rebin', 'rr_zscan',
            'rr_zscan_workspace_bytes', 'rr_pack_mixed', 'rr_zscan_tiles', 'rr_zscan_tile_table',
-           'rr_zscan_grad')
+           'rr_zscan_grad', 'rr_arch_prep', 'rr_arch_workspace_bytes', 'rr_arch_fit')
+RR_ARCH_MAX_NA = 4
+RR_ARCH_MAX_NLEG = 12
 
 _vp = ctypes.c_void_p
 _i = ctypes.c_int
@@ -53,6 +55,19 @@ class ZscanDesc(ctypes.Structure):
 
 
 RR_STAGE_STATS, RR_STAGE_MAIN, RR_STAGE_FINISH = 1, 2, 4
+
+
+class ArchDesc(ctypes.Structure):
+    _fields_ = [
+        ('npair', _i), ('nprob', _i), ('na', _i), ('nleg', _i), ('nbound', _i),
+        ('npix_pad', _i), ('halo', _i),
+        ('d_pair_target', _vp), ('d_acache', _vp), ('ld', _ll), ('narch', _i), ('d_columns', _vp),
+        ('d_rband', _vp), ('d_w', _vp), ('d_wf', _vp),
+        ('d_wrl', _vp), ('d_gll', _vp), ('d_yl', _vp), ('d_fwf', _vp),
+        ('d_prior', _vp), ('d_wave', _vp), ('d_zigm', _vp),
+        ('lines', ctypes.POINTER(LymanLine)), ('nlines', _i),
+        ('d_coeff', _vp), ('d_chi2', _vp), ('d_work', _vp), ('work_bytes', _ll),
+    ]
 
 
 class HipError(RuntimeError):
@@ -87,6 +102,12 @@ def lib():
     L.rr_zscan_tiles.argtypes = [ctypes.POINTER(_i), _i]
     L.rr_zscan_tile_table.restype = _i
     L.rr_zscan_tile_table.argtypes = [_i, ctypes.POINTER(_i), _i]
+    L.rr_arch_prep.restype = _i
+    L.rr_arch_prep.argtypes = [_vp, _vp, _vp, _i, _i, _vp, _i, _vp, _vp, _vp, _vp, _vp]
+    L.rr_arch_workspace_bytes.restype = _ll
+    L.rr_arch_workspace_bytes.argtypes = [ctypes.POINTER(ArchDesc)]
+    L.rr_arch_fit.restype = _i
+    L.rr_arch_fit.argtypes = [ctypes.POINTER(ArchDesc), _vp]
     v = L.rr_abi_version()
     if v != RR_ABI_VERSION:
         raise HipError(f'libredrock_hip ABI {v}, expected {RR_ABI_VERSION}')

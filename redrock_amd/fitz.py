@@ -97,6 +97,14 @@ class _Refiner(object):
             for k, i in enumerate(idx):
                 self.slot[i] = (sig, k)
 
+    def batch_of(self, targets):
+        """The TargetBatch of all targets when they share one layout, else None."""
+        if len(self._tb) == 1:
+            tb = next(iter(self._tb.values()))
+            if tb.nt == len(targets):
+                return tb
+        return None
+
     @staticmethod
     def _cache_budget():
         """HBM bytes for one fine-grid template cache (nbasis x padded pixels x
@@ -190,6 +198,7 @@ class _Refiner(object):
                               [pairs[j][0] for j in sel], 1)
         for k, j in enumerate(sel):
             out[j]['coeff'] = cf[k, 0]
+            out[j]['zigm'] = out[j]['zmin']          # transmission at zmin (fitz.py:266-284)
         for j in good:
             if 'coeff' in out[j]:
                 continue
@@ -199,6 +208,7 @@ class _Refiner(object):
                 out[j]['coeff'] = np.zeros(self.template.nbasis)
                 out[j]['zwarn'] |= ZW.Z_FITLIMIT
                 out[j]['zwarn'] |= ZW.BAD_MINFIT
+                out[j]['zigm'] = np.nan                   # no transmission computed there
             else:
                 out[j]['error'] = ValueError('edges must be within input x range')
         return out
@@ -241,9 +251,18 @@ def _finalize(template, target, redshifts, nminima, cands, get):
         if np.any(np.abs(dv) < max_velo_diff):
             continue
         results.append(dict(z=zbest, zerr=zerr, zwarn=zwarn, chi2=chi2min, zz=zz,
-                            zzchi2=zzchi2, coeff=r['coeff'], fitmethod=template.method))
+                            zzchi2=zzchi2, coeff=r['coeff'], fitmethod=template.method,
+                            _zigm=r.get('zigm', np.nan)))
+    return results
+
+
+def _stack(results, target):
+    """Sort candidates by chi2 and stack them into the reference's dict of arrays
+    (fitz.py:359-395)."""
     ii = np.argsort([tmp['chi2'] for tmp in results])
     results = [results[i] for i in ii]
+    for r in results:
+        r.pop('_zigm', None)
     if len(results) == 0:
         xfloat = np.zeros((0, 1), dtype=float)
         xint = np.zeros((0, 1), dtype=int)
@@ -257,17 +276,23 @@ def _finalize(template, target, redshifts, nminima, cands, get):
 
 
 def fitz_targets(zchi2, redshifts, targets, template, nminima=3, zminfit_npoints=15,
-                 precision='f64', speculate=2, newton=False):
+                 precision='f64', speculate=2, newton=False, archetype=None, deg_legendre=None,
+                 per_camera=False, n_nearest=None, prior_sigma=None):
     """fitz for every target of one template (zchi2: (nt, nz) incl. penalty).
 
     Returns a list with the reference's per-target result dict.  With
     newton=True every minimum is also refined by Newton iterations on the
     analytic d chi2/dz inside its fine grid (redrock_amd.grad.newton_refine)
     and the dicts gain 'znewton' and 'chi2newton' beside the parabola's 'z'
-    and 'chi2'.
+    and 'chi2'.  With an archetype (redrock_amd.archetypes.Archetype) every
+    minimum's chi2 and coefficients come from the archetype fit at its
+    redshift (fitz.py:329-357): deg_legendre Legendre polynomials (per
+    camera with per_camera), prior_sigma on their coefficients, n_nearest
+    archetypes combined.
     """
     out = _fitz_targets(zchi2, redshifts, targets, template, nminima, zminfit_npoints,
-                        precision, speculate)
+                        precision, speculate, archetype, deg_legendre, per_camera, n_nearest,
+                        prior_sigma)
     if newton:
         _add_newton(out, targets, template)
     return out
@@ -295,7 +320,8 @@ def _add_newton(out, targets, template):
 
 
 def _fitz_targets(zchi2, redshifts, targets, template, nminima, zminfit_npoints, precision,
-                  speculate):
+                  speculate, archetype=None, deg_legendre=None, per_camera=False, n_nearest=None,
+                  prior_sigma=None):
     zchi2 = np.atleast_2d(np.asarray(zchi2, dtype=np.float64))
     assert zchi2.shape == (len(targets), len(redshifts))
     nz = 15 if zminfit_npoints is None else zminfit_npoints
@@ -322,7 +348,44 @@ def _fitz_targets(zchi2, redshifts, targets, template, nminima, zminfit_npoints,
                      if int(c) not in done[t]])
             return done[t][imin]
         out.append(_finalize(template, tg, redshifts, nminima, cands[t], get))
-    return out
+    if archetype is not None:
+        _archetype_pass(out, targets, template, archetype, ref, deg_legendre, per_camera, n_nearest,
+                        prior_sigma)
+    return [_stack(r, tg) for r, tg in zip(out, targets)]
+
+
+def _archetype_pass(rows, targets, template, archetype, ref, deg_legendre, per_camera, n_nearest,
+                    prior_sigma):
+    """Replace every candidate's fit by the archetype fit at its redshift
+    (fitz.py:333-357), all candidates of all targets in one ArchetypeFit."""
+    from .archetypes import ArchetypeFit, prior_on_coeffs, split_archetype_coeff
+    from .igm import lyman_table
+    from .templates import parse_fulltype
+    nleg = 0 if deg_legendre is None else int(deg_legendre)
+    ncamera = len({s.wavehash for s in targets[0].spectra}) if per_camera else 1
+    prior = None
+    if prior_sigma is not None:
+        prior = prior_on_coeffs(1 if n_nearest is None else n_nearest, nleg, prior_sigma, ncamera)
+    owner = [(t, k) for t, r in enumerate(rows) for k in range(len(r))]
+    if not owner:
+        return
+    for tg in targets:
+        tg.legendre(nleg)
+    fit = ArchetypeFit(archetype, targets, per_camera=per_camera, nleg=nleg, prior=prior,
+                       n_nearest=n_nearest, ncamera=ncamera, batch=ref.batch_of(targets))
+    tidx = np.array([t for t, _ in owner])
+    zb = np.array([rows[t][k]['z'] for t, k in owner], dtype=np.float64)
+    zigm = np.array([rows[t][k]['_zigm'] for t, k in owner], dtype=np.float64)
+    lines = lyman_table(template.igm_model) or []
+    chi2, coeff, ftype = fit.run(tidx, zb, zigm=zigm, lines=lines)
+    for (t, k), c2, cf, ft in zip(owner, chi2, coeff, ftype):
+        r = rows[t][k]
+        archcoeff, legcoeff = split_archetype_coeff(parse_fulltype(ft)[1], cf, ncamera, nleg)
+        if per_camera:
+            legcoeff = [np.vstack(legcoeff)]
+        r.update(chi2=c2, pca_coeff=r['coeff'], coeff=archcoeff, legcoeff=legcoeff, fulltype=ft,
+                 fitmethod=archetype.method, pca_spectype=template._rrtype,
+                 pca_subtype=template.sub_type)
 
 
 def fitz(zchi2, redshifts, target, template, nminima=3, archetype=None, use_gpu=False,
@@ -330,7 +393,7 @@ def fitz(zchi2, redshifts, target, template, nminima=3, archetype=None, use_gpu=
          prior_sigma=None):
     """Refine the redshift around up to nminima minima (fitz.py:133-395)."""
     assert len(zchi2) == len(redshifts)
-    if archetype is not None:
-        raise NotImplementedError('archetype fits are outside the chi2-scan path')
     return fitz_targets(np.asarray(zchi2)[None, :], redshifts, [target], template,
-                        nminima=nminima, zminfit_npoints=zminfit_npoints)[0]
+                        nminima=nminima, zminfit_npoints=zminfit_npoints, archetype=archetype,
+                        deg_legendre=deg_legendre, per_camera=per_camera, n_nearest=n_nearest,
+                        prior_sigma=prior_sigma)[0]

@@ -73,6 +73,9 @@ def write_zscan(filename, zscan, zfit, clobber=False):
         os.makedirs(outdir)
 
     zfit = {k: np.asarray(v) for k, v in zfit.items()}
+    for col, v in zfit.items():
+        if v.dtype.kind == 'O':       # e.g. pca_subtype of templates made without a subtype (None)
+            zfit[col] = np.array(['' if x is None else str(x) for x in v.ravel()]).reshape(v.shape)
     for col in _BYTES_COLUMNS:
         if col in zfit and zfit[col].dtype.kind == 'U':
             zfit[col] = np.char.encode(zfit[col], 'ascii')
@@ -147,7 +150,8 @@ def read_zscan(filename, select_targetids=None, upper=False, nozfit=False):
     for key in dict.fromkeys(keys):
         rows = np.array([k == key for k in keys])
         ft = make_fulltype(key[1], key[2])
-        zscan[key[0]][ft]['zfit'] = {k: v[rows] for k, v in zfit.items()}
+        if ft in zscan[key[0]]:       # archetype fits name no scanned template: table only
+            zscan[key[0]][ft]['zfit'] = {k: v[rows] for k, v in zfit.items()}
     return zscan, zfit
 
 

@@ -268,3 +268,16 @@ def mixed_targets(ntarget, seed=0, snr=1.0, mask_frac=(0.0, 0.3), templates=None
     for new_id, i in enumerate(order):
         targets.append(Target(new_id, tg[i].spectra))
     return targets, [types[i] for i in order], np.array([zt[i] for i in order])
+
+
+def archetypes_from(template, narch, seed=7, subtypes=None):
+    """narch positive archetype spectra on the template's wavelength grid:
+    |random mixtures of its basis| (an Archetype held in memory)."""
+    from .archetypes import Archetype
+    rng = np.random.default_rng(seed)
+    c = rng.normal(size=(narch, template.nbasis))
+    flux = np.abs(c @ template.flux)
+    if subtypes is None:
+        subtypes = [('ELG', 'LRG', 'BGS')[i % 3] for i in range(narch)]
+    return Archetype.from_arrays(template.template_type, template.wave, flux, subtypes=subtypes,
+                                 igm_model=template.igm_model)

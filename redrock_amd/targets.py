@@ -98,6 +98,21 @@ class Target(object):
         self.meta = dict() if meta is None else meta
         self.bands = [s.band for s in spectra]
         self._device = None   # cached device-side arrays, see zscan.TargetBatch
+        self._legendre = None
+        self.nleg = 0
+
+    def legendre(self, nleg, use_gpu=False):
+        """{wavehash: (nleg, nwave)} Legendre polynomials of degree 0..nleg-1
+        of each camera's reduced wavelength, cached (targets.py:237-254)."""
+        from .utils import legendre_rows
+        if self._legendre is None or self.nleg != nleg:
+            dwave = {s.wavehash: s.wave for s in self.spectra}
+            self._legendre = {hs: legendre_rows(w, nleg) for hs, w in dwave.items()}
+            self.nleg = nleg
+        if use_gpu:
+            import torch
+            return {hs: torch.as_tensor(v, device='cuda') for hs, v in self._legendre.items()}
+        return self._legendre
 
     def gpu_spectral_data(self):
         """Concatenated (weights, flux, wflux) as device tensors (targets.py:256-263)."""

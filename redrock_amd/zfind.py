@@ -185,12 +185,14 @@ def zfind(targets, templates, mp_procs=1, nminima=3, archetypes=None, priors=Non
     instead of scanning again; the fine fits still run.
     newton=True adds 'znewton'/'chi2newton' columns (Newton refinement on the
     analytic d chi2 / dz beside the parabola fit).
+    archetypes: {spectype: Archetype} (All_archetypes(...).archetypes): the
+    minima of those types are fit by their archetypes (redrock_amd.archetypes)
+    with deg_legendre Legendre polynomials (per camera when per_camera),
+    prior_sigma on their coefficients and n_nearest archetypes combined.
 
     Returns (allresults, allzfit): allresults[targetid][fulltype] =
     dict(redshifts, zchi2, penalty, zcoeff); allzfit the dict of columns.
     """
-    if archetypes:
-        raise NotImplementedError('archetype fits are outside the chi2-scan path')
     if isinstance(priors, str):
         priors = Priors(priors)
     local = _local_targets(targets)
@@ -221,22 +223,56 @@ def zfind(targets, templates, mp_procs=1, nminima=3, archetypes=None, priors=Non
             continue
         chi2 = np.stack([results[tg.id][ft]['zchi2'] + results[tg.id][ft]['penalty']
                          for tg in local])
+        archetype = None
+        if archetypes and t.template.template_type in archetypes:
+            archetype = archetypes[t.template.template_type]
         fits = fitz_targets(chi2, t.template.redshifts, local, t.template, nminima=nminima,
-                            zminfit_npoints=zminfit_npoints, newton=newton)
+                            zminfit_npoints=zminfit_npoints, newton=newton, archetype=archetype,
+                            deg_legendre=deg_legendre, per_camera=bool(per_camera),
+                            n_nearest=n_nearest, prior_sigma=prior_sigma)
         for tg, zfit in zip(local, fits):
             results[tg.id][ft]['zfit'] = zfit
 
     for tg in local:
         results[tg.id]['meta'] = getattr(tg, 'meta', dict())
 
-    allzfit = _assemble(results, _all_ids(targets), templates, nminima)
+    arch = None
+    if archetypes:
+        if ncamera is None:
+            ncamera = len({s.wavehash for s in local[0].spectra}) if local else 1
+        arch = dict(spectypes=list(archetypes.keys()), per_camera=bool(per_camera),
+                    deg_legendre=0 if deg_legendre is None else int(deg_legendre),
+                    n_nearest=n_nearest, ncamera=ncamera)
+    allzfit = _assemble(results, _all_ids(targets), templates, nminima, arch=arch)
     for tid in results:
         results[tid].pop('meta', None)
     return results, allzfit
 
 
-def _assemble(allresults, target_ids, templates, nminima):
-    """Per-target candidate tables -> one table (zfind.py:467-714, no archetypes)."""
+def _pad_cols(val, width):
+    """Columns of a 2-D numeric array padded with zeros / cut to `width` (zfind.py:537-546)."""
+    val = np.asarray(val)
+    if val.ndim < 2 or val.shape[1] == width or val.dtype.kind in 'US':
+        return val
+    if val.shape[1] < width:
+        val = np.pad(val, ((0, 0), (0, width - val.shape[1])), constant_values=0)
+    return val[:, :width]
+
+
+def _assemble(allresults, target_ids, templates, nminima, arch=None):
+    """Per-target candidate tables -> one table (zfind.py:467-714).
+
+    arch: None, or the archetype settings (spectypes with archetypes,
+    per_camera, deg_legendre, n_nearest, ncamera) for the archetype-mode
+    columns: per-row spectype/subtype from the archetype fulltype, pca_*
+    columns, legcoeff, NEGATIVE_MODEL and the COEFF width (zfind.py:476-693).
+    """
+    from .utils import extra_columns_in_archetype_mode
+    pca_map, zero_like_keys = extra_columns_in_archetype_mode()
+    all_spectype = []
+    for t in (templates if arch is not None else []):
+        if t.template.template_type not in all_spectype:
+            all_spectype.append(t.template.template_type)
     rows = []
     for tid in target_ids:
         tzfit = []
@@ -244,15 +280,26 @@ def _assemble(allresults, target_ids, templates, nminima):
             if fulltype == 'meta':
                 continue
             tmp = allresults[tid][fulltype].pop('zfit')
-            spectype, subtype = parse_fulltype(fulltype)
+            if arch is not None and 'fulltype' in tmp:
+                parsed = [parse_fulltype(el[0]) for el in tmp.pop('fulltype')]
+                spectype = [p[0] for p in parsed]
+                subtype = [p[1] for p in parsed]
+            else:
+                spectype, subtype = parse_fulltype(fulltype)
             nmin = len(tmp['chi2'])
             if nmin == 0:
                 print(f'WARNING: no {fulltype} chi2 vs. z minima for targetid {tid}')
                 continue
-            tmp['spectype'] = np.full((nmin, 1), spectype)
-            tmp['subtype'] = np.full((nmin, 1), subtype)
+            if isinstance(spectype, str):
+                tmp['spectype'] = np.full((nmin, 1), spectype)
+                tmp['subtype'] = np.full((nmin, 1), subtype)
+                star = spectype.upper() == 'STAR'
+            else:
+                tmp['spectype'] = np.array([spectype]).reshape((nmin, 1))
+                tmp['subtype'] = np.array([subtype]).reshape((nmin, 1))
+                star = spectype[0].upper() == 'STAR'
             tmp['ncoeff'] = np.full((nmin, 1), tmp['coeff'].shape[1])
-            mv = constants.max_velo_diff_star if spectype.upper() == 'STAR' else constants.max_velo_diff
+            mv = constants.max_velo_diff_star if star else constants.max_velo_diff
             tmp['max_velo_diff'] = np.full((nmin, 1), mv)
             tzfit.append(tmp)
         if not tzfit:
@@ -264,10 +311,33 @@ def _assemble(allresults, target_ids, templates, nminima):
                 tmp['coeff'] = np.append(tmp['coeff'], np.zeros((tmp['coeff'].shape[0], n)), axis=1)
         tz = dict()
         for k in tzfit[0].keys():
-            tz[k] = np.vstack([tmp[k] for tmp in tzfit])
+            ref = tzfit[0][k]
+            vals, val = [], None
+            for tmp in tzfit:
+                if k in tmp:
+                    val = tmp[k]
+                    if arch is not None and k in pca_map:
+                        val = _pad_cols(val, ref.shape[1])
+                elif arch is not None and k in pca_map and pca_map[k] in tmp:
+                    val = _pad_cols(tmp[pca_map[k]], ref.shape[1])
+                elif k in zero_like_keys:
+                    val = np.zeros((len(tmp['chi2']),) + np.shape(ref)[1:], dtype=np.asarray(ref).dtype)
+                vals.append(val)
+            tz[k] = np.vstack(vals)
             if k != 'coeff' and tz[k].shape[1] == 1:
                 tz[k] = tz[k].flatten()
         tz['targetid'] = np.full(len(tz['chi2']), tid)
+        if arch is not None:                                   # zfind.py:558-579
+            nn = arch['n_nearest']
+            coeff = tz['coeff'] if tz['coeff'].ndim > 1 else tz['coeff'].reshape(-1, 1)
+            if len(arch['spectypes']) == len(all_spectype):
+                ibad = coeff[:, 0] <= 0. if nn is None else np.any(coeff[:, :nn] < 0, axis=1)
+            else:
+                ibad = np.isin(tz['spectype'], np.array(arch['spectypes']))
+                for k in np.where(ibad)[0]:
+                    if (coeff[k, 0] >= 0.) if nn is None else np.all(coeff[k, :nn] >= 0):
+                        ibad[k] = False
+            tz['zwarn'][ibad] |= ZW.NEGATIVE_MODEL
         tz['zwarn'][tz['npixels'] == 0] |= ZW.NODATA
         tz['zwarn'][tz['npixels'] < 10 * tz['ncoeff']] |= ZW.LITTLE_COVERAGE
         sort_zfit_dict(tz)
@@ -292,7 +362,9 @@ def _assemble(allresults, target_ids, templates, nminima):
         n = maxnc - r['coeff'].shape[1]
         if n > 0:
             r['coeff'] = np.append(r['coeff'], np.zeros((r['coeff'].shape[0], n)), axis=1)
-    names = ['targetid'] + [k for k in rows[0].keys() if k != 'targetid']
+    # targetid first, pca_* columns last (zfind.py:624-631)
+    keys = [k for k in rows[0].keys() if k != 'targetid']
+    names = ['targetid'] + [k for k in keys if 'pca' not in k.lower()] + [k for k in keys if 'pca' in k.lower()]
     allzfit = {k: np.concatenate([r[k] for r in rows]) for k in names}
     # target metadata columns (zfind.py:598-636): '<key>_datatype' entries give dtypes
     meta0 = allresults[allzfit['targetid'][0]].get('meta', dict())
@@ -305,6 +377,8 @@ def _assemble(allresults, target_ids, templates, nminima):
         f"<U{max(20, max((len(s) for s in allzfit['subtype']), default=0))}")
     allzfit['spectype'] = allzfit['spectype'].astype('<U6')
     maxcoeff = max(t.template.nbasis for t in templates)
+    if arch is not None and arch['per_camera']:                # zfind.py:677-683
+        maxcoeff = max(maxcoeff, arch['ncamera'] * arch['deg_legendre'] + (arch['n_nearest'] or 1))
     coeff = allzfit['coeff']
     if coeff.shape[1] != maxcoeff:
         c = np.zeros((coeff.shape[0], maxcoeff), dtype=coeff.dtype)

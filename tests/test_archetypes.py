@@ -1,0 +1,117 @@
+"""Archetype mode, host side: file reader, helpers, and the pins of the
+oracle's bounded solve (tests that need no GPU)."""
+import itertools
+import os
+import shutil
+
+import numpy as np
+import pytest
+import scipy.special
+
+from oracle import rr_oracle as orc
+from redrock_amd import archetypes
+from redrock_amd.utils import legendre_rows, reduced_wavelength
+
+GOLDEN = os.path.join(os.path.dirname(__file__), 'golden')
+ARCHFILE = os.path.join(GOLDEN, 'rrarchetype-galaxy-test.fits')
+
+
+def _bvls_bruteforce(M, y, nbound):
+    """Unique minimiser of |Mx - y|^2 with x_j >= 0 for j < nbound: the active
+    set whose free least-squares solution is feasible and whose fixed columns
+    have a non-negative gradient (KKT)."""
+    nb = len(y)
+    for k in range(nbound + 1):
+        for S in itertools.combinations(range(nbound), k):
+            F = [j for j in range(nb) if j not in S]
+            x = np.zeros(nb)
+            x[F] = np.linalg.lstsq(M[:, F], y, rcond=None)[0]
+            if any(x[j] < 0 for j in F if j < nbound):
+                continue
+            g = M.T @ (M @ x - y)
+            if all(g[j] >= -1e-9 * np.abs(M).sum() * np.abs(y).sum() for j in S):
+                return x
+    raise AssertionError('no KKT point')
+
+
+@pytest.mark.parametrize('seed', range(12))
+def test_oracle_bvls_is_the_kkt_point(seed):
+    """scipy lsq_linear 'bvls' on (M, y) -- the reference's solve -- equals the
+    brute-force KKT point; with no active bound it is np.linalg.solve."""
+    rng = np.random.default_rng(seed)
+    nbound = 1 + seed % 3
+    nb = nbound + rng.integers(0, 7)
+    A = rng.normal(size=(200, nb))
+    A[:, :nbound] = np.abs(A[:, :nbound]) + 0.1 * rng.normal(size=(200, nbound))
+    w = rng.uniform(0.5, 2.0, 200)
+    M = A.T @ (w[:, None] * A)
+    if seed % 2:
+        M[nbound:, nbound:] += np.eye(nb - nbound) * 100.0           # prior_on_coeffs shape
+    truth = rng.normal(size=nb)
+    truth[:nbound] *= (-1) ** seed                                     # some bounds active
+    y = M @ truth + rng.normal(size=nb)
+    bounds = np.zeros((2, nb))
+    bounds[0][nbound:] = -np.inf
+    bounds[1] = np.inf
+    got = orc.solve_matrices_bvls(M, y, bounds)
+    want = _bvls_bruteforce(M, y, nbound)
+    assert np.allclose(got, want, rtol=1e-9, atol=1e-9 * np.abs(want).max())
+    if np.all(np.linalg.solve(M, y)[:nbound] >= 0):
+        assert np.allclose(got, np.linalg.solve(M, y), rtol=1e-9, atol=1e-12)
+
+
+def test_split_archetype_coeff_reference_vectors():
+    """test_archetypes.py:31-57 of the reference."""
+    f = archetypes.split_archetype_coeff
+    assert f('ELG_12', [1, 2, 3, 4], nbands=3, nleg=1) == ([1], [[2], [3], [4]])
+    assert f('ELG_12', [1, 2, 3, 4, 5, 6, 7], nbands=3, nleg=2) == ([1], [[2, 3], [4, 5], [6, 7]])
+    assert f('ELG_12', [1, 2, 3, 4, 5, 6, 7, 0, 0, 0], nbands=3, nleg=2) == ([1], [[2, 3], [4, 5], [6, 7]])
+    assert f('ELG_12;LRG_5', [1, 0, 2, 3, 4, 5, 6, 7, 0, 0, 0], nbands=3, nleg=2) == \
+        ([1, 0], [[2, 3], [4, 5], [6, 7]])
+    assert f('ELG_12;LRG_5', [1, 0], nbands=3, nleg=0) == ([1, 0], [[], [], []])
+
+
+def test_prior_on_coeffs():
+    p = archetypes.prior_on_coeffs(2, 3, 0.1, 3)
+    assert p.shape == (11, 11)
+    assert np.array_equal(np.diag(p), np.r_[0, 0, np.full(9, 1 / 0.1 ** 2)])
+    assert np.count_nonzero(p - np.diag(np.diag(p))) == 0
+    assert np.array_equal(p, orc.prior_on_coeffs(2, 3, 0.1, 3))
+
+
+def test_legendre_rows_match_scipy():
+    w = np.linspace(3600, 5800, 2751)
+    x = reduced_wavelength(w)
+    assert x[0] == -1 and x[-1] == 1
+    want = np.array([scipy.special.legendre(i)(x) for i in range(5)])
+    assert np.allclose(legendre_rows(w, 5), want, rtol=0, atol=1e-13)
+    assert legendre_rows(w, 0).shape == (0, len(w))
+
+
+def test_archetype_file():
+    a = archetypes.Archetype(ARCHFILE)
+    assert a.narch == 2 and a.flux.shape == (2, 48860) and a.flux.dtype == np.float64
+    assert a.template_type == 'GALAXY' and a.version == 'v3.0' and a.method == 'ARCH'
+    assert list(a.sub_type) == ['LRG_0', 'ELG_1']
+    assert list(a.full_type) == ['GALAXY:::LRG_0', 'GALAXY:::ELG_1']
+    assert a.wave[1] - a.wave[0] == pytest.approx(0.2) and a.minwave == a.wave[0]
+    assert a.igm_model == 'Inoue14'
+
+
+def test_find_and_load_all(tmp_path, monkeypatch):
+    shutil.copy(ARCHFILE, tmp_path / 'rrarchetype-GALAXY.fits')
+    (tmp_path / 'other.fits').write_bytes(b'')
+    assert archetypes.find_archetypes(str(tmp_path)) == [str(tmp_path / 'rrarchetype-GALAXY.fits')]
+    monkeypatch.setenv('RR_ARCHETYPE_DIR', str(tmp_path))
+    allarch = archetypes.All_archetypes().archetypes
+    assert list(allarch.keys()) == ['GALAXY'] and allarch['GALAXY'].narch == 2
+    monkeypatch.delenv('RR_ARCHETYPE_DIR')
+    with pytest.raises(IOError):
+        archetypes.find_archetypes()
+
+
+def test_interp_extrapolate():
+    x = np.array([1.0, 2.0, 4.0])
+    y = np.array([1.0, 3.0, 4.0])
+    got = archetypes._interp_extrapolate(np.array([0.0, 1.5, 3.0, 6.0]), x, y)
+    assert np.allclose(got, [-1.0, 2.0, 3.5, 5.0])
