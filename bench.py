@@ -268,6 +268,134 @@ def bench_grad(args):
     }), flush=True)
 
 
+def _cpu_arch_one(k):
+    from oracle import rr_oracle as orc
+    a, targets, z, prior = _CPU['arch'], _CPU['targets'], _CPU['z'], _CPU['prior']
+    t = targets[k]
+    orc.get_best_archetype(t.spectra, t.bands, a.wave, a.flux, list(a.full_type), z[k], 2, True,
+                           prior=prior)
+    return k
+
+
+def cpu_baseline_archetype(arch, targets, z, prior, seconds):
+    """Oracle archetype fit (scipy lsq_linear BVLS per archetype, as the
+    reference) on host cores: archetype fits/s over a bounded sample of pairs."""
+    import multiprocessing as mp
+    ncores = min(16, len(os.sched_getaffinity(0)))
+    _CPU.update(arch=arch, targets=targets, z=z, prior=prior)
+    t0 = time.perf_counter()
+    _cpu_arch_one(0)
+    one = time.perf_counter() - t0
+    nsample = min(len(targets), max(ncores, int(seconds * ncores / max(one, 1e-3))))
+    ctx = mp.get_context('fork')
+    with ctx.Pool(ncores) as pool:
+        t0 = time.perf_counter()
+        list(pool.imap_unordered(_cpu_arch_one, range(nsample)))
+        wall = time.perf_counter() - t0
+    return dict(value=nsample * arch.narch / wall, unit='archetype fits/s', cores=ncores, kind='port',
+                sample=f'{nsample} (target, z) pairs x {arch.narch} archetypes, float64 oracle '
+                       f'(CSR R-apply, normal equations, scipy lsq_linear BVLS, explicit residual), '
+                       f'one process per core, {wall:.1f} s wall')
+
+
+def bench_archetype(args):
+    """Archetype final fit: every GALAXY archetype at the 3 lowest minima of
+    each target's coarse scan, per-camera Legendre polynomials of degree < 2
+    and a 0.1 prior on them (rrdesi's defaults), bounded archetype
+    coefficient; one step = ArchetypeFit.run over all pairs (archetype rebin
+    into HBM, rr_arch_fit, best archetype per pair)."""
+    import numpy as np
+    from redrock_amd import synth
+    from redrock_amd.archetypes import prior_on_coeffs
+    tm = synth.galaxy_template()
+    targets, ztrue = synth.make_targets(tm, args.ntargets, seed=args.seed, snr=args.snr)
+    arch = synth.archetypes_from(tm, args.narch, seed=args.seed + 1)
+    prior = prior_on_coeffs(1, 2, 0.1, 3)
+    cpu = None
+    if not args.no_cpu:
+        log('CPU baseline (oracle archetype fit) ...')
+        cpu = cpu_baseline_archetype(arch, targets, ztrue, prior, args.cpu_seconds)
+        log(f"CPU baseline: {cpu['value']:.4g} archetype fits/s on {cpu['cores']} cores")
+    import torch
+    from redrock_amd import zscan
+    from redrock_amd import fitz as fz
+    from redrock_amd.templates import DistTemplate
+    from redrock_amd.archetypes import ArchetypeFit
+    from oracle import rr_oracle as orc
+    assert torch.cuda.is_available(), 'bench.py needs a GPU'
+    dwave = {s.wavehash: s.wave for s in targets[0].spectra}
+    nt, zg = len(targets), tm.redshifts
+    nz = len(zg)
+    dt = DistTemplate(tm, dwave)
+    batch = zscan.TargetBatch(targets)
+    plan = zscan.ScanPlan(dt.cache, batch, np.arange(nt), np.zeros(nt), np.full(nt, nz),
+                          precision='f16x3', oiisum=zscan._oiisum(tm))
+    plan.run()
+    c2 = (plan.zchi2 + plan.penalty).reshape(nt, nz).cpu().numpy()
+    del plan
+    tidx, zp = [], []
+    for t in range(nt):
+        for i in fz.find_minima(c2[t])[:3]:
+            tidx.append(t)
+            zp.append(zg[i])
+    tidx, zp = np.array(tidx), np.array(zp)
+    npair = len(tidx)
+    fit = ArchetypeFit(arch, targets, per_camera=True, nleg=2, prior=prior, batch=batch)
+    events = []
+    inner = fit._fit
+
+    def timed_fit(*a, **k):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        r = inner(*a, **k)
+        e1.record()
+        events.append((e0, e1))
+        return r
+    fit._fit = timed_fit
+    for _ in range(max(1, args.warmup)):
+        fit.run(tidx, zp)
+    torch.cuda.synchronize()
+    events.clear()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        chi2, coeff, ftype = fit.run(tidx, zp)
+    torch.cuda.synchronize()
+    step = (time.perf_counter() - t0) / args.steps
+    fit_ms = sum(a.elapsed_time(b) for a, b in events) / args.steps
+    nfit = npair * arch.narch
+    log(f'archetype workload: {nt} targets, {npair} pairs x {arch.narch} archetypes, '
+        f'{step * 1e3:.1f} ms/step (rr_arch_fit {fit_ms:.1f} ms)')
+    # parity on a sample of pairs
+    rng = np.random.default_rng(0)
+    sample = rng.choice(npair, size=min(8, npair), replace=False)
+    errs, same = [], 0
+    for k in sample:
+        t = targets[tidx[k]]
+        wc2, wcf, wty = orc.get_best_archetype(t.spectra, t.bands, arch.wave, arch.flux,
+                                               list(arch.full_type), zp[k], 2, True, prior=prior)
+        errs.append(abs(chi2[k] - wc2) / wc2)
+        same += ftype[k] == wty
+    log(f'oracle sample: max rel chi2 err {max(errs):.2e}, same archetype {same}/{len(sample)}')
+    npp = batch.layout.npix_pad
+    bytes_launch = 8.0 * npp * arch.narch * npair
+    print(json.dumps({
+        'metric': 'archetype fits/sec on 1 GPU (target x minimum x archetype, BVLS + Legendre)',
+        'value': nfit / step, 'unit': 'archetype fits/s', 'n_gpus': 1, 'steps': args.steps,
+        'warmup': args.warmup, 'ms_per_step': step * 1e3, 'higher_is_better': True, 'scaling': 'weak',
+        'vs_baseline': None, 'dtype': 'f64', 'data': 'synthetic',
+        'config': {'workload': f'{nt} GALAXY targets, 3 lowest minima each, {arch.narch} archetypes, '
+                               f'per-camera Legendre degree < 2, prior sigma 0.1',
+                   'targets': nt, 'pairs': int(npair), 'archetypes': arch.narch},
+        'roofline': {'bound': 'hbm', 'achieved': bytes_launch / (fit_ms * 1e-3) / 1e9, 'peak': 8000.0,
+                     'unit': 'GB/s', 'frac': bytes_launch / (fit_ms * 1e-3) / 8e12, 'traffic': None,
+                     'kernel': 'rr_arch_fit (arch_accum_kernel<1, 6> + arch_solve_kernel<1, 6>)',
+                     'kernel_ms': fit_ms, 'bytes_per_launch': bytes_launch},
+        'oracle_sample_max_rel_chi2_err': float(max(errs)),
+        'oracle_sample_same_archetype': f'{same}/{len(sample)}',
+        'cpu_baseline': cpu,
+    }), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
@@ -285,13 +413,15 @@ def main():
                     help='tile configuration of the mixed kernel (-1: autotune)')
     ap.add_argument('--calibrate-copy-mb', type=int, default=0,
                     help='only run a plain device copy of this many MiB (HBM counter calibration)')
-    ap.add_argument('--workload', default='galaxy', choices=('galaxy', 'qso', 'mixed', 'grad'),
+    ap.add_argument('--workload', default='galaxy', choices=('galaxy', 'qso', 'mixed', 'grad', 'archetype'),
                     help="galaxy: the 500-target galaxy petal (headline); qso: 500 QSO targets "
                          "with the fused Lyman-alpha IGM; mixed: GALAXY/QSO/STAR targets with "
                          "masked runs over all three templates in one HIP graph; grad: adjoint "
-                         "gradient and Newton refinement of the 3 lowest minima per target")
+                         "gradient and Newton refinement of the 3 lowest minima per target; "
+                         "archetype: the archetype final fit at the 3 lowest minima per target")
     ap.add_argument('--mixed-targets', type=int, default=5000)
     ap.add_argument('--grad-targets', type=int, default=5000)
+    ap.add_argument('--narch', type=int, default=256, help='archetypes of the archetype workload')
     args = ap.parse_args()
     if args.calibrate_copy_mb:
         calibrate_copy(args.calibrate_copy_mb)
@@ -306,6 +436,8 @@ def main():
         return bench_mixed(args)
     if args.workload == 'grad':
         return bench_grad(args)
+    if args.workload == 'archetype':
+        return bench_archetype(args)
 
     import numpy as np
     from redrock_amd import synth
