@@ -376,8 +376,12 @@ def bench_archetype(args):
         errs.append(abs(chi2[k] - wc2) / wc2)
         same += ftype[k] == wty
     log(f'oracle sample: max rel chi2 err {max(errs):.2e}, same archetype {same}/{len(sample)}')
-    npp = batch.layout.npix_pad
-    bytes_launch = 8.0 * npp * arch.narch * npair
+    # reference-form FLOPs of one archetype fit (the eval formula of the scan
+    # with nb = 1 + 6 columns): R-apply, weights, Gram, y, solve, model, chi2
+    nb, npix = 7, batch.layout.npix
+    nnz = 11 * npix
+    flop_fit = (2 * nnz * nb + npix * nb + 2 * npix * nb * nb + 2 * npix * nb
+                + (2 * nb ** 3) // 3 + 2 * nb * nb + 2 * npix * nb + 3 * npix)
     print(json.dumps({
         'metric': 'archetype fits/sec on 1 GPU (target x minimum x archetype, BVLS + Legendre)',
         'value': nfit / step, 'unit': 'archetype fits/s', 'n_gpus': 1, 'steps': args.steps,
@@ -386,10 +390,12 @@ def bench_archetype(args):
         'config': {'workload': f'{nt} GALAXY targets, 3 lowest minima each, {arch.narch} archetypes, '
                                f'per-camera Legendre degree < 2, prior sigma 0.1',
                    'targets': nt, 'pairs': int(npair), 'archetypes': arch.narch},
-        'roofline': {'bound': 'hbm', 'achieved': bytes_launch / (fit_ms * 1e-3) / 1e9, 'peak': 8000.0,
-                     'unit': 'GB/s', 'frac': bytes_launch / (fit_ms * 1e-3) / 8e12, 'traffic': None,
-                     'kernel': 'rr_arch_fit (arch_accum_kernel<1, 6> + arch_solve_kernel<1, 6>)',
-                     'kernel_ms': fit_ms, 'bytes_per_launch': bytes_launch},
+        'roofline': {'bound': 'fp64', 'achieved': nfit * flop_fit / (fit_ms * 1e-3) / 1e12, 'peak': 78.6,
+                     'unit': 'TFLOP/s', 'frac': nfit * flop_fit / (fit_ms * 1e-3) / 78.6e12, 'traffic': None,
+                     'note': 'FP64 vector ALU; reference-form FLOPs per fit x fits per launch',
+                     'kernel': 'rr_arch_fit (arch_geom_kernel + arch_accum_kernel<1, 6, true> + '
+                               'arch_solve_kernel<1, 6>)',
+                     'kernel_ms': fit_ms, 'flop_per_fit': int(flop_fit), 'fits_per_launch': int(nfit)},
         'oracle_sample_max_rel_chi2_err': float(max(errs)),
         'oracle_sample_same_archetype': f'{same}/{len(sample)}',
         'cpu_baseline': cpu,

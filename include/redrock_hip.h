@@ -20,7 +20,7 @@
 extern "C" {
 #endif
 
-#define RR_ABI_VERSION 4
+#define RR_ABI_VERSION 5
 
 /* Error codes. */
 #define RR_OK 0
@@ -346,6 +346,17 @@ typedef struct {
     double *d_chi2;             /* (npair, nprob) out                                */
     void *d_work;               /* rr_arch_workspace_bytes bytes                     */
     long long work_bytes;
+    /* Fused rebin, used when d_acache is NULL: every window row of the fit is
+       the bin mean of the archetype (rebin.py:126-176 expression by
+       expression, as rr_rebin) computed in the kernel, so no cache exists.   */
+    const double *d_x;          /* (nx) rest-frame archetype wavelengths, increasing */
+    int nx;
+    const double *d_yt;         /* (nx, ldy) archetype fluxes, archetype fastest     */
+    long long ldy;              /* >= narch                                          */
+    const double *d_pe_lo;      /* (npix) lower observed-frame edge of every pixel   */
+    const double *d_pe_hi;      /* (npix) upper edge                                 */
+    int npix;                   /* real pixels; rows npix..npix_pad-1 are zero       */
+    const double *d_zrebin;     /* (npair) redshift of the archetype rebin           */
 } rr_arch_desc;
 
 long long rr_arch_workspace_bytes(const rr_arch_desc *desc);
@@ -353,7 +364,9 @@ long long rr_arch_workspace_bytes(const rr_arch_desc *desc);
 /*
  * rr_arch_fit — normal equations, bounded solve and chi2 of every problem.
  * A basis column with zero weighted norm (and no prior) gets coefficient 0,
- * the minimum-norm choice of the least-squares solve.
+ * the minimum-norm choice of the least-squares solve.  The caller checks the
+ * rebin range (each pixel's edges inside the redshifted archetype range,
+ * rebin.py:332-341) for the fused path; searches are clamped in bounds.
  */
 int rr_arch_fit(const rr_arch_desc *desc, void *stream);
 

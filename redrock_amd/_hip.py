@@ -11,7 +11,7 @@ import os
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, 'libredrock_hip.so')
 
-RR_ABI_VERSION = 4
+RR_ABI_VERSION = 5
 RR_F64, RR_F32, RR_F16, RR_BF16 = 0, 1, 2, 3
 RR_PREC_F64, RR_PREC_F16, RR_PREC_BF16, RR_PREC_F32, RR_PREC_F16X3 = 0, 1, 2, 3, 4
 RR_STATUS_OK, RR_STATUS_SINGULAR = 0, 1
@@ -67,6 +67,8 @@ class ArchDesc(ctypes.Structure):
         ('d_prior', _vp), ('d_wave', _vp), ('d_zigm', _vp),
         ('lines', ctypes.POINTER(LymanLine)), ('nlines', _i),
         ('d_coeff', _vp), ('d_chi2', _vp), ('d_work', _vp), ('work_bytes', _ll),
+        ('d_x', _vp), ('nx', _i), ('d_yt', _vp), ('ldy', _ll), ('d_pe_lo', _vp), ('d_pe_hi', _vp),
+        ('npix', _i), ('d_zrebin', _vp),
     ]
 
 

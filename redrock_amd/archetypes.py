@@ -324,6 +324,12 @@ class ArchetypeFit(object):
         self.wave_pix = torch.as_tensor(np.concatenate(
             [np.concatenate([s.wave for s in sp0]), np.zeros(P - self.layout.npix)]), device=dev).contiguous()
         self.edges = [torch.as_tensor(centers2edges(s.wave), device=dev).contiguous() for s in sp0]
+        # fused rebin inputs: archetype fluxes wavelength-major, each pixel's bin edges
+        pe = [centers2edges(s.wave) for s in sp0]
+        pad = np.zeros(P - self.layout.npix)
+        self.pe_lo = torch.as_tensor(np.concatenate([e[:-1] for e in pe] + [pad]), device=dev).contiguous()
+        self.pe_hi = torch.as_tensor(np.concatenate([e[1:] for e in pe] + [pad]), device=dev).contiguous()
+        self.yt = None
         self.edges_host = [(float(centers2edges(s.wave)[0]), float(centers2edges(s.wave)[-1])) for s in sp0]
         # Legendre blocks in band order (zscan.py:336-347) when the bands name the grids
         self.perm = np.arange(1 + self.NL)
@@ -345,14 +351,33 @@ class ArchetypeFit(object):
         byband = {b: blocks[i] for i, b in enumerate(bands)}
         return list(range(na)) + [c for b in sorted(bands) for c in byband[b]]
 
-    def _chunk(self):
+    def _orders(self, tidx, na):
+        """(len(tidx), na + NL) result column order of every pair's target."""
+        cache = dict()
+        rows = []
+        for t in tidx:
+            t = int(t)
+            if t not in cache:
+                cache[t] = np.array(self._band_order(t, na))
+            rows.append(cache[t])
+        return np.stack(rows) if rows else np.zeros((0, na + self.NL), dtype=int)
+
+    def _chunk(self, fused):
         import torch
         free, _ = torch.cuda.mem_get_info()
-        per = (self.P + 2 * self.HALO) * self.arch.narch * 8
+        # per pair: the archetype cache, or the fused path's geometry and normal equations
+        per = self.P * 8 + self.arch.narch * 24 * 8 if fused else (self.P + 2 * self.HALO) * self.arch.narch * 8
         budget = int(min(max(free // 8, 256 << 20), 8 << 30))
         return int(min(max(1, budget // per), 4 * 65535))      # 4 pairs per grid row
 
-    def _fit(self, cache, npair, tgt, zigm, lines, na, nprob, columns, prior, nbound):
+    def _fused_inputs(self):
+        import torch
+        if self.yt is None:
+            x, y = self.arch.device()
+            self.yt = y.t().contiguous()
+        return self.arch.device()[0], self.yt
+
+    def _fit(self, cache, npair, tgt, zigm, lines, na, nprob, columns, prior, nbound, zrebin=None):
         import torch
         dev = torch.device('cuda')
         nb = na + self.NL
@@ -363,6 +388,11 @@ class ArchetypeFit(object):
         d.npix_pad, d.halo = self.P, self.HALO
         d.d_pair_target = _hip.ptr(tgt)
         d.d_acache = _hip.ptr(cache)
+        if cache is None:
+            x, yt = self._fused_inputs()
+            d.d_x, d.nx, d.d_yt, d.ldy = _hip.ptr(x), int(x.numel()), _hip.ptr(yt), self.arch.narch
+            d.d_pe_lo, d.d_pe_hi, d.npix = _hip.ptr(self.pe_lo), _hip.ptr(self.pe_hi), self.layout.npix
+            d.d_zrebin = _hip.ptr(zrebin)
         d.ld = self.arch.narch
         d.narch = self.arch.narch
         d.d_columns = _hip.ptr(columns)
@@ -389,18 +419,20 @@ class ArchetypeFit(object):
         _hip.check(rc, 'rr_arch_fit')
         return chi2, coeff
 
-    def run_all(self, tidx, z, zigm=None, lines=None):
+    def run_all(self, tidx, z, zigm=None, lines=None, fused=True):
         """Every archetype alone at every pair: chi2 (npair, narch) and coeff
         (npair, narch, 1 + nleg columns in band order)."""
-        return self.run(tidx, z, zigm=zigm, lines=lines, keep_all=True)
+        return self.run(tidx, z, zigm=zigm, lines=lines, keep_all=True, fused=fused)
 
-    def run(self, tidx, z, zigm=None, lines=None, tpix=None, keep_all=False):
+    def run(self, tidx, z, zigm=None, lines=None, tpix=None, keep_all=False, fused=True):
         """Fit every pair (targets[tidx[k]], z[k]).
 
         zigm[k]: redshift of the Lyman-series factor of pair k (NaN: none);
         lines: (line, A, B) list of that factor.  tpix: per pair a pixel
         transmission array multiplied into the archetypes instead (the
-        one-target API).  Returns (chi2 (npair,), [coeff], [fulltype]) with
+        one-target API; it uses the cache path).  fused=False rebins into an
+        HBM cache first instead of inside the fit kernel (same doubles).
+        Returns (chi2 (npair,), [coeff], [fulltype]) with
         coeff = archetype coefficient(s) then the Legendre coefficients.
         Raises ValueError when a redshifted archetype does not cover the
         pixels (rebin.py:332-341).
@@ -421,8 +453,9 @@ class ArchetypeFit(object):
         x, y = self.arch.device()
         narch = self.arch.narch
         rows = self.P + 2 * self.HALO
-        chunk = min(npair, self._chunk())
-        cache = torch.zeros((max(chunk, 1), rows, narch), dtype=torch.float64, device=dev)
+        fused = bool(fused) and tpix is None          # the cache path applies tpix
+        chunk = min(npair, self._chunk(fused))
+        cache = None if fused else torch.zeros((max(chunk, 1), rows, narch), dtype=torch.float64, device=dev)
         nn = self.n_nearest
         na1 = 1
         prior1 = None
@@ -437,42 +470,44 @@ class ArchetypeFit(object):
         for c0 in range(0, npair, max(chunk, 1)):
             sl = slice(c0, min(npair, c0 + chunk))
             n = sl.stop - sl.start
-            cache.zero_()
             zt = torch.as_tensor(z[sl], device=dev).contiguous()
-            flat = cache.view(-1)
-            for e, off in zip(self.edges, self.layout.offsets):
-                base = (self.HALO + off) * narch
-                rebin_into(flat[base:], x, y, e, zt, rows * narch, narch, 1)
+            if not fused:
+                cache.zero_()
+                flat = cache.view(-1)
+                for e, off in zip(self.edges, self.layout.offsets):
+                    base = (self.HALO + off) * narch
+                    rebin_into(flat[base:], x, y, e, zt, rows * narch, narch, 1)
             if tpix is not None:
                 T = torch.as_tensor(np.stack([np.concatenate([tp, np.ones(self.P - len(tp))])
                                               for tp in tpix[sl]]), device=dev)
                 cache[:n, self.HALO:self.HALO + self.P, :] *= T[:, :, None]
             tgt = torch.as_tensor(tidx[sl].astype(np.int32), device=dev).contiguous()
             zi = torch.as_tensor(np.asarray(zigm[sl], dtype=np.float64), device=dev).contiguous()
-            chi2, coeff = self._fit(cache, n, tgt, zi, lines, na1, narch, None, prior1, 1)
-            chi2 = chi2.cpu().numpy()
-            coeff = coeff.cpu().numpy()
+            chi2, coeff = self._fit(cache, n, tgt, zi, lines, na1, narch, None, prior1, 1, zrebin=zt)
             zero = self.wsum[tidx[sl]] == 0
-            chi2[zero] = HUGE_CHI2                       # zscan.py:656-660
-            coeff[zero] = 0.0
+            chi2h = chi2.cpu().numpy()
+            chi2h[zero] = HUGE_CHI2                      # zscan.py:656-660
             if keep_all:
+                coeff = coeff.cpu().numpy()
+                coeff[zero] = 0.0
                 for k in range(n):
-                    all_chi2[c0 + k] = chi2[k]
+                    all_chi2[c0 + k] = chi2h[k]
                     all_coeff[c0 + k] = coeff[k][:, self._band_order(int(tidx[sl][k]), 1)]
                 continue
             if nn is None:
-                best = np.argmin(chi2, axis=1)
-                for k in range(n):
-                    t = int(tidx[sl][k])
-                    order = self._band_order(t, 1)
-                    out_chi2[c0 + k] = chi2[k, best[k]]
-                    out_coeff[c0 + k] = coeff[k, best[k]][order]
-                    out_type[c0 + k] = self.arch.full_type[best[k]]
+                # only the best archetype's coefficients come back to the host
+                best = np.argmin(chi2h, axis=1)
+                cb = coeff[torch.arange(n, device=dev), torch.as_tensor(best, device=dev)].cpu().numpy()
+                cb[zero] = 0.0
+                cb = np.take_along_axis(cb, self._orders(tidx[sl], 1), axis=1)
+                out_chi2[sl] = chi2h[np.arange(n), best]
+                out_coeff[c0:c0 + n] = list(cb)
+                out_type[c0:c0 + n] = list(self.arch.full_type[best])
                 continue
-            ibest = np.argsort(chi2, axis=1)[:, :nn]
+            ibest = np.argsort(chi2h, axis=1)[:, :nn]
             cols = torch.as_tensor(ibest.astype(np.int32).reshape(n, 1, nn), device=dev).contiguous()
             prior2 = self.prior_full if self.per_camera else None
-            chi2b, coeffb = self._fit(cache, n, tgt, zi, lines, nn, 1, cols, prior2, nn)
+            chi2b, coeffb = self._fit(cache, n, tgt, zi, lines, nn, 1, cols, prior2, nn, zrebin=zt)
             chi2b = chi2b.cpu().numpy()[:, 0]
             coeffb = coeffb.cpu().numpy()[:, 0]
             chi2b[zero] = HUGE_CHI2

@@ -14,9 +14,12 @@
 // nearest_neighbour_model (archetypes.py:241-400) over
 // per_camera_coeff_with_least_square_batch (zscan.py:245-348).
 //
-// Three kernels:
+// Kernels:
 //   arch_prep_kernel   per target, z-independent: w (R L), (RL)'W(RL),
 //                      (RL)'Wf, f'Wf.  One workgroup per target.
+//   arch_geom_kernel   (fused rebin) per (pair, pixel): the sample index j of
+//                      the bin's lower edge and the last interior sample jend
+//                      (rebin.py:142,154), the same for every archetype.
 //   arch_accum_kernel  per (pair, problem): the archetype rows of M and y.
 //                      A wavefront is one pair (one target, wave-uniform R, w,
 //                      w f and w R L: scalar loads) and 64 problems, one per
@@ -24,10 +27,19 @@
 //                      is archetype `lane`, so every cache row is one
 //                      coalesced 64 x 8-byte run.  FP64 vector FMA, 13 + nleg
 //                      FMAs per problem and pixel; the cache is read once.
+//                      With the fused rebin the window rows are bin means
+//                      computed here from the archetype fluxes stored
+//                      wavelength-major (a row of samples is again one
+//                      coalesced run across the lanes), each new row once,
+//                      the window sliding by 4 pixels; no cache is written.
 //   arch_solve_kernel  per problem: assemble M and y, bounded least squares
 //                      by active sets (Householder QR of the free columns),
 //                      chi2 as c'M0c - 2c'y + f'Wf with M0 = M - prior.
 #include "common.hpp"
+
+// the bin-mean expressions match rr_rebin's double for double; every other
+// product-sum here is an explicit fma()
+#pragma clang fp contract(off)
 
 namespace {
 
@@ -49,6 +61,11 @@ struct ArchArgs {
     int nlines;
     double *coeff, *chi2;
     double *ne;   // (K, nproblems), structure of arrays
+    // fused rebin
+    const double *x, *yt, *pe_lo, *pe_hi, *zrebin;
+    long long ldy;
+    int nx, npix;
+    int2 *geom;   // (npair, npix_pad): (j, jend), jend -1 = bin inside one sample step, -2 = zero row
 };
 
 __device__ __forceinline__ double wave_sum(double v) {
@@ -123,6 +140,74 @@ __global__ __launch_bounds__(256) void arch_prep_kernel(const double *__restrict
     }
 }
 
+// ---------------------------------------------------------------- fused rebin
+__global__ __launch_bounds__(256) void arch_geom_kernel(ArchArgs a) {
+    const long long idx = (long long)blockIdx.x * 256 + threadIdx.x;
+    const long long n = (long long)a.npair * a.npix_pad;
+    if (idx >= n) return;
+    const int pair = (int)(idx / a.npix_pad);
+    const int r = (int)(idx - (long long)pair * a.npix_pad);
+    int2 g = make_int2(1, -2);
+    if (r < a.npix) {
+        const double *x = a.x;
+        const int nx = a.nx;
+        const double opz = 1. + a.zrebin[pair];
+        const double elo = a.pe_lo[r], ehi = a.pe_hi[r];
+        int lo = 0, hi = nx;                  // first x[j]*(1+z) above the lower edge
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if (x[mid] * opz <= elo) lo = mid + 1;
+            else hi = mid;
+        }
+        const int j = lo < 1 ? 1 : (lo > nx - 1 ? nx - 1 : lo);
+        int jend = -1;
+        if (j < nx - 1 && x[j] * opz < ehi) {     // x[jend + 1] stays in bounds
+            jend = j;
+            while (jend + 1 < nx - 1 && x[jend + 1] * opz < ehi) ++jend;
+        }
+        g = make_int2(j, jend);
+    }
+    a.geom[idx] = g;
+}
+
+// Bin mean of archetype column `col` for geometry g at 1+z = opz (the
+// expressions of rr_rebin, rebin.py:144-176).
+__device__ __forceinline__ double bin_mean(const ArchArgs &a, int2 g, double opz, double elo, double ehi,
+                                           int col) {
+    if (g.y == -2) return 0.0;
+    const double *x = a.x;
+    const double *Y = a.yt + col;
+    const long long ld = a.ldy;
+    const int j = g.x;
+    const double xj = x[j] * opz;
+    const double xjm1 = x[j - 1] * opz;
+    const double width = ehi - elo;
+    const double yj = Y[j * ld], yjm1 = Y[(j - 1) * ld];
+    double r = 0.0;
+    if (g.y >= 0) {
+        const int jend = g.y;
+        const double yedge = yjm1 + (elo - xjm1) * (yj - yjm1) / (xj - xjm1);
+        r += 0.5 * (yj + yedge) * (xj - elo);
+        double xprev = xj, yprev = yj;
+        for (int jj = j + 1; jj <= jend; ++jj) {
+            const double xc = x[jj] * opz;
+            const double yc = Y[jj * ld];
+            r += 0.5 * (yc + yprev) * (xc - xprev);
+            xprev = xc;
+            yprev = yc;
+        }
+        const double xe1 = x[jend + 1] * opz;
+        const double ye1 = Y[(jend + 1) * ld];
+        const double yedge2 = yprev + (ehi - xprev) * (ye1 - yprev) / (xe1 - xprev);
+        r += 0.5 * (yedge2 + yprev) * (ehi - xprev);
+    } else {
+        const double ylo = yj + (elo - xj) * (yj - yjm1) / (xj - xjm1);
+        const double yhi = yj + (ehi - xj) * (yj - yjm1) / (xj - xjm1);
+        r += 0.5 * (ylo + yhi) * width;
+    }
+    return r / width;
+}
+
 // ---------------------------------------------------------------- accumulate
 __device__ __forceinline__ double igm_factor(const ArchArgs &a, double lobs, double opz) {
     double T = 1.0;
@@ -137,7 +222,7 @@ __device__ __forceinline__ double igm_factor(const ArchArgs &a, double lobs, dou
     return T;
 }
 
-template <int NA, int NL>
+template <int NA, int NL, bool FUSED>
 __global__ __launch_bounds__(256) void arch_accum_kernel(ArchArgs a) {
     constexpr int P = 4;
     constexpr int NG = NA * (NA + 1) / 2;
@@ -162,10 +247,25 @@ __global__ __launch_bounds__(256) void arch_accum_kernel(ArchArgs a) {
     const double *__restrict__ WFt = a.wf + (long long)t * npp;
     const double *__restrict__ WRL = a.wrl + (long long)t * NL * npp;
     const long long rows = npp + 2LL * a.halo;
-    const double *__restrict__ Ap = a.acache + (long long)pair * rows * a.ld;
+    const double *__restrict__ Ap = FUSED ? nullptr : a.acache + (long long)pair * rows * a.ld;
     const double zi = a.zigm ? a.zigm[pair] : 0.0;
     const bool igm = a.nlines > 0 && zi == zi;
     const double opz = 1.0 + zi;
+    const double opzr = FUSED ? 1.0 + a.zrebin[pair] : 1.0;
+    const int2 *G = FUSED ? a.geom + (long long)pair * npp : nullptr;
+    // fused: the window of rows p0-5 .. p0+P+4 slides by P; row i enters once
+    double fw[NA][P + 10];
+    auto fused_row = [&](int pix, int j) -> double {
+        if (pix < 0 || pix >= a.npix) return 0.0;
+        const double v = bin_mean(a, G[pix], opzr, a.pe_lo[pix], a.pe_hi[pix], col[j]);
+        return igm ? v * igm_factor(a, a.wave[pix], opz) : v;
+    };
+    if (FUSED) {
+#pragma unroll
+        for (int j = 0; j < NA; ++j)
+#pragma unroll
+            for (int i = 0; i < 10; ++i) fw[j][i + P] = fused_row(i - 5, j);
+    }
 
     double gaa[NG], ya[NA], gal[NA][NL > 0 ? NL : 1];
 #pragma unroll
@@ -186,20 +286,29 @@ __global__ __launch_bounds__(256) void arch_accum_kernel(ArchArgs a) {
         double T[P + 10];
 #pragma unroll
         for (int i = 0; i < P + 10; ++i) T[i] = 1.0;
-        if (igm) {
+        if (!FUSED && igm) {
 #pragma unroll
             for (int i = 0; i < P + 10; ++i) {
                 const int pix = p0 - 5 + i;
                 if (pix >= 0 && pix < npp) T[i] = igm_factor(a, a.wave[pix], opz);
             }
         }
+        if (FUSED) {
+#pragma unroll
+            for (int j = 0; j < NA; ++j) {
+#pragma unroll
+                for (int i = 0; i < 10; ++i) fw[j][i] = fw[j][i + P];
+#pragma unroll
+                for (int i = 0; i < P; ++i) fw[j][10 + i] = fused_row(p0 + 5 + i, j);
+            }
+        }
         double ra[P][NA];
 #pragma unroll
         for (int j = 0; j < NA; ++j) {
-            const double *colp = Ap + (long long)(a.halo + p0 - 5) * a.ld + col[j];
+            const double *colp = FUSED ? nullptr : Ap + (long long)(a.halo + p0 - 5) * a.ld + col[j];
             double win[P + 10];
 #pragma unroll
-            for (int i = 0; i < P + 10; ++i) win[i] = colp[(long long)i * a.ld] * T[i];
+            for (int i = 0; i < P + 10; ++i) win[i] = FUSED ? fw[j][i] : colp[(long long)i * a.ld] * T[i];
 #pragma unroll
             for (int i = 0; i < P; ++i) {
                 double s = r[i][0] * win[i];
@@ -451,7 +560,15 @@ __global__ __launch_bounds__(64) void arch_solve_kernel(ArchArgs a) {
 template <int NA, int NL>
 int launch_fit(const ArchArgs &a, hipStream_t s) {
     dim3 g1((a.nprob + 63) / 64, (a.npair + 3) / 4);
-    hipLaunchKernelGGL((arch_accum_kernel<NA, NL>), g1, dim3(256), 0, s, a);
+    if (a.geom) {
+        const long long n = (long long)a.npair * a.npix_pad;
+        hipLaunchKernelGGL(arch_geom_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, a);
+        int rc = rr::check_launch("rr_arch_fit(geometry)");
+        if (rc) return rc;
+        hipLaunchKernelGGL((arch_accum_kernel<NA, NL, true>), g1, dim3(256), 0, s, a);
+    } else {
+        hipLaunchKernelGGL((arch_accum_kernel<NA, NL, false>), g1, dim3(256), 0, s, a);
+    }
     int rc = rr::check_launch("rr_arch_fit(accumulate)");
     if (rc) return rc;
     const unsigned nblk = (unsigned)((a.nproblems + 63) / 64);
@@ -488,6 +605,12 @@ void launch_prep(const double *rband, const double *w, const double *f, int nt, 
 
 long long ne_count(int na, int nleg) { return (long long)na * (na + 1) / 2 + (long long)na * nleg + na; }
 
+// normal-equation part of the workspace, 256-byte aligned; the fused rebin's
+// (npair, npix_pad) int2 geometry follows it
+long long ne_bytes(const rr_arch_desc *d) {
+    return (ne_count(d->na, d->nleg) * (long long)d->npair * d->nprob * 8 + 255) / 256 * 256;
+}
+
 }  // namespace
 
 extern "C" int rr_arch_prep(const double *d_rband, const double *d_w, const double *d_f, int ntarget,
@@ -521,7 +644,8 @@ extern "C" int rr_arch_prep(const double *d_rband, const double *d_w, const doub
 extern "C" long long rr_arch_workspace_bytes(const rr_arch_desc *d) {
     if (!d || d->npair < 0 || d->nprob < 0 || d->na < 1 || d->na > kMaxNA || d->nleg < 0 || d->nleg > kMaxNL)
         return -1;
-    return ne_count(d->na, d->nleg) * (long long)d->npair * d->nprob * 8;
+    const long long geom = d->d_acache ? 0 : (long long)d->npair * d->npix_pad * 8;
+    return ne_bytes(d) + geom;
 }
 
 extern "C" int rr_arch_fit(const rr_arch_desc *d, void *stream) {
@@ -537,7 +661,12 @@ extern "C" int rr_arch_fit(const rr_arch_desc *d, void *stream) {
     if (d->npair == 0 || d->nprob == 0) return RR_OK;
     if (!d->d_columns && (d->na != 1 || d->nprob > d->narch))
         return rr::fail(RR_EINVAL, "rr_arch_fit: without d_columns na must be 1 and nprob <= narch");
-    if (!d->d_pair_target || !d->d_acache || !d->d_rband || !d->d_w || !d->d_wf || !d->d_fwf ||
+    const bool fused = d->d_acache == nullptr;
+    if (fused && (!d->d_x || d->nx < 2 || !d->d_yt || d->ldy < d->narch || !d->d_pe_lo || !d->d_pe_hi ||
+                  d->npix < 1 || d->npix > d->npix_pad || !d->d_zrebin))
+        return rr::fail(RR_EINVAL, "rr_arch_fit: fused rebin needs d_x (nx >= 2), d_yt (ldy >= narch), "
+                                   "pixel edges, 0 < npix <= npix_pad and d_zrebin");
+    if (!d->d_pair_target || !d->d_rband || !d->d_w || !d->d_wf || !d->d_fwf ||
         !d->d_coeff || !d->d_chi2 || !d->d_work ||
         (d->nleg > 0 && (!d->d_wrl || !d->d_gll || !d->d_yl)) ||
         (d->nlines > 0 && (!d->d_wave || !d->d_zigm || !d->lines)))
@@ -572,6 +701,15 @@ extern "C" int rr_arch_fit(const rr_arch_desc *d, void *stream) {
     a.coeff = d->d_coeff;
     a.chi2 = d->d_chi2;
     a.ne = static_cast<double *>(d->d_work);
+    a.x = d->d_x;
+    a.yt = d->d_yt;
+    a.ldy = d->ldy;
+    a.nx = d->nx;
+    a.pe_lo = d->d_pe_lo;
+    a.pe_hi = d->d_pe_hi;
+    a.npix = fused ? d->npix : d->npix_pad;
+    a.zrebin = d->d_zrebin;
+    a.geom = fused ? reinterpret_cast<int2 *>(static_cast<char *>(d->d_work) + ne_bytes(d)) : nullptr;
     hipStream_t s = static_cast<hipStream_t>(stream);
     switch (d->na) {
         case 1: return launch_na<1>(a, d->nleg, s);

@@ -80,7 +80,9 @@ __global__ __launch_bounds__(256) void rebin_kernel(RebinArgs a, int nfast) {
     if (j > nx - 1) j = nx - 1;
     const double xj = x[j] * opz;
     const double xjm1 = x[j - 1] * opz;
-    const bool inside = xj < ehi;
+    // j == nx - 1 cannot be inside after the host's range check; the test
+    // keeps x[jend + 1] in bounds for an unchecked call
+    const bool inside = xj < ehi && j < nx - 1;
     // rebin.py:154 — last interior sample of this bin
     int jend = j;
     if (inside) {

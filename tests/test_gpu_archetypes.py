@@ -230,3 +230,30 @@ def test_zfind_archetype_mode(gpu, tmp_path):
     results.write_zscan(path, _, zt)
     back = results.read_zfit(path)
     assert np.array_equal(back['subtype'], zt['subtype']) and back['legcoeff'].shape == (n, 3, 2)
+
+
+def test_fused_rebin_equals_cache(gpu):
+    """The fit kernel's own bin means are rr_rebin's doubles: the fused and the
+    cached paths give identical chi2 and coefficients, with IGM and with the
+    n-nearest combination."""
+    from redrock_amd import synth, igm
+    from redrock_amd.archetypes import ArchetypeFit, prior_on_coeffs
+    q = synth.qso_template()
+    targets, zt = synth.make_targets(q, 3, seed=67, zrange=(2.2, 2.8), snr=2.0)
+    arch = synth.archetypes_from(q, 70, seed=67)
+    lines = igm.lyman_table(q.igm_model)
+    print('case: fused rebin vs cache, 3 QSO targets, 70 archetypes', flush=True)
+    for nn in (None, 3):
+        prior = prior_on_coeffs(nn or 1, 2, 0.2, 3)
+        fit = ArchetypeFit(arch, targets, per_camera=True, nleg=2, prior=prior, n_nearest=nn)
+        tidx = np.repeat(np.arange(3), 2)
+        z = np.concatenate([[zt[i], zt[i] + 0.013] for i in range(3)])
+        zigm = z + 0.0004
+        a = fit.run(tidx, z, zigm=zigm, lines=lines, fused=True)
+        b = fit.run(tidx, z, zigm=zigm, lines=lines, fused=False)
+        assert np.array_equal(a[0], b[0]) and a[2] == b[2]
+        for ca, cb in zip(a[1], b[1]):
+            assert np.array_equal(ca, cb)
+    ac2, acf = fit.run_all(tidx, z, zigm=zigm, lines=lines, fused=True)
+    bc2, bcf = fit.run_all(tidx, z, zigm=zigm, lines=lines, fused=False)
+    assert np.array_equal(ac2, bc2) and np.array_equal(acf, bcf)
