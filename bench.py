@@ -402,6 +402,57 @@ def bench_archetype(args):
     }), flush=True)
 
 
+def bench_zfind(args):
+    """zfind end to end (scan of every template, FP64 fine fits of the minima,
+    result table) on mixed GALAXY/QSO/STAR targets: targets per second."""
+    import numpy as np
+    import torch
+    from redrock_amd import synth, zfind as zf
+    from redrock_amd.templates import DistTemplate
+    from redrock_amd.targets import DistTargetsCopy
+    tms = (synth.galaxy_template(), synth.qso_template(), synth.star_template())
+    targets, types, ztrue = synth.mixed_targets(args.zfind_targets, seed=args.seed, snr=args.snr,
+                                                mask_frac=(0.0, 0.3), templates=tms)
+    assert torch.cuda.is_available(), 'bench.py needs a GPU'
+    dwave = {s.wavehash: s.wave for s in targets[0].spectra}
+    dts = [DistTemplate(t, dwave) for t in tms]
+    dist = DistTargetsCopy(targets)
+    zf.zfind(DistTargetsCopy(targets[:64]), dts, nminima=3, precision=args.precision)      # warm-up
+    out = {}
+    for p in (args.precision, 'f64'):
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        _, zt = zf.zfind(dist, dts, nminima=3, precision=p)
+        torch.cuda.synchronize()
+        out[p] = (time.perf_counter() - t, dict(zf.last_timings), zt)
+        tm = zf.last_timings
+        log(f"{p}: zfind {out[p][0]:.2f} s (scan {tm['scan']:.2f}, fitz "
+            + ', '.join(f'{k} {v:.2f}' for k, v in tm['fitz'].items()) + f", assemble {tm['assemble']:.2f})")
+    a, b = out[args.precision][2], out['f64'][2]
+    best = lambda zt: {t: (zt['z'][i], zt['spectype'][i]) for i, t in enumerate(zt['targetid']) if zt['znum'][i] == 0}
+    ba, bb = best(a), best(b)
+    same_z = np.mean([abs(ba[t][0] - bb[t][0]) <= 1e-9 * (1 + bb[t][0]) for t in bb])
+    same_type = np.mean([ba[t][1] == bb[t][1] for t in bb])
+    truth = {i: (z, ty) for i, (z, ty) in enumerate(zip(ztrue, types))}
+    right = np.mean([abs(bb[t][0] - truth[t][0]) <= 3e-3 * (1 + truth[t][0]) and bb[t][1] == truth[t][1]
+                     for t in bb])
+    nt = len(targets)
+    wall = out[args.precision][0]
+    print(json.dumps({
+        'metric': 'targets/sec on 1 GPU, zfind end to end (scan + fine fits + table)',
+        'value': nt / wall, 'unit': 'targets/s', 'n_gpus': 1, 'steps': 1, 'warmup': 1,
+        'ms_per_step': wall * 1e3, 'higher_is_better': True, 'scaling': 'weak', 'vs_baseline': None,
+        'dtype': args.precision, 'data': 'synthetic',
+        'config': {'workload': f'{nt} mixed GALAXY/QSO/STAR targets, 0-30% masked, templates GALAXY '
+                               f'(nb 10, 1977 z) + QSO (nb 4, 3795 z) + STAR (nb 5, 501 z), nminima 3',
+                   'targets': nt},
+        'phases_s': out[args.precision][1],
+        'fp64': {'value': nt / out['f64'][0], 'seconds': out['f64'][0], 'phases_s': out['f64'][1]},
+        'best_z_same_as_fp64': float(same_z), 'best_type_same_as_fp64': float(same_type),
+        'fp64_right_z_and_type': float(right),
+    }), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
@@ -419,15 +470,17 @@ def main():
                     help='tile configuration of the mixed kernel (-1: autotune)')
     ap.add_argument('--calibrate-copy-mb', type=int, default=0,
                     help='only run a plain device copy of this many MiB (HBM counter calibration)')
-    ap.add_argument('--workload', default='galaxy', choices=('galaxy', 'qso', 'mixed', 'grad', 'archetype'),
+    ap.add_argument('--workload', default='galaxy', choices=('galaxy', 'qso', 'mixed', 'grad', 'archetype', 'zfind'),
                     help="galaxy: the 500-target galaxy petal (headline); qso: 500 QSO targets "
                          "with the fused Lyman-alpha IGM; mixed: GALAXY/QSO/STAR targets with "
                          "masked runs over all three templates in one HIP graph; grad: adjoint "
                          "gradient and Newton refinement of the 3 lowest minima per target; "
-                         "archetype: the archetype final fit at the 3 lowest minima per target")
+                         "archetype: the archetype final fit at the 3 lowest minima per target; "
+                         "zfind: zfind end to end on mixed targets")
     ap.add_argument('--mixed-targets', type=int, default=5000)
     ap.add_argument('--grad-targets', type=int, default=5000)
     ap.add_argument('--narch', type=int, default=256, help='archetypes of the archetype workload')
+    ap.add_argument('--zfind-targets', type=int, default=2000)
     args = ap.parse_args()
     if args.calibrate_copy_mb:
         calibrate_copy(args.calibrate_copy_mb)
@@ -444,6 +497,8 @@ def main():
         return bench_grad(args)
     if args.workload == 'archetype':
         return bench_archetype(args)
+    if args.workload == 'zfind':
+        return bench_zfind(args)
 
     import numpy as np
     from redrock_amd import synth

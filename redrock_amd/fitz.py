@@ -277,7 +277,7 @@ def _stack(results, target):
 
 def fitz_targets(zchi2, redshifts, targets, template, nminima=3, zminfit_npoints=15,
                  precision='f64', speculate=2, newton=False, archetype=None, deg_legendre=None,
-                 per_camera=False, n_nearest=None, prior_sigma=None):
+                 per_camera=False, n_nearest=None, prior_sigma=None, batch=None):
     """fitz for every target of one template (zchi2: (nt, nz) incl. penalty).
 
     Returns a list with the reference's per-target result dict.  With
@@ -288,11 +288,12 @@ def fitz_targets(zchi2, redshifts, targets, template, nminima=3, zminfit_npoints
     minimum's chi2 and coefficients come from the archetype fit at its
     redshift (fitz.py:329-357): deg_legendre Legendre polynomials (per
     camera with per_camera), prior_sigma on their coefficients, n_nearest
-    archetypes combined.
+    archetypes combined.  batch: the TargetBatch of `targets` (one layout),
+    reused instead of uploading them again.
     """
     out = _fitz_targets(zchi2, redshifts, targets, template, nminima, zminfit_npoints,
                         precision, speculate, archetype, deg_legendre, per_camera, n_nearest,
-                        prior_sigma)
+                        prior_sigma, batch)
     if newton:
         _add_newton(out, targets, template)
     return out
@@ -319,13 +320,17 @@ def _add_newton(out, targets, template):
         out[t]['chi2newton'][k, 0] = cv
 
 
+class _NeedMore(Exception):
+    """A target's sequential fitz logic needs a candidate not refined yet."""
+
+
 def _fitz_targets(zchi2, redshifts, targets, template, nminima, zminfit_npoints, precision,
                   speculate, archetype=None, deg_legendre=None, per_camera=False, n_nearest=None,
-                  prior_sigma=None):
+                  prior_sigma=None, batch=None):
     zchi2 = np.atleast_2d(np.asarray(zchi2, dtype=np.float64))
     assert zchi2.shape == (len(targets), len(redshifts))
     nz = 15 if zminfit_npoints is None else zminfit_npoints
-    ref = _Refiner(template, targets, redshifts, nz, precision)
+    ref = _Refiner(template, targets, redshifts, nz, precision, batch=batch)
     cands = [find_minima(zchi2[t]) for t in range(len(targets))]
     done = [dict() for _ in targets]
 
@@ -338,16 +343,30 @@ def _fitz_targets(zchi2, redshifts, targets, template, nminima, zminfit_npoints,
     first = [(t, int(c)) for t in range(len(targets))
              for c in cands[t][:nminima + speculate]]
     run(first)
-    out = []
-    for t, tg in enumerate(targets):
-        def get(k, t=t):
-            imin = int(cands[t][k])
-            if imin not in done[t]:
-                # the sequential logic needs a candidate beyond the speculative set
-                run([(t, int(c)) for c in cands[t][k:k + nminima + speculate]
-                     if int(c) not in done[t]])
-            return done[t][imin]
-        out.append(_finalize(template, tg, redshifts, nminima, cands[t], get))
+    # The sequential logic of each target may reach past its speculative set
+    # (minima skipped within max_velo_diff): such targets stop, their next
+    # candidates are refined together in one batch, and they start over.
+    out = [None] * len(targets)
+    pending = list(range(len(targets)))
+    while pending:
+        need, again = [], []
+        for t in pending:
+            missing = []
+
+            def get(k, t=t, missing=missing):
+                imin = int(cands[t][k])
+                if imin not in done[t]:
+                    missing.extend(int(c) for c in cands[t][k:k + nminima + speculate]
+                                   if int(c) not in done[t])
+                    raise _NeedMore()
+                return done[t][imin]
+            try:
+                out[t] = _finalize(template, targets[t], redshifts, nminima, cands[t], get)
+            except _NeedMore:
+                again.append(t)
+                need.extend((t, c) for c in missing)
+        run(need)
+        pending = again
     if archetype is not None:
         _archetype_pass(out, targets, template, archetype, ref, deg_legendre, per_camera, n_nearest,
                         prior_sigma)

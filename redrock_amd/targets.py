@@ -25,6 +25,8 @@ def resolution_band(R, n=None):
     Raises NotImplementedError if R has entries farther than 5 from the
     diagonal.
     """
+    if scipy.sparse.issparse(R) and R.format == 'dia':
+        return _dia_band(R, n)
     coo = scipy.sparse.coo_matrix(R)
     if n is None:
         n = coo.shape[0]
@@ -34,6 +36,28 @@ def resolution_band(R, n=None):
         raise NotImplementedError('resolution matrix wider than 11 diagonals')
     band = np.zeros((n, RBAND_NDIAG), dtype=np.float64)
     np.add.at(band, (coo.row[keep], k[keep]), coo.data[keep])
+    return band
+
+
+def _dia_band(R, n=None):
+    """resolution_band of a dia_matrix by slicing its diagonals: row p of
+    diagonal offset `off` holds R[p, p+off] = data[d, p+off]."""
+    nrow, ncol = R.shape
+    if n is None:
+        n = nrow
+    band = np.zeros((n, RBAND_NDIAG), dtype=np.float64)
+    data = np.asarray(R.data)
+    L = min(ncol, data.shape[1])
+    for d, off in enumerate(np.asarray(R.offsets).astype(np.int64)):
+        p0, p1 = max(0, -off), min(n, nrow, L - off)
+        if p1 <= p0:
+            continue
+        vals = data[d, p0 + off:p1 + off]
+        if abs(off) > RBAND_HALF:
+            if np.any(vals != 0):
+                raise NotImplementedError('resolution matrix wider than 11 diagonals')
+            continue
+        band[p0:p1, off + RBAND_HALF] += vals
     return band
 
 

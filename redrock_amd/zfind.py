@@ -193,13 +193,20 @@ def zfind(targets, templates, mp_procs=1, nminima=3, archetypes=None, priors=Non
     Returns (allresults, allzfit): allresults[targetid][fulltype] =
     dict(redshifts, zchi2, penalty, zcoeff); allzfit the dict of columns.
     """
+    import time
+    clock = time.perf_counter
+    timings = dict(fitz=dict())
+    last_timings.clear()
+    t0 = clock()
     if isinstance(priors, str):
         priors = Priors(priors)
     local = _local_targets(targets)
     dist = targets if hasattr(targets, 'local_target_ids') else _Local(local)
+    from .zscan import target_batches
+    batches = target_batches(local) if local else dict()
     if chi2_scan is None:                                    # zfind.py:289-292
         results = calc_zchi2_targets(dist, templates, mp_procs=mp_procs, use_gpu=use_gpu,
-                                     precision=precision)
+                                     precision=precision, batches=batches)
     else:
         from .results import read_zscan_redrock
         saved = read_zscan_redrock(chi2_scan)
@@ -210,6 +217,7 @@ def zfind(targets, templates, mp_procs=1, nminima=3, archetypes=None, priors=Non
             results[tg.id] = {ft: {k: np.asarray(v, dtype=np.float64) for k, v in d.items()}
                               for ft, d in saved[tg.id].items()}
 
+    timings['scan'] = clock() - t0
     if priors is not None:                                   # zfind.py:318-322
         for tid in results:
             for ft in results[tid]:
@@ -223,15 +231,18 @@ def zfind(targets, templates, mp_procs=1, nminima=3, archetypes=None, priors=Non
             continue
         chi2 = np.stack([results[tg.id][ft]['zchi2'] + results[tg.id][ft]['penalty']
                          for tg in local])
+        tf = clock()
         archetype = None
         if archetypes and t.template.template_type in archetypes:
             archetype = archetypes[t.template.template_type]
         fits = fitz_targets(chi2, t.template.redshifts, local, t.template, nminima=nminima,
                             zminfit_npoints=zminfit_npoints, newton=newton, archetype=archetype,
                             deg_legendre=deg_legendre, per_camera=bool(per_camera),
-                            n_nearest=n_nearest, prior_sigma=prior_sigma)
+                            n_nearest=n_nearest, prior_sigma=prior_sigma,
+                            batch=next(iter(batches.values())) if len(batches) == 1 else None)
         for tg, zfit in zip(local, fits):
             results[tg.id][ft]['zfit'] = zfit
+        timings['fitz'][ft] = clock() - tf
 
     for tg in local:
         results[tg.id]['meta'] = getattr(tg, 'meta', dict())
@@ -243,10 +254,19 @@ def zfind(targets, templates, mp_procs=1, nminima=3, archetypes=None, priors=Non
         arch = dict(spectypes=list(archetypes.keys()), per_camera=bool(per_camera),
                     deg_legendre=0 if deg_legendre is None else int(deg_legendre),
                     n_nearest=n_nearest, ncamera=ncamera)
+    ta = clock()
     allzfit = _assemble(results, _all_ids(targets), templates, nminima, arch=arch)
     for tid in results:
         results[tid].pop('meta', None)
+    timings['assemble'] = clock() - ta
+    timings['total'] = clock() - t0
+    last_timings.update(timings)
     return results, allzfit
+
+
+# wall-clock seconds of the phases of the last zfind call (scan, fitz per
+# template, assemble, total): host timings for profiling, not part of the API
+last_timings = dict()
 
 
 def _pad_cols(val, width):

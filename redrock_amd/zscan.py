@@ -485,14 +485,17 @@ class ScanGraph(object):
         return out
 
 
-def calc_zchi2(target_ids, target_data, dtemplate, progress=None, use_gpu=False, precision=None):
+def calc_zchi2(target_ids, target_data, dtemplate, progress=None, use_gpu=False, precision=None,
+               batches=None):
     """chi2 vs redshift of every target for one template (zscan.py:807-889).
 
+    batches: {layout signature: TargetBatch} of these targets (target_batches),
+    reused instead of uploading the targets again.
     Returns (zchi2[nt, nz], zcoeff[nt, nz, nbasis], zchi2penalty[nt, nz]).
     """
     cache = dtemplate.cache
     oii = _oiisum(dtemplate.template)
-    zchi2, zcoeff, penalty = scan_targets(cache, list(target_data), precision, oii)
+    zchi2, zcoeff, penalty = scan_targets(cache, list(target_data), precision, oii, batches=batches)
     if progress is not None and dtemplate.comm is None:
         for _ in range(len(target_data)):
             progress.put(1)
@@ -598,17 +601,30 @@ class _ExplicitCache(object):
         return self._bufs[key]
 
 
-def calc_zchi2_targets(targets, templates, mp_procs=1, use_gpu=False, precision=None):
+def target_batches(targets):
+    """{layout signature: TargetBatch} of a target list, in list order per layout
+    (the grouping scan_targets uses)."""
+    groups = dict()
+    for tg in targets:
+        groups.setdefault(PixelLayout.of(tg.spectra).signature, []).append(tg)
+    return {sig: TargetBatch(tgs) for sig, tgs in groups.items()}
+
+
+def calc_zchi2_targets(targets, templates, mp_procs=1, use_gpu=False, precision=None, batches=None):
     """Coarse chi2 scan of every local target for every template (zscan.py:910-1105).
 
+    batches: target_batches(targets.local()) to reuse (built here otherwise,
+    once for all templates).
     Returns results[targetid][fulltype] = dict(redshifts, zchi2, penalty, zcoeff).
     """
     results = {tid: dict() for tid in targets.local_target_ids()}
     local = targets.local()
+    if batches is None and local:
+        batches = target_batches(local)
     for t in templates:
         ft = t.template.full_type
         zchi2, zcoeff, penalty = calc_zchi2(targets.local_target_ids(), local, t,
-                                            precision=precision)
+                                            precision=precision, batches=batches)
         for i, tg in enumerate(local):
             results[tg.id][ft] = dict(redshifts=t.template.redshifts, zchi2=zchi2[i],
                                       penalty=penalty[i], zcoeff=zcoeff[i])

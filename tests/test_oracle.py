@@ -284,3 +284,20 @@ def test_sort_zfit_dict_reference_vectors():
                  rank=np.array(rank))
         orc.sort_zfit_dict(z)
         assert list(z['rank']) == [0, 1, 2]
+
+
+def test_resolution_band_dia_fast_path():
+    """The dia_matrix slicing path of resolution_band equals the generic COO path
+    (non-square shapes, data longer than the matrix, zero wide diagonals)."""
+    import scipy.sparse
+    from redrock_amd.targets import resolution_band
+    rng = np.random.default_rng(4)
+    data = rng.normal(size=(5, 60))
+    offs = np.array([0, 2, -3, 5, 7])
+    data[4] = 0.0
+    for shape in ((50, 55), (55, 50), (60, 60)):
+        R = scipy.sparse.dia_matrix((data, offs), shape=shape)
+        assert np.array_equal(resolution_band(R), resolution_band(R.tocsr()))
+    data[4, 10] = 1.0
+    with pytest.raises(NotImplementedError):
+        resolution_band(scipy.sparse.dia_matrix((data, offs), shape=(50, 55)))
