@@ -68,3 +68,39 @@ def test_graph_f64_matches_oracle(gpu):
         assert np.max(np.abs(c2 - rc2) / np.abs(rc2)) <= 1e-9, tm.full_type
         assert np.array_equal(c2.argmin(axis=1), rc2.argmin(axis=1)), tm.full_type
         assert np.allclose(pen, rpen, rtol=1e-7, atol=1e-9 * max(np.abs(rpen).max(), 1.0))
+
+
+@pytest.mark.parametrize('precision', ['f64', 'f16x3', 'f16', 'f32'])
+def test_batch_invariance(gpu, precision):
+    """A target's chi2 does not depend on which targets share its launch, their
+    order or the tile shape: every (target, z) sum runs in a fixed order in its
+    own lane or MFMA column (the GPU analogue of the reference's
+    multi-process bit-identity test, test_zscan.py:113-140)."""
+    from redrock_amd import zscan, autotune
+    targets, types, dts, dwave = _setup(n=18)
+    print(f'case: batch invariance {precision}, 18 targets', flush=True)
+    for dt in dts[:2]:
+        oii = zscan._oiisum(dt.template)
+        full = zscan.scan_targets(dt.cache, targets, precision, oii)
+        perm = np.random.default_rng(1).permutation(len(targets))
+        shuffled = zscan.scan_targets(dt.cache, [targets[i] for i in perm], precision, oii)
+        alone = [zscan.scan_targets(dt.cache, [targets[i]], precision, oii) for i in (0, 7, 17)]
+        for k in range(3):
+            assert np.array_equal(shuffled[k], full[k][perm])
+        for i, one in zip((0, 7, 17), alone):
+            for k in range(3):
+                assert np.array_equal(one[k][0], full[k][i])
+        if precision in ('f16x3', 'f16'):
+            # every tile configuration of the kernel gives the same doubles
+            batch = zscan.TargetBatch(targets)
+            nt, nz = len(targets), dt.cache.nz
+            ref = None
+            for tile in range(len(autotune.tile_table(precision))):
+                plan = zscan.ScanPlan(dt.cache, batch, np.arange(nt), np.zeros(nt), np.full(nt, nz),
+                                      precision, oiisum=oii)
+                plan.desc.tile = tile
+                plan.run()
+                c2 = plan.host()[0]
+                if ref is None:
+                    ref = c2
+                assert np.array_equal(c2, ref), f'tile {tile}'
