@@ -365,8 +365,10 @@ class ArchetypeFit(object):
     def _chunk(self, fused):
         import torch
         free, _ = torch.cuda.mem_get_info()
-        # per pair: the archetype cache, or the fused path's geometry and normal equations
-        per = self.P * 8 + self.arch.narch * 24 * 8 if fused else (self.P + 2 * self.HALO) * self.arch.narch * 8
+        # per pair: the archetype cache, or the fused path's per-pixel sample
+        # geometry and weights (8 + 6 x 8 bytes a row) and normal equations
+        per = ((self.P + 8) * 56 + self.arch.narch * 24 * 8 if fused
+               else (self.P + 2 * self.HALO) * self.arch.narch * 8)
         budget = int(min(max(free // 8, 256 << 20), 8 << 30))
         return int(min(max(1, budget // per), 4 * 65535))      # 4 pairs per grid row
 
@@ -431,7 +433,9 @@ class ArchetypeFit(object):
         lines: (line, A, B) list of that factor.  tpix: per pair a pixel
         transmission array multiplied into the archetypes instead (the
         one-target API; it uses the cache path).  fused=False rebins into an
-        HBM cache first instead of inside the fit kernel (same doubles).
+        HBM cache first (rr_rebin) instead of inside the fit kernel, where each
+        bin mean is the same trapezoid written as weights of its samples
+        (equal to a few ulp).
         Returns (chi2 (npair,), [coeff], [fulltype]) with
         coeff = archetype coefficient(s) then the Legendre coefficients.
         Raises ValueError when a redshifted archetype does not cover the

@@ -65,8 +65,15 @@ struct ArchArgs {
     const double *x, *yt, *pe_lo, *pe_hi, *zrebin;
     long long ldy;
     int nx, npix;
-    int2 *geom;   // (npair, npix_pad): (j, jend), jend -1 = bin inside one sample step, -2 = zero row
+    // fused rebin, per (pair, row) for rows 0 .. npix_pad + kGeomPad - 1:
+    //   geom (k0, n >= 0): the bin mean is sum_k wts[k] y[k0 + k], k < kNW
+    //   geom (j, -3 - jend): more than kNW samples, the exact expressions
+    int2 *geom;
+    double *wts;  // (npair, npix_pad + kGeomPad, kNW)
 };
+
+constexpr int kNW = 6;          // sample weights per bin (4 interior samples)
+constexpr int kGeomPad = 8;     // zero rows after npix_pad: the sliding window reads ahead
 
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
@@ -142,12 +149,16 @@ __global__ __launch_bounds__(256) void arch_prep_kernel(const double *__restrict
 
 // ---------------------------------------------------------------- fused rebin
 __global__ __launch_bounds__(256) void arch_geom_kernel(ArchArgs a) {
+    const int rowsg = a.npix_pad + kGeomPad;
     const long long idx = (long long)blockIdx.x * 256 + threadIdx.x;
-    const long long n = (long long)a.npair * a.npix_pad;
+    const long long n = (long long)a.npair * rowsg;
     if (idx >= n) return;
-    const int pair = (int)(idx / a.npix_pad);
-    const int r = (int)(idx - (long long)pair * a.npix_pad);
-    int2 g = make_int2(1, -2);
+    const int pair = (int)(idx / rowsg);
+    const int r = (int)(idx - (long long)pair * rowsg);
+    int2 g = make_int2(0, 0);                 // zero row: no weights
+    double w[kNW];
+#pragma unroll
+    for (int k = 0; k < kNW; ++k) w[k] = 0.0;
     if (r < a.npix) {
         const double *x = a.x;
         const int nx = a.nx;
@@ -165,9 +176,53 @@ __global__ __launch_bounds__(256) void arch_geom_kernel(ArchArgs a) {
             jend = j;
             while (jend + 1 < nx - 1 && x[jend + 1] * opz < ehi) ++jend;
         }
-        g = make_int2(j, jend);
+        // the trapezoid of rebin.py:144-176 as weights of the samples
+        // j-1 .. jend+1 (a linear map of y), divided by the bin width
+        const double xj = x[j] * opz, xjm1 = x[j - 1] * opz;
+        const double width = ehi - elo;
+        if (jend < 0) {
+            const double u = (0.5 * (elo + ehi) - xj) / (xj - xjm1);
+            w[0] = -u;
+            w[1] = 1.0 + u;
+            g = make_int2(j - 1, 2);
+        } else if (jend - j + 3 <= kNW) {
+            const double t0 = (elo - xjm1) / (xj - xjm1), d0 = xj - elo;
+            double c[kNW];
+#pragma unroll
+            for (int k = 0; k < kNW; ++k) c[k] = 0.0;
+            c[0] = 0.5 * d0 * (1.0 - t0);
+            c[1] = 0.5 * d0 * (1.0 + t0);
+            double xprev = xj;
+            for (int jj = j + 1; jj <= jend; ++jj) {
+                const double xc = x[jj] * opz, h = 0.5 * (xc - xprev);
+                const int m = jj - j;              // weight slot of sample jj - 1
+#pragma unroll
+                for (int k = 0; k < kNW - 1; ++k)
+                    if (k == m) {
+                        c[k] += h;
+                        c[k + 1] += h;
+                    }
+                xprev = xc;
+            }
+            const double xe1 = x[jend + 1] * opz, d1 = ehi - xprev;
+            const double t1 = d1 / (xe1 - xprev);
+            const int m = jend - j + 1;            // weight slot of sample jend
+#pragma unroll
+            for (int k = 0; k < kNW - 1; ++k)
+                if (k == m) {
+                    c[k] += 0.5 * d1 * (2.0 - t1);
+                    c[k + 1] += 0.5 * d1 * t1;
+                }
+#pragma unroll
+            for (int k = 0; k < kNW; ++k) w[k] = c[k] / width;
+            g = make_int2(j - 1, jend - j + 3);
+        } else {
+            g = make_int2(j, -3 - jend);          // exact expressions in the fit kernel
+        }
     }
     a.geom[idx] = g;
+#pragma unroll
+    for (int k = 0; k < kNW; ++k) a.wts[idx * kNW + k] = w[k];
 }
 
 // Bin mean of archetype column `col` for geometry g at 1+z = opz (the
@@ -252,13 +307,28 @@ __global__ __launch_bounds__(256) void arch_accum_kernel(ArchArgs a) {
     const bool igm = a.nlines > 0 && zi == zi;
     const double opz = 1.0 + zi;
     const double opzr = FUSED ? 1.0 + a.zrebin[pair] : 1.0;
-    const int2 *G = FUSED ? a.geom + (long long)pair * npp : nullptr;
-    // fused: the window of rows p0-5 .. p0+P+4 slides by P; row i enters once
+    const long long growb = (long long)pair * (npp + kGeomPad);
+    const int2 *G = FUSED ? a.geom + growb : nullptr;
+    const double *Wg = FUSED ? a.wts + growb * kNW : nullptr;
+    // fused: the window of rows p0-5 .. p0+P+4 slides by P; row i enters once.
+    // A row is sum_k w_k y[k0 + k]: the weights and k0 are the same for every
+    // lane (scalar loads), the samples one coalesced load each.
     double fw[NA][P + 10];
     auto fused_row = [&](int pix, int j) -> double {
-        if (pix < 0 || pix >= a.npix) return 0.0;
-        const double v = bin_mean(a, G[pix], opzr, a.pe_lo[pix], a.pe_hi[pix], col[j]);
-        return igm ? v * igm_factor(a, a.wave[pix], opz) : v;
+        if (pix < 0) return 0.0;
+        const int2 gg = G[pix];
+        double v = 0.0;
+        if (gg.y >= 0) {
+            const double *Yc = a.yt + col[j];
+#pragma unroll
+            for (int k = 0; k < kNW; ++k) {
+                const int sidx = gg.x + k < a.nx ? gg.x + k : a.nx - 1;
+                v = fma(Wg[(long long)pix * kNW + k], Yc[(long long)sidx * a.ldy], v);
+            }
+        } else {
+            v = bin_mean(a, make_int2(gg.x, -3 - gg.y), opzr, a.pe_lo[pix], a.pe_hi[pix], col[j]);
+        }
+        return (igm && pix < a.npix) ? v * igm_factor(a, a.wave[pix], opz) : v;
     };
     if (FUSED) {
 #pragma unroll
@@ -561,7 +631,7 @@ template <int NA, int NL>
 int launch_fit(const ArchArgs &a, hipStream_t s) {
     dim3 g1((a.nprob + 63) / 64, (a.npair + 3) / 4);
     if (a.geom) {
-        const long long n = (long long)a.npair * a.npix_pad;
+        const long long n = (long long)a.npair * (a.npix_pad + kGeomPad);
         hipLaunchKernelGGL(arch_geom_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, a);
         int rc = rr::check_launch("rr_arch_fit(geometry)");
         if (rc) return rc;
@@ -644,8 +714,8 @@ extern "C" int rr_arch_prep(const double *d_rband, const double *d_w, const doub
 extern "C" long long rr_arch_workspace_bytes(const rr_arch_desc *d) {
     if (!d || d->npair < 0 || d->nprob < 0 || d->na < 1 || d->na > kMaxNA || d->nleg < 0 || d->nleg > kMaxNL)
         return -1;
-    const long long geom = d->d_acache ? 0 : (long long)d->npair * d->npix_pad * 8;
-    return ne_bytes(d) + geom;
+    const long long rows = d->d_acache ? 0 : (long long)d->npair * (d->npix_pad + kGeomPad);
+    return ne_bytes(d) + rows * 8 + rows * kNW * 8;
 }
 
 extern "C" int rr_arch_fit(const rr_arch_desc *d, void *stream) {
@@ -710,6 +780,9 @@ extern "C" int rr_arch_fit(const rr_arch_desc *d, void *stream) {
     a.npix = fused ? d->npix : d->npix_pad;
     a.zrebin = d->d_zrebin;
     a.geom = fused ? reinterpret_cast<int2 *>(static_cast<char *>(d->d_work) + ne_bytes(d)) : nullptr;
+    a.wts = fused ? reinterpret_cast<double *>(static_cast<char *>(d->d_work) + ne_bytes(d) +
+                                               (long long)d->npair * (d->npix_pad + kGeomPad) * 8)
+                  : nullptr;
     hipStream_t s = static_cast<hipStream_t>(stream);
     switch (d->na) {
         case 1: return launch_na<1>(a, d->nleg, s);
