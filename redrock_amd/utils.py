@@ -4,6 +4,14 @@ from __future__ import annotations
 import numpy as np
 
 
+def native_endian(data):
+    """`data` in native byte order: the same array when it already is, else a
+    byte-swapped copy (utils.py:19-40)."""
+    if data.dtype.isnative:
+        return data
+    return data.astype(data.dtype.newbyteorder('='))
+
+
 def reduced_wavelength(wave):
     """wave mapped linearly onto [-1, 1], the domain of the Legendre
     polynomials of the archetype fit (utils.py:260-271)."""

@@ -115,3 +115,17 @@ def test_interp_extrapolate():
     y = np.array([1.0, 3.0, 4.0])
     got = archetypes._interp_extrapolate(np.array([0.0, 1.5, 3.0, 6.0]), x, y)
     assert np.allclose(got, [-1.0, 2.0, 3.5, 5.0])
+
+
+def test_reduced_wavelength_and_native_endian():
+    """test_utils.py:41-62 and test_io.py:35-43 of the reference."""
+    from redrock_amd.utils import native_endian
+    x = reduced_wavelength(np.arange(10))
+    assert x[0] == -1.0 and x[-1] == 1.0
+    x = reduced_wavelength(np.random.default_rng(0).uniform(-5, 20, size=100))   # out of order
+    assert np.min(x) == -1.0 and np.max(x) == 1.0
+    big, little = np.arange(5, dtype='>f4'), np.arange(5, dtype='<f4')
+    assert native_endian(big).dtype.isnative and native_endian(little).dtype.isnative
+    native = little if little.dtype.isnative else big
+    assert native_endian(native) is native
+    assert np.array_equal(native_endian(big), big)

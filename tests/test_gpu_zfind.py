@@ -120,3 +120,17 @@ def test_zfind_resumes_from_details_file(gpu, tmp_path):
     # read_zfit stacks the targets in the order of the zbest table: zfind's order
     zfit = results.read_zfit(path)
     assert np.array_equal(zfit['z'], a['z']) and np.array_equal(zfit['targetid'], a['targetid'])
+    # test_io.py:45-81 of the reference: same columns; z and targetid keep their
+    # precision, the other columns equal after the writer's conversions
+    zscan2, zfit2 = results.read_zscan(path)
+    assert list(zfit2.keys()) == list(a.keys())
+    assert zfit2['z'].dtype == a['z'].dtype and zfit2['targetid'].dtype == a['targetid'].dtype
+    for cn in a:
+        d2 = zfit2[cn]
+        assert np.array_equal(np.asarray(a[cn]).astype(d2.dtype), d2), cn
+    for tid in allresults:
+        for ft in allresults[tid]:
+            for key, d1 in allresults[tid][ft].items():
+                if key != 'redshifts':
+                    d1 = d1.astype(np.float32)
+                assert np.array_equal(d1, zscan2[tid][ft][key]), (tid, ft, key)

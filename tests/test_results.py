@@ -259,3 +259,27 @@ def test_write_zscan_roundtrip(tmp_path):
         results.write_zscan(str(tmp_path / 'sub' / 'rrdetails.h5' / 'x.h5'), allresults, zfit)
     results.write_zscan(path, allresults, zfit, clobber=True)
     assert results.read_zscan_redrock(path).keys() == zscan.keys()
+
+
+def test_select_targetids_like_reference():
+    """test_io.py:83-140 of the reference on its own details file."""
+    zscan, zfit = results.read_zscan(DETAILS)
+    for tid in np.unique(zfit['targetid']):
+        for ft, zz in zscan[tid].items():
+            assert len(zz['redshifts']) == len(zz['zchi2']) and len(zz['zfit']['z']) > 0
+    tids = np.unique(zfit['targetid'])
+    _, one = results.read_zscan(DETAILS, select_targetids=tids[1])
+    assert list(np.unique(one['targetid'])) == [tids[1]]
+    _, two = results.read_zscan(DETAILS, select_targetids=tids[0:2])
+    assert np.array_equal(np.unique(two['targetid']), tids[0:2])
+    _, up = results.read_zscan(DETAILS, upper=True)
+    assert 'TARGETID' in up
+    only = results.read_zscan(DETAILS, nozfit=True)
+    assert isinstance(only, dict) and tids[0] in only
+    assert len(np.unique(results.read_zfit(DETAILS)['targetid'])) == 3
+    z2 = results.read_zfit(DETAILS, select_targetids=tids[0:2])
+    assert np.array_equal(np.unique(z2['targetid']), tids[0:2])
+    z1 = results.read_zfit(DETAILS, select_targetids=int(tids[1]))
+    assert list(np.unique(z1['targetid'])) == [tids[1]]
+    zu = results.read_zfit(DETAILS, upper=True)
+    assert 'TARGETID' in zu and 'Z' in zu and 'z' not in zu
