@@ -138,6 +138,42 @@ class Target(object):
             return {hs: torch.as_tensor(v, device='cuda') for hs, v in self._legendre.items()}
         return self._legendre
 
+    def eval_model(self, bestfit, templates, archetypes=None, nleg=None):
+        """Best-fit model of every camera, {wavehash: flux}, also kept as
+        self.model (targets.py:184-234).
+
+        bestfit: one fit row with SPECTYPE, SUBTYPE, COEFF, Z, FITMETHOD
+        (upper or lower case; LEGCOEFF for archetype fits).  templates:
+        {(spectype, subtype): Template}; archetypes: {spectype: Archetype}.
+        All-zero coefficients (fully masked data) give zero models.
+        """
+        def field(name):
+            try:
+                return bestfit[name]
+            except (KeyError, ValueError):
+                return bestfit[name.lower()]
+
+        spectype, subtype = field('SPECTYPE'), field('SUBTYPE')
+        coeff, z, fitmethod = np.asarray(field('COEFF')), field('Z'), field('FITMETHOD')
+        self.model = dict()
+        if np.all(coeff == 0.0):
+            for sp in self.spectra:
+                self.model[sp.wavehash] = np.zeros(len(sp.wave), dtype=np.float32)
+        elif fitmethod == 'ARCH':
+            narch = len(subtype.split(';'))
+            legcoeff = np.asarray(field('LEGCOEFF'))
+            if legcoeff.ndim == 1:
+                legcoeff = [legcoeff for _ in self.spectra]
+            ax = archetypes[spectype]
+            for i, sp in enumerate(self.spectra):
+                self.model[sp.wavehash] = ax.eval(subtype, coeff[0:narch], sp.wave, z, R=sp.Rcsr,
+                                                  legcoeff=legcoeff[i])
+        else:
+            tx = templates[(spectype, subtype)]
+            for sp in self.spectra:
+                self.model[sp.wavehash] = tx.eval(coeff, sp.wave, z, sp.Rcsr)
+        return self.model
+
     def gpu_spectral_data(self):
         """Concatenated (weights, flux, wflux) as device tensors (targets.py:256-263)."""
         import torch
@@ -174,6 +210,21 @@ class DistTargets(object):
 
     def local(self):
         return self._local_data()
+
+    def eval_models(self, bestfit, templates, archetypes=None, nleg=None):
+        """Target.eval_model of each local target, with its row of bestfit
+        found by TARGETID (targets.py:442-461)."""
+        cols = bestfit.dtype.names if hasattr(bestfit, 'dtype') else list(bestfit.keys())
+        tid = bestfit['TARGETID'] if 'TARGETID' in cols else bestfit['targetid']
+        models = list()
+        for tgt in self.local():
+            i = np.where(np.asarray(tid) == tgt.id)[0][0]
+            if hasattr(bestfit, 'dtype'):
+                row = bestfit[i]
+            else:
+                row = {k: v[i] for k, v in bestfit.items()}
+            models.append(tgt.eval_model(row, templates, archetypes, nleg))
+        return models
 
     def wavegrids(self):
         """Dictionary of wavelength grids keyed by wavehash (targets.py:468-489)."""

@@ -9,6 +9,7 @@ keeps the cache in HBM for the whole scan.
 """
 from __future__ import annotations
 
+import glob
 import os
 
 import numpy as np
@@ -17,13 +18,94 @@ valid_template_methods = ('PCA', 'NMF')
 
 
 def get_template_dir(template_path=None):
-    """Template directory: template_path, else $RR_TEMPLATE_DIR (templates.py:320-345)."""
+    """template_path if given, else $RR_TEMPLATE_DIR, else the package's
+    data/templates directory (templates.py:300-308).  The path is returned
+    whether or not it exists; find_templates checks it."""
     if template_path is not None:
         return template_path
     d = os.environ.get('RR_TEMPLATE_DIR')
     if d is None:
-        raise IOError('RR_TEMPLATE_DIR is not set and no template path was given')
+        d = os.path.join(os.path.dirname(os.path.abspath(__file__)), 'data', 'templates')
     return d
+
+
+def find_templates(template_path=None):
+    """List of template files to use (templates.py:311-380).
+
+    template_path is None (get_template_dir()), a list/tuple/set/array of
+    files (returned as given), a directory (its templates-default.txt if
+    present, else every rrtemplate-*.fits in it, sorted), a .txt file naming
+    one template per line (blank lines and '#' comments skipped, trailing
+    text ignored, relative names taken from the .txt file's directory), or a
+    single .fits / .fits.gz / .fits.fz file.  ValueError for a path that does
+    not exist, an unrecognised suffix, or a listed file that is missing.
+    """
+    template_path = get_template_dir(template_path)
+    if isinstance(template_path, (list, tuple, set, np.ndarray)):
+        return template_path
+    template_path = os.fspath(template_path)
+    if not os.path.exists(template_path):
+        raise ValueError(f"can't find template_path {template_path!r}, $RR_TEMPLATE_DIR "
+                         'or the package template directory')
+    if os.path.isdir(template_path):
+        listfile = os.path.join(template_path, 'templates-default.txt')
+        template_dir = template_path
+    elif template_path.endswith('.txt'):
+        listfile = template_path
+        template_dir = os.path.dirname(os.path.abspath(template_path))
+    elif template_path.endswith(('.fits', '.fits.gz', '.fits.fz')):
+        return [template_path]
+    else:
+        raise ValueError(f'Unrecognized template_path={template_path!r}')
+
+    if not os.path.exists(listfile):
+        return sorted(glob.glob(os.path.join(template_dir, 'rrtemplate-*.fits')))
+    files = list()
+    with open(listfile) as fx:
+        for line in fx:
+            line = line.strip()
+            if len(line) < 2 or line.startswith('#'):
+                continue
+            name = line.split()[0]
+            if not name.startswith('/'):
+                name = f'{template_dir}/{name}'
+            if not os.path.exists(name):
+                raise ValueError(f'missing {name} given in {listfile}')
+            files.append(name)
+    return files
+
+
+def header2templatefiles(hdr, template_dir=None):
+    """Template file names from header keywords (templates.py:383-423).
+
+    For nn = 00..99: TEMFILnn names the file; otherwise TEMNAMnn (a fulltype)
+    and TEMVERnn give rrtemplate-{spectype}-{subtype}-v{version}.fits, with
+    'None' for a blank subtype and version 0.1 for an 'unknown' STAR version
+    (ValueError for another spectype).  Each name is joined to template_dir
+    (default get_template_dir()).
+    """
+    names = list()
+    for i in range(100):
+        filekey, typekey, verkey = f'TEMFIL{i:02d}', f'TEMNAM{i:02d}', f'TEMVER{i:02d}'
+        if filekey in hdr:
+            names.append(hdr[filekey])
+        elif typekey in hdr and verkey in hdr:
+            version = hdr[verkey].strip()
+            spectype, subtype = parse_fulltype(hdr[typekey])
+            if version == 'unknown':
+                if spectype != 'STAR':
+                    raise ValueError(f'unknown version for spectype={spectype!r}')
+                version = '0.1'
+            subtype = subtype if subtype != '' else None
+            names.append(f'rrtemplate-{spectype}-{subtype}-v{version}.fits')
+    if template_dir is None:
+        template_dir = get_template_dir()
+    return [os.path.join(template_dir, n) for n in names]
+
+
+def load_templates_from_header(hdr, template_dir=None, asdict=False):
+    """Templates named by header keywords (templates.py:426-438)."""
+    return load_templates(header2templatefiles(hdr, template_dir=template_dir), asdict=asdict)
 
 
 def _read_template_file(filename):
@@ -163,6 +245,19 @@ class Template(object):
         if self._rrtype == 'GALAXY' and self._method == 'PCA':
             isOII = (3724 <= self.wave) & (self.wave <= 3733)
             self.OIItemplate = self.flux[:, isOII]
+
+    def eval(self, coeff, wave, z, R=None):
+        """Template flux for coefficients coeff at redshift z on wavelengths
+        wave, convolved with R if given (templates.py:247-272).  No (1+z)
+        factor is applied and there is no IGM: model = rebin(flux'coeff).
+        """
+        from .rebin import trapz_rebin
+        coeff = np.asarray(coeff, dtype=np.float64)[0:self._nbasis]
+        flux = self.flux.T.dot(coeff)
+        model = trapz_rebin(self.wave * (1 + z), flux, wave)
+        if R is not None:
+            model = R.dot(model)
+        return model
 
     def __str__(self):
         zmin = round(self._redshifts[0], 4)
@@ -311,7 +406,75 @@ def write_template(path, template, with_redshifts=True, loglam=False):
     fits.write(path, hdus)
 
 
-def load_templates(filenames, zscan_galaxy=None, zscan_qso=None, zscan_star=None):
-    """Template objects for a list of template files (templates.py:394-444)."""
-    return [Template(f, zscan_galaxy=zscan_galaxy, zscan_qso=zscan_qso, zscan_star=zscan_star)
-            for f in filenames]
+def load_templates(template_path=None, zscan_galaxy=None, zscan_star=None, zscan_qso=None,
+                   asdict=False):
+    """Template objects (templates.py:440-474).
+
+    template_path: a list of template files, or anything find_templates
+    takes (None, a directory, a .txt list, one .fits file).  zscan_*:
+    'zmin,zmax,dz' redshift ranges for files without a REDSHIFTS HDU.
+    Returns a list, or with asdict a dict keyed by (spectype, subtype).
+    """
+    if isinstance(template_path, (str, os.PathLike, type(None))):
+        template_path = find_templates(template_path)
+    templates = [Template(f, zscan_galaxy=zscan_galaxy, zscan_star=zscan_star, zscan_qso=zscan_qso)
+                 for f in template_path]
+    if asdict:
+        return {(t.template_type, t.sub_type): t for t in templates}
+    return templates
+
+
+def load_dist_templates(dwave, templates=None, comm=None, mp_procs=1,
+                        zscan_galaxy=None, zscan_qso=None, zscan_star=None,
+                        redistribute=False, use_gpu=False, gpu_mode=False):
+    """One DistTemplate per template, each rebinned to every grid of dwave
+    (templates.py:715-778).
+
+    templates is what load_templates takes.  With one process the whole
+    redshift grid is local, so redistribute (the reference's allgather of
+    the slices) changes nothing; the rebinned caches always live in HBM.
+    """
+    if comm is not None:
+        raise NotImplementedError('MPI distribution is outside the chi2-scan path')
+    tdict = load_templates(templates, asdict=True, zscan_galaxy=zscan_galaxy,
+                           zscan_star=zscan_star, zscan_qso=zscan_qso)
+    return [DistTemplate(t, dwave, mp_procs=mp_procs, comm=comm, use_gpu=use_gpu, gpu_mode=gpu_mode)
+            for t in tdict.values()]
+
+
+def eval_model(data, wave, R=None, templates=None):
+    """Model spectra of a table of fits (templates.py:781-832).
+
+    data: table-like with Z, COEFF, SPECTYPE, SUBTYPE columns (upper or
+    lower case).  wave: wavelengths, or a dict of them per camera, in which
+    case R must be None or a dict with the same keys and the result is a dict.
+    R: per-row resolution matrices (anything with .dot), or None for no
+    convolution.  templates: {(spectype, subtype): Template}, default
+    load_templates(asdict=True).  Returns float32 (nrow, nwave).
+    """
+    if templates is None:
+        templates = load_templates(asdict=True)
+    if isinstance(wave, dict):
+        if R is None:
+            R = {k: None for k in wave}
+        elif not isinstance(R, dict) or set(R.keys()) != set(wave.keys()):
+            raise ValueError('R must be dict with same keys as wave')
+        return {k: eval_model(data, wave[k], R=R[k], templates=templates) for k in wave}
+
+    def col(name):
+        return data[name] if name in _columns(data) else data[name.lower()]
+
+    z, coeff, spectype, subtype = col('Z'), col('COEFF'), col('SPECTYPE'), col('SUBTYPE')
+    models = np.zeros((len(z), len(wave)), dtype='f4')
+    for i in range(len(z)):
+        tx = templates[(spectype[i], subtype[i])]
+        models[i] = tx.eval(np.asarray(coeff[i])[0:tx.nbasis], wave, z[i],
+                            R=None if R is None else R[i])
+    return models
+
+
+def _columns(data):
+    names = getattr(getattr(data, 'dtype', None), 'names', None)
+    if names is not None:
+        return names
+    return data.keys() if hasattr(data, 'keys') else data.colnames
