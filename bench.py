@@ -135,8 +135,11 @@ def bench_mixed(args):
     evals = nt * sum(len(t.redshifts) for t in tms)
     out = {}
     res = {}
-    for p in (args.precision, 'f64'):
-        sg = zscan.ScanGraph(caches, batch, p, tune=(p != 'f64'))
+    runs = [(args.precision, not args.no_polish)] + ([('raw', False)] if not args.no_polish else [])
+    runs.append(('f64', False))
+    for key, pol in runs:
+        p = args.precision if key == 'raw' else key
+        sg = zscan.ScanGraph(caches, batch, p, tune=(p != 'f64'), polish=pol)
         sg.capture()
         for _ in range(args.warmup):
             sg.replay()
@@ -147,9 +150,10 @@ def bench_mixed(args):
             sg.replay()
         torch.cuda.synchronize()
         dt_s = (time.perf_counter() - t) / k
-        out[p] = dt_s
-        res[p] = sg.results()
-        log(f'{p}: {dt_s * 1e3:.2f} ms per graph replay ({evals / dt_s / 1e6:.1f} M evals/s)')
+        out[key] = dt_s
+        res[key] = sg.results()
+        log(f'{key}{" + FP64 polish" if pol else ""}: {dt_s * 1e3:.2f} ms per graph replay '
+            f'({evals / dt_s / 1e6:.1f} M evals/s)')
         del sg
     rel, agree, agree_type = [], [], []
     best = {p: np.stack([r[0].min(axis=1) for r in res[p]], axis=1) for p in res}
@@ -166,6 +170,17 @@ def bench_mixed(args):
             f'best-z agreement {agree[-1]:.4f}')
     agree_type = float(np.mean(best[args.precision].argmin(axis=1) == best['f64'].argmin(axis=1)))
     rel = np.concatenate([r.ravel() for r in rel])
+    unpolished = None
+    if 'raw' in res:
+        unpolished = {'value': evals / out['raw'], 'ms_per_step': out['raw'] * 1e3, 'per_template': {}}
+        for tm, (c2m, _, _), (c2d, _, _) in zip(tms, res['raw'], res['f64']):
+            r = np.abs(c2m - c2d) / np.abs(c2d)
+            a = float(np.mean(c2m.argmin(axis=1) == c2d.argmin(axis=1)))
+            unpolished['per_template'][tm.template_type] = {
+                'best_z_agreement': a, 'max_rel_chi2_err': float(r.max()),
+                'max_rel_chi2_err_at_best_z': float(r[np.arange(nt), c2d.argmin(axis=1)].max())}
+            log(f'{tm.template_type} without the polish: best-z agreement {a:.4f}, '
+                f'max rel err {r.max():.2e}')
     print(json.dumps({
         'metric': 'zchi2 evals/sec on 1 GPU, mixed GALAXY/QSO/STAR batch in one HIP graph',
         'value': evals / out[args.precision], 'unit': 'evals/s', 'n_gpus': 1,
@@ -181,6 +196,10 @@ def bench_mixed(args):
         'best_template_agreement': agree_type,
         'max_rel_chi2_err': float(rel.max()), 'median_rel_chi2_err': float(np.median(rel)),
         'per_template': per_template,
+        'polish': (None if args.no_polish else
+                   {'tol': zscan.POLISH_TOL[args.precision], 'max_windows': zscan.POLISH_WINDOWS,
+                    'window': 16}),
+        'unpolished': unpolished,
     }), flush=True)
 
 
@@ -481,6 +500,9 @@ def main():
     ap.add_argument('--grad-targets', type=int, default=5000)
     ap.add_argument('--narch', type=int, default=256, help='archetypes of the archetype workload')
     ap.add_argument('--zfind-targets', type=int, default=2000)
+    ap.add_argument('--no-polish', action='store_true',
+                    help='leave out the FP64 polish of the rows near each minimum after the '
+                         'mixed-precision scan')
     args = ap.parse_args()
     if args.calibrate_copy_mb:
         calibrate_copy(args.calibrate_copy_mb)
@@ -527,7 +549,8 @@ def main():
     batch = zscan.TargetBatch(targets)
     oii = tm.OIItemplate.sum(axis=1) if hasattr(tm, 'OIItemplate') else None
     items = (np.arange(nt), np.zeros(nt), np.full(nt, nz))
-    plans = {p: zscan.ScanPlan(dt.cache, batch, *items, precision=p, oiisum=oii)
+    plans = {p: zscan.ScanPlan(dt.cache, batch, *items, precision=p, oiisum=oii,
+                               polish=(p != 'f64' and not args.no_polish))
              for p in ('f64', args.precision)}
     torch.cuda.synchronize()
     setup_s = time.perf_counter() - t0
@@ -571,6 +594,33 @@ def main():
     f64 = plans['f64']
     t_f64 = timed(f64, max(1, args.steps // 2))
     k_f64 = kernel_ms(f64, max(1, args.steps // 2))
+    polish = None
+    c_raw = None
+    if mix.polish is not None:
+        # the same plan without its polish: time and accuracy of the bare scan,
+        # then the polish stage alone on the bare scan's rows (HIP events)
+        bare = _hip.RR_STAGE_STATS | _hip.RR_STAGE_MAIN | _hip.RR_STAGE_FINISH
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(args.steps):
+            mix.run(stages=bare)
+        torch.cuda.synchronize()
+        t_raw = (time.perf_counter() - t) / args.steps
+        c_raw = mix.zchi2.reshape(nt, nz).cpu().numpy()
+        evs = []
+        for _ in range(args.steps):
+            mix.run(stages=bare)
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            mix.run(stages=zscan.STAGE_POLISH)
+            e1.record()
+            evs.append((e0, e1))
+        torch.cuda.synchronize()
+        polish = {'tol': zscan.polish_tol(args.precision, True), 'max_windows': zscan.POLISH_WINDOWS,
+                  'window': _hip.RR_POLISH_WINDOW,
+                  'ms': sum(a.elapsed_time(b) for a, b in evs) / len(evs),
+                  'windows_per_target': float(mix.nwindows.float().mean()),
+                  'unpolished_ms_per_step': t_raw * 1e3, 'unpolished_value': evals / t_raw}
     # final full runs so the outputs hold complete results
     mix.run()
     f64.run()
@@ -578,6 +628,15 @@ def main():
 
     c_mix = mix.zchi2.reshape(nt, nz).cpu().numpy()
     c_f64 = f64.zchi2.reshape(nt, nz).cpu().numpy()
+    if polish is not None:
+        rr_ = np.abs(c_raw - c_f64) / np.abs(c_f64)
+        polish.update({
+            'unpolished_best_z_agreement': float(np.mean(c_raw.argmin(axis=1) == c_f64.argmin(axis=1))),
+            'unpolished_max_rel_chi2_err': float(rr_.max()),
+            'unpolished_max_rel_chi2_err_at_best_z': float(rr_[np.arange(nt), c_f64.argmin(axis=1)].max())})
+        log(f'polish: {polish["ms"]:.2f} ms, {polish["windows_per_target"]:.2f} windows per target; '
+            f'without it {t_raw * 1e3:.2f} ms/step, best-z agreement '
+            f'{polish["unpolished_best_z_agreement"]:.4f}')
     bz_mix, bz_f64 = c_mix.argmin(axis=1), c_f64.argmin(axis=1)
     agree = float(np.mean(bz_mix == bz_f64))
     rel = np.abs(c_mix - c_f64) / np.abs(c_f64)
@@ -592,7 +651,8 @@ def main():
     # other precisions on the same inputs: speed, chi2 error vs FP64, best-z agreement
     variants = {}
     for p in [v for v in args.variants.split(',') if v and v != args.precision]:
-        plan = zscan.ScanPlan(dt.cache, batch, *items, precision=p, oiisum=oii)
+        plan = zscan.ScanPlan(dt.cache, batch, *items, precision=p, oiisum=oii,
+                              polish=not args.no_polish)
         plan.run()
         tv = timed(plan, max(2, args.steps // 2))
         plan.run()
@@ -631,6 +691,10 @@ def main():
                                + '3 cameras 7958 px, 11-diagonal R',
                    'targets': nt, 'redshifts': nz, 'nbasis': nb, 'snr_per_pixel': args.snr,
                    'solve': 'FP64 Cholesky',
+                   'polish': (None if args.no_polish else
+                              f'FP64 re-evaluation of the 16-redshift windows within '
+                              f'{zscan.polish_tol(args.precision, True):g} relative of each '
+                              f'target minimum (in the timed step)'),
                    'accumulate': ('fp32 MFMA partial sums over 64 pixels, added in FP64'
                                   if args.precision == 'f16x3' else 'fp32 (MFMA), FP64 after the scan'),
                    'tile': dict(zip(('waves_items', 'waves_z', 'items_per_wave', 'z_per_wave'),
@@ -654,6 +718,7 @@ def main():
                      'kernel_ms': k_mix,
                      'flop_per_eval': FLOP_PER_EVAL[kind], 'evals_per_launch': evals},
         'median_rel_chi2_err': float(np.median(rel)),
+        'polish': polish,
         'variants': variants,
         'cpu_baseline': cpu,
     }

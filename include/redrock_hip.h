@@ -20,7 +20,7 @@
 extern "C" {
 #endif
 
-#define RR_ABI_VERSION 5
+#define RR_ABI_VERSION 6
 
 /* Error codes. */
 #define RR_OK 0
@@ -250,6 +250,42 @@ int rr_zscan_tile_table(int precision, int *cfg, int max);
 int rr_pack_mixed(const double *d_rband, const double *d_w, const double *d_f, int ntarget,
                   int npix_pad, int precision, void *d_rtile, void *d_sfcol,
                   double *d_scales, void *stream);
+
+/*
+ * rr_zscan_polish — FP64 re-evaluation of the rows of a mixed-precision scan
+ * near each item's minimum, so that the item's best redshift is the FP64
+ * scan's.  No counterpart in the reference (its scan is FP64 throughout);
+ * it stands between the mixed scan and the consumers of zchi2 (find_minima
+ * in fitz.py:28-54, the best-redshift choice of zfind.py:458-500).
+ *
+ * For every item of `scan` (a descriptor that rr_zscan has run in full, in a
+ * mixed precision): the windows of RR_POLISH_WINDOW consecutive redshifts
+ * (item z index 16k .. 16k+15) holding a chi2, or chi2 + penalty, within
+ * tol * |minimum| of the item's minimum of that quantity are re-evaluated in
+ * FP64 (R-apply, normal equations, LU solve, compensated chi2, penalty,
+ * status) from the FP64 template cache, and their rows of d_zchi2,
+ * d_zcoeff, d_penalty and d_status are overwritten.  If more than
+ * max_windows windows qualify, the tolerance is halved until they fit (the
+ * windows nearest the minimum are kept).  Items whose minimum chi2 is
+ * HUGE_CHI2 or more are left alone.  With |chi2_mixed - chi2_f64| <= e chi2
+ * and tol >= 2e + e^2 the argmin of every polished item equals the FP64
+ * scan's (ties within FP64 rounding aside).
+ *
+ * scan->precision RR_PREC_F64: nothing to do, returns RR_OK.
+ * Reads the scan's workspace statistics (stage RR_STAGE_STATS); the FP64
+ * cache has the scan's rows, halo and pixel layout.  max_item_nz <= 65536.
+ */
+#define RR_POLISH_WINDOW 16
+#define RR_POLISH_MAX_WINDOWS 64
+typedef struct {
+    const double *d_tcache64;  /* FP64 cache [b][row][z] of the scanned template  */
+    long long cache_ldz;       /* >= scan->cache_nz                                 */
+    double tol;                /* relative threshold, >= 0                          */
+    int max_windows;           /* 1 .. RR_POLISH_MAX_WINDOWS per item               */
+    int *d_windows;            /* (nitems, max_windows) out: polished window k's    */
+    int *d_nwindows;           /* (nitems) out: windows polished per item           */
+} rr_polish_desc;
+int rr_zscan_polish(const rr_zscan_desc *scan, const rr_polish_desc *polish, void *stream);
 
 /*
  * Adjoint problem descriptor for rr_zscan_grad.  One item = one target at one

@@ -11,7 +11,7 @@ import os
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, 'libredrock_hip.so')
 
-RR_ABI_VERSION = 5
+RR_ABI_VERSION = 6
 RR_F64, RR_F32, RR_F16, RR_BF16 = 0, 1, 2, 3
 RR_PREC_F64, RR_PREC_F16, RR_PREC_BF16, RR_PREC_F32, RR_PREC_F16X3 = 0, 1, 2, 3, 4
 RR_STATUS_OK, RR_STATUS_SINGULAR = 0, 1
@@ -23,7 +23,8 @@ PRECISIONS = {'f64': RR_PREC_F64, 'f16': RR_PREC_F16, 'bf16': RR_PREC_BF16, 'f32
 
 EXPORTS = ('rr_abi_version', 'rr_last_error', 'rr_rebin', 'rr_zscan',
            'rr_zscan_workspace_bytes', 'rr_pack_mixed', 'rr_zscan_tiles', 'rr_zscan_tile_table',
-           'rr_zscan_grad', 'rr_arch_prep', 'rr_arch_workspace_bytes', 'rr_arch_fit')
+           'rr_zscan_grad', 'rr_arch_prep', 'rr_arch_workspace_bytes', 'rr_arch_fit',
+           'rr_zscan_polish')
 RR_ARCH_MAX_NA = 4
 RR_ARCH_MAX_NLEG = 12
 
@@ -55,6 +56,16 @@ class ZscanDesc(ctypes.Structure):
 
 
 RR_STAGE_STATS, RR_STAGE_MAIN, RR_STAGE_FINISH = 1, 2, 4
+
+RR_POLISH_WINDOW = 16
+RR_POLISH_MAX_WINDOWS = 64
+
+
+class PolishDesc(ctypes.Structure):
+    _fields_ = [
+        ('d_tcache64', _vp), ('cache_ldz', _ll), ('tol', _d), ('max_windows', _i),
+        ('d_windows', _vp), ('d_nwindows', _vp),
+    ]
 
 
 class ArchDesc(ctypes.Structure):
@@ -96,6 +107,8 @@ def lib():
                            ctypes.POINTER(LymanLine), _i, _vp, _i, _ll, _ll, _ll, _d, _vp]
     L.rr_zscan.restype = _i
     L.rr_zscan.argtypes = [ctypes.POINTER(ZscanDesc), _vp]
+    L.rr_zscan_polish.restype = _i
+    L.rr_zscan_polish.argtypes = [ctypes.POINTER(ZscanDesc), ctypes.POINTER(PolishDesc), _vp]
     L.rr_zscan_workspace_bytes.restype = _ll
     L.rr_zscan_workspace_bytes.argtypes = [ctypes.POINTER(ZscanDesc)]
     L.rr_pack_mixed.restype = _i

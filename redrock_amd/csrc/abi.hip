@@ -30,7 +30,9 @@ extern "C" long long rr_zscan_workspace_bytes(const rr_zscan_desc *d) {
     return rr::workspace_bytes(d->nitems, d->nb, is_mixed(d->precision), d->total_rows);
 }
 
-extern "C" int rr_zscan(const rr_zscan_desc *d, void *stream) {
+// Validate a scan descriptor and fill the kernel arguments.  Returns RR_OK,
+// an error code, or 1 for a valid call with nothing to do.
+static int scan_args(const rr_zscan_desc *d, rr::ZArgs &a) {
     if (!d) return rr::fail(RR_EINVAL, "rr_zscan: null descriptor");
     if (d->nitems < 0 || d->nitems > 65535)
         return rr::fail(RR_EINVAL, "rr_zscan: nitems=%d outside 0..65535", d->nitems);
@@ -44,7 +46,7 @@ extern "C" int rr_zscan(const rr_zscan_desc *d, void *stream) {
                         d->cache_rows, d->npix_pad, d->halo);
     // no items (or only empty ones): a valid call with nothing to do; zero-size
     // device arrays may then have null addresses
-    if (d->nitems == 0 || d->max_item_nz <= 0) return RR_OK;
+    if (d->nitems == 0 || d->max_item_nz <= 0) return 1;
     if (!d->d_tcache || !d->d_rband || !d->d_w || !d->d_wf || !d->d_f || !d->d_item_target ||
         !d->d_item_z0 || !d->d_item_nz || !d->d_item_out || !d->d_zchi2 || !d->d_zcoeff)
         return rr::fail(RR_EINVAL, "rr_zscan: null required pointer");
@@ -64,7 +66,6 @@ extern "C" int rr_zscan(const rr_zscan_desc *d, void *stream) {
         return rr::fail(RR_EINVAL, "rr_zscan: unknown precision %d", d->precision);
     }
 
-    rr::ZArgs a;
     a.nitems = d->nitems; a.npix = d->npix; a.npix_pad = d->npix_pad; a.nb = d->nb;
     a.precision = d->precision; a.halo = d->halo;
     a.tcache = d->d_tcache; a.cache_rows = d->cache_rows; a.cache_ldz = d->cache_ldz;
@@ -78,6 +79,14 @@ extern "C" int rr_zscan(const rr_zscan_desc *d, void *stream) {
     a.zchi2 = d->d_zchi2; a.zcoeff = d->d_zcoeff; a.penalty = d->d_penalty; a.status = d->d_status;
     a.max_item_nz = d->max_item_nz;
     a.work = d->d_work; a.total_rows = d->total_rows;
+    return RR_OK;
+}
+
+extern "C" int rr_zscan(const rr_zscan_desc *d, void *stream) {
+    rr::ZArgs a;
+    const int v = scan_args(d, a);
+    if (v) return v < 0 ? v : RR_OK;
+    const bool mixed = is_mixed(d->precision);
     hipStream_t s = static_cast<hipStream_t>(stream);
 
     const int st = d->stages ? d->stages : (RR_STAGE_STATS | RR_STAGE_MAIN | RR_STAGE_FINISH);
@@ -92,6 +101,35 @@ extern "C" int rr_zscan(const rr_zscan_desc *d, void *stream) {
     if (rc) return rc;
     if (st & RR_STAGE_FINISH) rc = rr::zscan_finish(a, s);
     return rc;
+}
+
+extern "C" int rr_zscan_polish(const rr_zscan_desc *d, const rr_polish_desc *p, void *stream) {
+    rr::ZArgs a;
+    const int v = scan_args(d, a);
+    if (v) return v < 0 ? v : RR_OK;
+    if (d->precision == RR_PREC_F64) return RR_OK;
+    if (!p) return rr::fail(RR_EINVAL, "rr_zscan_polish: null polish descriptor");
+    if (!p->d_tcache64 || !p->d_windows || !p->d_nwindows)
+        return rr::fail(RR_EINVAL, "rr_zscan_polish: null required pointer");
+    if (p->cache_ldz < d->cache_nz)
+        return rr::fail(RR_EINVAL, "rr_zscan_polish: cache_ldz %lld < cache_nz %d", p->cache_ldz,
+                        d->cache_nz);
+    if (!(p->tol >= 0.0) || p->tol > 1e300)
+        return rr::fail(RR_EINVAL, "rr_zscan_polish: tol must be finite and >= 0");
+    if (p->max_windows < 1 || p->max_windows > RR_POLISH_MAX_WINDOWS)
+        return rr::fail(RR_EINVAL, "rr_zscan_polish: max_windows %d outside 1..%d", p->max_windows,
+                        RR_POLISH_MAX_WINDOWS);
+    if (d->max_item_nz > 65536)
+        return rr::fail(RR_EUNSUPPORTED, "rr_zscan_polish: items of more than 65536 redshifts");
+    rr::PolishArgs pa;
+    pa.tcache64 = p->d_tcache64;
+    pa.ldz = p->cache_ldz;
+    pa.tol = p->tol;
+    pa.max_windows = p->max_windows;
+    pa.nw_all = (d->max_item_nz + RR_POLISH_WINDOW - 1) / RR_POLISH_WINDOW;
+    pa.windows = p->d_windows;
+    pa.nwindows = p->d_nwindows;
+    return rr::zscan_polish(a, pa, static_cast<hipStream_t>(stream));
 }
 
 extern "C" int rr_zscan_tile_table(int precision, int *cfg, int max) {

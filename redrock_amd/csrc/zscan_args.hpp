@@ -26,6 +26,16 @@ struct ZArgs {
     long long total_rows;  // rows of the output arrays (>= max item_out + nz)
 };
 
+// Kernel-argument form of rr_polish_desc.
+struct PolishArgs {
+    const double *tcache64;
+    long long ldz;
+    double tol;
+    int max_windows;
+    int nw_all;            // windows of the longest item, ceil(max_item_nz / 16)
+    int *windows, *nwindows;
+};
+
 // Workspace layout (all offsets 256-byte aligned):
 //   stats  double[nitems][2]            f'Wf, sum(w) of the item's target
 //   ne     double[ne_count][total_rows]   (k-major, coalesced); the mixed
@@ -56,6 +66,7 @@ int zscan_f64(const ZArgs &a, hipStream_t s);
 int zscan_mfma(const ZArgs &a, hipStream_t s);
 int zscan_mfma32(const ZArgs &a, hipStream_t s);
 int zscan_mfma3(const ZArgs &a, hipStream_t s);
+int zscan_polish(const ZArgs &a, const PolishArgs &p, hipStream_t s);
 int pack_mixed3(const double *d_rband, const double *d_w, const double *d_f, int ntarget, int npix_pad,
                 void *d_rtile, double *d_scales, hipStream_t s);
 int zscan_tiles3(int *cfg, int max);

@@ -174,13 +174,16 @@ class _Local(object):
 def zfind(targets, templates, mp_procs=1, nminima=3, archetypes=None, priors=None,
           chi2_scan=None, use_gpu=False, zminfit_npoints=15, per_camera=None,
           deg_legendre=None, n_nearest=None, prior_sigma=None, ncamera=None,
-          precision=None, newton=False):
+          precision=None, newton=False, polish=None):
     """All redshift fits of the targets (zfind.py:220-714).
 
     targets: DistTargets (redrock_amd.targets) or a list of Target;
     templates: DistTemplate list.  priors: a Priors object (or None).
     precision: scan precision ('f64', 'f16x3', ...; default
     zscan.default_precision()); the fine fits always run in FP64.
+    polish: FP64 re-evaluation of the scan rows near each minimum after a
+    mixed-precision scan (zscan.ScanPlan; default on for mixed precisions,
+    False turns it off).
     chi2_scan: a details file (results.write_zscan) whose scan is used
     instead of scanning again; the fine fits still run.
     newton=True adds 'znewton'/'chi2newton' columns (Newton refinement on the
@@ -206,7 +209,7 @@ def zfind(targets, templates, mp_procs=1, nminima=3, archetypes=None, priors=Non
     batches = target_batches(local) if local else dict()
     if chi2_scan is None:                                    # zfind.py:289-292
         results = calc_zchi2_targets(dist, templates, mp_procs=mp_procs, use_gpu=use_gpu,
-                                     precision=precision, batches=batches)
+                                     precision=precision, batches=batches, polish=polish)
     else:
         from .results import read_zscan_redrock
         saved = read_zscan_redrock(chi2_scan)

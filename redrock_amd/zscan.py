@@ -53,6 +53,30 @@ def default_precision():
     return os.environ.get('RR_PRECISION', 'f64')
 
 
+# Relative tolerance of the FP64 polish per mixed precision: at least four
+# times the largest |chi2_mixed - chi2_f64| / chi2 measured for that mode on
+# the galaxy petal, the QSO petal and the mixed GALAXY/QSO/STAR batch.
+POLISH_TOL = {'f16x3': 1e-4, 'f32': 1e-4, 'f16': 3e-3, 'bf16': 1.5e-2}
+POLISH_WINDOWS = 8
+STAGE_POLISH = 8      # ScanPlan.run stage bit (rr_zscan_polish), beside RR_STAGE_*
+
+
+def polish_tol(precision, polish):
+    """Tolerance of the FP64 polish for `precision`, or None when it does not run.
+
+    polish: None or True = the mode's POLISH_TOL, False = off, a number = that
+    tolerance.  FP64 scans are never polished.
+    """
+    if precision == 'f64' or polish is False:
+        return None
+    if polish is None or polish is True:
+        return POLISH_TOL[precision]
+    tol = float(polish)
+    if not (tol >= 0.0 and math.isfinite(tol)):
+        raise ValueError(f'polish tolerance must be finite and >= 0, got {polish!r}')
+    return tol
+
+
 def spectral_data(spectra):
     """Concatenated (weights, flux, wflux) of a list of spectra (zscan.py:192-209)."""
     weights = np.concatenate([s.ivar for s in spectra])
@@ -285,7 +309,10 @@ class ScanPlan(object):
     """
 
     def __init__(self, cache, batch, item_target, item_z0, item_nz, precision='f64',
-                 prior=None, oiisum=None, tile=0):
+                 prior=None, oiisum=None, tile=0, polish=None, polish_windows=POLISH_WINDOWS):
+        """polish: FP64 re-evaluation of the rows near each item's minimum after a
+        mixed-precision scan (rr_zscan_polish), see polish_tol(); it needs the
+        FP64 template cache as well."""
         import torch
         if precision not in _hip.PRECISIONS:
             raise ValueError(f'unknown precision {precision!r}')
@@ -366,15 +393,40 @@ class ScanPlan(object):
         d.d_work = self.work.data_ptr()
         d.work_bytes = self.work.numel()
         self.desc = d
+        self.precision = precision
+        self.polish = None
+        tol = polish_tol(precision, polish)
+        if tol is not None and self.nitems > 0:
+            if not 1 <= polish_windows <= _hip.RR_POLISH_MAX_WINDOWS:
+                raise ValueError(f'polish_windows must be in 1..{_hip.RR_POLISH_MAX_WINDOWS}')
+            t64, ldz64 = cache.get(layout, 'f64')
+            self.windows = torch.zeros((self.nitems, polish_windows), dtype=torch.int32, device=dev)
+            self.nwindows = torch.zeros(self.nitems, dtype=torch.int32, device=dev)
+            pd = _hip.PolishDesc()
+            pd.d_tcache64 = t64.data_ptr()
+            pd.cache_ldz = ldz64
+            pd.tol = tol
+            pd.max_windows = polish_windows
+            pd.d_windows = self.windows.data_ptr()
+            pd.d_nwindows = self.nwindows.data_ptr()
+            self._keep.append(t64)
+            self.polish = pd
 
     def run(self, stream=None, stages=0):
         """Enqueue the scan on `stream` (default: torch's current stream).
 
-        stages: RR_STAGE_* bits (0 = stats, main and finish).
+        stages: RR_STAGE_* and STAGE_POLISH bits (0 = stats, main, finish and,
+        when the plan has one, the polish).
         """
-        self.desc.stages = stages
-        rc = _hip.lib().rr_zscan(ctypes.byref(self.desc), _hip.stream_handle(stream))
-        _hip.check(rc, 'rr_zscan')
+        scan = stages & (_hip.RR_STAGE_STATS | _hip.RR_STAGE_MAIN | _hip.RR_STAGE_FINISH)
+        h = _hip.stream_handle(stream)
+        if stages == 0 or scan:
+            self.desc.stages = scan
+            rc = _hip.lib().rr_zscan(ctypes.byref(self.desc), h)
+            _hip.check(rc, 'rr_zscan')
+        if self.polish is not None and (stages == 0 or stages & STAGE_POLISH):
+            rc = _hip.lib().rr_zscan_polish(ctypes.byref(self.desc), ctypes.byref(self.polish), h)
+            _hip.check(rc, 'rr_zscan_polish')
         return self
 
     def host(self):
@@ -389,10 +441,11 @@ def _oiisum(template):
     return None
 
 
-def scan_targets(cache, targets, precision=None, oiisum=None, batches=None):
+def scan_targets(cache, targets, precision=None, oiisum=None, batches=None, polish=None):
     """Scan every target over every redshift of `cache`.
 
     Targets are grouped by wavelength layout (one launch per layout).
+    polish: FP64 polish of a mixed-precision scan (ScanPlan).
     Returns numpy (zchi2 (nt, nz), zcoeff (nt, nz, nb), penalty (nt, nz)).
     """
     precision = precision or default_precision()
@@ -409,7 +462,7 @@ def scan_targets(cache, targets, precision=None, oiisum=None, batches=None):
             batch = TargetBatch([targets[i] for i in idx])
         n = len(idx)
         plan = ScanPlan(cache, batch, np.arange(n), np.zeros(n), np.full(n, nz), precision,
-                        oiisum=oiisum)
+                        oiisum=oiisum, polish=polish)
         plan.run()
         c2, cf, pen, _ = plan.host()
         zchi2[idx] = c2.reshape(n, nz)
@@ -432,8 +485,9 @@ class ScanGraph(object):
     targets that share one wavelength layout.
     """
 
-    def __init__(self, caches, batch, precision=None, tune=False):
-        """caches: list of (TemplateCache, oiisum or None); tune: autotune each plan's tile."""
+    def __init__(self, caches, batch, precision=None, tune=False, polish=None):
+        """caches: list of (TemplateCache, oiisum or None); tune: autotune each plan's
+        tile; polish: FP64 polish of each plan (ScanPlan), captured with it."""
         precision = precision or default_precision()
         self.precision = precision
         self.batch = batch
@@ -441,7 +495,7 @@ class ScanGraph(object):
         self.plans = []
         for cache, oii in caches:
             plan = ScanPlan(cache, batch, np.arange(nt), np.zeros(nt), np.full(nt, cache.nz),
-                            precision, oiisum=oii)
+                            precision, oiisum=oii, polish=polish)
             if tune:
                 from . import autotune
                 autotune.choose(plan)
@@ -486,16 +540,18 @@ class ScanGraph(object):
 
 
 def calc_zchi2(target_ids, target_data, dtemplate, progress=None, use_gpu=False, precision=None,
-               batches=None):
+               batches=None, polish=None):
     """chi2 vs redshift of every target for one template (zscan.py:807-889).
 
     batches: {layout signature: TargetBatch} of these targets (target_batches),
-    reused instead of uploading the targets again.
+    reused instead of uploading the targets again.  polish: FP64 polish of a
+    mixed-precision scan (ScanPlan; on by default for mixed precisions).
     Returns (zchi2[nt, nz], zcoeff[nt, nz, nbasis], zchi2penalty[nt, nz]).
     """
     cache = dtemplate.cache
     oii = _oiisum(dtemplate.template)
-    zchi2, zcoeff, penalty = scan_targets(cache, list(target_data), precision, oii, batches=batches)
+    zchi2, zcoeff, penalty = scan_targets(cache, list(target_data), precision, oii, batches=batches,
+                                          polish=polish)
     if progress is not None and dtemplate.comm is None:
         for _ in range(len(target_data)):
             progress.put(1)
@@ -610,7 +666,8 @@ def target_batches(targets):
     return {sig: TargetBatch(tgs) for sig, tgs in groups.items()}
 
 
-def calc_zchi2_targets(targets, templates, mp_procs=1, use_gpu=False, precision=None, batches=None):
+def calc_zchi2_targets(targets, templates, mp_procs=1, use_gpu=False, precision=None, batches=None,
+                       polish=None):
     """Coarse chi2 scan of every local target for every template (zscan.py:910-1105).
 
     batches: target_batches(targets.local()) to reuse (built here otherwise,
@@ -624,7 +681,7 @@ def calc_zchi2_targets(targets, templates, mp_procs=1, use_gpu=False, precision=
     for t in templates:
         ft = t.template.full_type
         zchi2, zcoeff, penalty = calc_zchi2(targets.local_target_ids(), local, t,
-                                            precision=precision, batches=batches)
+                                            precision=precision, batches=batches, polish=polish)
         for i, tg in enumerate(local):
             results[tg.id][ft] = dict(redshifts=t.template.redshifts, zchi2=zchi2[i],
                                       penalty=penalty[i], zcoeff=zcoeff[i])

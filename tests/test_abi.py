@@ -57,6 +57,58 @@ def test_abi_version_and_error_text():
     assert lib.rr_zscan_workspace_bytes(None) < 0
 
 
+def test_polish_argument_errors():
+    """rr_zscan_polish validates before any device call; an empty scan is a no-op."""
+    from redrock_amd import _hip
+    lib = _hip.lib()
+    assert lib.rr_zscan_polish(None, None, None) == -1
+    assert b'null descriptor' in lib.rr_last_error()
+    d = _hip.ZscanDesc()
+    d.nitems, d.npix, d.npix_pad, d.nb, d.halo = 0, 10, 32, 3, 8
+    d.cache_rows = 64
+    d.precision = _hip.RR_PREC_F16X3
+    assert lib.rr_zscan_polish(ctypes.byref(d), None, None) == 0
+    d.nitems, d.max_item_nz, d.total_rows = 1, 5, 5
+    d.d_tcache = d.d_rband = d.d_w = d.d_wf = d.d_f = 8
+    d.d_item_target = d.d_item_z0 = d.d_item_nz = d.d_item_out = d.d_zchi2 = d.d_zcoeff = 8
+    d.d_rtile = d.d_scales = 8
+    d.d_work, d.work_bytes = 8, 1 << 20
+    assert lib.rr_zscan_polish(ctypes.byref(d), None, None) == -1
+    assert b'null polish descriptor' in lib.rr_last_error()
+    p = _hip.PolishDesc()
+    p.d_tcache64 = p.d_windows = p.d_nwindows = 8
+    p.cache_ldz, p.tol, p.max_windows = 0, 1e-4, 8
+    d.cache_nz = 5
+    assert lib.rr_zscan_polish(ctypes.byref(d), ctypes.byref(p), None) == -1
+    assert b'cache_ldz' in lib.rr_last_error()
+    p.cache_ldz = 64
+    for tol in (-1.0, float('nan'), float('inf')):
+        p.tol = tol
+        assert lib.rr_zscan_polish(ctypes.byref(d), ctypes.byref(p), None) == -1
+        assert b'tol' in lib.rr_last_error()
+    p.tol = 1e-4
+    for mw in (0, _hip.RR_POLISH_MAX_WINDOWS + 1):
+        p.max_windows = mw
+        assert lib.rr_zscan_polish(ctypes.byref(d), ctypes.byref(p), None) == -1
+        assert b'max_windows' in lib.rr_last_error()
+    # an FP64 scan has nothing to polish
+    d.precision = _hip.RR_PREC_F64
+    d.cache_ldz = 64
+    assert lib.rr_zscan_polish(ctypes.byref(d), ctypes.byref(p), None) == 0
+
+
+def test_polish_tolerance_argument():
+    import math
+    from redrock_amd import zscan
+    assert zscan.polish_tol('f64', None) is None
+    assert zscan.polish_tol('f16x3', False) is None
+    assert zscan.polish_tol('bf16', True) == zscan.POLISH_TOL['bf16']
+    assert zscan.polish_tol('f16', 0.25) == 0.25
+    for bad in (-1.0, math.inf, math.nan):
+        with pytest.raises(ValueError):
+            zscan.polish_tol('f16', bad)
+
+
 def test_descriptor_matches_header_field_order():
     """ctypes struct mirrors rr_zscan_desc field for field."""
     from redrock_amd import _hip
