@@ -119,7 +119,7 @@ def bench_mixed(args):
     each precision's whole scan replayed as one captured HIP graph."""
     import numpy as np
     import torch
-    from redrock_amd import synth, zscan
+    from redrock_amd import synth, zscan, _hip
     from redrock_amd.templates import DistTemplate
     t0 = time.perf_counter()
     tms = (synth.galaxy_template(), synth.qso_template(), synth.star_template())
@@ -198,7 +198,7 @@ def bench_mixed(args):
         'per_template': per_template,
         'polish': (None if args.no_polish else
                    {'tol': zscan.POLISH_TOL[args.precision], 'max_windows': zscan.POLISH_WINDOWS,
-                    'window': 16}),
+                    'window': _hip.RR_POLISH_WINDOW}),
         'unpolished': unpolished,
     }), flush=True)
 
@@ -620,6 +620,7 @@ def main():
                   'window': _hip.RR_POLISH_WINDOW,
                   'ms': sum(a.elapsed_time(b) for a, b in evs) / len(evs),
                   'windows_per_target': float(mix.nwindows.float().mean()),
+                  'targets_polished': int((mix.nwindows > 0).sum()),
                   'unpolished_ms_per_step': t_raw * 1e3, 'unpolished_value': evals / t_raw}
     # final full runs so the outputs hold complete results
     mix.run()
@@ -634,7 +635,8 @@ def main():
             'unpolished_best_z_agreement': float(np.mean(c_raw.argmin(axis=1) == c_f64.argmin(axis=1))),
             'unpolished_max_rel_chi2_err': float(rr_.max()),
             'unpolished_max_rel_chi2_err_at_best_z': float(rr_[np.arange(nt), c_f64.argmin(axis=1)].max())})
-        log(f'polish: {polish["ms"]:.2f} ms, {polish["windows_per_target"]:.2f} windows per target; '
+        log(f'polish: {polish["ms"]:.3f} ms, {polish["targets_polished"]} targets with '
+            f'{polish["windows_per_target"] * nt:.0f} windows; '
             f'without it {t_raw * 1e3:.2f} ms/step, best-z agreement '
             f'{polish["unpolished_best_z_agreement"]:.4f}')
     bz_mix, bz_f64 = c_mix.argmin(axis=1), c_f64.argmin(axis=1)
@@ -692,7 +694,7 @@ def main():
                    'targets': nt, 'redshifts': nz, 'nbasis': nb, 'snr_per_pixel': args.snr,
                    'solve': 'FP64 Cholesky',
                    'polish': (None if args.no_polish else
-                              f'FP64 re-evaluation of the 16-redshift windows within '
+                              f'FP64 re-evaluation of the {_hip.RR_POLISH_WINDOW}-redshift windows within '
                               f'{zscan.polish_tol(args.precision, True):g} relative of each '
                               f'target minimum (in the timed step)'),
                    'accumulate': ('fp32 MFMA partial sums over 64 pixels, added in FP64'

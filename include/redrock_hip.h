@@ -260,22 +260,24 @@ int rr_pack_mixed(const double *d_rband, const double *d_w, const double *d_f, i
  *
  * For every item of `scan` (a descriptor that rr_zscan has run in full, in a
  * mixed precision): the windows of RR_POLISH_WINDOW consecutive redshifts
- * (item z index 16k .. 16k+15) holding a chi2, or chi2 + penalty, within
+ * (item z index 4k .. 4k+3) holding a chi2, or chi2 + penalty, within
  * tol * |minimum| of the item's minimum of that quantity are re-evaluated in
  * FP64 (R-apply, normal equations, LU solve, compensated chi2, penalty,
  * status) from the FP64 template cache, and their rows of d_zchi2,
  * d_zcoeff, d_penalty and d_status are overwritten.  If more than
  * max_windows windows qualify, the tolerance is halved until they fit (the
- * windows nearest the minimum are kept).  Items whose minimum chi2 is
+ * windows nearest the minimum are kept). An item with a single row under the
+ * full thresholds gets no window (that row is the FP64 argmin already; its
+ * chi2 keeps the mixed value).  Items whose minimum chi2 is
  * HUGE_CHI2 or more are left alone.  With |chi2_mixed - chi2_f64| <= e chi2
  * and tol >= 2e + e^2 the argmin of every polished item equals the FP64
  * scan's (ties within FP64 rounding aside).
  *
  * scan->precision RR_PREC_F64: nothing to do, returns RR_OK.
  * Reads the scan's workspace statistics (stage RR_STAGE_STATS); the FP64
- * cache has the scan's rows, halo and pixel layout.  max_item_nz <= 65536.
+ * cache has the scan's rows, halo and pixel layout.  max_item_nz <= 16384.
  */
-#define RR_POLISH_WINDOW 16
+#define RR_POLISH_WINDOW 4
 #define RR_POLISH_MAX_WINDOWS 64
 typedef struct {
     const double *d_tcache64;  /* FP64 cache [b][row][z] of the scanned template  */
@@ -284,6 +286,8 @@ typedef struct {
     int max_windows;           /* 1 .. RR_POLISH_MAX_WINDOWS per item               */
     int *d_windows;            /* (nitems, max_windows) out: polished window k's    */
     int *d_nwindows;           /* (nitems) out: windows polished per item           */
+    int *d_queue;              /* (1 + nitems * max_windows) workspace: count and
+                                  the (item, window) work list                      */
 } rr_polish_desc;
 int rr_zscan_polish(const rr_zscan_desc *scan, const rr_polish_desc *polish, void *stream);
 

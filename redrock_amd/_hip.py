@@ -9,7 +9,8 @@ import ctypes
 import os
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, 'libredrock_hip.so')
+# RR_LIB_PATH: load the library from another path (a build kept elsewhere)
+LIB_PATH = os.environ.get('RR_LIB_PATH') or os.path.join(HERE, 'libredrock_hip.so')
 
 RR_ABI_VERSION = 6
 RR_F64, RR_F32, RR_F16, RR_BF16 = 0, 1, 2, 3
@@ -57,14 +58,15 @@ class ZscanDesc(ctypes.Structure):
 
 RR_STAGE_STATS, RR_STAGE_MAIN, RR_STAGE_FINISH = 1, 2, 4
 
-RR_POLISH_WINDOW = 16
+RR_POLISH_WINDOW = 4
 RR_POLISH_MAX_WINDOWS = 64
+RR_POLISH_MAX_NZ = 16384
 
 
 class PolishDesc(ctypes.Structure):
     _fields_ = [
         ('d_tcache64', _vp), ('cache_ldz', _ll), ('tol', _d), ('max_windows', _i),
-        ('d_windows', _vp), ('d_nwindows', _vp),
+        ('d_windows', _vp), ('d_nwindows', _vp), ('d_queue', _vp),
     ]
 
 

@@ -109,7 +109,7 @@ extern "C" int rr_zscan_polish(const rr_zscan_desc *d, const rr_polish_desc *p, 
     if (v) return v < 0 ? v : RR_OK;
     if (d->precision == RR_PREC_F64) return RR_OK;
     if (!p) return rr::fail(RR_EINVAL, "rr_zscan_polish: null polish descriptor");
-    if (!p->d_tcache64 || !p->d_windows || !p->d_nwindows)
+    if (!p->d_tcache64 || !p->d_windows || !p->d_nwindows || !p->d_queue)
         return rr::fail(RR_EINVAL, "rr_zscan_polish: null required pointer");
     if (p->cache_ldz < d->cache_nz)
         return rr::fail(RR_EINVAL, "rr_zscan_polish: cache_ldz %lld < cache_nz %d", p->cache_ldz,
@@ -119,8 +119,8 @@ extern "C" int rr_zscan_polish(const rr_zscan_desc *d, const rr_polish_desc *p, 
     if (p->max_windows < 1 || p->max_windows > RR_POLISH_MAX_WINDOWS)
         return rr::fail(RR_EINVAL, "rr_zscan_polish: max_windows %d outside 1..%d", p->max_windows,
                         RR_POLISH_MAX_WINDOWS);
-    if (d->max_item_nz > 65536)
-        return rr::fail(RR_EUNSUPPORTED, "rr_zscan_polish: items of more than 65536 redshifts");
+    if (d->max_item_nz > 16384)   // window minima in 64 KB of LDS
+        return rr::fail(RR_EUNSUPPORTED, "rr_zscan_polish: items of more than 16384 redshifts");
     rr::PolishArgs pa;
     pa.tcache64 = p->d_tcache64;
     pa.ldz = p->cache_ldz;
@@ -129,6 +129,7 @@ extern "C" int rr_zscan_polish(const rr_zscan_desc *d, const rr_polish_desc *p, 
     pa.nw_all = (d->max_item_nz + RR_POLISH_WINDOW - 1) / RR_POLISH_WINDOW;
     pa.windows = p->d_windows;
     pa.nwindows = p->d_nwindows;
+    pa.queue = p->d_queue;
     return rr::zscan_polish(a, pa, static_cast<hipStream_t>(stream));
 }
 

@@ -34,6 +34,7 @@ struct PolishArgs {
     int max_windows;
     int nw_all;            // windows of the longest item, ceil(max_item_nz / 16)
     int *windows, *nwindows;
+    int *queue;            // [0] = entries, then item * RR_POLISH_MAX_WINDOWS + slot
 };
 
 // Workspace layout (all offsets 256-byte aligned):

@@ -10,23 +10,26 @@
 // minimum is among the re-evaluated rows and every other row stays above it,
 // so the item's argmin is the FP64 scan's.
 //
-// polish_select_kernel (one wavefront per item): per aligned window of 16
+// polish_select_kernel (one wavefront per item): per aligned window of 4
 //   redshifts the minimum of chi2 and of chi2 + penalty (in LDS), the item's
 //   minima, then the windows holding a value under the thresholds; if more
 //   than max_windows qualify the tolerance is halved until they fit (so the
 //   windows kept are the ones nearest the minimum), and they are written in
-//   increasing redshift order.
-// polish_eval_kernel (one workgroup of 4 waves per (item, window)): thread
-//   (zl, chunk) accumulates the FP64 normal equations of redshift 16 w + zl
-//   over the 2-pixel blocks 2 chunk + 32 j (the per-pixel arithmetic of
-//   zscan_f64.hip, on the FP64 cache [b][row][z], so 16 lanes of one pixel
-//   read one 128-byte run and the workgroup's 16 chunks read one 32-pixel
-//   span at a time), the 16 chunks are summed in a fixed order (lane
-//   shuffles, then the four waves through LDS), and 16 lanes solve: LU with
+//   increasing redshift order.  An item with a single row under the full
+//   thresholds gets no window: that row is the FP64 argmin already.
+// polish_eval_kernel (4 waves per (item, window), persistent over a work
+//   queue): thread (zl, chunk) accumulates the FP64 normal equations of
+//   redshift 4 w + zl over the 2-pixel blocks 2 chunk + 128 j (the per-pixel
+//   arithmetic of zscan_f64.hip, on the FP64 cache [b][row][z], so the 4
+//   lanes of one pixel read one 32-byte run and the workgroup's 64 chunks read
+//   one 128-pixel span at a time), the 64 chunks are summed in a fixed order
+//   (lane shuffles, then the four waves through LDS), and 4 lanes solve: LU
+//   with
 //   partial pivoting, chi2 = f'Wf - 2 c'y + c'Mc compensated, HUGE_CHI2 rules
-//   as zscan_finish.hip.  Items are the fast grid axis: consecutive
-//   workgroups go to different XCDs, so the (usually one) window of every
-//   item spreads over the whole chip.
+//   as zscan_finish.hip.  One workgroup per CU walks the queue, so windows
+//   spread over every XCD and items without a window cost nothing.  A
+//   window's time is latency (one pass over the pixels at one wave per
+//   SIMD), hence narrow windows and many pixel chunks.
 //
 // No counterpart in the reference (its scan is FP64 throughout).
 #include "common.hpp"
@@ -83,7 +86,8 @@ __global__ __launch_bounds__(64) void polish_select_kernel(rr::ZArgs a, rr::Poli
     if (m1 < HUGE_CHI2) {
         double d1 = p.tol * fabs(m1), d2 = p.tol * fabs(m2);
         double t1 = m1 + d1, t2 = m2 + d2;
-        for (int it = 0;; ++it) {
+        int it = 0;
+        for (;; ++it) {
             t1 = m1 + d1;
             t2 = m2 + d2;
             int cnt = 0;
@@ -98,6 +102,18 @@ __global__ __launch_bounds__(64) void polish_select_kernel(rr::ZArgs a, rr::Poli
                 d2 *= 0.5;
             }
         }
+        // An isolated minimum (one row under the full tolerance) is already the
+        // FP64 argmin: nothing to re-evaluate.
+        int rows = 0;
+        if (it == 0) {
+            for (int iz = lane; iz < nzi; iz += 64) {
+                const double c = a.zchi2[row0 + iz];
+                const double s = c + (a.penalty ? a.penalty[row0 + iz] : 0.0);
+                rows += (c <= t1 || s <= t2) ? 1 : 0;
+            }
+            rows = wave_sum(rows);
+        }
+        if (it == 0 && rows <= 1) t1 = t2 = -__builtin_huge_val();   // selects no window
         int base = 0;
         for (int k0 = 0; k0 < nw; k0 += 64) {
             const int k = k0 + lane;
@@ -110,21 +126,44 @@ __global__ __launch_bounds__(64) void polish_select_kernel(rr::ZArgs a, rr::Poli
         nsel = base < p.max_windows ? base : p.max_windows;
     }
     if (lane == 0) p.nwindows[item] = nsel;
+    // queue entries item * RR_POLISH_MAX_WINDOWS + slot for the eval kernel
+    int q0 = 0;
+    if (lane == 0 && nsel > 0) q0 = atomicAdd(p.queue, nsel);
+    q0 = __shfl(q0, 0, 64);
+    if (lane < nsel) p.queue[1 + q0 + lane] = item * RR_POLISH_MAX_WINDOWS + lane;
 }
 
+__global__ void polish_reset_kernel(int *queue) { queue[0] = 0; }
+
 constexpr int kEvalWaves = 4;                   // 8 would cap a wave at 256 registers and spill
-constexpr int kChunks = kEvalWaves * 64 / WZ;   // 16 pixel chunks per redshift
+constexpr int kChunks = kEvalWaves * 64 / WZ;   // 64 pixel chunks per redshift
 
 template <int NB>
+__device__ __forceinline__ void eval_window(const rr::ZArgs &a, const rr::PolishArgs &p, int item,
+                                            int slot, double (&red)[kEvalWaves][WZ][NB * (NB + 1) / 2 + NB]);
+
+// Persistent over the work queue that polish_select_kernel filled (one
+// workgroup per CU): no workgroup is launched for items without a window.
+template <int NB>
 __global__ __launch_bounds__(kEvalWaves * 64) void polish_eval_kernel(rr::ZArgs a, rr::PolishArgs p) {
+    constexpr int NV = NB * (NB + 1) / 2 + NB;
+    __shared__ double red[kEvalWaves][WZ][NV];
+
+    const int nwork = __builtin_amdgcn_readfirstlane(*(volatile const int *)p.queue);
+    for (int wk = blockIdx.x; wk < nwork; wk += gridDim.x) {   // uniform over the workgroup
+        const int entry = p.queue[1 + wk];
+        eval_window<NB>(a, p, entry / RR_POLISH_MAX_WINDOWS, entry % RR_POLISH_MAX_WINDOWS, red);
+        __syncthreads();                     // red is reused by the next window
+    }
+}
+
+// One (item, window): FP64 normal equations of its 16 redshifts, solve, chi2.
+template <int NB>
+__device__ __forceinline__ void eval_window(const rr::ZArgs &a, const rr::PolishArgs &p, int item,
+                                            int slot, double (&red)[kEvalWaves][WZ][NB * (NB + 1) / 2 + NB]) {
     constexpr int NM = NB * (NB + 1) / 2;
     constexpr int NV = NM + NB;
     constexpr int P = 2;
-    __shared__ double red[kEvalWaves][WZ][NV];
-
-    const int item = blockIdx.x;
-    const int slot = blockIdx.y;
-    if (slot >= p.nwindows[item]) return;     // uniform over the workgroup
     const int tid = threadIdx.x;
     const int zl = tid & (WZ - 1);
     const int chunk = tid / WZ;              // 0..kChunks-1
@@ -146,8 +185,8 @@ __global__ __launch_bounds__(kEvalWaves * 64) void polish_eval_kernel(rr::ZArgs 
 #pragma unroll
     for (int i = 0; i < NV; ++i) v[i] = 0.0;
 
-    // chunk c takes the 2-pixel blocks 2c + 32j: at any time the workgroup's
-    // chunks cover one 32-pixel span and share their window rows in L1
+    // chunk c takes the 2-pixel blocks 2c + 128j: at any time the workgroup's
+    // chunks cover one 128-pixel span and share their window rows in L1
     for (int p0 = P * chunk; p0 < a.npix_pad; p0 += P * kChunks) {
         double r[P][11];
 #pragma unroll
@@ -187,18 +226,18 @@ __global__ __launch_bounds__(kEvalWaves * 64) void polish_eval_kernel(rr::ZArgs 
         }
     }
 
-    // chunks 4 wave + {0..3} sit in lanes zl + 16 c of this wave
+    // this wave's 64 / WZ chunks sit in lanes zl + WZ c: butterfly over c
 #pragma unroll
     for (int i = 0; i < NV; ++i) {
-        v[i] += __shfl_xor(v[i], 16, 64);
-        v[i] += __shfl_xor(v[i], 32, 64);
+#pragma unroll
+        for (int o = WZ; o < 64; o <<= 1) v[i] += __shfl_xor(v[i], o, 64);
     }
     if ((tid & 63) < WZ) {
 #pragma unroll
         for (int i = 0; i < NV; ++i) red[wave][zl][i] = v[i];
     }
     __syncthreads();
-    if (tid >= WZ || iz >= nzi) return;
+    if (tid >= WZ || iz >= nzi) return;      // only the end of this window's work
 
 #pragma unroll
     for (int i = 0; i < NV; ++i) {
@@ -277,8 +316,14 @@ __global__ __launch_bounds__(kEvalWaves * 64) void polish_eval_kernel(rr::ZArgs 
 
 template <int NB>
 int launch_eval(const rr::ZArgs &a, const rr::PolishArgs &p, hipStream_t s) {
-    hipLaunchKernelGGL((polish_eval_kernel<NB>), dim3(a.nitems, p.max_windows), dim3(kEvalWaves * 64), 0,
-                       s, a, p);
+    static int cus = 0;
+    if (cus == 0) {
+        int dev = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1)
+            cus = 256;
+    }
+    hipLaunchKernelGGL((polish_eval_kernel<NB>), dim3(cus), dim3(kEvalWaves * 64), 0, s, a, p);
     return rr::check_launch("rr_zscan_polish(eval)");
 }
 
@@ -287,9 +332,12 @@ int launch_eval(const rr::ZArgs &a, const rr::PolishArgs &p, hipStream_t s) {
 namespace rr {
 
 int zscan_polish(const ZArgs &a, const PolishArgs &p, hipStream_t s) {
+    hipLaunchKernelGGL(polish_reset_kernel, dim3(1), dim3(1), 0, s, p.queue);
+    int rc = check_launch("rr_zscan_polish(reset)");
+    if (rc) return rc;
     const long long lds = 2LL * p.nw_all * 8;
     hipLaunchKernelGGL(polish_select_kernel, dim3(a.nitems), dim3(64), (size_t)lds, s, a, p);
-    int rc = check_launch("rr_zscan_polish(select)");
+    rc = check_launch("rr_zscan_polish(select)");
     if (rc) return rc;
     switch (a.nb) {
         case 1: return launch_eval<1>(a, p, s);

@@ -57,7 +57,7 @@ def default_precision():
 # times the largest |chi2_mixed - chi2_f64| / chi2 measured for that mode on
 # the galaxy petal, the QSO petal and the mixed GALAXY/QSO/STAR batch.
 POLISH_TOL = {'f16x3': 1e-4, 'f32': 1e-4, 'f16': 3e-3, 'bf16': 1.5e-2}
-POLISH_WINDOWS = 8
+POLISH_WINDOWS = 16
 STAGE_POLISH = 8      # ScanPlan.run stage bit (rr_zscan_polish), beside RR_STAGE_*
 
 
@@ -396,12 +396,18 @@ class ScanPlan(object):
         self.precision = precision
         self.polish = None
         tol = polish_tol(precision, polish)
+        if tol is not None and self.nitems > 0 and int(item_nz.max()) > _hip.RR_POLISH_MAX_NZ:
+            if polish is not None:
+                raise ValueError(f'the FP64 polish handles items of at most '
+                                 f'{_hip.RR_POLISH_MAX_NZ} redshifts')
+            tol = None                       # default polish: left out for such long items
         if tol is not None and self.nitems > 0:
             if not 1 <= polish_windows <= _hip.RR_POLISH_MAX_WINDOWS:
                 raise ValueError(f'polish_windows must be in 1..{_hip.RR_POLISH_MAX_WINDOWS}')
             t64, ldz64 = cache.get(layout, 'f64')
             self.windows = torch.zeros((self.nitems, polish_windows), dtype=torch.int32, device=dev)
             self.nwindows = torch.zeros(self.nitems, dtype=torch.int32, device=dev)
+            self.queue = torch.zeros(1 + self.nitems * polish_windows, dtype=torch.int32, device=dev)
             pd = _hip.PolishDesc()
             pd.d_tcache64 = t64.data_ptr()
             pd.cache_ldz = ldz64
@@ -409,6 +415,7 @@ class ScanPlan(object):
             pd.max_windows = polish_windows
             pd.d_windows = self.windows.data_ptr()
             pd.d_nwindows = self.nwindows.data_ptr()
+            pd.d_queue = self.queue.data_ptr()
             self._keep.append(t64)
             self.polish = pd
 

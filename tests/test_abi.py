@@ -77,6 +77,9 @@ def test_polish_argument_errors():
     assert b'null polish descriptor' in lib.rr_last_error()
     p = _hip.PolishDesc()
     p.d_tcache64 = p.d_windows = p.d_nwindows = 8
+    assert lib.rr_zscan_polish(ctypes.byref(d), ctypes.byref(p), None) == -1
+    assert b'null required pointer' in lib.rr_last_error()
+    p.d_queue = 8
     p.cache_ldz, p.tol, p.max_windows = 0, 1e-4, 8
     d.cache_nz = 5
     assert lib.rr_zscan_polish(ctypes.byref(d), ctypes.byref(p), None) == -1
@@ -117,6 +120,15 @@ def test_descriptor_matches_header_field_order():
     body = re.sub(r'/\*.*?\*/', '', body, flags=re.S)
     fields = re.findall(r'(\w+)\s*;', body)
     assert fields == [f for f, _ in _hip.ZscanDesc._fields_]
+
+
+def test_polish_descriptor_matches_header_field_order():
+    from redrock_amd import _hip
+    text = open(os.path.join(ROOT, 'include', 'redrock_hip.h')).read()
+    body = text[text.index('typedef struct {\n    const double *d_tcache64;'):text.index('} rr_polish_desc;')]
+    body = re.sub(r'/\*.*?\*/', '', body, flags=re.S)
+    fields = re.findall(r'(\w+)\s*;', body)
+    assert fields == [f for f, _ in _hip.PolishDesc._fields_]
 
 
 def test_workspace_size_formula():

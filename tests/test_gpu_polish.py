@@ -12,7 +12,8 @@ held to the oracle at 1e-9 by test_gpu_zscan.py):
   for bit;
 * the windows chosen equal a CPU restatement of the selection rule, written
   below from the rule's statement (window minima of chi2 and chi2 + penalty,
-  threshold halved until at most max_windows windows qualify);
+  threshold halved until at most max_windows windows qualify, no window for
+  an item with a single row under the full threshold);
 * a target with no weight (HUGE_CHI2 everywhere) is left alone.
 """
 import numpy as np
@@ -21,7 +22,7 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 HUGE = 9999999.0
-WZ = 16
+WZ = 4
 
 
 def select_windows(chi2, pen, tol, maxw):
@@ -50,13 +51,16 @@ def select_windows(chi2, pen, tol, maxw):
             d1 *= 0.5
             d2 *= 0.5
         it += 1
+    if it == 0 and np.sum((chi2 <= t1) | (chi2 + pen <= t2)) <= 1:
+        return np.zeros(0, dtype=np.int64)      # isolated minimum
     return np.flatnonzero(sel)[:maxw]
 
 
-def _plans(tm, targets, precision, polish, windows=8):
+def _plans(tm, targets, precision, polish, windows=None):
     import torch
     from redrock_amd import zscan
     from redrock_amd.templates import DistTemplate
+    windows = windows or zscan.POLISH_WINDOWS
     dwave = {s.wavehash: s.wave for s in targets[0].spectra}
     dt = DistTemplate(tm, dwave)
     batch = zscan.TargetBatch(targets)
@@ -120,7 +124,10 @@ def test_polish_galaxy_best_z(gpu, precision):
     assert np.array_equal(cpol[live].argmin(axis=1), c64[live].argmin(axis=1))
     best = c64[live].argmin(axis=1)
     rel = np.abs(cpol[live, best] - c64[live, best]) / c64[live, best]
-    assert rel.max() <= 1e-12
+    did = pol.nwindows.cpu().numpy()[live] > 0
+    print(f'{precision}: {did.sum()} of {live.size} targets polished, the rest isolated')
+    assert rel[did].max(initial=0.0) <= 1e-12
+    assert rel.max() <= tol
 
 
 def test_polish_star_fine_grid(gpu):
