@@ -1,14 +1,17 @@
 // Plain FP64 chi2 scan (rr_zscan with RR_PREC_F64).
 //
 // Work decomposition: one wavefront = one item (a target scanned over a z
-// range) x 64 consecutive redshifts, one lane per redshift.  A workgroup's 4
-// waves are 4 items on the same 64-z block, so their template-cache reads hit
+// range) x 64 consecutive redshifts, one lane per redshift (items of at most
+// 16 or 4 redshifts: 4 or 16 items per wavefront, see zscan_f64_kernel).  A
+// workgroup's 4 waves are 4 items on the same 64-z block, so their
+// template-cache reads hit
 // the same lines in L1/L2.  Per lane and per block of P pixels:
 //   A[p][b] = sum_k R[p][k] T[b][p+k-5][z]      (banded R-apply, 11 taps)
 //   M[b1][b2] += w_p A[p][b1] A[p][b2],  y[b] += A[p][b] (w f)_p
-// R, w and w*f are wave-uniform (the target is uniform per wave), so they come
-// through scalar loads; the cache is laid out [b][pixel][z] so every load of a
-// window row is one coalesced 512-byte line run.  The kernel writes the
+// With one item per wave R, w and w*f are wave-uniform and come through
+// scalar loads; the cache is laid out [b][pixel][z] so every load of a window
+// row is one coalesced 512-byte line run (64 / G consecutive redshifts per
+// item with G items per wave).  The kernel writes the
 // per-redshift normal equations [M | y] to the workspace; the FP64 solve and
 // chi2 follow in zscan_finish.hip.
 //
@@ -18,18 +21,41 @@
 
 namespace {
 
-template <int NB, int P>
+// G items per wavefront, 64 / G redshifts each (G = 1 for full redshift
+// grids; 4 or 16 for the short items of fine-grid fits and Newton steps, whose
+// redshifts would leave most lanes of a one-item wave idle).  Every lane runs
+// the same per-pixel arithmetic in the same order whatever G is, so the
+// results do not depend on G.
+template <int NB, int P, int G>
 __global__ __launch_bounds__(256) void zscan_f64_kernel(rr::ZArgs a, long long ne_off) {
+    constexpr int LZ = 64 / G;           // lanes (redshifts) per item
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int item = blockIdx.y * 4 + wave;
-    if (item >= a.nitems) return;
+    int item, zl;
+    bool live = true;
+    if constexpr (G == 1) {
+        item = blockIdx.y * 4 + wave;    // wave-uniform: R, w, w*f through scalar loads
+        if (item >= a.nitems) return;
+        zl = lane;
+    } else {
+        const int first = (blockIdx.y * 4 + wave) * G;
+        if (first >= a.nitems) return;   // uniform over the wave
+        item = first + lane / LZ;
+        zl = lane % LZ;
+        if (item >= a.nitems) {          // idle lanes repeat the last item, write nothing
+            item = a.nitems - 1;
+            live = false;
+        }
+    }
     const int t = a.item_target[item];
     const int z0 = a.item_z0[item];
     const int nzi = a.item_nz[item];
-    const int zbase = blockIdx.x * 64;
-    if (zbase >= nzi) return;
-    const int iz = zbase + lane;
+    const int zbase = blockIdx.x * LZ;
+    if constexpr (G == 1) {
+        if (zbase >= nzi) return;
+    }
+    const int iz = zbase + zl;
+    live = live && iz < nzi;
     const int zc = z0 + (iz < nzi ? iz : nzi - 1);
 
     const long long npp = a.npix_pad;
@@ -90,7 +116,7 @@ __global__ __launch_bounds__(256) void zscan_f64_kernel(rr::ZArgs a, long long n
         }
     }
 
-    if (iz >= nzi) return;
+    if (!live) return;
     // normal equations of this (item, z) -> workspace, k-major (coalesced)
     const long long row = a.item_out[item] + iz;
     double *ne = reinterpret_cast<double *>(static_cast<char *>(a.work) + ne_off);
@@ -107,13 +133,20 @@ __global__ __launch_bounds__(256) void zscan_f64_kernel(rr::ZArgs a, long long n
     }
 }
 
+template <int NB, int G>
+int launch_g(const rr::ZArgs &a, hipStream_t s) {
+    constexpr int LZ = 64 / G;
+    dim3 grid((a.max_item_nz + LZ - 1) / LZ, (a.nitems + 4 * G - 1) / (4 * G));
+    const long long ne_off = rr::align256((long long)a.nitems * 2 * 8);
+    hipLaunchKernelGGL((zscan_f64_kernel<NB, 2, G>), grid, dim3(256), 0, s, a, ne_off);
+    return rr::check_launch("rr_zscan(f64)");
+}
+
 template <int NB>
 int launch_nb(const rr::ZArgs &a, hipStream_t s) {
-    dim3 block(256);
-    dim3 grid((a.max_item_nz + 63) / 64, (a.nitems + 3) / 4);
-    const long long ne_off = rr::align256((long long)a.nitems * 2 * 8);
-    hipLaunchKernelGGL((zscan_f64_kernel<NB, 2>), grid, block, 0, s, a, ne_off);
-    return rr::check_launch("rr_zscan(f64)");
+    if (a.max_item_nz <= 4) return launch_g<NB, 16>(a, s);
+    if (a.max_item_nz <= 16) return launch_g<NB, 4>(a, s);
+    return launch_g<NB, 1>(a, s);
 }
 
 }  // namespace

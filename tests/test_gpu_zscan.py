@@ -39,6 +39,32 @@ def test_f64_matches_oracle(galaxy):
     assert np.allclose(pen, rpen, rtol=1e-7, atol=1e-9 * max(np.abs(rpen).max(), 1.0))
 
 
+def test_f64_short_items_match_full_grid(galaxy):
+    """Items of few redshifts (4 or 16 items per wavefront) and ragged item
+    lengths give the full-grid scan's rows bit for bit: every lane runs the
+    same arithmetic whatever the packing."""
+    from redrock_amd import zscan
+    tm, targets, dt, *_ = galaxy
+    batch = zscan.TargetBatch(targets)
+    nt, nz = len(targets), dt.cache.nz
+    full = zscan.ScanPlan(dt.cache, batch, np.arange(nt), np.zeros(nt), np.full(nt, nz), 'f64').run()
+    c_full = full.zchi2.cpu().numpy().reshape(nt, nz)
+    f_full = full.zcoeff.cpu().numpy().reshape(nt, nz, -1)
+    rng = np.random.default_rng(3)
+    for case in ('ragged', 3, 15, 40):
+        it = np.repeat(np.arange(nt), 23)
+        rng.shuffle(it)
+        L = rng.integers(1, 17, size=it.size) if case == 'ragged' else np.full(it.size, case)
+        z0 = rng.integers(0, nz - L + 1)
+        print(f'case: f64 short items {case}, {it.size} items', flush=True)
+        plan = zscan.ScanPlan(dt.cache, batch, it, z0, L, 'f64').run()
+        c, f = plan.zchi2.cpu().numpy(), plan.zcoeff.cpu().numpy()
+        want_c = np.concatenate([c_full[t, z:z + n] for t, z, n in zip(it, z0, L)])
+        want_f = np.concatenate([f_full[t, z:z + n] for t, z, n in zip(it, z0, L)])
+        assert np.array_equal(c, want_c), case
+        assert np.array_equal(f, want_f), case
+
+
 def test_f64_deterministic(galaxy):
     """Run-to-run bit identity (the reference checks mp_procs=1 vs 2, test_zscan.py:113-140)."""
     from redrock_amd import zscan
