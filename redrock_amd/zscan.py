@@ -567,10 +567,11 @@ def calc_zchi2(target_ids, target_data, dtemplate, progress=None, use_gpu=False,
 
 def calc_zchi2_batch(spectra, tdata, weights, flux, wflux, nz, nbasis, solve_matrices_algorithm=None,
                      solver_args=None, use_gpu=False, fullprecision=True, prior=None,
-                     precision='f64'):
+                     precision='f64', polish=None):
     """chi2 and coefficients of one target at every redshift of `tdata` (zscan.py:626-781).
 
     tdata: {wavehash: (nz, nwave, nbasis)} numpy arrays or device tensors.
+    polish: FP64 polish of a mixed-precision scan (ScanPlan).
     Returns numpy (zchi2 (nz,), zcoeff (nz, nbasis)).
     """
     import torch
@@ -599,7 +600,7 @@ def calc_zchi2_batch(spectra, tdata, weights, flux, wflux, nz, nbasis, solve_mat
     wfa[0, :layout.npix] = _host(wflux)
     batch = TargetBatch(arrays=(layout, rband, wa, fa, wfa))
     cache = _ExplicitCache(tdata, layout, nz, nbasis)
-    plan = ScanPlan(cache, batch, [0], [0], [nz], precision, prior=prior)
+    plan = ScanPlan(cache, batch, [0], [0], [nz], precision, prior=prior, polish=polish)
     plan.run()
     c2, cf, _, _ = plan.host()
     return c2, cf

@@ -118,6 +118,13 @@ def test_calc_zchi2_batch_with_prior(gpu):
                                          prior=prior)
     assert np.max(np.abs(chi2 - rchi2) / rchi2) <= F64_RTOL
     assert np.allclose(coeff, rcoeff, rtol=1e-7, atol=1e-9 * np.abs(rcoeff).max())
+    # a polish tolerance that takes in every row (15 windows of 4 redshifts)
+    # turns a bf16 scan into the FP64 one, prior included
+    print('case: bf16 + polish of every row, prior, 60 z', flush=True)
+    chi2, coeff = zscan.calc_zchi2_batch(spectra, tdata, w, f, wf, len(zsel), tm.nbasis, 'PCA',
+                                         prior=prior, precision='bf16', polish=10.0)
+    assert np.max(np.abs(chi2 - rchi2) / rchi2) <= F64_RTOL
+    assert np.allclose(coeff, rcoeff, rtol=1e-7, atol=1e-9 * np.abs(rcoeff).max())
 
 
 def test_huge_chi2_conventions(gpu):
