@@ -156,6 +156,7 @@ def bench_mixed(args):
             f'({evals / dt_s / 1e6:.1f} M evals/s)')
         del sg
     rel, agree, agree_type = [], [], []
+    fwf = (batch.wf * batch.f).sum(dim=1).cpu().numpy()
     best = {p: np.stack([r[0].min(axis=1) for r in res[p]], axis=1) for p in res}
     per_template = {}
     for tm, (c2m, _, pm), (c2d, _, pd) in zip(tms, res[args.precision], res['f64']):
@@ -165,6 +166,7 @@ def bench_mixed(args):
         rb = r[np.arange(nt), c2d.argmin(axis=1)]
         per_template[tm.template_type] = {
             'max_rel_chi2_err': float(r.max()), 'median_rel_chi2_err': float(np.median(r)),
+            'max_chi2_err_over_fwf': float((np.abs(c2m - c2d) / fwf[:, None]).max()),
             'max_rel_chi2_err_at_best_z': float(rb.max()), 'best_z_agreement': float(agree[-1])}
         log(f'{tm.template_type}: max rel err {r.max():.2e} (at best z {rb.max():.2e}), '
             f'best-z agreement {agree[-1]:.4f}')
@@ -629,6 +631,8 @@ def main():
 
     c_mix = mix.zchi2.reshape(nt, nz).cpu().numpy()
     c_f64 = f64.zchi2.reshape(nt, nz).cpu().numpy()
+    # f'Wf per target: the scale of a mixed chi2's error (the sum chi2 cancels against)
+    fwf = (batch.wf * batch.f).sum(dim=1).cpu().numpy()
     if polish is not None:
         rr_ = np.abs(c_raw - c_f64) / np.abs(c_f64)
         polish.update({
@@ -708,6 +712,7 @@ def main():
         'best_z_agreement': agree,
         'max_rel_chi2_err': float(rel.max()),
         'max_rel_chi2_err_at_best_z': float(rel_best.max()),
+        'max_chi2_err_over_fwf': float((np.abs(c_mix - c_f64) / fwf[:, None]).max()),
         'setup_s': setup_s,
         'roofline': {'bound': 'mfma', 'achieved': ach / 1e12, 'peak': PEAK_FLOPS[args.precision] / 1e12,
                      'unit': 'TFLOP/s', 'frac': ach / PEAK_FLOPS[args.precision],

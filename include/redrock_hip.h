@@ -261,17 +261,19 @@ int rr_pack_mixed(const double *d_rband, const double *d_w, const double *d_f, i
  * For every item of `scan` (a descriptor that rr_zscan has run in full, in a
  * mixed precision): the windows of RR_POLISH_WINDOW consecutive redshifts
  * (item z index 4k .. 4k+3) holding a chi2, or chi2 + penalty, within
- * tol * |minimum| of the item's minimum of that quantity are re-evaluated in
+ * tol * max(f'Wf, |minimum|) of the item's minimum of that quantity (f'Wf of
+ * the item's target, from the scan's statistics stage) are re-evaluated in
  * FP64 (R-apply, normal equations, LU solve, compensated chi2, penalty,
- * status) from the FP64 template cache, and their rows of d_zchi2,
- * d_zcoeff, d_penalty and d_status are overwritten.  If more than
- * max_windows windows qualify, the tolerance is halved until they fit (the
- * windows nearest the minimum are kept). An item with a single row under the
- * full thresholds gets no window (that row is the FP64 argmin already; its
- * chi2 keeps the mixed value).  Items whose minimum chi2 is
- * HUGE_CHI2 or more are left alone.  With |chi2_mixed - chi2_f64| <= e chi2
- * and tol >= 2e + e^2 the argmin of every polished item equals the FP64
- * scan's (ties within FP64 rounding aside).
+ * status) from the FP64 template cache, and their rows of d_zchi2, d_zcoeff,
+ * d_penalty and d_status are overwritten.  If more than max_windows windows
+ * qualify, the tolerance is halved until they fit (the windows nearest the
+ * minimum are kept).  An item with a single row under the full thresholds
+ * gets no window (that row is the FP64 argmin already; its chi2 keeps the
+ * mixed value).  Items whose minimum chi2 is HUGE_CHI2 or more are left
+ * alone.  A mixed scan's chi2 error scales with f'Wf (the sum chi2 cancels
+ * against): with |chi2_mixed - chi2_f64| <= e f'Wf and tol >= 2e the argmin
+ * of every polished item equals the FP64 scan's (ties within FP64 rounding
+ * aside).
  *
  * scan->precision RR_PREC_F64: nothing to do, returns RR_OK.
  * Reads the scan's workspace statistics (stage RR_STAGE_STATS); the FP64

@@ -1,14 +1,17 @@
 // FP64 polish of a mixed-precision scan (rr_zscan_polish).
 //
-// A mixed-precision scan gives chi2 with a relative error e (about 1e-5 for
-// split fp16, 1e-3 for fp16, 1e-2 for bf16).  Where two redshifts' chi2
-// differ by less than that, the scan's best redshift can differ from the FP64
-// scan's.  The polish re-evaluates in FP64 every redshift whose mixed chi2,
-// or chi2 + penalty, lies within tol * |minimum| of the item's minimum, and
-// overwrites those rows (chi2, coefficients, penalty, status).  If
-// |chi2_mixed - chi2_f64| <= e chi2 everywhere and tol >= 2e + e^2, the FP64
-// minimum is among the re-evaluated rows and every other row stays above it,
-// so the item's argmin is the FP64 scan's.
+// A mixed-precision scan gives chi2 with an absolute error that scales with
+// the target's f'Wf, the sum chi2 = f'Wf - 2 c'y + c'Mc cancels against:
+// |chi2_mixed - chi2_f64| <= e f'Wf with e about 1e-6 for split fp16, 1e-4
+// for fp16, 1e-3 for bf16 (relative to chi2 the error grows with the
+// signal-to-noise ratio).  Where two redshifts' chi2 differ by less than
+// that, the scan's best redshift can differ from the FP64 scan's.  The
+// polish re-evaluates in FP64 every redshift whose mixed chi2, or chi2 +
+// penalty, lies within tol * max(f'Wf, |minimum|) of the item's minimum, and
+// overwrites those rows (chi2, coefficients, penalty, status).  If the error
+// is at most e f'Wf everywhere and tol >= 2e, the FP64 minimum is among the
+// re-evaluated rows and every other row stays above it, so the item's argmin
+// is the FP64 scan's.
 //
 // polish_select_kernel (one wavefront per item): per aligned window of 4
 //   redshifts the minimum of chi2 and of chi2 + penalty (in LDS), the item's
@@ -84,7 +87,10 @@ __global__ __launch_bounds__(64) void polish_select_kernel(rr::ZArgs a, rr::Poli
     __syncthreads();
     int nsel = 0;
     if (m1 < HUGE_CHI2) {
-        double d1 = p.tol * fabs(m1), d2 = p.tol * fabs(m2);
+        // error scale of a mixed-precision chi2: the item's f'Wf (the sum the
+        // chi2 cancels against), at least |minimum|
+        const double fwf = static_cast<const double *>(a.work)[2 * item];
+        double d1 = p.tol * fmax(fwf, fabs(m1)), d2 = p.tol * fmax(fwf, fabs(m2));
         double t1 = m1 + d1, t2 = m2 + d2;
         int it = 0;
         for (;; ++it) {

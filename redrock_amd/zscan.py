@@ -53,10 +53,12 @@ def default_precision():
     return os.environ.get('RR_PRECISION', 'f64')
 
 
-# Relative tolerance of the FP64 polish per mixed precision: at least four
-# times the largest |chi2_mixed - chi2_f64| / chi2 measured for that mode on
-# the galaxy petal, the QSO petal and the mixed GALAXY/QSO/STAR batch.
-POLISH_TOL = {'f16x3': 1e-4, 'f32': 1e-4, 'f16': 3e-3, 'bf16': 2e-2}
+# Tolerance of the FP64 polish per mixed precision, relative to the target's
+# f'Wf (a mixed chi2's error scales with f'Wf, the sum chi2 cancels against,
+# not with chi2): at least four times the largest |chi2_mixed - chi2_f64| /
+# f'Wf measured for that mode over GALAXY / QSO / STAR targets at per-pixel
+# S/N 0.3 to 30 and the masked GALAXY/QSO/STAR batch.
+POLISH_TOL = {'f16x3': 1e-5, 'f32': 1e-5, 'f16': 1e-3, 'bf16': 5e-3}
 POLISH_WINDOWS = 16
 STAGE_POLISH = 8      # ScanPlan.run stage bit (rr_zscan_polish), beside RR_STAGE_*
 
