@@ -145,9 +145,11 @@ class _Refiner(object):
             if not sel:
                 continue
             slots = np.array([self.slot[owners[j]][1] for j in sel])
-            if len(sel) == nitem and self.precision == 'f64':
+            if len(sel) == nitem and self.precision == 'f64' and (per_item > 16 or per_item <= 4):
                 # consecutive candidates of one target share a scan item (their
-                # fine grids are consecutive in zlist): more redshifts per wavefront
+                # fine grids are consecutive in zlist): more redshifts per
+                # wavefront.  Grids of 5-16 points stay one item each: the FP64
+                # kernel packs such items 4 to a wavefront
                 tr, zr, nr = target_runs(np.repeat(slots, per_item))
                 plan = ScanPlan(cache, self._tb[sig], tr, zr, nr, self.precision, oiisum=self.oii)
             else:
