@@ -269,8 +269,10 @@ int rr_pack_mixed(const double *d_rband, const double *d_w, const double *d_f, i
  * qualify, the tolerance is halved until they fit (the windows nearest the
  * minimum are kept).  An item with a single row under the full thresholds
  * gets no window (that row is the FP64 argmin already; its chi2 keeps the
- * mixed value).  Items whose minimum chi2 is HUGE_CHI2 or more are left
- * alone.  A mixed scan's chi2 error scales with f'Wf (the sum chi2 cancels
+ * mixed value).  Rows with a status other than RR_STATUS_OK take no part,
+ * and an item whose target has no weight (HUGE_CHI2 everywhere) is left
+ * alone; a real chi2 above HUGE_CHI2 (a bright target) is polished like any
+ * other.  A mixed scan's chi2 error scales with f'Wf (the sum chi2 cancels
  * against): with |chi2_mixed - chi2_f64| <= e f'Wf and tol >= 2e the argmin
  * of every polished item equals the FP64 scan's (ties within FP64 rounding
  * aside).

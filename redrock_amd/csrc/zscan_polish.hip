@@ -66,11 +66,15 @@ __global__ __launch_bounds__(64) void polish_select_kernel(rr::ZArgs a, rr::Poli
     double *wm1 = wmin, *wm2 = wmin + p.nw_all;
     const double inf = __builtin_huge_val();
     double m1 = inf, m2 = inf;
+    // rows that hold the HUGE_CHI2 convention (singular solve: status != OK)
+    // take no part; a target without weight is HUGE_CHI2 everywhere.  A real
+    // chi2 may exceed HUGE_CHI2 (bright targets), so the value itself is no test.
+    const double sumw = static_cast<const double *>(a.work)[2 * item + 1];
     for (int k = lane; k < nw; k += 64) {
         double w1 = inf, w2 = inf;
         for (int i = 0; i < WZ; ++i) {
             const int iz = k * WZ + i;
-            if (iz < nzi) {
+            if (iz < nzi && (!a.status || a.status[row0 + iz] == RR_STATUS_OK)) {
                 const double c = a.zchi2[row0 + iz];
                 const double s = c + (a.penalty ? a.penalty[row0 + iz] : 0.0);
                 w1 = fmin(w1, c);
@@ -86,7 +90,7 @@ __global__ __launch_bounds__(64) void polish_select_kernel(rr::ZArgs a, rr::Poli
     m2 = wave_min(m2);
     __syncthreads();
     int nsel = 0;
-    if (m1 < HUGE_CHI2) {
+    if (sumw > 0.0 && m1 < inf) {
         // error scale of a mixed-precision chi2: the item's f'Wf (the sum the
         // chi2 cancels against), at least |minimum|
         const double fwf = static_cast<const double *>(a.work)[2 * item];
@@ -113,6 +117,7 @@ __global__ __launch_bounds__(64) void polish_select_kernel(rr::ZArgs a, rr::Poli
         int rows = 0;
         if (it == 0) {
             for (int iz = lane; iz < nzi; iz += 64) {
+                if (a.status && a.status[row0 + iz] != RR_STATUS_OK) continue;
                 const double c = a.zchi2[row0 + iz];
                 const double s = c + (a.penalty ? a.penalty[row0 + iz] : 0.0);
                 rows += (c <= t1 || s <= t2) ? 1 : 0;

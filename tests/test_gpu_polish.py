@@ -26,18 +26,19 @@ HUGE = 9999999.0
 WZ = 4
 
 
-def select_windows(chi2, pen, fwf, tol, maxw):
+def select_windows(chi2, pen, status, fwf, sumw, tol, maxw):
     nz = chi2.size
     nw = (nz + WZ - 1) // WZ
+    ok = status == 0                 # singular rows (HUGE_CHI2 by convention) take no part
     pad = np.full(nw * WZ, np.inf)
     c = pad.copy()
-    c[:nz] = chi2
+    c[:nz] = np.where(ok, chi2, np.inf)
     s = pad.copy()
-    s[:nz] = chi2 + pen
+    s[:nz] = np.where(ok, chi2 + pen, np.inf)
     w1 = c.reshape(nw, WZ).min(axis=1)
     w2 = s.reshape(nw, WZ).min(axis=1)
     m1, m2 = w1.min(), w2.min()
-    if not m1 < HUGE:
+    if not (sumw > 0 and m1 < np.inf):  # no weight: HUGE_CHI2 everywhere
         return np.zeros(0, dtype=np.int64)
     d1, d2 = tol * max(fwf, abs(m1)), tol * max(fwf, abs(m2))
     it = 0
@@ -52,15 +53,16 @@ def select_windows(chi2, pen, fwf, tol, maxw):
             d1 *= 0.5
             d2 *= 0.5
         it += 1
-    if it == 0 and np.sum((chi2 <= t1) | (chi2 + pen <= t2)) <= 1:
+    if it == 0 and np.sum(ok & ((chi2 <= t1) | (chi2 + pen <= t2))) <= 1:
         return np.zeros(0, dtype=np.int64)      # isolated minimum
     return np.flatnonzero(sel)[:maxw]
 
 
-def fwf_of(plan):
-    """f'Wf of every item, as the scan's statistics stage left it in the workspace."""
+def fwf_of(plan, k=0):
+    """f'Wf (k = 0) or sum(w) (k = 1) of every item, as the scan's statistics
+    stage left them in the workspace."""
     import torch
-    return plan.work[:16 * plan.nitems].view(torch.float64).cpu().numpy()[0::2]
+    return plan.work[:16 * plan.nitems].view(torch.float64).cpu().numpy()[k::2]
 
 
 def _plans(tm, targets, precision, polish, windows=None):
@@ -85,16 +87,16 @@ def _plans(tm, targets, precision, polish, windows=None):
 
 def _check(f64, raw, pol, nt, nz, tol, windows, dead=()):
     c64, f64c, p64, _ = f64.host()
-    craw, rawc, praw, _ = raw.host()
+    craw, rawc, praw, sraw = raw.host()
     cpol, polc, ppol, spol = pol.host()
-    c64, craw, cpol = (a.reshape(nt, nz) for a in (c64, craw, cpol))
+    c64, craw, cpol, sraw = (a.reshape(nt, nz) for a in (c64, craw, cpol, sraw))
     praw, ppol, p64 = (a.reshape(nt, nz) for a in (praw, ppol, p64))
     nwin = pol.nwindows.cpu().numpy()
     win = pol.windows.cpu().numpy()
-    fwf = fwf_of(pol)
+    fwf, sumw = fwf_of(pol), fwf_of(pol, 1)
     nrows = 0
     for i in range(nt):
-        want = select_windows(craw[i], praw[i], fwf[i], tol, windows)
+        want = select_windows(craw[i], praw[i], sraw[i], fwf[i], sumw[i], tol, windows)
         assert nwin[i] == want.size, i
         assert np.array_equal(win[i, :nwin[i]], want), i
         mask = np.zeros(nz, dtype=bool)
@@ -140,6 +142,15 @@ def test_polish_galaxy_best_z(gpu, precision, snr):
     assert rel[did].max(initial=0.0) <= 1e-12
     err_f = np.abs(cpol[live, best] - c64[live, best]) / fwf_of(pol)[live]
     assert err_f.max() <= tol
+    if snr >= 30:
+        # bright targets: real chi2 minima above HUGE_CHI2 are polished like any
+        # other (a wide tolerance puts several rows of every target in range)
+        big = c64[live].min(axis=1) > HUGE
+        assert big.any()
+        _, _, wide, _, _ = _plans(tm, targets, precision, 0.05)
+        assert np.all(wide.nwindows.cpu().numpy()[live][big] > 0)
+        assert np.array_equal(wide.zchi2.cpu().numpy().reshape(nt, nz)[live].argmin(axis=1),
+                              c64[live].argmin(axis=1))
 
 
 def test_polish_star_fine_grid(gpu):
