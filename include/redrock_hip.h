@@ -20,7 +20,7 @@
 extern "C" {
 #endif
 
-#define RR_ABI_VERSION 6
+#define RR_ABI_VERSION 7
 
 /* Error codes. */
 #define RR_OK 0
@@ -183,7 +183,18 @@ typedef struct {
      * The stages communicate through the workspace, so a partial call
      * relies on an earlier call having run the other stages on it.        */
     int stages;
+    /* Coefficient solve per (item, redshift), the template's method
+     * (solve_matrices, zscan.py:1107-1205):
+     *   RR_SOLVE_PCA : M c = y (LU in FP64, Cholesky in the mixed modes)
+     *   RR_SOLVE_NNLS: min |M c - y|^2 over c >= 0, the reference's NMF / NNLS
+     *                  templates (scipy.optimize.nnls(M, y), zscan.py:1153-1172),
+     *                  Lawson-Hanson active sets; no solution within 3 nb
+     *                  iterations -> HUGE_CHI2, status RR_STATUS_SINGULAR
+     *                  (the reference's LinAlgError, zscan.py:769-771)         */
+    int solve;
 } rr_zscan_desc;
+#define RR_SOLVE_PCA 0
+#define RR_SOLVE_NNLS 1
 
 #define RR_STAGE_STATS 1    /* per-item f'Wf and sum(w)                     */
 #define RR_STAGE_MAIN 2     /* normal equations (the scan kernel)           */

@@ -39,6 +39,10 @@ calc_deltachi2             reference vectors (test_zfind.py:141-177,
 sort_zfit_dict             test_zscan.py:101-111, test_zfind.py:86-139), and
                            the details file's deltachi2, SMALL_DELTA_CHI2
                            bits and row order (exact)
+solve_matrices_nnls        scipy.optimize.nnls (the reference's routine); pinned
+                           against brute-force active sets with the KKT
+                           conditions and against np.linalg.solve where no
+                           bound is active (tests/test_oracle.py)
 solve_matrices_bvls        scipy lsq_linear 'bvls' (the reference's routine);
                            pinned against brute-force active sets with the
                            KKT conditions and against np.linalg.solve where
@@ -301,6 +305,16 @@ def solve_matrices_pca(M, y):
     return np.linalg.solve(M, y.reshape(y.shape + (1,))).reshape(y.shape)
 
 
+def solve_matrices_nnls(M, y):
+    """zscan.py:1153-1172 (NMF / NNLS branch, CPU): scipy.optimize.nnls(M, y) --
+    the library routine the reference calls; its failure becomes LinAlgError."""
+    from scipy.optimize import nnls
+    try:
+        return nnls(M, y)[0]
+    except Exception:
+        raise np.linalg.LinAlgError
+
+
 def solve_matrices_bvls(M, y, bounds):
     """zscan.py:1175-1201 (BVLS branch): scipy's lsq_linear(M, y, bounds,
     method='bvls') -- the library routine the reference calls."""
@@ -308,8 +322,10 @@ def solve_matrices_bvls(M, y, bounds):
     return lsq_linear(M, y, bounds=bounds, method='bvls').x
 
 
-def calc_zchi2_batch(spectra, tdata, weights, flux, wflux, nz, nbasis, prior=None, bounds=None):
-    """CPU branch of zscan.py:626-781: PCA solve, or BVLS when bounds (2, nbasis) are given."""
+def calc_zchi2_batch(spectra, tdata, weights, flux, wflux, nz, nbasis, prior=None, bounds=None,
+                     algo='PCA'):
+    """CPU branch of zscan.py:626-781: PCA solve (algo 'PCA'), NNLS (algo 'NMF' or
+    'NNLS'), or BVLS when bounds (2, nbasis) are given."""
     zchi2 = np.zeros(nz)
     if weights.sum() == 0:
         zchi2[:] = HUGE_CHI2
@@ -323,10 +339,12 @@ def calc_zchi2_batch(spectra, tdata, weights, flux, wflux, nz, nbasis, prior=Non
         if prior is not None:
             M += prior
         try:
-            if bounds is None:
-                zcoeff[i, :] = solve_matrices_pca(M, y)
-            else:
+            if bounds is not None:
                 zcoeff[i, :] = solve_matrices_bvls(M, y, bounds)
+            elif algo.upper() in ('NMF', 'NNLS'):
+                zcoeff[i, :] = solve_matrices_nnls(M, y)
+            else:
+                zcoeff[i, :] = solve_matrices_pca(M, y)
         except np.linalg.LinAlgError:
             zchi2[i] = HUGE_CHI2
             continue
@@ -341,8 +359,9 @@ def calc_negOII_penalty(OIItemplate, coeff):
     return -OIIflux * (OIIflux < 0)
 
 
-def calc_zchi2(targets, tdata, nbasis, OIItemplate=None):
-    """Serial form of zscan.py:807-889 over a list of targets."""
+def calc_zchi2(targets, tdata, nbasis, OIItemplate=None, algo='PCA'):
+    """Serial form of zscan.py:807-889 over a list of targets (algo: the
+    template's solve_matrices_algorithm)."""
     nt = len(targets)
     nz = next(iter(tdata.values())).shape[0]
     zchi2 = np.zeros((nt, nz))
@@ -353,7 +372,7 @@ def calc_zchi2(targets, tdata, nbasis, OIItemplate=None):
         if np.sum(w) == 0:
             zchi2[j, :] = HUGE_CHI2
             continue
-        zchi2[j], zcoeff[j] = calc_zchi2_batch(tg.spectra, tdata, w, f, wf, nz, nbasis)
+        zchi2[j], zcoeff[j] = calc_zchi2_batch(tg.spectra, tdata, w, f, wf, nz, nbasis, algo=algo)
         if OIItemplate is not None:
             zpen[j] = calc_negOII_penalty(OIItemplate, zcoeff[j])
     return zchi2, zcoeff, zpen

@@ -65,6 +65,8 @@ static int scan_args(const rr_zscan_desc *d, rr::ZArgs &a) {
     } else {
         return rr::fail(RR_EINVAL, "rr_zscan: unknown precision %d", d->precision);
     }
+    if (d->solve != RR_SOLVE_PCA && d->solve != RR_SOLVE_NNLS)
+        return rr::fail(RR_EINVAL, "rr_zscan: unknown solve %d", d->solve);
 
     a.nitems = d->nitems; a.npix = d->npix; a.npix_pad = d->npix_pad; a.nb = d->nb;
     a.precision = d->precision; a.halo = d->halo;
@@ -79,6 +81,7 @@ static int scan_args(const rr_zscan_desc *d, rr::ZArgs &a) {
     a.zchi2 = d->d_zchi2; a.zcoeff = d->d_zcoeff; a.penalty = d->d_penalty; a.status = d->d_status;
     a.max_item_nz = d->max_item_nz;
     a.work = d->d_work; a.total_rows = d->total_rows;
+    a.solve = d->solve;
     return RR_OK;
 }
 

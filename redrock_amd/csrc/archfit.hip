@@ -36,6 +36,7 @@
 //                      by active sets (Householder QR of the free columns),
 //                      chi2 as c'M0c - 2c'y + f'Wf with M0 = M - prior.
 #include "common.hpp"
+#include "solve.hpp"
 
 // the bin-mean expressions match rr_rebin's double for double; every other
 // product-sum here is an explicit fma()
@@ -421,83 +422,6 @@ __global__ __launch_bounds__(256) void arch_accum_kernel(ArchArgs a) {
 }
 
 // ---------------------------------------------------------------- solve
-// min |M[:, F] x - y|^2 (x_j = 0 outside the free set F) by Householder QR of
-// the free columns taken in index order.  Columns with zero weighted norm are
-// excluded from F by the caller.
-template <int NB>
-__device__ void ls_free(const double (&M)[NB][NB], const double (&y)[NB], unsigned F, double (&x)[NB]) {
-    double A[NB][NB], b[NB];
-#pragma unroll
-    for (int i = 0; i < NB; ++i) {
-        b[i] = y[i];
-#pragma unroll
-        for (int j = 0; j < NB; ++j) A[i][j] = M[i][j];
-    }
-    int piv[NB];
-    int r = 0;
-#pragma unroll
-    for (int j = 0; j < NB; ++j) {
-        piv[j] = -1;
-        if (!((F >> j) & 1u)) continue;
-        // reflect rows r.. of column j onto row r
-        double nrm2 = 0.0;
-#pragma unroll
-        for (int i = 0; i < NB; ++i)
-            if (i >= r) nrm2 = fma(A[i][j], A[i][j], nrm2);
-        double arr = 0.0;
-#pragma unroll
-        for (int i = 0; i < NB; ++i)
-            if (i == r) arr = A[i][j];
-        const double alpha = arr >= 0.0 ? -sqrt(nrm2) : sqrt(nrm2);
-        double v[NB];
-#pragma unroll
-        for (int i = 0; i < NB; ++i) v[i] = (i > r) ? A[i][j] : (i == r ? arr - alpha : 0.0);
-        const double vv = (arr - alpha) * (arr - alpha) + (nrm2 - arr * arr);
-        if (vv > 0.0) {
-            const double s2 = 2.0 / vv;
-#pragma unroll
-            for (int j2 = 0; j2 < NB; ++j2) {
-                if (j2 <= j || !((F >> j2) & 1u)) continue;
-                double d = 0.0;
-#pragma unroll
-                for (int i = 0; i < NB; ++i) d = fma(v[i], A[i][j2], d);
-                d *= s2;
-#pragma unroll
-                for (int i = 0; i < NB; ++i) A[i][j2] = fma(-d, v[i], A[i][j2]);
-            }
-            double d = 0.0;
-#pragma unroll
-            for (int i = 0; i < NB; ++i) d = fma(v[i], b[i], d);
-            d *= s2;
-#pragma unroll
-            for (int i = 0; i < NB; ++i) b[i] = fma(-d, v[i], b[i]);
-#pragma unroll
-            for (int i = 0; i < NB; ++i)
-                if (i == r) A[i][j] = alpha;
-        }
-        piv[j] = r;
-        ++r;
-    }
-    // back substitution, last free column first
-#pragma unroll
-    for (int j = NB - 1; j >= 0; --j) {
-        x[j] = 0.0;
-        if (piv[j] < 0) continue;
-        const int pr = piv[j];
-        double s = 0.0, diag = 0.0;
-#pragma unroll
-        for (int i = 0; i < NB; ++i)
-            if (i == pr) {
-                s = b[i];
-                diag = A[i][j];
-#pragma unroll
-                for (int j2 = 0; j2 < NB; ++j2)
-                    if (j2 > j) s = fma(-A[i][j2], x[j2], s);
-            }
-        x[j] = s / diag;
-    }
-}
-
 template <int NA, int NL>
 __global__ __launch_bounds__(64) void arch_solve_kernel(ArchArgs a) {
     constexpr int NB = NA + NL;
@@ -562,7 +486,7 @@ __global__ __launch_bounds__(64) void arch_solve_kernel(ArchArgs a) {
     const int nbound = a.nbound;
     for (unsigned S0 = 0; S0 < (1u << nbound) && !done; ++S0) {
         const unsigned Fm = live & ~S0;
-        ls_free<NB>(M, y, Fm, c);
+        rr::ls_free<NB>(M, y, Fm, c);
         bool feasible = true;
 #pragma unroll
         for (int j = 0; j < NA; ++j)

@@ -96,6 +96,191 @@ __device__ __forceinline__ bool chol_solve(double (&a)[NB][NB], double (&c)[NB])
     return ok;
 }
 
+// min |M[:, F] x - y|^2 (x_j = 0 outside the free set F) by Householder QR of
+// the free columns taken in index order.  Columns with zero weighted norm are
+// excluded from F by the caller.
+template <int NB>
+__device__ inline void ls_free(const double (&M)[NB][NB], const double (&y)[NB], unsigned F, double (&x)[NB]) {
+    double A[NB][NB], b[NB];
+#pragma unroll
+    for (int i = 0; i < NB; ++i) {
+        b[i] = y[i];
+#pragma unroll
+        for (int j = 0; j < NB; ++j) A[i][j] = M[i][j];
+    }
+    int piv[NB];
+    int r = 0;
+#pragma unroll
+    for (int j = 0; j < NB; ++j) {
+        piv[j] = -1;
+        if (!((F >> j) & 1u)) continue;
+        // reflect rows r.. of column j onto row r
+        double nrm2 = 0.0;
+#pragma unroll
+        for (int i = 0; i < NB; ++i)
+            if (i >= r) nrm2 = fma(A[i][j], A[i][j], nrm2);
+        double arr = 0.0;
+#pragma unroll
+        for (int i = 0; i < NB; ++i)
+            if (i == r) arr = A[i][j];
+        const double alpha = arr >= 0.0 ? -sqrt(nrm2) : sqrt(nrm2);
+        double v[NB];
+#pragma unroll
+        for (int i = 0; i < NB; ++i) v[i] = (i > r) ? A[i][j] : (i == r ? arr - alpha : 0.0);
+        const double vv = (arr - alpha) * (arr - alpha) + (nrm2 - arr * arr);
+        if (vv > 0.0) {
+            const double s2 = 2.0 / vv;
+#pragma unroll
+            for (int j2 = 0; j2 < NB; ++j2) {
+                if (j2 <= j || !((F >> j2) & 1u)) continue;
+                double d = 0.0;
+#pragma unroll
+                for (int i = 0; i < NB; ++i) d = fma(v[i], A[i][j2], d);
+                d *= s2;
+#pragma unroll
+                for (int i = 0; i < NB; ++i) A[i][j2] = fma(-d, v[i], A[i][j2]);
+            }
+            double d = 0.0;
+#pragma unroll
+            for (int i = 0; i < NB; ++i) d = fma(v[i], b[i], d);
+            d *= s2;
+#pragma unroll
+            for (int i = 0; i < NB; ++i) b[i] = fma(-d, v[i], b[i]);
+#pragma unroll
+            for (int i = 0; i < NB; ++i)
+                if (i == r) A[i][j] = alpha;
+        }
+        piv[j] = r;
+        ++r;
+    }
+    // back substitution, last free column first
+#pragma unroll
+    for (int j = NB - 1; j >= 0; --j) {
+        x[j] = 0.0;
+        if (piv[j] < 0) continue;
+        const int pr = piv[j];
+        double s = 0.0, diag = 0.0;
+#pragma unroll
+        for (int i = 0; i < NB; ++i)
+            if (i == pr) {
+                s = b[i];
+                diag = A[i][j];
+#pragma unroll
+                for (int j2 = 0; j2 < NB; ++j2)
+                    if (j2 > j) s = fma(-A[i][j2], x[j2], s);
+            }
+        x[j] = s / diag;
+    }
+}
+
+
+// Non-negative least squares min |M x - y|^2 over x >= 0 by the active-set
+// method of Lawson and Hanson (Solving Least Squares Problems, 1974, ch. 23),
+// the problem scipy.optimize.nnls(M, y) solves for the reference's NMF
+// templates (zscan.py:1153-1172).  Each passive-set subproblem is ls_free
+// (Householder QR of the passive columns).  The solution is unique when M has
+// full rank.  Returns false when 3 NB inner iterations did not settle (scipy
+// raises there; the reference then gives HUGE_CHI2, zscan.py:769-771).
+template <int NB>
+__device__ inline bool nnls_solve(const double (&M)[NB][NB], const double (&y)[NB], double (&x)[NB]) {
+    double scale = 0.0, ymax = 0.0;          // |M|_1 |y|_inf: the scale of the gradient
+#pragma unroll
+    for (int j = 0; j < NB; ++j) {
+        double cs = 0.0;
+#pragma unroll
+        for (int i = 0; i < NB; ++i) cs += fabs(M[i][j]);
+        scale = fmax(scale, cs);
+        ymax = fmax(ymax, fabs(y[j]));
+    }
+    const double tol = 10.0 * NB * 2.220446049250313e-16 * scale * ymax;
+    double w[NB];                            // M^T (y - M x), the negative gradient
+    auto gradient = [&]() {
+        double r[NB];
+#pragma unroll
+        for (int i = 0; i < NB; ++i) {
+            double v = y[i];
+#pragma unroll
+            for (int j = 0; j < NB; ++j) v = fma(-M[i][j], x[j], v);
+            r[i] = v;
+        }
+#pragma unroll
+        for (int j = 0; j < NB; ++j) {
+            double v = 0.0;
+#pragma unroll
+            for (int i = 0; i < NB; ++i) v = fma(M[i][j], r[i], v);
+            w[j] = v;
+        }
+    };
+#pragma unroll
+    for (int j = 0; j < NB; ++j) x[j] = 0.0;
+    gradient();
+    unsigned P = 0u;                         // passive (free) set
+    unsigned rejected = 0u;                  // indices that cannot enter at this x
+    int iter = 0;
+    for (int outer = 0; outer < 4 * NB; ++outer) {
+        int t = -1;
+        double wt = tol;
+#pragma unroll
+        for (int j = 0; j < NB; ++j)
+            if (!(((P | rejected) >> j) & 1u) && w[j] > wt) {
+                wt = w[j];
+                t = j;
+            }
+        if (t < 0) return true;              // Kuhn-Tucker conditions hold
+        const unsigned P0 = P;
+        P |= 1u << t;
+        double s[NB];
+        ls_free<NB>(M, y, P, s);
+        bool tpos = false;
+#pragma unroll
+        for (int j = 0; j < NB; ++j)
+            if (j == t) tpos = s[j] > 0.0;
+        if (!tpos) {                         // t would enter at a non-positive value
+            P = P0;
+            rejected |= 1u << t;
+            continue;
+        }
+        for (;;) {
+            bool feasible = true;
+#pragma unroll
+            for (int j = 0; j < NB; ++j)
+                if (((P >> j) & 1u) && !(s[j] > 0.0)) feasible = false;
+            if (feasible) {
+#pragma unroll
+                for (int j = 0; j < NB; ++j) x[j] = s[j];
+                break;
+            }
+            if (++iter > 3 * NB) return false;
+            // move from x towards s until the first passive coefficient reaches 0
+            double alpha = 2.0;
+            int jmin = -1;
+#pragma unroll
+            for (int j = 0; j < NB; ++j)
+                if (((P >> j) & 1u) && !(s[j] > 0.0)) {
+                    const double d = x[j] - s[j];
+                    const double a = d > 0.0 ? x[j] / d : 0.0;
+                    if (a < alpha) {
+                        alpha = a;
+                        jmin = j;
+                    }
+                }
+#pragma unroll
+            for (int j = 0; j < NB; ++j) {
+                x[j] = fma(alpha, s[j] - x[j], x[j]);
+                if (j == jmin || (((P >> j) & 1u) && !(x[j] > 0.0))) {
+                    P &= ~(1u << j);
+                    x[j] = 0.0;
+                }
+            }
+            if (P == 0u) break;
+            ls_free<NB>(M, y, P, s);
+        }
+        rejected = 0u;
+        gradient();
+    }
+    return false;
+}
+
 // Error-free transformations for the compensated chi2 = f'Wf - c'y.
 __device__ __forceinline__ void two_sum(double a, double b, double &s, double &e) {
     s = a + b;

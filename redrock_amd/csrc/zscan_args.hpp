@@ -24,6 +24,7 @@ struct ZArgs {
     int max_item_nz;
     void *work;            // workspace: item stats then normal equations
     long long total_rows;  // rows of the output arrays (>= max item_out + nz)
+    int solve;             // RR_SOLVE_PCA or RR_SOLVE_NNLS
 };
 
 // Kernel-argument form of rr_polish_desc.

@@ -10,6 +10,8 @@
 //              zero pivot -> HUGE_CHI2 and zero coefficients (:769-771).
 //   mixed    : Cholesky (M is a Gram matrix), LU when Cholesky meets a
 //              non-positive pivot.
+//   NNLS     : (the NMF / NNLS templates, either precision) min |Mc - y|^2
+//              over c >= 0 (scipy.optimize.nnls(M, y), zscan.py:1153-1172).
 // chi2 = f'Wf - 2 c'y + c'Mc with the unregularised M (compensated sum): the
 // reference's explicit residual sum(w (f - A c)^2) (:777-780) expanded, which
 // stays exact with a prior and is stationary in c.
@@ -46,7 +48,7 @@ __global__ __launch_bounds__(64) void stats_kernel(rr::ZArgs a) {
     }
 }
 
-template <int NB, bool MIXED>
+template <int NB, bool MIXED, bool NNLS>
 __global__ __launch_bounds__(64) void finish_kernel(rr::ZArgs a, long long ne_off) {
     const int item = blockIdx.y;
     const int iz = blockIdx.x * 64 + threadIdx.x;
@@ -102,7 +104,9 @@ __global__ __launch_bounds__(64) void finish_kernel(rr::ZArgs a, long long ne_of
 #pragma unroll
     for (int i = 0; i < NB; ++i) c[i] = y[i];
     bool ok;
-    if constexpr (MIXED) {
+    if constexpr (NNLS) {
+        ok = rr::nnls_solve<NB>(M, y, c);
+    } else if constexpr (MIXED) {
         double L[NB][NB];
 #pragma unroll
         for (int i = 0; i < NB; ++i)
@@ -159,10 +163,18 @@ int launch_finish(const rr::ZArgs &a, hipStream_t s) {
     const bool mixed = a.precision != RR_PREC_F64;
     const long long ne_off = rr::align256((long long)a.nitems * 2 * 8);
     dim3 grid((a.max_item_nz + 63) / 64, a.nitems);
-    if (mixed)
-        hipLaunchKernelGGL((finish_kernel<NB, true>), grid, dim3(64), 0, s, a, ne_off);
-    else
-        hipLaunchKernelGGL((finish_kernel<NB, false>), grid, dim3(64), 0, s, a, ne_off);
+    // the NNLS solve is its own instantiation: its active-set loop would
+    // raise the register use (and spill) of the plain solves
+    if (a.solve == RR_SOLVE_NNLS) {
+        if (mixed)
+            hipLaunchKernelGGL((finish_kernel<NB, true, true>), grid, dim3(64), 0, s, a, ne_off);
+        else
+            hipLaunchKernelGGL((finish_kernel<NB, false, true>), grid, dim3(64), 0, s, a, ne_off);
+    } else if (mixed) {
+        hipLaunchKernelGGL((finish_kernel<NB, true, false>), grid, dim3(64), 0, s, a, ne_off);
+    } else {
+        hipLaunchKernelGGL((finish_kernel<NB, false, false>), grid, dim3(64), 0, s, a, ne_off);
+    }
     return rr::check_launch("rr_zscan(finish)");
 }
 

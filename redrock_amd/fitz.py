@@ -151,10 +151,12 @@ class _Refiner(object):
                 # wavefront.  Grids of 5-16 points stay one item each: the FP64
                 # kernel packs such items 4 to a wavefront
                 tr, zr, nr = target_runs(np.repeat(slots, per_item))
-                plan = ScanPlan(cache, self._tb[sig], tr, zr, nr, self.precision, oiisum=self.oii)
+                plan = ScanPlan(cache, self._tb[sig], tr, zr, nr, self.precision, oiisum=self.oii,
+                                solve=self.template.solve_matrices_algorithm)
             else:
                 plan = ScanPlan(cache, self._tb[sig], slots, [j * per_item for j in sel],
-                                [per_item] * len(sel), self.precision, oiisum=self.oii)
+                                [per_item] * len(sel), self.precision, oiisum=self.oii,
+                                solve=self.template.solve_matrices_algorithm)
             plan.run()
             c2, cf, pn, _ = plan.host()
             chi2[sel] = c2.reshape(len(sel), per_item)

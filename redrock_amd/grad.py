@@ -85,7 +85,8 @@ class GradProblem(object):
         zh = z.detach().cpu().numpy() if hasattr(z, 'detach') else np.asarray(z, dtype=np.float64)
         cache = TemplateCache(self.template, self.dwave, redshifts=zh, flux=flux)
         # consecutive entries of one target share a scan item (one z per lane)
-        plan = ScanPlan(cache, self.batch, *target_runs(self.item_target), 'f64')
+        plan = ScanPlan(cache, self.batch, *target_runs(self.item_target), 'f64',
+                        solve=self.template.solve_matrices_algorithm)
         plan.run()
         return plan.zchi2, plan.zcoeff
 

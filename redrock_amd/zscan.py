@@ -63,6 +63,18 @@ POLISH_WINDOWS = 16
 STAGE_POLISH = 8      # ScanPlan.run stage bit (rr_zscan_polish), beside RR_STAGE_*
 
 
+SOLVES = {'PCA': _hip.RR_SOLVE_PCA, 'NMF': _hip.RR_SOLVE_NNLS, 'NNLS': _hip.RR_SOLVE_NNLS}
+
+
+def solve_code(method):
+    """RR_SOLVE_* of a template method / solve_matrices_algorithm name
+    (zscan.py:1107-1205): PCA -> M c = y, NMF or NNLS -> nnls(M, y)."""
+    key = 'PCA' if method is None else str(method).upper()
+    if key not in SOLVES:
+        raise NotImplementedError(f'solve_matrices algorithm {method!r} is not available in the scan')
+    return SOLVES[key]
+
+
 def polish_tol(precision, polish):
     """Tolerance of the FP64 polish for `precision`, or None when it does not run.
 
@@ -311,10 +323,12 @@ class ScanPlan(object):
     """
 
     def __init__(self, cache, batch, item_target, item_z0, item_nz, precision='f64',
-                 prior=None, oiisum=None, tile=0, polish=None, polish_windows=POLISH_WINDOWS):
+                 prior=None, oiisum=None, tile=0, polish=None, polish_windows=POLISH_WINDOWS,
+                 solve='PCA'):
         """polish: FP64 re-evaluation of the rows near each item's minimum after a
         mixed-precision scan (rr_zscan_polish), see polish_tol(); it needs the
-        FP64 template cache as well."""
+        FP64 template cache as well.  solve: the template's method, 'PCA' or
+        'NMF' / 'NNLS' (non-negative coefficients, no polish)."""
         import torch
         if precision not in _hip.PRECISIONS:
             raise ValueError(f'unknown precision {precision!r}')
@@ -388,6 +402,7 @@ class ScanPlan(object):
         d.d_status = self.status.data_ptr()
         d.max_item_nz = int(item_nz.max()) if self.nitems else 0
         d.total_rows = rows
+        d.solve = solve_code(solve)
         nbytes = _hip.lib().rr_zscan_workspace_bytes(ctypes.byref(d))
         if nbytes < 0:
             _hip.check(int(nbytes), 'rr_zscan_workspace_bytes')
@@ -398,6 +413,10 @@ class ScanPlan(object):
         self.precision = precision
         self.polish = None
         tol = polish_tol(precision, polish)
+        if tol is not None and d.solve != _hip.RR_SOLVE_PCA:
+            if polish not in (None, False):
+                raise ValueError('the FP64 polish re-solves with M c = y: not for NNLS templates')
+            tol = None
         if tol is not None and self.nitems > 0 and int(item_nz.max()) > _hip.RR_POLISH_MAX_NZ:
             if polish is not None:
                 raise ValueError(f'the FP64 polish handles items of at most '
@@ -444,17 +463,25 @@ class ScanPlan(object):
                 self.penalty.cpu().numpy(), self.status.cpu().numpy())
 
 
+def _method(cache):
+    """solve_matrices_algorithm of the template behind a cache ('PCA' if none)."""
+    tm = getattr(cache, 'template', None)
+    return getattr(tm, 'solve_matrices_algorithm', 'PCA') if tm is not None else 'PCA'
+
+
 def _oiisum(template):
     if hasattr(template, 'OIItemplate'):
         return np.asarray(template.OIItemplate, dtype=np.float64).sum(axis=1)
     return None
 
 
-def scan_targets(cache, targets, precision=None, oiisum=None, batches=None, polish=None):
+def scan_targets(cache, targets, precision=None, oiisum=None, batches=None, polish=None,
+                 solve=None):
     """Scan every target over every redshift of `cache`.
 
     Targets are grouped by wavelength layout (one launch per layout).
-    polish: FP64 polish of a mixed-precision scan (ScanPlan).
+    polish: FP64 polish of a mixed-precision scan (ScanPlan); solve: the
+    template's method (default: cache.template's).
     Returns numpy (zchi2 (nt, nz), zcoeff (nt, nz, nb), penalty (nt, nz)).
     """
     precision = precision or default_precision()
@@ -471,7 +498,7 @@ def scan_targets(cache, targets, precision=None, oiisum=None, batches=None, poli
             batch = TargetBatch([targets[i] for i in idx])
         n = len(idx)
         plan = ScanPlan(cache, batch, np.arange(n), np.zeros(n), np.full(n, nz), precision,
-                        oiisum=oiisum, polish=polish)
+                        oiisum=oiisum, polish=polish, solve=solve or _method(cache))
         plan.run()
         c2, cf, pen, _ = plan.host()
         zchi2[idx] = c2.reshape(n, nz)
@@ -504,7 +531,7 @@ class ScanGraph(object):
         self.plans = []
         for cache, oii in caches:
             plan = ScanPlan(cache, batch, np.arange(nt), np.zeros(nt), np.full(nt, cache.nz),
-                            precision, oiisum=oii, polish=polish)
+                            precision, oiisum=oii, polish=polish, solve=_method(cache))
             if tune:
                 from . import autotune
                 autotune.choose(plan)
@@ -574,6 +601,8 @@ def calc_zchi2_batch(spectra, tdata, weights, flux, wflux, nz, nbasis, solve_mat
 
     tdata: {wavehash: (nz, nwave, nbasis)} numpy arrays or device tensors.
     polish: FP64 polish of a mixed-precision scan (ScanPlan).
+    solve_matrices_algorithm: 'PCA' (default) or 'NMF' / 'NNLS' (coefficients
+    >= 0, scipy.optimize.nnls(M, y) in the reference, zscan.py:1153-1172).
     Returns numpy (zchi2 (nz,), zcoeff (nz, nbasis)).
     """
     import torch
@@ -583,9 +612,10 @@ def calc_zchi2_batch(spectra, tdata, weights, flux, wflux, nz, nbasis, solve_mat
         zchi2[:] = HUGE_CHI2
         return zchi2, np.zeros((nz, nbasis))
     algo = 'PCA' if solve_matrices_algorithm is None else solve_matrices_algorithm.upper()
-    if algo in ('NMF', 'NNLS', 'BVLS'):
-        raise NotImplementedError(f'{algo} solves are outside the chi2-scan path')
-    if algo != 'PCA':
+    if algo == 'BVLS':
+        raise NotImplementedError(f'{algo} solves are outside the chi2-scan path '
+                                  f'(the archetype fit has its own bounded solve)')
+    if algo not in SOLVES:
         # solve_matrices raises NotImplementedError, caught by the caller (zscan.py:721-724)
         zchi2[:] = HUGE_CHI2
         return zchi2, np.zeros((nz, nbasis))
@@ -602,7 +632,7 @@ def calc_zchi2_batch(spectra, tdata, weights, flux, wflux, nz, nbasis, solve_mat
     wfa[0, :layout.npix] = _host(wflux)
     batch = TargetBatch(arrays=(layout, rband, wa, fa, wfa))
     cache = _ExplicitCache(tdata, layout, nz, nbasis)
-    plan = ScanPlan(cache, batch, [0], [0], [nz], precision, prior=prior, polish=polish)
+    plan = ScanPlan(cache, batch, [0], [0], [nz], precision, prior=prior, polish=polish, solve=algo)
     plan.run()
     c2, cf, _, _ = plan.host()
     return c2, cf

@@ -100,6 +100,27 @@ def test_polish_argument_errors():
     assert lib.rr_zscan_polish(ctypes.byref(d), ctypes.byref(p), None) == 0
 
 
+def test_solve_argument():
+    """Unknown solve codes are refused before any device call; the method names
+    map as the reference's solve_matrices does (zscan.py:1107-1205)."""
+    from redrock_amd import _hip, zscan
+    assert zscan.solve_code(None) == _hip.RR_SOLVE_PCA
+    assert zscan.solve_code('pca') == _hip.RR_SOLVE_PCA
+    assert zscan.solve_code('NMF') == zscan.solve_code('NNLS') == _hip.RR_SOLVE_NNLS
+    with pytest.raises(NotImplementedError):
+        zscan.solve_code('BVLS')
+    lib = _hip.lib()
+    d = _hip.ZscanDesc()
+    d.nitems, d.npix, d.npix_pad, d.nb, d.halo, d.cache_rows = 1, 10, 32, 3, 8, 64
+    d.max_item_nz, d.total_rows, d.cache_nz, d.cache_ldz = 5, 5, 5, 64
+    d.d_tcache = d.d_rband = d.d_w = d.d_wf = d.d_f = 8
+    d.d_item_target = d.d_item_z0 = d.d_item_nz = d.d_item_out = d.d_zchi2 = d.d_zcoeff = 8
+    d.d_work, d.work_bytes = 8, 1 << 20
+    d.solve = 7
+    assert lib.rr_zscan(ctypes.byref(d), None) == -1
+    assert b'unknown solve' in lib.rr_last_error()
+
+
 def test_polish_tolerance_argument():
     import math
     from redrock_amd import zscan

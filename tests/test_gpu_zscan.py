@@ -257,3 +257,54 @@ def test_ragged_items_every_precision(gpu, kind):
     empty = zscan.ScanPlan(dt.cache, batch, np.zeros(0, int), np.zeros(0, int), np.zeros(0, int), 'f16x3')
     empty.run()
     assert empty.zchi2.numel() == 0
+
+
+@pytest.fixture(scope='module')
+def nmf(gpu):
+    """An NMF template (non-negative basis: |galaxy basis|) fit to targets made
+    with the signed basis, so that coefficient bounds are often active."""
+    from redrock_amd import synth
+    from redrock_amd.templates import Template, DistTemplate
+    base = synth.galaxy_template()
+    tm = Template(spectype='GALAXY', redshifts=base.redshifts[600:664], wave=base.wave,
+                  flux=np.abs(base.flux), method='NMF', igm_model='None')
+    targets, _ = synth.make_targets(base, 4, seed=17, zrange=(0.37, 0.40))
+    dwave = {s.wavehash: s.wave for s in targets[0].spectra}
+    dt = DistTemplate(tm, dwave)
+    tdata = orc.template_cache(tm, dwave)
+    ref = orc.calc_zchi2(targets, tdata, tm.nbasis, algo='NMF')
+    return tm, targets, dt, ref
+
+
+def test_nnls_template_matches_oracle(nmf):
+    """NMF templates (RRMETHOD = NMF) are solved as scipy.optimize.nnls(M, y)
+    (zscan.py:1153-1172): FP64 chi2 within 1e-9 of the oracle, same best
+    redshift, coefficients >= 0 and equal to the oracle's."""
+    from redrock_amd import zscan
+    tm, targets, dt, (rchi2, rcoeff, _) = nmf
+    print('case: NMF template, 4 targets x 64 z, f64', flush=True)
+    chi2, coeff, _ = zscan.calc_zchi2([t.id for t in targets], targets, dt, precision='f64')
+    rel = np.abs(chi2 - rchi2) / np.abs(rchi2)
+    print(f'nnls: max rel chi2 err {rel.max():.2e}; active bounds {np.mean(rcoeff == 0):.2f}')
+    assert np.mean(rcoeff == 0) > 0.1            # the case exercises active bounds
+    assert rel.max() <= F64_RTOL
+    assert np.array_equal(chi2.argmin(axis=1), rchi2.argmin(axis=1))
+    assert np.all(coeff >= 0)
+    assert np.allclose(coeff, rcoeff, rtol=1e-6, atol=1e-8 * np.abs(rcoeff).max())
+
+
+def test_nnls_template_mixed_precision(nmf):
+    """The same NMF fit in split fp16 (no polish for NNLS): chi2 within
+    1e-5 of the target's f'Wf of the oracle, same best redshift."""
+    from redrock_amd import zscan
+    tm, targets, dt, (rchi2, rcoeff, _) = nmf
+    print('case: NMF template, f16x3', flush=True)
+    chi2, coeff, _ = zscan.calc_zchi2([t.id for t in targets], targets, dt, precision='f16x3')
+    fwf = np.array([np.sum(s_ivar * s_flux ** 2) for s_ivar, s_flux in
+                    ((np.concatenate([s.ivar for s in t.spectra]),
+                      np.concatenate([s.flux for s in t.spectra])) for t in targets)])
+    err = np.abs(chi2 - rchi2) / fwf[:, None]
+    print(f'nnls f16x3: max |dchi2|/fWf {err.max():.2e}')
+    assert err.max() <= 1e-5
+    assert np.array_equal(chi2.argmin(axis=1), rchi2.argmin(axis=1))
+    assert np.all(coeff >= 0)

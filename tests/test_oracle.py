@@ -301,3 +301,66 @@ def test_resolution_band_dia_fast_path():
     data[4, 10] = 1.0
     with pytest.raises(NotImplementedError):
         resolution_band(scipy.sparse.dia_matrix((data, offs), shape=(50, 55)))
+
+
+def _nnls_bruteforce(M, y):
+    """min |M x - y|^2 over x >= 0 by enumerating every free set: the feasible
+    least-squares solution whose gradient M^T (M x - y) is >= 0 on the fixed
+    (zero) coefficients -- the Kuhn-Tucker point, unique for full-rank M."""
+    import itertools
+    n = M.shape[1]
+    best = None
+    for k in range(n + 1):
+        for F in itertools.combinations(range(n), k):
+            x = np.zeros(n)
+            if F:
+                x[list(F)] = np.linalg.lstsq(M[:, F], y, rcond=None)[0]
+            if np.any(x[list(F)] <= 0):
+                continue
+            g = M.T @ (M @ x - y)
+            fixed = [j for j in range(n) if j not in F]
+            scale = np.abs(M).sum() * np.abs(y).max()
+            if np.all(g[fixed] >= -1e-10 * scale):
+                r = np.sum((M @ x - y) ** 2)
+                if best is None or r < best[0]:
+                    best = (r, x)
+    return best[1]
+
+
+@pytest.mark.parametrize('seed', range(8))
+def test_oracle_nnls_is_the_kkt_point(seed):
+    """scipy.optimize.nnls(M, y) -- the reference's NMF / NNLS solve
+    (zscan.py:1153-1172) -- equals the brute-force Kuhn-Tucker point of
+    min |M x - y|^2, x >= 0; with no active bound it is np.linalg.solve."""
+    rng = np.random.default_rng(seed)
+    nb = 2 + seed % 5
+    A = np.abs(rng.normal(size=(300, nb))) + 0.2 * rng.normal(size=(300, nb))
+    w = rng.uniform(0.5, 2.0, 300)
+    M = A.T @ (w[:, None] * A)
+    truth = rng.normal(size=nb) * (1.0 if seed % 2 else 0.3)
+    truth[0] = abs(truth[0])
+    y = M @ truth + rng.normal(size=nb)
+    got = orc.solve_matrices_nnls(M, y)
+    want = _nnls_bruteforce(M, y)
+    assert np.all(got >= 0)
+    assert np.allclose(got, want, rtol=1e-8, atol=1e-10 * max(np.abs(want).max(), 1.0))
+    free = np.linalg.solve(M, y)
+    if np.all(free > 0):
+        assert np.allclose(got, free, rtol=1e-8)
+
+
+def test_oracle_nnls_in_the_scan_clamps_coefficients():
+    """calc_zchi2_batch with algo 'NMF' gives non-negative coefficients and a
+    chi2 no smaller than the unconstrained (PCA) one at every redshift."""
+    from redrock_amd import synth
+    tm = synth.galaxy_template()
+    targets, _ = synth.make_targets(tm, 1, seed=3)
+    dwave = {s.wavehash: s.wave for s in targets[0].spectra}
+    zsel = tm.redshifts[400:412]
+    tdata = orc.template_cache(tm, dwave, myz=zsel)
+    w, f, wf = orc.spectral_data(targets[0].spectra)
+    c_pca, _ = orc.calc_zchi2_batch(targets[0].spectra, tdata, w, f, wf, len(zsel), tm.nbasis)
+    c_nn, cf = orc.calc_zchi2_batch(targets[0].spectra, tdata, w, f, wf, len(zsel), tm.nbasis,
+                                    algo='NMF')
+    assert np.all(cf >= 0)
+    assert np.all(c_nn >= c_pca * (1 - 1e-12))
