@@ -449,7 +449,9 @@ def fitz(zchi2, redshifts, target, template, nminima=3, zminfit_npoints=15, line
             if T is None:
                 continue
             binned[k] *= T[:, :, None]
-        zzchi2, zzcoeff = calc_zchi2_batch(spectra, binned, weights, flux, wflux, nz, nbasis)
+        algo = getattr(template, 'solve_matrices_algorithm', 'PCA')      # fitz.py:246-253
+        zzchi2, zzcoeff = calc_zchi2_batch(spectra, binned, weights, flux, wflux, nz, nbasis,
+                                           algo=algo)
         if oii is not None:
             zzchi2 += calc_negOII_penalty(oii, zzcoeff)
         i = min(max(np.argmin(zzchi2), 1), len(zz) - 2)
@@ -464,7 +466,8 @@ def fitz(zchi2, redshifts, target, template, nminima=3, zminfit_npoints=15, line
                 if T is None:
                     continue
                 binned[k] *= T[:, :, None]
-            chi2, coeff = calc_zchi2_batch(spectra, binned, weights, flux, wflux, 1, nbasis)
+            chi2, coeff = calc_zchi2_batch(spectra, binned, weights, flux, wflux, 1, nbasis,
+                                           algo=algo)                 # fitz.py:290-292
         except ValueError as err:
             if zmin < redshifts[0] or redshifts[-1] < zmin:
                 coeff = np.zeros(template.nbasis)

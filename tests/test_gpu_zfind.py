@@ -37,7 +37,8 @@ def _oracle_zfind(targets, dts, dwave, nminima, priors=None):
         lines = QSO_LINES if tm.template_type == 'QSO' else None
         tdata = orc.template_cache(tm, dwave, lines=lines)
         oii = getattr(tm, 'OIItemplate', None)
-        chi2, coeff, pen = orc.calc_zchi2(targets, tdata, tm.nbasis, oii)
+        chi2, coeff, pen = orc.calc_zchi2(targets, tdata, tm.nbasis, oii,
+                                          algo=tm.solve_matrices_algorithm)
         for i, t in enumerate(targets):
             c = chi2[i] + (priors.eval(t.id, tm.redshifts) if priors is not None else 0.)
             res[t.id][tm.full_type] = dict(zfit=orc.fitz(c + pen[i], tm.redshifts, t, tm,
@@ -81,6 +82,33 @@ def test_zfind_matches_oracle(gpu, with_prior):
         for dt in dts:
             r = allresults[t.id][dt.template.full_type]
             assert set(r.keys()) == {'redshifts', 'zchi2', 'penalty', 'zcoeff'}
+
+
+def test_zfind_nmf_template_matches_oracle(gpu):
+    """A PCA and an NMF galaxy template side by side (RRMETHOD = NMF: the scan
+    and the fine fits solve nnls(M, y), fitz.py:246-253,290-292)."""
+    from redrock_amd import synth, zfind as zf
+    from redrock_amd.templates import Template, DistTemplate
+    from redrock_amd.targets import DistTargetsCopy
+    g = synth.galaxy_template()
+    tg = Template(spectype='GALAXY', redshifts=g.redshifts[600:900], wave=g.wave, flux=g.flux,
+                  igm_model=g.igm_model)
+    tn = Template(spectype='GALAXY', subtype='NMF', redshifts=g.redshifts[600:900], wave=g.wave,
+                  flux=np.abs(g.flux), method='NMF', igm_model=g.igm_model)
+    targets, _ = synth.make_targets(g, 3, seed=31, zrange=(0.38, 0.58), snr=0.7)
+    dwave = {sp.wavehash: sp.wave for sp in targets[0].spectra}
+    dts = [DistTemplate(t, dwave) for t in (tg, tn)]
+    print('case: zfind PCA + NMF galaxy templates, 3 targets', flush=True)
+    allresults, got = zf.zfind(DistTargetsCopy(targets), dts, nminima=3, precision='f64')
+    want = _oracle_zfind(targets, dts, dwave, 3)
+    for k in ('targetid', 'spectype', 'subtype', 'znum', 'fitmethod', 'zwarn'):
+        assert np.array_equal(got[k].astype(str), want[k].astype(str)), k
+    assert np.allclose(got['z'], want['z'], rtol=1e-9, atol=1e-12)
+    assert np.allclose(got['chi2'], want['chi2'], rtol=1e-9, atol=0)
+    # the NMF template's scan coefficients are non-negative (the table keeps
+    # the 3 best minima per spectype, here all from the better-fitting PCA one)
+    for t in targets:
+        assert np.all(allresults[t.id][tn.full_type]['zcoeff'] >= 0)
 
 
 def test_zfind_mixed_precision_best_z(gpu):
