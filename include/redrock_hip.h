@@ -288,7 +288,9 @@ int rr_pack_mixed(const double *d_rband, const double *d_w, const double *d_f, i
  * of every polished item equals the FP64 scan's (ties within FP64 rounding
  * aside).
  *
- * scan->precision RR_PREC_F64: nothing to do, returns RR_OK.
+ * scan->precision RR_PREC_F64: nothing to do, returns RR_OK.  scan->solve
+ * must be RR_SOLVE_PCA (the re-evaluation solves M c = y): RR_EUNSUPPORTED
+ * otherwise.
  * Reads the scan's workspace statistics (stage RR_STAGE_STATS); the FP64
  * cache has the scan's rows, halo and pixel layout.  max_item_nz <= 16384.
  */

@@ -111,6 +111,8 @@ extern "C" int rr_zscan_polish(const rr_zscan_desc *d, const rr_polish_desc *p, 
     const int v = scan_args(d, a);
     if (v) return v < 0 ? v : RR_OK;
     if (d->precision == RR_PREC_F64) return RR_OK;
+    if (d->solve != RR_SOLVE_PCA)
+        return rr::fail(RR_EUNSUPPORTED, "rr_zscan_polish: re-solves M c = y; NNLS scans are not polished");
     if (!p) return rr::fail(RR_EINVAL, "rr_zscan_polish: null polish descriptor");
     if (!p->d_tcache64 || !p->d_windows || !p->d_nwindows || !p->d_queue)
         return rr::fail(RR_EINVAL, "rr_zscan_polish: null required pointer");

@@ -75,6 +75,10 @@ def test_polish_argument_errors():
     d.d_work, d.work_bytes = 8, 1 << 20
     assert lib.rr_zscan_polish(ctypes.byref(d), None, None) == -1
     assert b'null polish descriptor' in lib.rr_last_error()
+    d.solve = _hip.RR_SOLVE_NNLS                   # NNLS scans are not polished
+    assert lib.rr_zscan_polish(ctypes.byref(d), None, None) == -3
+    assert b'NNLS' in lib.rr_last_error()
+    d.solve = _hip.RR_SOLVE_PCA
     p = _hip.PolishDesc()
     p.d_tcache64 = p.d_windows = p.d_nwindows = 8
     assert lib.rr_zscan_polish(ctypes.byref(d), ctypes.byref(p), None) == -1
