@@ -20,7 +20,7 @@
 extern "C" {
 #endif
 
-#define RR_ABI_VERSION 7
+#define RR_ABI_VERSION 8
 
 /* Error codes. */
 #define RR_OK 0
@@ -192,6 +192,11 @@ typedef struct {
      *                  iterations -> HUGE_CHI2, status RR_STATUS_SINGULAR
      *                  (the reference's LinAlgError, zscan.py:769-771)         */
     int solve;
+    /* RR_SOLVE_NNLS: bit j set = coefficient j >= 0, the others free.  Every
+     * bit for NMF / NNLS templates; the archetype bounds of the reference's
+     * BVLS calls (zscan.py:289-294: archetype coefficients >= 0, Legendre
+     * terms free) otherwise.  Ignored by RR_SOLVE_PCA.                       */
+    int nonneg_mask;
 } rr_zscan_desc;
 #define RR_SOLVE_PCA 0
 #define RR_SOLVE_NNLS 1

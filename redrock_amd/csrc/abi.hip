@@ -82,6 +82,7 @@ static int scan_args(const rr_zscan_desc *d, rr::ZArgs &a) {
     a.max_item_nz = d->max_item_nz;
     a.work = d->d_work; a.total_rows = d->total_rows;
     a.solve = d->solve;
+    a.nonneg_mask = d->nonneg_mask;
     return RR_OK;
 }
 

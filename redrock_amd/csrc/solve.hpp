@@ -97,8 +97,9 @@ __device__ __forceinline__ bool chol_solve(double (&a)[NB][NB], double (&c)[NB])
 }
 
 // min |M[:, F] x - y|^2 (x_j = 0 outside the free set F) by Householder QR of
-// the free columns taken in index order.  Columns with zero weighted norm are
-// excluded from F by the caller.
+// the free columns taken in index order.  A column with nothing left after
+// the reflections of the earlier ones (zero, or exactly dependent on them)
+// gets x_j = 0, the basic solution.
 template <int NB>
 __device__ inline void ls_free(const double (&M)[NB][NB], const double (&y)[NB], unsigned F, double (&x)[NB]) {
     double A[NB][NB], b[NB];
@@ -119,6 +120,7 @@ __device__ inline void ls_free(const double (&M)[NB][NB], const double (&y)[NB],
 #pragma unroll
         for (int i = 0; i < NB; ++i)
             if (i >= r) nrm2 = fma(A[i][j], A[i][j], nrm2);
+        if (!(nrm2 > 0.0)) continue;
         double arr = 0.0;
 #pragma unroll
         for (int i = 0; i < NB; ++i)
@@ -174,15 +176,20 @@ __device__ inline void ls_free(const double (&M)[NB][NB], const double (&y)[NB],
 }
 
 
-// Non-negative least squares min |M x - y|^2 over x >= 0 by the active-set
-// method of Lawson and Hanson (Solving Least Squares Problems, 1974, ch. 23),
-// the problem scipy.optimize.nnls(M, y) solves for the reference's NMF
-// templates (zscan.py:1153-1172).  Each passive-set subproblem is ls_free
-// (Householder QR of the passive columns).  The solution is unique when M has
-// full rank.  Returns false when 3 NB inner iterations did not settle (scipy
-// raises there; the reference then gives HUGE_CHI2, zscan.py:769-771).
+// Non-negative least squares min |M x - y|^2 over x_j >= 0 for the
+// coefficients j in `bounded` (the others free) by the active-set method of
+// Lawson and Hanson (Solving Least Squares Problems, 1974, ch. 23, with the
+// free coefficients kept in the passive set throughout).  All coefficients
+// bounded: the problem scipy.optimize.nnls(M, y) solves for the reference's
+// NMF templates (zscan.py:1153-1172); some bounded below by 0 and the rest
+// free: scipy lsq_linear(M, y, bounds, 'bvls') with the archetype bounds of
+// zscan.py:289-294.  Each passive-set subproblem is ls_free (Householder QR
+// of the passive columns).  The solution is unique when M has full rank.
+// Returns false when 3 NB inner iterations did not settle (scipy raises
+// there; the reference then gives HUGE_CHI2, zscan.py:769-771).
 template <int NB>
-__device__ inline bool nnls_solve(const double (&M)[NB][NB], const double (&y)[NB], double (&x)[NB]) {
+__device__ inline bool nnls_solve(const double (&M)[NB][NB], const double (&y)[NB], double (&x)[NB],
+                                  unsigned bounded) {
     double scale = 0.0, ymax = 0.0;          // |M|_1 |y|_inf: the scale of the gradient
 #pragma unroll
     for (int j = 0; j < NB; ++j) {
@@ -211,10 +218,16 @@ __device__ inline bool nnls_solve(const double (&M)[NB][NB], const double (&y)[N
             w[j] = v;
         }
     };
+    constexpr unsigned all = NB >= 32 ? 0xffffffffu : ((1u << NB) - 1u);
+    bounded &= all;
+    unsigned P = all & ~bounded;             // passive set: the free coefficients always
+    if (P)
+        ls_free<NB>(M, y, P, x);
+    else {
 #pragma unroll
-    for (int j = 0; j < NB; ++j) x[j] = 0.0;
+        for (int j = 0; j < NB; ++j) x[j] = 0.0;
+    }
     gradient();
-    unsigned P = 0u;                         // passive (free) set
     unsigned rejected = 0u;                  // indices that cannot enter at this x
     int iter = 0;
     for (int outer = 0; outer < 4 * NB; ++outer) {
@@ -222,7 +235,7 @@ __device__ inline bool nnls_solve(const double (&M)[NB][NB], const double (&y)[N
         double wt = tol;
 #pragma unroll
         for (int j = 0; j < NB; ++j)
-            if (!(((P | rejected) >> j) & 1u) && w[j] > wt) {
+            if (((bounded & ~(P | rejected)) >> j & 1u) && w[j] > wt) {
                 wt = w[j];
                 t = j;
             }
@@ -244,7 +257,7 @@ __device__ inline bool nnls_solve(const double (&M)[NB][NB], const double (&y)[N
             bool feasible = true;
 #pragma unroll
             for (int j = 0; j < NB; ++j)
-                if (((P >> j) & 1u) && !(s[j] > 0.0)) feasible = false;
+                if ((((P & bounded) >> j) & 1u) && !(s[j] > 0.0)) feasible = false;
             if (feasible) {
 #pragma unroll
                 for (int j = 0; j < NB; ++j) x[j] = s[j];
@@ -256,7 +269,7 @@ __device__ inline bool nnls_solve(const double (&M)[NB][NB], const double (&y)[N
             int jmin = -1;
 #pragma unroll
             for (int j = 0; j < NB; ++j)
-                if (((P >> j) & 1u) && !(s[j] > 0.0)) {
+                if ((((P & bounded) >> j) & 1u) && !(s[j] > 0.0)) {
                     const double d = x[j] - s[j];
                     const double a = d > 0.0 ? x[j] / d : 0.0;
                     if (a < alpha) {
@@ -267,7 +280,8 @@ __device__ inline bool nnls_solve(const double (&M)[NB][NB], const double (&y)[N
 #pragma unroll
             for (int j = 0; j < NB; ++j) {
                 x[j] = fma(alpha, s[j] - x[j], x[j]);
-                if (j == jmin || (((P >> j) & 1u) && !(x[j] > 0.0))) {
+                if (((bounded >> j) & 1u) &&
+                    (j == jmin || (((P >> j) & 1u) && !(x[j] > 0.0)))) {
                     P &= ~(1u << j);
                     x[j] = 0.0;
                 }

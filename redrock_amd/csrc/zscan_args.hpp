@@ -25,6 +25,7 @@ struct ZArgs {
     void *work;            // workspace: item stats then normal equations
     long long total_rows;  // rows of the output arrays (>= max item_out + nz)
     int solve;             // RR_SOLVE_PCA or RR_SOLVE_NNLS
+    int nonneg_mask;       // RR_SOLVE_NNLS: bit j set = coefficient j >= 0
 };
 
 // Kernel-argument form of rr_polish_desc.

@@ -11,7 +11,9 @@
 //   mixed    : Cholesky (M is a Gram matrix), LU when Cholesky meets a
 //              non-positive pivot.
 //   NNLS     : (the NMF / NNLS templates, either precision) min |Mc - y|^2
-//              over c >= 0 (scipy.optimize.nnls(M, y), zscan.py:1153-1172).
+//              over c >= 0 (scipy.optimize.nnls(M, y), zscan.py:1153-1172),
+//              or over c_j >= 0 for the coefficients of nonneg_mask only
+//              (the reference's BVLS archetype bounds, zscan.py:289-294).
 // chi2 = f'Wf - 2 c'y + c'Mc with the unregularised M (compensated sum): the
 // reference's explicit residual sum(w (f - A c)^2) (:777-780) expanded, which
 // stays exact with a prior and is stationary in c.
@@ -105,7 +107,7 @@ __global__ __launch_bounds__(64) void finish_kernel(rr::ZArgs a, long long ne_of
     for (int i = 0; i < NB; ++i) c[i] = y[i];
     bool ok;
     if constexpr (NNLS) {
-        ok = rr::nnls_solve<NB>(M, y, c);
+        ok = rr::nnls_solve<NB>(M, y, c, (unsigned)a.nonneg_mask);
     } else if constexpr (MIXED) {
         double L[NB][NB];
 #pragma unroll

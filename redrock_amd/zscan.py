@@ -63,16 +63,37 @@ POLISH_WINDOWS = 16
 STAGE_POLISH = 8      # ScanPlan.run stage bit (rr_zscan_polish), beside RR_STAGE_*
 
 
-SOLVES = {'PCA': _hip.RR_SOLVE_PCA, 'NMF': _hip.RR_SOLVE_NNLS, 'NNLS': _hip.RR_SOLVE_NNLS}
+SOLVES = ('PCA', 'NMF', 'NNLS', 'BVLS')
+
+
+def solve_spec(method, nb, bounds=None):
+    """(RR_SOLVE_*, non-negativity mask) of a solve_matrices algorithm
+    (zscan.py:1107-1205): PCA -> M c = y; NMF / NNLS -> nnls(M, y), every
+    coefficient >= 0; BVLS -> lsq_linear(M, y, bounds, 'bvls') for bounds
+    (2, nb) whose lower bounds are 0 or -inf and upper bounds +inf (the
+    archetype bounds of zscan.py:289-294; None: no bound at all)."""
+    key = 'PCA' if method is None else str(method).upper()
+    if key == 'PCA':
+        return _hip.RR_SOLVE_PCA, 0
+    if key in ('NMF', 'NNLS'):
+        return _hip.RR_SOLVE_NNLS, (1 << nb) - 1
+    if key == 'BVLS':
+        if bounds is None:
+            return _hip.RR_SOLVE_NNLS, 0
+        b = np.asarray(bounds, dtype=np.float64)
+        if b.shape != (2, nb):
+            raise ValueError(f'BVLS bounds must have shape (2, {nb}), got {b.shape}')
+        lo, hi = b
+        if not (np.all(np.isposinf(hi)) and np.all((lo == 0.0) | np.isneginf(lo))):
+            raise NotImplementedError('BVLS bounds other than c >= 0 or free per coefficient '
+                                      '(the archetype bounds) are not available in the scan')
+        return _hip.RR_SOLVE_NNLS, int(sum(1 << j for j in range(nb) if lo[j] == 0.0))
+    raise NotImplementedError(f'solve_matrices algorithm {method!r} is not available in the scan')
 
 
 def solve_code(method):
-    """RR_SOLVE_* of a template method / solve_matrices_algorithm name
-    (zscan.py:1107-1205): PCA -> M c = y, NMF or NNLS -> nnls(M, y)."""
-    key = 'PCA' if method is None else str(method).upper()
-    if key not in SOLVES:
-        raise NotImplementedError(f'solve_matrices algorithm {method!r} is not available in the scan')
-    return SOLVES[key]
+    """RR_SOLVE_* of a template method (PCA or NMF / NNLS)."""
+    return solve_spec(method, 1)[0]
 
 
 def polish_tol(precision, polish):
@@ -324,11 +345,12 @@ class ScanPlan(object):
 
     def __init__(self, cache, batch, item_target, item_z0, item_nz, precision='f64',
                  prior=None, oiisum=None, tile=0, polish=None, polish_windows=POLISH_WINDOWS,
-                 solve='PCA'):
+                 solve='PCA', bounds=None):
         """polish: FP64 re-evaluation of the rows near each item's minimum after a
         mixed-precision scan (rr_zscan_polish), see polish_tol(); it needs the
         FP64 template cache as well.  solve: the template's method, 'PCA' or
-        'NMF' / 'NNLS' (non-negative coefficients, no polish)."""
+        'NMF' / 'NNLS' (non-negative coefficients, no polish), or 'BVLS' with
+        `bounds` (solve_spec)."""
         import torch
         if precision not in _hip.PRECISIONS:
             raise ValueError(f'unknown precision {precision!r}')
@@ -402,7 +424,7 @@ class ScanPlan(object):
         d.d_status = self.status.data_ptr()
         d.max_item_nz = int(item_nz.max()) if self.nitems else 0
         d.total_rows = rows
-        d.solve = solve_code(solve)
+        d.solve, d.nonneg_mask = solve_spec(solve, nb, bounds)
         nbytes = _hip.lib().rr_zscan_workspace_bytes(ctypes.byref(d))
         if nbytes < 0:
             _hip.check(int(nbytes), 'rr_zscan_workspace_bytes')
@@ -601,8 +623,9 @@ def calc_zchi2_batch(spectra, tdata, weights, flux, wflux, nz, nbasis, solve_mat
 
     tdata: {wavehash: (nz, nwave, nbasis)} numpy arrays or device tensors.
     polish: FP64 polish of a mixed-precision scan (ScanPlan).
-    solve_matrices_algorithm: 'PCA' (default) or 'NMF' / 'NNLS' (coefficients
-    >= 0, scipy.optimize.nnls(M, y) in the reference, zscan.py:1153-1172).
+    solve_matrices_algorithm: 'PCA' (default), 'NMF' / 'NNLS' (coefficients
+    >= 0, scipy.optimize.nnls(M, y) in the reference, zscan.py:1153-1172) or
+    'BVLS' with solver_args['bounds'] of the archetype form (solve_spec).
     Returns numpy (zchi2 (nz,), zcoeff (nz, nbasis)).
     """
     import torch
@@ -612,9 +635,9 @@ def calc_zchi2_batch(spectra, tdata, weights, flux, wflux, nz, nbasis, solve_mat
         zchi2[:] = HUGE_CHI2
         return zchi2, np.zeros((nz, nbasis))
     algo = 'PCA' if solve_matrices_algorithm is None else solve_matrices_algorithm.upper()
-    if algo == 'BVLS':
-        raise NotImplementedError(f'{algo} solves are outside the chi2-scan path '
-                                  f'(the archetype fit has its own bounded solve)')
+    bounds = None
+    if algo == 'BVLS' and solver_args is not None and 'bounds' in solver_args:
+        bounds = solver_args['bounds']                          # zscan.py:1175-1181
     if algo not in SOLVES:
         # solve_matrices raises NotImplementedError, caught by the caller (zscan.py:721-724)
         zchi2[:] = HUGE_CHI2
@@ -632,7 +655,8 @@ def calc_zchi2_batch(spectra, tdata, weights, flux, wflux, nz, nbasis, solve_mat
     wfa[0, :layout.npix] = _host(wflux)
     batch = TargetBatch(arrays=(layout, rband, wa, fa, wfa))
     cache = _ExplicitCache(tdata, layout, nz, nbasis)
-    plan = ScanPlan(cache, batch, [0], [0], [nz], precision, prior=prior, polish=polish, solve=algo)
+    plan = ScanPlan(cache, batch, [0], [0], [nz], precision, prior=prior, polish=polish, solve=algo,
+                    bounds=bounds)
     plan.run()
     c2, cf, _, _ = plan.host()
     return c2, cf

@@ -8,6 +8,7 @@ import os
 import re
 import subprocess
 
+import numpy as np
 import pytest
 
 from tests.conftest import ROOT
@@ -111,8 +112,20 @@ def test_solve_argument():
     assert zscan.solve_code(None) == _hip.RR_SOLVE_PCA
     assert zscan.solve_code('pca') == _hip.RR_SOLVE_PCA
     assert zscan.solve_code('NMF') == zscan.solve_code('NNLS') == _hip.RR_SOLVE_NNLS
+    assert zscan.solve_spec('NMF', 4) == (_hip.RR_SOLVE_NNLS, 0b1111)
+    # BVLS with the archetype bounds (zscan.py:289-294): archetype >= 0,
+    # Legendre terms free; no bounds at all = every coefficient free
+    inf = np.inf
+    assert zscan.solve_spec('BVLS', 4, [[0, 0, -inf, -inf], [inf] * 4]) == (_hip.RR_SOLVE_NNLS, 0b0011)
+    assert zscan.solve_spec('bvls', 3) == (_hip.RR_SOLVE_NNLS, 0)
+    with pytest.raises(NotImplementedError):          # a finite upper bound
+        zscan.solve_spec('BVLS', 2, [[0, 0], [1, inf]])
+    with pytest.raises(NotImplementedError):          # a non-zero lower bound
+        zscan.solve_spec('BVLS', 2, [[-1, 0], [inf, inf]])
+    with pytest.raises(ValueError):
+        zscan.solve_spec('BVLS', 3, [[0, 0], [inf, inf]])
     with pytest.raises(NotImplementedError):
-        zscan.solve_code('BVLS')
+        zscan.solve_code('LSQ')
     lib = _hip.lib()
     d = _hip.ZscanDesc()
     d.nitems, d.npix, d.npix_pad, d.nb, d.halo, d.cache_rows = 1, 10, 32, 3, 8, 64

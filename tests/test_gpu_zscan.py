@@ -127,6 +127,49 @@ def test_calc_zchi2_batch_with_prior(gpu):
     assert np.allclose(coeff, rcoeff, rtol=1e-7, atol=1e-9 * np.abs(rcoeff).max())
 
 
+def test_calc_zchi2_batch_bvls_archetype_bounds(gpu):
+    """solve_matrices_algorithm 'BVLS' with the archetype bounds of
+    zscan.py:289-294 (the first coefficients >= 0, the rest free):
+    scipy lsq_linear(M, y, bounds, 'bvls') in the reference (zscan.py:1175-1201).
+    FP64 within 1e-9 of the oracle with bounds active; split fp16 within 1e-5
+    of f'Wf; bounds of another form are refused."""
+    from redrock_amd import synth, zscan
+    tm = synth.galaxy_template()
+    tg, _ = synth.make_targets(tm, 1, seed=23)
+    spectra = tg[0].spectra
+    dwave = {s.wavehash: s.wave for s in spectra}
+    zsel = tm.redshifts[500:580]
+    tdata = orc.template_cache(tm, dwave, myz=zsel)
+    w, f, wf = orc.spectral_data(spectra)
+    nbound = 4
+    bounds = np.zeros((2, tm.nbasis))
+    bounds[0][nbound:] = -np.inf
+    bounds[1] = np.inf
+    rchi2, rcoeff = orc.calc_zchi2_batch(spectra, tdata, w, f, wf, len(zsel), tm.nbasis, bounds=bounds)
+    active = np.mean(rcoeff[:, :nbound] == 0)
+    print(f'case: BVLS, {nbound} bounded of {tm.nbasis}, {len(zsel)} z; active bounds {active:.2f}', flush=True)
+    assert active > 0.05
+    args = {'bounds': bounds}
+    chi2, coeff = zscan.calc_zchi2_batch(spectra, tdata, w, f, wf, len(zsel), tm.nbasis, 'BVLS',
+                                         solver_args=args)
+    assert np.max(np.abs(chi2 - rchi2) / rchi2) <= F64_RTOL
+    assert np.all(coeff[:, :nbound] >= 0)
+    assert np.allclose(coeff, rcoeff, rtol=1e-6, atol=1e-8 * np.abs(rcoeff).max())
+    chi2, coeff = zscan.calc_zchi2_batch(spectra, tdata, w, f, wf, len(zsel), tm.nbasis, 'BVLS',
+                                         solver_args=args, precision='f16x3')
+    assert np.max(np.abs(chi2 - rchi2)) / np.sum(wf * f) <= 1e-5
+    assert chi2.argmin() == rchi2.argmin()
+    # no bounds: the unconstrained solve, as the PCA scan
+    chi2, _ = zscan.calc_zchi2_batch(spectra, tdata, w, f, wf, len(zsel), tm.nbasis, 'BVLS')
+    pchi2, _ = zscan.calc_zchi2_batch(spectra, tdata, w, f, wf, len(zsel), tm.nbasis, 'PCA')
+    assert np.max(np.abs(chi2 - pchi2) / pchi2) <= F64_RTOL
+    bad = bounds.copy()
+    bad[1][0] = 1.0
+    with pytest.raises(NotImplementedError):
+        zscan.calc_zchi2_batch(spectra, tdata, w, f, wf, len(zsel), tm.nbasis, 'BVLS',
+                               solver_args={'bounds': bad})
+
+
 def test_huge_chi2_conventions(gpu):
     """Zero weights -> HUGE everywhere; a vanishing basis -> singular -> HUGE (zscan.py:657-660,769-771)."""
     import torch
