@@ -27,9 +27,11 @@ find_minima                reference docstring vector (fitz.py:52-53); the
                            rrdetails-test.h5, tests/test_results.py): the
                            candidate choice replayed on its stored scans
                            gives every stored fine grid
-minfit                     exact parabola recovery (textbook); the stored
-                           fits of the details file (z 1e-11, zerr 3e-8 =
-                           polyfit conditioning, chi2 1e-10, zwarn bits)
+minfit                     exact parabola recovery (textbook); exact rational
+                           arithmetic on fine-grid triples (vertex 4e-16);
+                           the stored fits of the details file (z 1e-11,
+                           zerr 3e-8 = polyfit conditioning, chi2 1e-10,
+                           zwarn bits)
 fitz                       restatement only for the control flow; its pieces
                            are pinned above, and candidate selection and the
                            BAD_MINFIT / Z_FITLIMIT rules by the details file.
@@ -395,7 +397,28 @@ def find_minima(x):
 
 
 def minfit(x, y):
-    """fitz.py:70-111."""
+    """fitz.py:70-111: the parabola y0 + ((x - x0)/xerr)^2 through the points.
+
+    The reference gets its coefficients from np.polyfit(x, y, 2) on the raw
+    redshifts, whose Vandermonde matrix has condition ~ (x/dx)^2 ~ 1e8 on a
+    fine grid: its vertex value carries rounding errors up to ~1e-9 relative
+    (measured against exact rational arithmetic).  For 3 points with
+    increasing x, finite y and non-zero curvature -- every call fitz makes --
+    the same parabola (the interpolant: a degree-2 least-squares fit to 3
+    points is exact) is evaluated here by Newton divided differences about
+    the middle point, exact to rounding.  Other inputs: np.polyfit as the
+    reference does."""
+    x = np.asarray(x, dtype=np.float64)
+    y = np.asarray(y, dtype=np.float64)
+    if len(x) == 3 and x[0] < x[1] < x[2] and np.all(np.isfinite(y)):
+        h0, h1 = x[1] - x[0], x[2] - x[1]
+        s0, s1 = (y[1] - y[0]) / h0, (y[2] - y[1]) / h1      # first divided differences
+        a = (s1 - s0) / (x[2] - x[0])                          # second divided difference
+        if a != 0.0 and np.isfinite(a):
+            slope = s0 + a * h0                                # p'(x[1])
+            x0 = x[1] - slope / (2 * a)
+            y0 = y[1] - slope ** 2 / (4 * a)
+            return _minfit_flags(x, x0, y0, a)
     if len(x) < 3:
         return (-1, -1, -1, ZW_BAD_MINFIT)
     try:
@@ -406,6 +429,11 @@ def minfit(x, y):
         return (-1, -1, -1, ZW_BAD_MINFIT)
     x0 = -b / (2 * a)
     y0 = -(b ** 2) / (4 * a) + c
+    return _minfit_flags(x, x0, y0, a)
+
+
+def _minfit_flags(x, x0, y0, a):
+    """The BAD_MINFIT tests and xerr of fitz.py:94-111."""
     zwarn = 0
     if (x0 <= np.min(x)) or (np.max(x) <= x0):
         zwarn |= ZW_BAD_MINFIT

@@ -37,8 +37,57 @@ def find_minima(x):
     return ii[jj]
 
 
+def minfit_batch(x, y):
+    """minfit of many 3-point sets at once: x, y (K, 3), x increasing in each
+    row.  Returns arrays (x0, xerr, y0, zwarn) of length K.
+
+    A degree-2 fit to 3 distinct points is the interpolating parabola, here
+    in closed form about the middle point, p(u) = y1 + b u + a u^2 with
+    u = x - x1: vertex x1 - b/(2a), value y1 - b^2/(4a), xerr = 1/sqrt(|a|),
+    and the reference's BAD_MINFIT tests (fitz.py:94-111).  np.polyfit on the
+    raw Vandermonde matrix (the reference) has condition ~ (x/dx)^2 ~ 1e8 for
+    a fine redshift grid, so its vertex carries rounding errors up to ~1e-11
+    relative, which this form does not.  Rows that are not 3 increasing
+    points with finite y, or whose curvature is exactly 0, go through the
+    reference's np.polyfit path (minfit) unchanged."""
+    x = np.asarray(x, dtype=np.float64)
+    y = np.asarray(y, dtype=np.float64)
+    K = x.shape[0]
+    x0 = np.full(K, -1.0)
+    xerr = np.full(K, -1.0)
+    y0 = np.full(K, -1.0)
+    zwarn = np.zeros(K, dtype=np.int64)
+    if K == 0:
+        return x0, xerr, y0, zwarn
+    u0 = x[:, 0] - x[:, 1]
+    u2 = x[:, 2] - x[:, 1]
+    with np.errstate(all='ignore'):
+        d1 = (y[:, 0] - y[:, 1]) / u0
+        d2 = (y[:, 2] - y[:, 1]) / u2
+        a = (d2 - d1) / (u2 - u0)
+        b = d1 - a * u0
+        ok = (u0 < 0) & (u2 > 0) & np.all(np.isfinite(y), axis=1) & np.isfinite(a) & (a != 0.0)
+        x0[ok] = x[ok, 1] - b[ok] / (2 * a[ok])
+        y0[ok] = y[ok, 1] - b[ok] ** 2 / (4 * a[ok])
+        xerr[ok] = 1 / np.sqrt(np.abs(a[ok]))
+    bad = ok & ((x0 <= x[:, 0]) | (x[:, 2] <= x0) | (y0 <= 0.) | (a < 0.0))
+    zwarn[bad] |= ZW.BAD_MINFIT
+    for k in np.where(~ok)[0]:
+        x0[k], xerr[k], y0[k], zwarn[k] = _minfit_polyfit(x[k], y[k])
+    return x0, xerr, y0, zwarn
+
+
 def minfit(x, y):
-    """Fit y = y0 + ((x-x0)/xerr)**2 on 3 points (fitz.py:70-111)."""
+    """Fit y = y0 + ((x-x0)/xerr)**2 on 3 points (fitz.py:70-111): the closed
+    form of minfit_batch for 3 points, np.polyfit as in the reference else."""
+    if len(x) == 3:
+        r = minfit_batch(np.asarray(x)[None, :], np.asarray(y)[None, :])
+        return (float(r[0][0]), float(r[1][0]), float(r[2][0]), int(r[3][0]))
+    return _minfit_polyfit(x, y)
+
+
+def _minfit_polyfit(x, y):
+    """fitz.py:70-111 as written: least squares by np.polyfit."""
     if len(x) < 3:
         return (-1, -1, -1, ZW.BAD_MINFIT)
     try:
@@ -164,6 +213,15 @@ class _Refiner(object):
             pen[sel] = pn.reshape(len(sel), per_item)
         return chi2, coeff, pen
 
+    def _edges_ok_many(self, zlo, zhi):
+        """_edges_ok for arrays of (zlo, zhi)."""
+        xmin = self.template.minwave * (1 + np.asarray(zhi))
+        xmax = self.template.maxwave * (1 + np.asarray(zlo))
+        ok = np.ones(xmin.shape, dtype=bool)
+        for e in self.edges:
+            ok &= ~((e[0] < xmin) | (e[-1] > xmax))
+        return ok
+
     def refine(self, pairs, zchi2):
         """Refine candidates pairs = [(target, imin)]: returns per-pair dicts.
 
@@ -174,30 +232,30 @@ class _Refiner(object):
         nz = self.nz_fine
         zs = self.redshifts
         out = [None] * len(pairs)
-        good, zz_all = [], []
-        for j, (t, imin) in enumerate(pairs):
-            ilo = max(0, imin - 1)
-            ihi = min(imin + 1, len(zs) - 1)
-            zz = np.linspace(zs[ilo], zs[ihi], nz)
-            if not _edges_ok(self.template, zz.min(), zz.max(), self.edges):
-                out[j] = dict(imin=imin, error=ValueError('edges must be within input x range'))
-                continue
-            good.append(j)
-            zz_all.append(zz)
-        zzchi2, _, zzpen = self._scan(np.concatenate(zz_all) if zz_all else np.zeros(0),
-                                      [pairs[j][0] for j in good], nz)
+        imins = np.array([imin for _, imin in pairs], dtype=np.int64).reshape(-1)
+        lo = zs[np.maximum(imins - 1, 0)]
+        hi = zs[np.minimum(imins + 1, len(zs) - 1)]
+        # row by row what np.linspace(zs[ilo], zs[ihi], nz) gives (fitz.py:226-231)
+        zz_rows = np.linspace(lo, hi, nz, axis=1)
+        ok = self._edges_ok_many(np.minimum(lo, hi), np.maximum(lo, hi))
+        for j in np.where(~ok)[0]:
+            out[j] = dict(imin=pairs[j][1], error=ValueError('edges must be within input x range'))
+        good = np.where(ok)[0]
+        zzchi2, _, zzpen = self._scan(zz_rows[good].reshape(-1), [pairs[j][0] for j in good], nz)
+        c2 = zzchi2[:len(good)].reshape(len(good), nz).copy()
+        if self.oii is not None:
+            c2 += zzpen[:len(good)].reshape(len(good), nz)
+        # the 3 points about each fine-grid minimum (fitz.py:233-250)
+        i = np.clip(np.argmin(c2, axis=1), 1, nz - 2) if len(good) else np.zeros(0, np.int64)
+        r3 = np.arange(len(good))[:, None]
+        c3 = i[:, None] + np.arange(-1, 2)[None, :]
+        zmin, sigma, chi2min, zwarn = minfit_batch(zz_rows[good][r3, c3], c2[r3, c3])
         for k, j in enumerate(good):
-            zz = zz_all[k]
-            c2 = zzchi2[k].copy()
-            if self.oii is not None:
-                c2 += zzpen[k]
-            i = min(max(np.argmin(c2), 1), len(zz) - 2)
-            zmin, sigma, chi2min, zwarn = minfit(zz[i - 1:i + 2], c2[i - 1:i + 2])
-            out[j] = dict(imin=pairs[j][1], zz=zz, zzchi2=c2, zmin=zmin, sigma=sigma,
-                          chi2min=chi2min, zwarn=zwarn)
+            out[j] = dict(imin=pairs[j][1], zz=zz_rows[j], zzchi2=c2[k], zmin=float(zmin[k]),
+                          sigma=float(sigma[k]), chi2min=float(chi2min[k]), zwarn=int(zwarn[k]))
         # coefficients at each parabola minimum (fitz.py:265-304)
-        sel = [j for j in good if _edges_ok(self.template, out[j]['zmin'], out[j]['zmin'],
-                                            self.edges)]
+        zv = np.array([out[j]['zmin'] for j in good], dtype=np.float64)
+        sel = [j for j, o in zip(good, self._edges_ok_many(zv, zv)) if o]
         _, cf, _ = self._scan(np.array([out[j]['zmin'] for j in sel], dtype=np.float64),
                               [pairs[j][0] for j in sel], 1)
         for k, j in enumerate(sel):

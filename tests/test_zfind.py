@@ -208,3 +208,66 @@ def test_priors_from_fits(tmp_path):
     z = np.array([0.4, 0.5, 0.6])
     assert np.allclose(p.eval(10, z), [2 * np.log(2), 0.0, 2 * np.log(2)])
     assert p.eval(12, z) == 0.
+
+
+def _vertex_exact(x, y):
+    """Vertex, curvature and value of the parabola through 3 points in exact
+    rational arithmetic (divided differences)."""
+    from fractions import Fraction as Fr
+    X = [Fr(float(v)) for v in x]
+    Y = [Fr(float(v)) for v in y]
+    d1 = (Y[1] - Y[0]) / (X[1] - X[0])
+    d2 = (Y[2] - Y[1]) / (X[2] - X[1])
+    a = (d2 - d1) / (X[2] - X[0])
+    b = d1 - a * (X[0] + X[1])
+    c = Y[0] - a * X[0] ** 2 - b * X[0]
+    return float(-b / (2 * a)), float(a), float(c - b * b / (4 * a))
+
+
+def test_minfit_batch_is_the_exact_parabola():
+    """minfit on 3 points (fitz.py:70-111) is the interpolating parabola: the
+    batched closed form against exact rational arithmetic on fine-grid-like
+    triples (dz ~ 1e-4, chi2 ~ 1e4), and the reference's BAD_MINFIT bits."""
+    from redrock_amd.fitz import minfit_batch, minfit
+    rng = np.random.default_rng(3)
+    K = 300
+    zc = rng.uniform(0.05, 3.5, K)
+    h = rng.choice([2e-5, 1e-4, 4e-4], K) * (1 + zc)
+    x = zc[:, None] + h[:, None] * np.array([-1.0, 0.0, 1.0])
+    curv = rng.uniform(1e-3, 1e2, K) / h ** 2
+    xt = zc + rng.uniform(-1.5, 1.5, K) * h          # some vertices outside the points
+    y = rng.uniform(5e3, 5e4, K)[:, None] + curv[:, None] * (x - xt[:, None]) ** 2
+    y[::7] *= -1                                     # negative curvature and value
+    x0, xerr, y0, zw = minfit_batch(x, y)
+    for k in range(K):
+        ex0, ea, ey0 = _vertex_exact(x[k], y[k])
+        assert abs(x0[k] - ex0) <= 4e-16 * (1 + abs(ex0)), k
+        assert abs(xerr[k] - 1 / np.sqrt(abs(ea))) <= 1e-9 * xerr[k], k
+        assert abs(y0[k] - ey0) <= 1e-12 * abs(ey0), k
+        bad = (ex0 <= x[k, 0]) or (x[k, 2] <= ex0) or (ey0 <= 0) or (ea < 0)
+        assert bool(zw[k] & ZW.BAD_MINFIT) == bad, k
+        assert minfit(x[k], y[k]) == (x0[k], xerr[k], y0[k], zw[k])
+        ox0, oxerr, oy0, ozw = orc.minfit(x[k], y[k])     # the oracle's restatement
+        assert abs(ox0 - ex0) <= 4e-16 * (1 + abs(ex0)), k
+        assert abs(oxerr - 1 / np.sqrt(abs(ea))) <= 1e-9 * oxerr, k
+        assert abs(oy0 - ey0) <= 1e-12 * abs(ey0), k
+        assert bool(ozw & ZW.BAD_MINFIT) == bad, k
+
+
+def test_minfit_degenerate_rows_take_the_reference_path():
+    """Rows the closed form does not cover give exactly what np.polyfit-based
+    minfit (fitz.py:70-111) gives: a straight line (a == 0), NaN values,
+    repeated abscissae; fewer than 3 points is BAD_MINFIT."""
+    from redrock_amd.fitz import minfit_batch, _minfit_polyfit, minfit
+    x = np.array([[0.5, 0.6, 0.7], [0.5, 0.6, 0.7], [0.5, 0.5, 0.7]])
+    y = np.array([[1.0, 2.0, 3.0], [1.0, np.nan, 3.0], [3.0, 1.0, 2.0]])
+    import warnings
+    with warnings.catch_warnings():
+        warnings.simplefilter('ignore')
+        got = minfit_batch(x, y)
+        for k in range(3):
+            want = _minfit_polyfit(x[k], y[k])
+            assert want[3] & ZW.BAD_MINFIT
+            for g, w in zip((got[0][k], got[1][k], got[2][k], got[3][k]), want):
+                assert g == w or (np.isnan(g) and np.isnan(w))
+    assert minfit([0.1, 0.2], [1.0, 2.0]) == (-1, -1, -1, ZW.BAD_MINFIT)
