@@ -70,7 +70,7 @@ def solve_spec(method, nb, bounds=None):
     """(RR_SOLVE_*, non-negativity mask) of a solve_matrices algorithm
     (zscan.py:1107-1205): PCA -> M c = y; NMF / NNLS -> nnls(M, y), every
     coefficient >= 0; BVLS -> lsq_linear(M, y, bounds, 'bvls') for bounds
-    (2, nb) whose lower bounds are 0 or -inf and upper bounds +inf (the
+    (2, nb) or (2,) whose lower bounds are 0 or -inf and upper bounds +inf (the
     archetype bounds of zscan.py:289-294; None: no bound at all)."""
     key = 'PCA' if method is None else str(method).upper()
     if key == 'PCA':
@@ -81,8 +81,10 @@ def solve_spec(method, nb, bounds=None):
         if bounds is None:
             return _hip.RR_SOLVE_NNLS, 0
         b = np.asarray(bounds, dtype=np.float64)
+        if b.shape == (2,):                      # scalar bounds, broadcast as lsq_linear does
+            b = np.repeat(b[:, None], nb, axis=1)
         if b.shape != (2, nb):
-            raise ValueError(f'BVLS bounds must have shape (2, {nb}), got {b.shape}')
+            raise ValueError(f'BVLS bounds must have shape (2, {nb}) or (2,), got {b.shape}')
         lo, hi = b
         if not (np.all(np.isposinf(hi)) and np.all((lo == 0.0) | np.isneginf(lo))):
             raise NotImplementedError('BVLS bounds other than c >= 0 or free per coefficient '

@@ -118,6 +118,8 @@ def test_solve_argument():
     inf = np.inf
     assert zscan.solve_spec('BVLS', 4, [[0, 0, -inf, -inf], [inf] * 4]) == (_hip.RR_SOLVE_NNLS, 0b0011)
     assert zscan.solve_spec('bvls', 3) == (_hip.RR_SOLVE_NNLS, 0)
+    assert zscan.solve_spec('BVLS', 3, (0, inf)) == (_hip.RR_SOLVE_NNLS, 0b111)
+    assert zscan.solve_spec('BVLS', 3, (-inf, inf)) == (_hip.RR_SOLVE_NNLS, 0)
     with pytest.raises(NotImplementedError):          # a finite upper bound
         zscan.solve_spec('BVLS', 2, [[0, 0], [1, inf]])
     with pytest.raises(NotImplementedError):          # a non-zero lower bound
