@@ -159,6 +159,15 @@ def test_calc_zchi2_batch_bvls_archetype_bounds(gpu):
                                          solver_args=args, precision='f16x3')
     assert np.max(np.abs(chi2 - rchi2)) / np.sum(wf * f) <= 1e-5
     assert chi2.argmin() == rchi2.argmin()
+    # with a prior on the free coefficients, as the archetype fit passes one
+    # (zscan.py:245-348: prior_on_coeffs on the Legendre terms)
+    prior = np.diag(np.r_[np.zeros(nbound), np.full(tm.nbasis - nbound, 50.0)])
+    rchi2, rcoeff = orc.calc_zchi2_batch(spectra, tdata, w, f, wf, len(zsel), tm.nbasis, prior=prior,
+                                         bounds=bounds)
+    chi2, coeff = zscan.calc_zchi2_batch(spectra, tdata, w, f, wf, len(zsel), tm.nbasis, 'BVLS',
+                                         solver_args=args, prior=prior)
+    assert np.max(np.abs(chi2 - rchi2) / rchi2) <= F64_RTOL
+    assert np.allclose(coeff, rcoeff, rtol=1e-6, atol=1e-8 * np.abs(rcoeff).max())
     # no bounds: the unconstrained solve, as the PCA scan
     chi2, _ = zscan.calc_zchi2_batch(spectra, tdata, w, f, wf, len(zsel), tm.nbasis, 'BVLS')
     pchi2, _ = zscan.calc_zchi2_batch(spectra, tdata, w, f, wf, len(zsel), tm.nbasis, 'PCA')
