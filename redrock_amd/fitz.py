@@ -111,13 +111,6 @@ def _minfit_polyfit(x, y):
     return (x0, xerr, y0, zwarn)
 
 
-def _edges_ok(template, zlo, zhi, edges):
-    """rebin.py:332-341 as used by rebin_template with xmin/xmax passed by fitz."""
-    xmin = template.minwave * (1 + zhi)
-    xmax = template.maxwave * (1 + zlo)
-    return all(not (e[0] < xmin or e[-1] > xmax) for e in edges)
-
-
 class _Refiner(object):
     """GPU refinement of (target, candidate-minimum) pairs for one template."""
 
@@ -214,7 +207,9 @@ class _Refiner(object):
         return chi2, coeff, pen
 
     def _edges_ok_many(self, zlo, zhi):
-        """_edges_ok for arrays of (zlo, zhi)."""
+        """rebin.py:332-341 as rebin_template applies it with the xmin / xmax
+        fitz passes (the template's wavelength range at zhi / zlo), for
+        arrays of (zlo, zhi): True where every camera's edges lie inside."""
         xmin = self.template.minwave * (1 + np.asarray(zhi))
         xmax = self.template.maxwave * (1 + np.asarray(zlo))
         ok = np.ones(xmin.shape, dtype=bool)
