@@ -329,7 +329,23 @@ def _stack(results, target):
     npix = sum(int((s.ivar > 0.).sum()) for s in target.spectra)
     for r in results:
         r['npixels'] = npix
-    return {k: np.vstack([r[k] for r in results]) for k in results[0].keys()}
+    return {k: _vstack([r[k] for r in results]) for k in results[0].keys()}
+
+
+_SCALARS = (float, int, str, np.floating, np.integer, np.str_)
+
+
+def _vstack(vals):
+    """np.vstack of one result field over the candidates: scalars -> (n, 1),
+    equal-length 1-D arrays -> (n, m), with np.vstack's dtype; the common
+    cases skip its per-element atleast_2d."""
+    v0 = vals[0]
+    if isinstance(v0, _SCALARS) and all(isinstance(v, _SCALARS) for v in vals):
+        return np.array(vals).reshape(len(vals), 1)
+    if isinstance(v0, np.ndarray) and v0.ndim == 1 and all(
+            isinstance(v, np.ndarray) and v.shape == v0.shape for v in vals):
+        return np.array(vals)
+    return np.vstack(vals)
 
 
 def fitz_targets(zchi2, redshifts, targets, template, nminima=3, zminfit_npoints=15,
