@@ -271,3 +271,19 @@ def test_minfit_degenerate_rows_take_the_reference_path():
             for g, w in zip((got[0][k], got[1][k], got[2][k], got[3][k]), want):
                 assert g == w or (np.isnan(g) and np.isnan(w))
     assert minfit([0.1, 0.2], [1.0, 2.0]) == (-1, -1, -1, ZW.BAD_MINFIT)
+
+
+def test_candidate_stacking_equals_vstack():
+    """fitz stacks each result field over the candidates (fitz.py:359-395
+    uses np.vstack): same shape, dtype and values for scalars, strings,
+    equal-length 1-D arrays, mixed int / float and 2-D rows."""
+    from redrock_amd.fitz import _vstack
+    rng = np.random.default_rng(0)
+    cases = [[1.5, 2.5, np.float64(3)], [np.int64(3), 1, 0], ['PCA', 'PCA'],
+             [rng.normal(size=15) for _ in range(3)], [rng.normal(size=10)], [np.nan],
+             [rng.normal(size=(1, 4)), rng.normal(size=(1, 4))], [1, 2.5],
+             [np.int32(2), np.int32(5)], [rng.normal(size=4).astype(np.float32), rng.normal(size=4)]]
+    for c in cases:
+        got, want = _vstack(c), np.vstack(c)
+        assert got.dtype == want.dtype and got.shape == want.shape, c
+        assert np.array_equal(got, want, equal_nan=got.dtype.kind == 'f'), c
