@@ -20,7 +20,7 @@
 extern "C" {
 #endif
 
-#define RR_ABI_VERSION 8
+#define RR_ABI_VERSION 9
 
 /* Error codes. */
 #define RR_OK 0
@@ -42,6 +42,10 @@ extern "C" {
 #define RR_PREC_F16X3 4 /* MFMA fp16 hi+lo split inputs (three products per
                            term), fp32 partial sums added in FP64, FP64
                            Cholesky                                           */
+#define RR_PREC_F16R 5  /* fp16 MFMA normal equations and FP64 Cholesky for the
+                           coefficients, then chi2 as the explicit residual
+                           sum(w (f - R T c)^2) at those coefficients in FP32
+                           with FP64 partial sums (uniform items only)        */
 
 /* Resolution band: half width 5 (11 diagonals), the DESI layout. */
 #define RR_BAND_HALF 5
@@ -151,10 +155,25 @@ typedef struct {
                                 (ntarget, npix_pad/32, 4, 8) fp32 sqrt(w) f in
                                 MFMA operand order                              */
     const double *d_scales;  /* (ntarget, 2) power-of-two scales of R and f     */
+    /* Lane-per-item kernels (NULL = not used):
+     *  d_tcache_r: template cache [z][row][b] (basis fastest), element
+     *         (zc, row, b) at (zc*cache_rows + row)*nb + b; double for F64,
+     *         float for F16R (values not scaled by tscale).  Needs
+     *         cache_rows >= halo + rr_lane_pixels(npix_pad) + 5.
+     *  d_rlane: rr_pack_lanes output in item order (slot i = the target
+     *         of item i), same precision (F64 or F16R).
+     * F64 with both set and uniform_z: the scan runs one item per lane
+     * (template values through scalar loads); otherwise zscan_f64 reads
+     * d_tcache.  F16R needs both (and the F16 cache and tiles above).      */
+    const void *d_tcache_r;
+    const void *d_rlane;
     int uniform_z;           /* 1 if every item has the same z0 and nz: lets a
                                 wavefront share template tiles across items     */
     int tile;                /* mixed: tile configuration, 0 = default (see
-                                rr_zscan_tiles)                                  */
+                                rr_zscan_tiles); F64 with the lane inputs: the
+                                lane kernel's (rr_zscan_tile_table)             */
+    int resid_tile;          /* F16R: configuration of the explicit-residual
+                                kernel, 0 = default (rr_zscan_resid_tiles)      */
 
     /* Work list, one entry per item. */
     const int *d_item_target;  /* target slot                                   */
@@ -204,6 +223,7 @@ typedef struct {
 #define RR_STAGE_STATS 1    /* per-item f'Wf and sum(w)                     */
 #define RR_STAGE_MAIN 2     /* normal equations (the scan kernel)           */
 #define RR_STAGE_FINISH 4   /* FP64 solve, chi2, penalty, status            */
+#define RR_STAGE_RESID 8    /* F16R: explicit residual chi2 (after FINISH)   */
 
 /* Workspace size rr_zscan needs for this descriptor (nitems, nb, precision,
  * total_rows are read).  Returns a negative code on a bad descriptor. */
@@ -226,10 +246,35 @@ long long rr_zscan_workspace_bytes(const rr_zscan_desc *desc);
  * and status RR_STATUS_SINGULAR (zscan.py:769-771).  Zero total weight gives
  * HUGE_CHI2 and zero coeffs with status OK (zscan.py:657-660).
  *
+ * RR_PREC_F16R: the fp16 normal equations (as RR_PREC_F16) give the
+ * coefficients; stage RR_STAGE_RESID then overwrites chi2 with the explicit
+ * residual sum(w (f - R T c)^2) (:777-780 itself, no expansion) at those
+ * coefficients, whose error is second order in the coefficients' error.
+ *
  * Mixed modes need halo == 8 and the rr_pack_mixed outputs; F64 needs
- * cache_ldz >= cache_nz.
+ * cache_ldz >= cache_nz, or the lane-per-item inputs (d_tcache_r, d_rlane)
+ * with uniform_z.
  */
 int rr_zscan(const rr_zscan_desc *desc, void *stream);
+
+/*
+ * rr_pack_lanes — per-item data of the lane-per-item kernels, lane-major:
+ * value v of pixel p of slot i at ((i/64)*L + p)*S*64 + v*64 + i%64 with
+ * L = rr_lane_pixels(npix_pad) (pixels from npix_pad on, and slots from
+ * nslots on, are zero).  Slot i holds target d_order[i] (d_order NULL:
+ * target i); rr_zscan reads slot i for item i, so pack the plan's
+ * item_target as d_order (or NULL when item i is target i).
+ *   RR_PREC_F64 : double, S = 13: R[p][0..10] (R[p, p+k-5]), w, w*f
+ *   RR_PREC_F16R: float,  S = 12: sqrt(w) R[p][0..10], sqrt(w) f
+ * d_out holds rr_pack_lanes_bytes(nslots, ...) bytes (the packed slots plus
+ * 1 KB the kernels' staging copies may read past the end).  Inputs as in
+ * rr_zscan_desc.  Replaces the per-call CSR upload of zscan.py:428-430
+ * (done once per target batch here).
+ */
+#define rr_lane_pixels(npix_pad) ((((npix_pad) + 21) / 22) * 22)
+long long rr_pack_lanes_bytes(int nslots, int npix_pad, int precision);
+int rr_pack_lanes(const double *d_rband, const double *d_w, const double *d_f, const int *d_order,
+                  int nslots, int npix_pad, int precision, void *d_out, void *stream);
 
 /*
  * rr_pack_mixed — per-target preparation for the mixed precision modes:
@@ -258,10 +303,19 @@ int rr_zscan_tiles(int *cfg, int max);
 
 /*
  * rr_zscan_tile_table — as rr_zscan_tiles for the kernel of `precision`
- * (RR_PREC_F16 / _BF16: the table of rr_zscan_tiles; RR_PREC_F16X3: its own
- * table; other modes have a single configuration and return 0).
+ * (RR_PREC_F16 / _BF16 / _F16R: the table of rr_zscan_tiles; RR_PREC_F16X3:
+ * its own table; RR_PREC_F64: the lane-per-item kernel's, entry i =
+ * (1, waves per workgroup, 64, 1), each wave one redshift of the same 64
+ * items; other modes have a single configuration and return 0).
  */
 int rr_zscan_tile_table(int precision, int *cfg, int max);
+
+/*
+ * rr_zscan_resid_tiles — configurations of the explicit-residual kernel of
+ * RR_PREC_F16R (rr_zscan_desc.resid_tile): entry i = (1, waves per
+ * workgroup, 64 items per wave, redshifts per wave); entry 0 is the default.
+ */
+int rr_zscan_resid_tiles(int *cfg, int max);
 
 int rr_pack_mixed(const double *d_rband, const double *d_w, const double *d_f, int ntarget,
                   int npix_pad, int precision, void *d_rtile, void *d_sfcol,

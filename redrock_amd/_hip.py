@@ -12,20 +12,20 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 # RR_LIB_PATH: load the library from another path (a build kept elsewhere)
 LIB_PATH = os.environ.get('RR_LIB_PATH') or os.path.join(HERE, 'libredrock_hip.so')
 
-RR_ABI_VERSION = 8
+RR_ABI_VERSION = 9
 RR_F64, RR_F32, RR_F16, RR_BF16 = 0, 1, 2, 3
-RR_PREC_F64, RR_PREC_F16, RR_PREC_BF16, RR_PREC_F32, RR_PREC_F16X3 = 0, 1, 2, 3, 4
+RR_PREC_F64, RR_PREC_F16, RR_PREC_BF16, RR_PREC_F32, RR_PREC_F16X3, RR_PREC_F16R = 0, 1, 2, 3, 4, 5
 RR_STATUS_OK, RR_STATUS_SINGULAR = 0, 1
 RR_MAX_NBASIS = 12
 RR_MAX_LYMAN_LINES = 8
 
 PRECISIONS = {'f64': RR_PREC_F64, 'f16': RR_PREC_F16, 'bf16': RR_PREC_BF16, 'f32': RR_PREC_F32,
-              'f16x3': RR_PREC_F16X3}
+              'f16x3': RR_PREC_F16X3, 'f16r': RR_PREC_F16R}
 
 EXPORTS = ('rr_abi_version', 'rr_last_error', 'rr_rebin', 'rr_zscan',
            'rr_zscan_workspace_bytes', 'rr_pack_mixed', 'rr_zscan_tiles', 'rr_zscan_tile_table',
            'rr_zscan_grad', 'rr_arch_prep', 'rr_arch_workspace_bytes', 'rr_arch_fit',
-           'rr_zscan_polish')
+           'rr_zscan_polish', 'rr_pack_lanes', 'rr_pack_lanes_bytes', 'rr_zscan_resid_tiles')
 RR_ARCH_MAX_NA = 4
 RR_ARCH_MAX_NLEG = 12
 
@@ -46,7 +46,8 @@ class ZscanDesc(ctypes.Structure):
         ('d_tcache', _vp), ('cache_rows', _ll), ('cache_ldz', _ll), ('cache_nz', _i),
         ('tscale', _d),
         ('d_rband', _vp), ('d_w', _vp), ('d_wf', _vp), ('d_f', _vp),
-        ('d_rtile', _vp), ('d_sfcol', _vp), ('d_scales', _vp), ('uniform_z', _i), ('tile', _i),
+        ('d_rtile', _vp), ('d_sfcol', _vp), ('d_scales', _vp),
+        ('d_tcache_r', _vp), ('d_rlane', _vp), ('uniform_z', _i), ('tile', _i), ('resid_tile', _i),
         ('d_item_target', _vp), ('d_item_z0', _vp), ('d_item_nz', _vp), ('d_item_out', _vp),
         ('d_prior', _vp), ('d_oiisum', _vp),
         ('d_zchi2', _vp), ('d_zcoeff', _vp), ('d_penalty', _vp), ('d_status', _vp),
@@ -56,7 +57,7 @@ class ZscanDesc(ctypes.Structure):
     ]
 
 
-RR_STAGE_STATS, RR_STAGE_MAIN, RR_STAGE_FINISH = 1, 2, 4
+RR_STAGE_STATS, RR_STAGE_MAIN, RR_STAGE_FINISH, RR_STAGE_RESID = 1, 2, 4, 8
 RR_SOLVE_PCA, RR_SOLVE_NNLS = 0, 1
 
 RR_POLISH_WINDOW = 4
@@ -116,6 +117,12 @@ def lib():
     L.rr_zscan_workspace_bytes.argtypes = [ctypes.POINTER(ZscanDesc)]
     L.rr_pack_mixed.restype = _i
     L.rr_pack_mixed.argtypes = [_vp, _vp, _vp, _i, _i, _i, _vp, _vp, _vp, _vp]
+    L.rr_pack_lanes_bytes.restype = _ll
+    L.rr_pack_lanes_bytes.argtypes = [_i, _i, _i]
+    L.rr_pack_lanes.restype = _i
+    L.rr_pack_lanes.argtypes = [_vp, _vp, _vp, _vp, _i, _i, _i, _vp, _vp]
+    L.rr_zscan_resid_tiles.restype = _i
+    L.rr_zscan_resid_tiles.argtypes = [ctypes.POINTER(_i), _i]
     L.rr_zscan_tiles.restype = _i
     L.rr_zscan_tiles.argtypes = [ctypes.POINTER(_i), _i]
     L.rr_zscan_tile_table.restype = _i
@@ -131,6 +138,11 @@ def lib():
         raise HipError(f'libredrock_hip ABI {v}, expected {RR_ABI_VERSION}')
     _lib = L
     return L
+
+
+def lane_pixels(npix_pad):
+    """Pixels of the lane-major per-target pack (rr_lane_pixels in the header)."""
+    return (npix_pad + 21) // 22 * 22
 
 
 def check(rc, what):

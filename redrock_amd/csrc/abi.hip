@@ -21,7 +21,13 @@ extern "C" const char *rr_last_error(void) { return rr::g_err; }
 
 static bool is_mixed(int precision) {
     return precision == RR_PREC_F16 || precision == RR_PREC_BF16 || precision == RR_PREC_F32 ||
-           precision == RR_PREC_F16X3;
+           precision == RR_PREC_F16X3 || precision == RR_PREC_F16R;
+}
+
+// The lane-per-item kernels read rows up to halo + lane_pixels + 4 of their cache.
+static bool lane_inputs(const rr_zscan_desc *d) {
+    return d->d_tcache_r && d->d_rlane && d->uniform_z &&
+           d->cache_rows >= d->halo + rr::lane_pixels(d->npix_pad) + 5;
 }
 
 extern "C" long long rr_zscan_workspace_bytes(const rr_zscan_desc *d) {
@@ -47,7 +53,11 @@ static int scan_args(const rr_zscan_desc *d, rr::ZArgs &a) {
     // no items (or only empty ones): a valid call with nothing to do; zero-size
     // device arrays may then have null addresses
     if (d->nitems == 0 || d->max_item_nz <= 0) return 1;
-    if (!d->d_tcache || !d->d_rband || !d->d_w || !d->d_wf || !d->d_f || !d->d_item_target ||
+    if (d->precision == RR_PREC_F64 && !d->d_tcache && !lane_inputs(d))
+        return rr::fail(RR_EINVAL, "rr_zscan: F64 needs d_tcache, or d_tcache_r and d_rlane with "
+                                   "uniform items and cache_rows >= halo + rr_lane_pixels(npix_pad) + 5");
+    if ((d->precision != RR_PREC_F64 && !d->d_tcache) || !d->d_rband || !d->d_w || !d->d_wf || !d->d_f ||
+        !d->d_item_target ||
         !d->d_item_z0 || !d->d_item_nz || !d->d_item_out || !d->d_zchi2 || !d->d_zcoeff)
         return rr::fail(RR_EINVAL, "rr_zscan: null required pointer");
     const bool mixed = is_mixed(d->precision);
@@ -56,8 +66,12 @@ static int scan_args(const rr_zscan_desc *d, rr::ZArgs &a) {
         return rr::fail(RR_EINVAL, "rr_zscan: workspace of %lld bytes needed, %lld given", need,
                         d->work_bytes);
     if (d->precision == RR_PREC_F64) {
-        if (d->cache_ldz < d->cache_nz) return rr::fail(RR_EINVAL, "rr_zscan: cache_ldz < cache_nz");
+        if (!lane_inputs(d) && d->cache_ldz < d->cache_nz)
+            return rr::fail(RR_EINVAL, "rr_zscan: cache_ldz < cache_nz");
     } else if (mixed) {
+        if (d->precision == RR_PREC_F16R && !lane_inputs(d))
+            return rr::fail(RR_EINVAL, "rr_zscan: F16R needs d_tcache_r and d_rlane, uniform items "
+                                       "and cache_rows >= halo + rr_lane_pixels(npix_pad) + 5");
         if (!d->d_rtile || !d->d_scales || (d->precision == RR_PREC_F32 && !d->d_sfcol))
             return rr::fail(RR_EINVAL, "rr_zscan: mixed precision needs rr_pack_mixed outputs");
         if (d->halo != 8) return rr::fail(RR_EINVAL, "rr_zscan: mixed precision needs halo == 8");
@@ -74,8 +88,10 @@ static int scan_args(const rr_zscan_desc *d, rr::ZArgs &a) {
     a.cache_nz = d->cache_nz; a.tscale = d->tscale;
     a.rband = d->d_rband; a.w = d->d_w; a.wf = d->d_wf; a.f = d->d_f;
     a.rtile = d->d_rtile; a.sfcol = d->d_sfcol; a.scales = d->d_scales;
+    a.tcache_r = d->d_tcache_r; a.rlane = d->d_rlane;
     a.uniform_z = d->uniform_z;
     a.tile = d->tile;
+    a.resid_tile = d->resid_tile;
     a.item_target = d->d_item_target; a.item_z0 = d->d_item_z0; a.item_nz = d->d_item_nz;
     a.item_out = d->d_item_out; a.prior = d->d_prior; a.oiisum = d->d_oiisum;
     a.zchi2 = d->d_zchi2; a.zcoeff = d->d_zcoeff; a.penalty = d->d_penalty; a.status = d->d_status;
@@ -93,17 +109,28 @@ extern "C" int rr_zscan(const rr_zscan_desc *d, void *stream) {
     const bool mixed = is_mixed(d->precision);
     hipStream_t s = static_cast<hipStream_t>(stream);
 
-    const int st = d->stages ? d->stages : (RR_STAGE_STATS | RR_STAGE_MAIN | RR_STAGE_FINISH);
+    const int st = d->stages ? d->stages
+                             : (RR_STAGE_STATS | RR_STAGE_MAIN | RR_STAGE_FINISH | RR_STAGE_RESID);
     int rc = RR_OK;
     if (st & RR_STAGE_STATS) rc = rr::zscan_stats(a, s);
     if (rc) return rc;
-    if (st & RR_STAGE_MAIN)
-        rc = d->precision == RR_PREC_F32     ? rr::zscan_mfma32(a, s)
-             : d->precision == RR_PREC_F16X3 ? rr::zscan_mfma3(a, s)
-             : mixed                         ? rr::zscan_mfma(a, s)
-                                             : rr::zscan_f64(a, s);
+    if (st & RR_STAGE_MAIN) {
+        if (d->precision == RR_PREC_F16R) {
+            rr::ZArgs a16 = a;            // the fp16 normal equations of RR_PREC_F16
+            a16.precision = RR_PREC_F16;
+            rc = rr::zscan_mfma(a16, s);
+        } else {
+            rc = d->precision == RR_PREC_F32     ? rr::zscan_mfma32(a, s)
+                 : d->precision == RR_PREC_F16X3 ? rr::zscan_mfma3(a, s)
+                 : mixed                         ? rr::zscan_mfma(a, s)
+                 : lane_inputs(d)                ? rr::zscan_f64r(a, s)
+                                                 : rr::zscan_f64(a, s);
+        }
+    }
     if (rc) return rc;
     if (st & RR_STAGE_FINISH) rc = rr::zscan_finish(a, s);
+    if (rc) return rc;
+    if ((st & RR_STAGE_RESID) && d->precision == RR_PREC_F16R) rc = rr::zscan_resid(a, s);
     return rc;
 }
 
@@ -141,7 +168,14 @@ extern "C" int rr_zscan_polish(const rr_zscan_desc *d, const rr_polish_desc *p, 
 
 extern "C" int rr_zscan_tile_table(int precision, int *cfg, int max) {
     if (!cfg || max < 0) return rr::fail(RR_EINVAL, "rr_zscan_tile_table: bad arguments");
-    if (precision == RR_PREC_F16 || precision == RR_PREC_BF16) return rr::zscan_tiles16(cfg, max);
+    if (precision == RR_PREC_F16 || precision == RR_PREC_BF16 || precision == RR_PREC_F16R)
+        return rr::zscan_tiles16(cfg, max);
     if (precision == RR_PREC_F16X3) return rr::zscan_tiles3(cfg, max);
+    if (precision == RR_PREC_F64) return rr::zscan_f64r_tiles(cfg, max);
     return 0;
+}
+
+extern "C" int rr_zscan_resid_tiles(int *cfg, int max) {
+    if (!cfg || max < 0) return rr::fail(RR_EINVAL, "rr_zscan_resid_tiles: bad arguments");
+    return rr::zscan_resid_tiles(cfg, max);
 }

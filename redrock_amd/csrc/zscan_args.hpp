@@ -14,8 +14,11 @@ struct ZArgs {
     const void *rtile;
     const void *sfcol;
     const double *scales;
+    const void *tcache_r;  // [z][row][b] cache of the lane-per-item kernels
+    const void *rlane;     // rr_pack_lanes output
     int uniform_z;
     int tile;
+    int resid_tile;        // RR_PREC_F16R: configuration of the residual kernel
     const int *item_target, *item_z0, *item_nz;
     const long long *item_out;
     const double *prior, *oiisum;
@@ -66,6 +69,12 @@ int zscan_stats(const ZArgs &a, hipStream_t s);
 int zscan_finish(const ZArgs &a, hipStream_t s);
 
 int zscan_f64(const ZArgs &a, hipStream_t s);
+int zscan_f64r(const ZArgs &a, hipStream_t s);
+int zscan_f64r_tiles(int *cfg, int max);
+int zscan_resid(const ZArgs &a, hipStream_t s);
+int zscan_resid_tiles(int *cfg, int max);
+// pixels of the lane-major per-target pack (a multiple of 22, >= npix_pad)
+__host__ __device__ constexpr long long lane_pixels(long long npix_pad) { return rr_lane_pixels(npix_pad); }
 int zscan_mfma(const ZArgs &a, hipStream_t s);
 int zscan_mfma32(const ZArgs &a, hipStream_t s);
 int zscan_mfma3(const ZArgs &a, hipStream_t s);

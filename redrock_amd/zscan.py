@@ -14,7 +14,9 @@ Every call runs the HIP kernels of libredrock_hip.so; there is no CPU path.
 The ``use_gpu`` and ``fullprecision`` arguments are accepted for signature
 compatibility (the scan is always on the GPU and always deterministic).
 ``precision`` selects 'f64' (default, the reference's arithmetic) or a
-mixed-precision MFMA form: 'f16x3' (split fp16, chi2 within ~1e-5 of FP64),
+mixed-precision MFMA form: 'f16r' (fp16 normal equations for the
+coefficients, chi2 as the explicit FP32 residual at them: within ~1e-7 of
+FP64), 'f16x3' (split fp16, chi2 within ~1e-5 of FP64 at per-pixel S/N 1),
 'f16', 'bf16' or 'f32'; the default can be changed with the environment
 variable RR_PRECISION.
 
@@ -58,9 +60,9 @@ def default_precision():
 # not with chi2): at least four times the largest |chi2_mixed - chi2_f64| /
 # f'Wf measured for that mode over GALAXY / QSO / STAR targets at per-pixel
 # S/N 0.3 to 30 and the masked GALAXY/QSO/STAR batch.
-POLISH_TOL = {'f16x3': 1e-5, 'f32': 1e-5, 'f16': 1e-3, 'bf16': 5e-3}
+POLISH_TOL = {'f16r': 1e-6, 'f16x3': 1e-5, 'f32': 1e-5, 'f16': 1e-3, 'bf16': 5e-3}
 POLISH_WINDOWS = 16
-STAGE_POLISH = 8      # ScanPlan.run stage bit (rr_zscan_polish), beside RR_STAGE_*
+STAGE_POLISH = 16     # ScanPlan.run stage bit (rr_zscan_polish), beside RR_STAGE_*
 
 
 SOLVES = ('PCA', 'NMF', 'NNLS', 'BVLS')
@@ -142,7 +144,7 @@ class PixelLayout(object):
         self.npix = int(sum(self.sizes))
         self.npix_pad = _roundup(self.npix, 32)
         self.halo = HALO
-        self.cache_rows = self.npix_pad + 32
+        self.cache_rows = self.npix_pad + 48   # halo + pixels + the lane kernels' window overrun
 
     @property
     def signature(self):
@@ -214,6 +216,8 @@ class TemplateCache(object):
         """Device cache for `layout` in the layout of `precision`'s kernel.
 
         f64 : float64 [nb][cache_rows][ldz]   (z fastest), ldz = nz rounded to 64
+        f64r/f32r: float64 / float32 [nz][cache_rows][nb] (basis fastest),
+              unscaled: the caches of the lane-per-item kernels
         f16/bf16/f32: [nz][nb][cache_rows] (pixel fastest), values * tscale
         f16x3: fp16 [nz][hi, lo][nb][cache_rows], hi + lo = values * tscale
         Returns (tensor, ldz).
@@ -224,7 +228,15 @@ class TemplateCache(object):
             return self._bufs[key]
         d = self.device()
         nb, nz, rows = self.nb, self.nz, layout.cache_rows
-        if precision == 'f64':
+        if precision in ('f64r', 'f32r'):
+            ldz = 0
+            dt, ot = ((torch.float64, _hip.RR_F64) if precision == 'f64r'
+                      else (torch.float32, _hip.RR_F32))
+            buf = torch.zeros(nz * rows * nb, dtype=dt, device='cuda')
+            for k, off in zip(layout.keys, layout.offsets):
+                rebin_into(buf[(layout.halo + off) * nb:], d['x'], d['y'], d['edges'][k], d['z'],
+                           rows * nb, nb, 1, ot, 1.0, self.lyman(k), d['wave'][k])
+        elif precision == 'f64':
             ldz = _roundup(max(nz, 1), 64)
             buf = torch.zeros(nb * rows * ldz, dtype=torch.float64, device='cuda')
             for k, off in zip(layout.keys, layout.offsets):
@@ -302,9 +314,35 @@ class TargetBatch(object):
         self.wf = torch.as_tensor(np.ascontiguousarray(wf), dtype=torch.float64, device=dev)
         self._mixed = dict()
 
+    def lanes(self, precision, order=None):
+        """Lane-major pack of the lane-per-item kernels (rr_pack_lanes), 'f64' or
+        'f16r': slot i holds target order[i] (order None: target i, cached)."""
+        import torch
+        key = 'lanes-' + precision
+        if order is None and key in self._mixed:
+            return self._mixed[key]
+        P = self.layout.npix_pad
+        code = _hip.PRECISIONS[precision]
+        nslots = self.nt if order is None else len(order)
+        nbytes = _hip.lib().rr_pack_lanes_bytes(nslots, P, code)
+        if nbytes < 0:
+            _hip.check(int(nbytes), 'rr_pack_lanes_bytes')
+        buf = torch.empty(max(int(nbytes), 256), dtype=torch.uint8, device='cuda')
+        t_order = (None if order is None else
+                   torch.as_tensor(np.ascontiguousarray(order, dtype=np.int32), device='cuda'))
+        rc = _hip.lib().rr_pack_lanes(_hip.ptr(self.rband), _hip.ptr(self.w), _hip.ptr(self.f),
+                                      _hip.ptr(t_order), nslots, P, code, _hip.ptr(buf),
+                                      _hip.stream_handle())
+        _hip.check(rc, 'rr_pack_lanes')
+        if order is None:
+            self._mixed[key] = buf
+        return buf
+
     def mixed(self, precision):
         """(rtile, sfcol, scales) for a mixed precision (rr_pack_mixed), cached."""
         import torch
+        if precision == 'f16r':
+            precision = 'f16'                # the fp16 normal equations use the fp16 tiles
         if precision not in self._mixed:
             P = self.layout.npix_pad
             dt = torch.float32 if precision == 'f32' else torch.int16
@@ -347,12 +385,14 @@ class ScanPlan(object):
 
     def __init__(self, cache, batch, item_target, item_z0, item_nz, precision='f64',
                  prior=None, oiisum=None, tile=0, polish=None, polish_windows=POLISH_WINDOWS,
-                 solve='PCA', bounds=None):
+                 solve='PCA', bounds=None, lanes=None):
         """polish: FP64 re-evaluation of the rows near each item's minimum after a
         mixed-precision scan (rr_zscan_polish), see polish_tol(); it needs the
         FP64 template cache as well.  solve: the template's method, 'PCA' or
         'NMF' / 'NNLS' (non-negative coefficients, no polish), or 'BVLS' with
-        `bounds` (solve_spec)."""
+        `bounds` (solve_spec).  lanes: FP64 only, scan with one item per lane
+        (items sharing one redshift range); None = when there are at least
+        LANE_MIN_ITEMS such items ('f16r' always needs them)."""
         import torch
         if precision not in _hip.PRECISIONS:
             raise ValueError(f'unknown precision {precision!r}')
@@ -362,10 +402,27 @@ class ScanPlan(object):
         if nb > _hip.RR_MAX_NBASIS:
             raise NotImplementedError(f'nbasis={nb} > {_hip.RR_MAX_NBASIS}')
         layout = batch.layout
-        tbuf, ldz = cache.get(layout, precision)
         item_target = np.asarray(item_target, dtype=np.int32)
         item_z0 = np.asarray(item_z0, dtype=np.int32)
         item_nz = np.asarray(item_nz, dtype=np.int32)
+        uniform = bool(len(item_nz) > 0 and np.all(item_z0 == item_z0[0])
+                       and np.all(item_nz == item_nz[0]))
+        if precision == 'f16r':
+            if len(item_nz) and not uniform:
+                raise NotImplementedError("precision 'f16r' needs items that share one redshift range")
+            lanes = uniform
+        elif precision == 'f64':
+            if lanes is None:
+                lanes = uniform and len(item_nz) >= LANE_MIN_ITEMS
+            elif lanes and not uniform:
+                raise ValueError('the lane-per-item FP64 scan needs items sharing one redshift range')
+        else:
+            lanes = False
+        self.lanes = bool(lanes)
+        if precision == 'f64' and self.lanes:
+            tbuf, ldz = None, 0
+        else:
+            tbuf, ldz = cache.get(layout, 'f16' if precision == 'f16r' else precision)
         if np.any(item_z0 < 0) or np.any(item_z0 + item_nz > cache.nz):
             raise ValueError('item redshift range outside the template cache')
         if np.any(item_target < 0) or np.any(item_target >= batch.nt):
@@ -397,7 +454,7 @@ class ScanPlan(object):
         d.nb = nb
         d.precision = _hip.PRECISIONS[precision]
         d.halo = layout.halo
-        d.d_tcache = tbuf.data_ptr()
+        d.d_tcache = None if tbuf is None else tbuf.data_ptr()
         d.cache_rows = layout.cache_rows
         d.cache_ldz = ldz
         d.cache_nz = cache.nz
@@ -411,9 +468,14 @@ class ScanPlan(object):
             d.d_rtile, d.d_scales = rtile.data_ptr(), scales.data_ptr()
             d.d_sfcol = None if sfcol is None else sfcol.data_ptr()
             self._keep += [rtile, sfcol, scales]
+        if self.lanes and self.nitems > 0:
+            tr, _ = cache.get(layout, 'f64r' if precision == 'f64' else 'f32r')
+            identity = np.array_equal(item_target, np.arange(len(item_target)))
+            rl = batch.lanes(precision, None if identity else item_target)
+            d.d_tcache_r, d.d_rlane = tr.data_ptr(), rl.data_ptr()
+            self._keep += [tr, rl]
         d.tile = int(tile)
-        d.uniform_z = int(self.nitems > 0 and np.all(item_z0 == item_z0[0])
-                          and np.all(item_nz == item_nz[0]))
+        d.uniform_z = int(uniform)
         d.d_item_target = self.t_item_target.data_ptr()
         d.d_item_z0 = self.t_item_z0.data_ptr()
         d.d_item_nz = self.t_item_nz.data_ptr()
@@ -467,10 +529,11 @@ class ScanPlan(object):
     def run(self, stream=None, stages=0):
         """Enqueue the scan on `stream` (default: torch's current stream).
 
-        stages: RR_STAGE_* and STAGE_POLISH bits (0 = stats, main, finish and,
-        when the plan has one, the polish).
+        stages: RR_STAGE_* and STAGE_POLISH bits (0 = stats, main, finish, the
+        explicit residual of 'f16r' and, when the plan has one, the polish).
         """
-        scan = stages & (_hip.RR_STAGE_STATS | _hip.RR_STAGE_MAIN | _hip.RR_STAGE_FINISH)
+        scan = stages & (_hip.RR_STAGE_STATS | _hip.RR_STAGE_MAIN | _hip.RR_STAGE_FINISH
+                         | _hip.RR_STAGE_RESID)
         h = _hip.stream_handle(stream)
         if stages == 0 or scan:
             self.desc.stages = scan
@@ -485,6 +548,12 @@ class ScanPlan(object):
         """(zchi2, zcoeff, penalty, status) copied to numpy."""
         return (self.zchi2.cpu().numpy(), self.zcoeff.cpu().numpy(),
                 self.penalty.cpu().numpy(), self.status.cpu().numpy())
+
+
+# Smallest item count for which an FP64 plan of items sharing one redshift
+# range takes the lane-per-item kernel (fewer items would leave lanes idle;
+# the redshift-per-lane kernel serves them).
+LANE_MIN_ITEMS = 32
 
 
 def _method(cache):
@@ -697,7 +766,15 @@ class _ExplicitCache(object):
         if key in self._bufs:
             return self._bufs[key]
         nb, nz, rows = self.nb, self.nz, layout.cache_rows
-        if precision == 'f64':
+        if precision in ('f64r', 'f32r'):
+            ldz = 0
+            dt = torch.float64 if precision == 'f64r' else torch.float32
+            buf = torch.zeros((nz, rows, nb), dtype=dt, device='cuda')
+            for k, off, n in zip(layout.keys, layout.offsets, layout.sizes):
+                t = torch.as_tensor(self.tdata[k], dtype=torch.float64, device='cuda')
+                buf[:, layout.halo + off:layout.halo + off + n, :] = t.to(dt)
+            buf = buf.reshape(-1)
+        elif precision == 'f64':
             ldz = _roundup(max(nz, 1), 64)
             buf = torch.zeros((nb, rows, ldz), dtype=torch.float64, device='cuda')
             for k, off, n in zip(layout.keys, layout.offsets, layout.sizes):

@@ -1,0 +1,128 @@
+"""Lane-per-item scan kernels (zscan_lanes.hip) against the oracle and the
+redshift-per-lane FP64 kernel.
+
+* FP64, one item per lane: the same doubles as the redshift-per-lane kernel
+  (same operation order), so bit-identical rows; <= 1e-9 of the oracle with
+  identical best-z.
+* 'f16r' (fp16 normal equations for the coefficients, chi2 as the explicit
+  FP32 residual at them): |delta chi2|/chi2 <= 1e-6 of the oracle at every
+  (target, z) and identical best-z, GALAXY, QSO with the IGM, STAR, and at
+  per-pixel S/N 10; HUGE_CHI2 conventions kept.
+"""
+import numpy as np
+import pytest
+
+from oracle import rr_oracle as orc
+
+pytestmark = pytest.mark.gpu
+
+F64_RTOL = 1e-9
+F16R_RTOL = 1e-6
+LYA = [(1215.67, 0.0036, 3.46)]
+
+
+def _setup(kind, nt, seed, snr=1.0, zsel=None):
+    from redrock_amd import synth
+    from redrock_amd.templates import DistTemplate
+    tm = {'galaxy': synth.galaxy_template, 'qso': synth.qso_template,
+          'star': synth.star_template}[kind]()
+    if zsel is not None:
+        tm._redshifts = tm.redshifts[zsel]
+    targets, _ = synth.make_targets(tm, nt, seed=seed, snr=snr)
+    dwave = {s.wavehash: s.wave for s in targets[0].spectra}
+    return tm, targets, dwave, DistTemplate(tm, dwave)
+
+
+def _plan(dt, targets, precision, items=None, **kw):
+    from redrock_amd import zscan
+    batch = zscan.TargetBatch(targets)
+    nt, nz = len(targets), dt.cache.nz
+    it = np.arange(nt) if items is None else np.asarray(items)
+    plan = zscan.ScanPlan(dt.cache, batch, it, np.zeros(len(it)), np.full(len(it), nz), precision,
+                          **kw).run()
+    n = len(it)
+    return (plan.zchi2.cpu().numpy().reshape(n, nz), plan.zcoeff.cpu().numpy().reshape(n, nz, -1),
+            plan.penalty.cpu().numpy().reshape(n, nz), plan)
+
+
+@pytest.fixture(scope='module')
+def galaxy70(gpu):
+    # 70 targets: two lane groups, the second partly idle; one target without weight
+    tm, targets, dwave, dt = _setup('galaxy', 70, seed=21, zsel=slice(500, 820))
+    for s in targets[5].spectra:
+        s.ivar[:] = 0.0
+    return tm, targets, dwave, dt
+
+
+def test_f64_lanes_equal_zfast_kernel(galaxy70):
+    tm, targets, dwave, dt = galaxy70
+    print('case: f64 lanes vs z-per-lane, 70 galaxy targets x 320 z', flush=True)
+    a = _plan(dt, targets, 'f64', lanes=True)
+    b = _plan(dt, targets, 'f64', lanes=False)
+    assert a[3].lanes and not b[3].lanes
+    for x, y in zip(a[:3], b[:3]):
+        assert np.array_equal(x, y)
+    assert np.all(a[0][5] == 9999999.0) and np.all(a[1][5] == 0.0)
+
+
+def test_f64_lanes_shuffled_items(galaxy70):
+    """Items in any target order (lanes then gather their per-target data)."""
+    tm, targets, dwave, dt = galaxy70
+    rng = np.random.default_rng(4)
+    items = rng.permutation(np.r_[np.arange(70), np.arange(0, 70, 3)])
+    print(f'case: f64 lanes, {items.size} shuffled items', flush=True)
+    full = _plan(dt, targets, 'f64', lanes=True)
+    a = _plan(dt, targets, 'f64', items=items, lanes=True)
+    assert np.array_equal(a[0], full[0][items])
+    assert np.array_equal(a[1], full[1][items])
+
+
+def test_f64_lanes_match_oracle(galaxy70):
+    tm, targets, dwave, dt = galaxy70
+    sel = [0, 33, 64, 69]
+    print('case: f64 lanes vs oracle, targets', sel, flush=True)
+    chi2, coeff, pen, _ = _plan(dt, targets, 'f64', lanes=True)
+    tdata = orc.template_cache(tm, dwave)
+    rchi2, rcoeff, rpen = orc.calc_zchi2([targets[i] for i in sel], tdata, tm.nbasis, tm.OIItemplate)
+    rel = np.abs(chi2[sel] - rchi2) / np.abs(rchi2)
+    print('max rel chi2 err', rel.max())
+    assert rel.max() <= F64_RTOL
+    assert np.array_equal(chi2[sel].argmin(axis=1), rchi2.argmin(axis=1))
+    assert np.allclose(coeff[sel], rcoeff, rtol=1e-7, atol=1e-9 * np.abs(rcoeff).max())
+
+
+@pytest.mark.parametrize('kind,snr,zsel', [('galaxy', 1.0, slice(0, None, 6)),
+                                           ('galaxy', 10.0, slice(0, None, 6)),
+                                           ('qso', 1.0, slice(0, None, 10)),
+                                           ('star', 3.0, slice(None))])
+def test_f16r_matches_oracle(gpu, kind, snr, zsel):
+    tm, targets, dwave, dt = _setup(kind, 3, seed=31, snr=snr, zsel=zsel)
+    print(f'case: f16r {kind} S/N {snr}, 3 targets x {tm.redshifts.size} z', flush=True)
+    chi2, coeff, pen, plan = _plan(dt, targets, 'f16r', polish=False)
+    tdata = orc.template_cache(tm, dwave, lines=LYA if kind == 'qso' else None)
+    rchi2, rcoeff, rpen = orc.calc_zchi2(targets, tdata, tm.nbasis,
+                                         getattr(tm, 'OIItemplate', None))
+    rel = np.abs(chi2 - rchi2) / np.abs(rchi2)
+    print(f'f16r {kind} S/N {snr}: max rel err {rel.max():.3e}, median {np.median(rel):.3e}')
+    assert rel.max() <= F16R_RTOL
+    assert np.array_equal(chi2.argmin(axis=1), rchi2.argmin(axis=1))
+    # chi2 at approximate coefficients is never below the least-squares minimum
+    # (beyond FP32 rounding of the residual sum)
+    assert np.all(chi2 >= rchi2 * (1 - 2e-7))
+
+
+def test_f16r_conventions_and_polish(galaxy70):
+    """Zero weight -> HUGE_CHI2 and zero coefficients; the FP64 polish leaves
+    the best redshift of every target equal to the FP64 scan's."""
+    tm, targets, dwave, dt = galaxy70
+    print('case: f16r 70 targets, polished vs f64', flush=True)
+    chi2, coeff, pen, plan = _plan(dt, targets, 'f16r')
+    ref = _plan(dt, targets, 'f64')
+    assert plan.polish is not None
+    assert np.all(chi2[5] == 9999999.0) and np.all(coeff[5] == 0.0)
+    ok = np.arange(70) != 5
+    rel = np.abs(chi2[ok] - ref[0][ok]) / ref[0][ok]
+    print('f16r vs f64 max rel', rel.max())
+    assert rel.max() <= F16R_RTOL
+    assert np.array_equal(chi2.argmin(axis=1), ref[0].argmin(axis=1))
+    assert np.array_equal((chi2 + pen).argmin(axis=1), (ref[0] + ref[2]).argmin(axis=1))
