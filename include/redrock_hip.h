@@ -258,6 +258,21 @@ long long rr_zscan_workspace_bytes(const rr_zscan_desc *desc);
 int rr_zscan(const rr_zscan_desc *desc, void *stream);
 
 /*
+ * rr_zscan_noalloc — rr_zscan with no caller workspace: the statistics and
+ * normal equations live in a static device array of RR_NOALLOC_SCRATCH_BYTES
+ * held by every GPU the library runs on until the process ends.  No
+ * counterpart in the reference; it is for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops).
+ * desc->d_work and desc->work_bytes are ignored; everything else is as for
+ * rr_zscan, which it runs.  A scan whose rr_zscan_workspace_bytes() exceeds
+ * the array returns RR_EINVAL.  Calls that overlap on one GPU (other
+ * streams, other host threads) share the array and race.  The array is not
+ * rr_zscan_polish's workspace: polish scans made by rr_zscan.
+ */
+#define RR_NOALLOC_SCRATCH_BYTES (64LL << 20)
+int rr_zscan_noalloc(const rr_zscan_desc *desc, void *stream);
+
+/*
  * rr_pack_lanes — per-item data of the lane-per-item kernels, lane-major:
  * value v of pixel p of slot i at ((i/64)*L + p)*S*64 + v*64 + i%64 with
  * L = rr_lane_pixels(npix_pad) (pixels from npix_pad on, and slots from

@@ -25,7 +25,9 @@ PRECISIONS = {'f64': RR_PREC_F64, 'f16': RR_PREC_F16, 'bf16': RR_PREC_BF16, 'f32
 EXPORTS = ('rr_abi_version', 'rr_last_error', 'rr_rebin', 'rr_zscan',
            'rr_zscan_workspace_bytes', 'rr_pack_mixed', 'rr_zscan_tiles', 'rr_zscan_tile_table',
            'rr_zscan_grad', 'rr_arch_prep', 'rr_arch_workspace_bytes', 'rr_arch_fit',
-           'rr_zscan_polish', 'rr_pack_lanes', 'rr_pack_lanes_bytes', 'rr_zscan_resid_tiles')
+           'rr_zscan_polish', 'rr_pack_lanes', 'rr_pack_lanes_bytes', 'rr_zscan_resid_tiles',
+           'rr_zscan_noalloc')
+RR_NOALLOC_SCRATCH_BYTES = 64 << 20
 RR_ARCH_MAX_NA = 4
 RR_ARCH_MAX_NLEG = 12
 
@@ -111,6 +113,8 @@ def lib():
                            ctypes.POINTER(LymanLine), _i, _vp, _i, _ll, _ll, _ll, _d, _vp]
     L.rr_zscan.restype = _i
     L.rr_zscan.argtypes = [ctypes.POINTER(ZscanDesc), _vp]
+    L.rr_zscan_noalloc.restype = _i
+    L.rr_zscan_noalloc.argtypes = [ctypes.POINTER(ZscanDesc), _vp]
     L.rr_zscan_polish.restype = _i
     L.rr_zscan_polish.argtypes = [ctypes.POINTER(ZscanDesc), ctypes.POINTER(PolishDesc), _vp]
     L.rr_zscan_workspace_bytes.restype = _ll

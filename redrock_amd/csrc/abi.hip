@@ -3,6 +3,7 @@
 #include "zscan_args.hpp"
 
 #include <cstring>
+#include <mutex>
 
 namespace rr {
 static thread_local char g_err[1024] = "";
@@ -132,6 +133,61 @@ extern "C" int rr_zscan(const rr_zscan_desc *d, void *stream) {
     if (rc) return rc;
     if ((st & RR_STAGE_RESID) && d->precision == RR_PREC_F16R) rc = rr::zscan_resid(a, s);
     return rc;
+}
+
+// ---------------------------------------------------------------------------
+// rr_zscan_noalloc: rr_zscan with its workspace in a static device array.
+// `scratch` is a host-side stand-in for the array; on every device the
+// library runs on, its device address comes from hipGetSymbolAddress (once
+// per device, with that device current) and only that address is passed to
+// the kernels.  Every call on a device shares the device's array: calls that
+// overlap on one device race.  Nothing in the workspace must start at zero
+// (the statistics and normal-equation stages write every entry the later
+// stages read).
+// ---------------------------------------------------------------------------
+static __device__ __align__(256) unsigned char scratch[RR_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+void *g_scratch_ptr[kMaxDevices] = {};
+
+int scratch_address(void **out) {
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return rr::fail(RR_ELAUNCH, "rr_zscan_noalloc: hipGetDevice: %s", hipGetErrorString(e));
+    if (dev < 0 || dev >= kMaxDevices)
+        return rr::fail(RR_EUNSUPPORTED, "rr_zscan_noalloc: device %d beyond %d", dev, kMaxDevices);
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return rr::fail(RR_ELAUNCH, "rr_zscan_noalloc: hipGetSymbolAddress(scratch) on device %d: %s", dev,
+                            hipGetErrorString(e));
+        g_scratch_ptr[dev] = p;
+    }
+    *out = g_scratch_ptr[dev];
+    return RR_OK;
+}
+}  // namespace
+
+extern "C" int rr_zscan_noalloc(const rr_zscan_desc *d, void *stream) {
+    if (!d) return rr::fail(RR_EINVAL, "rr_zscan_noalloc: null descriptor");
+    const long long need = rr_zscan_workspace_bytes(d);
+    if (need < 0) return (int)need;
+    if (need > (long long)RR_NOALLOC_SCRATCH_BYTES)
+        return rr::fail(RR_EINVAL,
+                        "rr_zscan_noalloc: this scan needs a %lld-byte workspace, the static scratch holds "
+                        "%lld bytes (fewer items or redshifts per call, or rr_zscan with a caller workspace)",
+                        need, (long long)RR_NOALLOC_SCRATCH_BYTES);
+    void *ptr = nullptr;
+    const int rc = scratch_address(&ptr);
+    if (rc) return rc;
+    rr_zscan_desc dd = *d;
+    dd.d_work = ptr;
+    dd.work_bytes = RR_NOALLOC_SCRATCH_BYTES;
+    return rr_zscan(&dd, stream);
 }
 
 extern "C" int rr_zscan_polish(const rr_zscan_desc *d, const rr_polish_desc *p, void *stream) {
