@@ -7,13 +7,16 @@ One eval = one (target, template, z) chi2 including its coefficient solve.
 A step is one full scan of all targets over all redshifts with every input
 already resident in HBM (template cache built once, as DistTemplate does).
 
-The headline value is the split-fp16 scan (f16x3: fp16 hi + lo operands on
-MFMA, fp32 partial sums added in FP64, FP64 Cholesky), the mixed mode whose
-chi2 stays within 1e-5 relative of FP64; the plain FP64 scan runs on the same
-inputs for the speedup and the best-z agreement, and the single-pass fp16 /
-bf16 / fp32-MFMA modes are reported beside it as variants.  The CPU baseline is the float64 oracle (a numpy restatement of
+The headline value is the 'f16r' scan: fp16 normal equations on MFMA (fp32
+accumulation) for the coefficients, FP64 Cholesky, then chi2 as the
+reference's explicit residual sum w (f - R T c)^2 at those coefficients in
+FP32 with FP64 partial sums, plus the FP64 polish of the rows near each
+minimum.  The plain FP64 scan (one target per lane) runs on the same inputs
+for the speedup and the best-z agreement; the split-fp16 (f16x3) and the
+single-pass fp16 / bf16 / fp32-MFMA modes are reported beside it as
+variants.  The CPU baseline is the float64 oracle (a numpy restatement of
 the reference's CPU loop) on a bounded sample of the same targets, one
-process per core, run before the GPU is initialised.
+process per core and on one core, run before the GPU is initialised.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--ntargets 500]
 """
@@ -33,7 +36,10 @@ sys.path.insert(0, ROOT)
 METRIC = 'zchi2 evals/sec on 1 GPU: mixed-precision vs FP64 plain at same best-z'
 # reference-form FLOPs per eval (R-apply, weights, Gram, y, solve, model, chi2)
 FLOP_PER_EVAL = {'GALAXY': 3763200, 'QSO': 1137348, 'STAR': 1495337}
-PEAK_FLOPS = {'f16': 2.5e15, 'bf16': 2.5e15, 'f16x3': 2.5e15, 'f64': 78.6e12}
+PEAK_FLOPS = {'f16': 2.5e15, 'bf16': 2.5e15, 'f16x3': 2.5e15, 'f16r': 2.5e15, 'f64': 78.6e12,
+              'f32': 157.3e12}
+# the GPU box allots this many host cores to a one-GPU job
+CPU_SHARE = 16
 
 
 def log(*a):
@@ -45,7 +51,7 @@ def kernel_signature(precision, nb, cfg):
     wa, wb, tw, zw = cfg
     if precision == 'f16x3':
         return f'void zscan_mfma3_kernel<{nb}, {wa}, {wb}, {tw}, {zw}>'
-    ty = {'f16': 1, 'bf16': 2}[precision]
+    ty = {'f16': 1, 'bf16': 2, 'f16r': 1}[precision]
     return f'void zscan_mfma_kernel<{nb}, {ty}, {wa}, {wb}, {tw}, {zw}>'
 
 
@@ -75,11 +81,22 @@ def _cpu_one(i):
     return i
 
 
-def cpu_baseline(tm, targets, dwave, seconds):
-    """Oracle scan on host cores: evals/s over a bounded sample of targets."""
+def _cpu_pool_run(ncores, nsample):
     import multiprocessing as mp
+    ctx = mp.get_context('fork')
+    with ctx.Pool(ncores) as pool:
+        t0 = time.perf_counter()
+        list(pool.imap_unordered(_cpu_one, range(nsample)))
+        return time.perf_counter() - t0
+
+
+def cpu_baseline(tm, targets, dwave, seconds):
+    """Oracle scan on host cores: evals/s over a bounded sample of targets, one
+    single-threaded process per core (targets split between them, as
+    rrdesi --mp N, zscan.py:1037-1094), and on one core."""
     from oracle import rr_oracle as orc
-    ncores = min(16, len(os.sched_getaffinity(0)))
+    visible = len(os.sched_getaffinity(0))
+    ncores = min(CPU_SHARE, visible)
     # the QSO workload's IGM: Lyman-alpha only, tau = 0.0036 (1+z_abs)^3.46
     lines = [(1215.67, 0.0036, 3.46)] if tm.template_type == 'QSO' else None
     tdata = orc.template_cache(tm, dwave, lines=lines)
@@ -87,18 +104,23 @@ def cpu_baseline(tm, targets, dwave, seconds):
     t0 = time.perf_counter()
     _cpu_one(0)
     one = time.perf_counter() - t0
+    nz = len(tm.redshifts)
+    # one core: a few targets in this process
+    n1 = max(1, min(len(targets), int(0.4 * seconds / max(one, 1e-3))))
+    t0 = time.perf_counter()
+    for i in range(n1):
+        _cpu_one(i)
+    wall1 = time.perf_counter() - t0
     nsample = max(ncores, min(len(targets), int(seconds * ncores / max(one, 1e-3))))
     nsample = min(nsample, len(targets))
-    ctx = mp.get_context('fork')
-    with ctx.Pool(ncores) as pool:
-        t0 = time.perf_counter()
-        list(pool.imap_unordered(_cpu_one, range(nsample)))
-        wall = time.perf_counter() - t0
-    nz = len(tm.redshifts)
+    wall = _cpu_pool_run(ncores, nsample)
     return dict(value=nsample * nz / wall, unit='evals/s', cores=ncores, kind='port',
+                value_1core=n1 * nz / wall1, host_cores_visible=visible,
                 sample=f'{nsample} of the workload targets x {nz} z, float64 oracle '
                        f'(numpy CSR R-apply, normal equations, LAPACK gesv, explicit residual), '
-                       f'one process per core, {wall:.1f} s wall')
+                       f'one process per core on {ncores} cores (the host share of a one-GPU job; '
+                       f'{visible} cores visible), {wall:.1f} s wall; 1 core: {n1} targets, '
+                       f'{wall1:.1f} s')
 
 
 def calibrate_copy(mib):
@@ -302,7 +324,7 @@ def cpu_baseline_archetype(arch, targets, z, prior, seconds):
     """Oracle archetype fit (scipy lsq_linear BVLS per archetype, as the
     reference) on host cores: archetype fits/s over a bounded sample of pairs."""
     import multiprocessing as mp
-    ncores = min(16, len(os.sched_getaffinity(0)))
+    ncores = min(CPU_SHARE, len(os.sched_getaffinity(0)))
     _CPU.update(arch=arch, targets=targets, z=z, prior=prior)
     t0 = time.perf_counter()
     _cpu_arch_one(0)
@@ -480,12 +502,14 @@ def main():
     ap.add_argument('--steps', type=int, default=10)
     ap.add_argument('--warmup', type=int, default=2)
     ap.add_argument('--ntargets', type=int, default=500)
-    ap.add_argument('--precision', default='f16x3', choices=('f16', 'bf16', 'f16x3'))
+    ap.add_argument('--precision', default='f16r', choices=('f16r', 'f16x3', 'f16', 'bf16'))
     ap.add_argument('--snr', type=float, default=1.0)
     ap.add_argument('--seed', type=int, default=0)
     ap.add_argument('--cpu-seconds', type=float, default=12.0)
     ap.add_argument('--no-cpu', action='store_true')
-    ap.add_argument('--variants', default='f16,bf16,f32',
+    ap.add_argument('--snr-sweep', default='3,10',
+                    help='per-pixel S/N of extra target sets whose chi2 error against FP64 is reported')
+    ap.add_argument('--variants', default='f16x3,f16,bf16,f32',
                     help='other mixed precisions to time and compare (comma separated)')
     ap.add_argument('--tile', type=int, default=-1,
                     help='tile configuration of the mixed kernel (-1: autotune)')
@@ -524,12 +548,23 @@ def main():
     if args.workload == 'zfind':
         return bench_zfind(args)
 
+    return bench_petal(args)
+
+
+def _petal_targets(tm, n, seed, snr):
+    from redrock_amd import synth
+    return synth.make_targets(tm, n, seed=seed, snr=snr)
+
+
+def bench_petal(args):
+    """The galaxy petal (headline) or the QSO petal: every target x every
+    redshift of one template, inputs resident in HBM."""
     import numpy as np
     from redrock_amd import synth
 
     tm = synth.galaxy_template() if args.workload == 'galaxy' else synth.qso_template()
     kind = tm.template_type
-    targets, ztrue = synth.make_targets(tm, args.ntargets, seed=args.seed, snr=args.snr)
+    targets, ztrue = _petal_targets(tm, args.ntargets, args.seed, args.snr)
     dwave = {s.wavehash: s.wave for s in targets[0].spectra}
     nt, nz, nb = len(targets), len(tm.redshifts), tm.nbasis
     evals = nt * nz
@@ -538,10 +573,11 @@ def main():
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(tm, targets, dwave, args.cpu_seconds)
-        log(f'cpu baseline: {cpu["value"]:.0f} evals/s on {cpu["cores"]} cores')
+        log(f'cpu baseline: {cpu["value"]:.0f} evals/s on {cpu["cores"]} cores, '
+            f'{cpu["value_1core"]:.0f} on 1 core')
 
     import torch
-    from redrock_amd import zscan, _hip
+    from redrock_amd import zscan, _hip, autotune
     from redrock_amd.templates import DistTemplate
     assert torch.cuda.is_available(), 'bench.py needs a GPU'
 
@@ -558,26 +594,25 @@ def main():
     setup_s = time.perf_counter() - t0
     log(f'setup (template caches, target upload, packing): {setup_s:.2f} s')
 
-    def timed(plan, k):
+    def timed(plan, k, stages=0):
         torch.cuda.synchronize()
         t = time.perf_counter()
         for _ in range(k):
-            plan.run()
+            plan.run(stages=stages)
         torch.cuda.synchronize()
         return (time.perf_counter() - t) / k
 
-    def kernel_ms(plan, k):
-        """Average duration of the scan kernel alone (HIP events on its stream)."""
+    def kernel_ms(plan, k, stage=_hip.RR_STAGE_MAIN):
+        """Average duration of one stage's kernel (HIP events on its stream)."""
         evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
                for _ in range(k)]
         for e0, e1 in evs:
             e0.record()
-            plan.run(stages=_hip.RR_STAGE_MAIN)
+            plan.run(stages=stage)
             e1.record()
         torch.cuda.synchronize()
         return sum(e0.elapsed_time(e1) for e0, e1 in evs) / k
 
-    from redrock_amd import autotune
     mix = plans[args.precision]
     table = autotune.tile_table(args.precision)
     if args.tile < 0:
@@ -593,21 +628,23 @@ def main():
     torch.cuda.synchronize()
     t_mix = timed(mix, args.steps)
     k_mix = kernel_ms(mix, args.steps)
+    k_resid = kernel_ms(mix, args.steps, _hip.RR_STAGE_RESID) if args.precision == 'f16r' else None
     f64 = plans['f64']
     t_f64 = timed(f64, max(1, args.steps // 2))
     k_f64 = kernel_ms(f64, max(1, args.steps // 2))
+    # the redshift-per-lane FP64 kernel (the FP64 scan before the lane-per-target one)
+    old = zscan.ScanPlan(dt.cache, batch, *items, precision='f64', oiisum=oii, lanes=False)
+    old.run()
+    k_old = kernel_ms(old, 2)
+    torch.cuda.synchronize()
+    del old
     polish = None
     c_raw = None
     if mix.polish is not None:
         # the same plan without its polish: time and accuracy of the bare scan,
         # then the polish stage alone on the bare scan's rows (HIP events)
-        bare = _hip.RR_STAGE_STATS | _hip.RR_STAGE_MAIN | _hip.RR_STAGE_FINISH
-        torch.cuda.synchronize()
-        t = time.perf_counter()
-        for _ in range(args.steps):
-            mix.run(stages=bare)
-        torch.cuda.synchronize()
-        t_raw = (time.perf_counter() - t) / args.steps
+        bare = _hip.RR_STAGE_STATS | _hip.RR_STAGE_MAIN | _hip.RR_STAGE_FINISH | _hip.RR_STAGE_RESID
+        t_raw = timed(mix, args.steps, bare)
         c_raw = mix.zchi2.reshape(nt, nz).cpu().numpy()
         evs = []
         for _ in range(args.steps):
@@ -624,12 +661,26 @@ def main():
                   'windows_per_target': float(mix.nwindows.float().mean()),
                   'targets_polished': int((mix.nwindows > 0).sum()),
                   'unpolished_ms_per_step': t_raw * 1e3, 'unpolished_value': evals / t_raw}
+    # the per-batch packing (R tiles, lane pack) the step leaves out, and the
+    # copy of the results to host numpy arrays (the reference returns numpy)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for _ in range(args.steps):
+        batch._mixed.clear()
+        batch.mixed(args.precision)
+        if args.precision == 'f16r':
+            batch.lanes('f16r')
+    torch.cuda.synchronize()
+    t_pack = (time.perf_counter() - t) / args.steps
     # final full runs so the outputs hold complete results
     mix.run()
     f64.run()
     torch.cuda.synchronize()
+    t = time.perf_counter()
+    host = mix.host()
+    t_d2h = time.perf_counter() - t
 
-    c_mix = mix.zchi2.reshape(nt, nz).cpu().numpy()
+    c_mix = host[0].reshape(nt, nz)
     c_f64 = f64.zchi2.reshape(nt, nz).cpu().numpy()
     # f'Wf per target: the scale of a mixed chi2's error (the sum chi2 cancels against)
     fwf = (batch.wf * batch.f).sum(dim=1).cpu().numpy()
@@ -649,8 +700,10 @@ def main():
     rel_best = rel[np.arange(nt), bz_f64]
     zgrid = tm.redshifts
     zerr_ok = float(np.mean(np.abs(zgrid[bz_f64] - ztrue) < 5e-3 * (1 + ztrue)))
-    log(f'{args.precision}: {t_mix*1e3:.2f} ms/step (kernel {k_mix:.2f} ms); '
-        f'f64: {t_f64*1e3:.2f} ms/step (kernel {k_f64:.2f} ms)')
+    log(f'{args.precision}: {t_mix*1e3:.2f} ms/step (normal equations {k_mix:.2f} ms'
+        + (f', residual {k_resid:.2f} ms' if k_resid is not None else '') + f'); '
+        f'f64: {t_f64*1e3:.2f} ms/step (kernel {k_f64:.2f} ms; redshift-per-lane kernel {k_old:.2f} ms); '
+        f'packing {t_pack*1e3:.2f} ms, results to host {t_d2h*1e3:.1f} ms')
     log(f'best-z agreement {agree:.4f}; max rel chi2 err {rel.max():.2e}, at best z {rel_best.max():.2e}; '
         f'f64 best z within 5e-3(1+z) of truth: {zerr_ok:.3f}')
 
@@ -670,14 +723,41 @@ def main():
                        'best_z_agreement': float(np.mean(cv.argmin(axis=1) == bz_f64)),
                        'max_rel_chi2_err': float(rv.max()),
                        'median_rel_chi2_err': float(np.median(rv)),
+                       'max_chi2_err_over_fwf': float((np.abs(cv - c_f64) / fwf[:, None]).max()),
                        'max_rel_chi2_err_at_best_z': float(rv[np.arange(nt), bz_f64].max())}
         log(f'{p}: {tv*1e3:.2f} ms/step, max rel err {rv.max():.2e}, median {np.median(rv):.2e}, '
             f'best-z agreement {variants[p]["best_z_agreement"]:.4f}')
         del plan
 
+    # error against FP64 at higher per-pixel S/N (same template caches)
+    sweep = []
+    for snr in [float(v) for v in args.snr_sweep.split(',') if v]:
+        tg2, _ = _petal_targets(tm, nt, args.seed + 7, snr)
+        b2 = zscan.TargetBatch(tg2)
+        res2 = {}
+        for p in (args.precision, 'f16x3', 'f64'):
+            pl = zscan.ScanPlan(dt.cache, b2, *items, precision=p, oiisum=oii,
+                                polish=(p != 'f64' and not args.no_polish))
+            pl.run()
+            res2[p] = pl.zchi2.reshape(nt, nz).cpu().numpy()
+            del pl
+        fwf2 = (b2.wf * b2.f).sum(dim=1).cpu().numpy()
+        row = {'snr_per_pixel': snr}
+        for p in (args.precision, 'f16x3'):
+            r2 = np.abs(res2[p] - res2['f64']) / np.abs(res2['f64'])
+            row[p] = {'max_rel_chi2_err': float(r2.max()),
+                      'max_chi2_err_over_fwf': float((np.abs(res2[p] - res2['f64']) / fwf2[:, None]).max()),
+                      'best_z_agreement': float(np.mean(res2[p].argmin(axis=1) == res2['f64'].argmin(axis=1)))}
+        sweep.append(row)
+        log(f'S/N {snr:g}: ' + '; '.join(f'{p} max rel {row[p]["max_rel_chi2_err"]:.2e}, best-z '
+                                        f'{row[p]["best_z_agreement"]:.4f}' for p in (args.precision, 'f16x3')))
+        del b2
+
     flops = FLOP_PER_EVAL[kind] * evals
     sig = kernel_signature(args.precision, nb, table[tile])
     ach = flops / (k_mix * 1e-3)
+    npix = batch.layout.npix
+    resid_flop = 2 * npix * (nb + 11 + 1)          # u = T c, R-apply, square-accumulate per eval
     out = {
         'metric': METRIC,
         'value': evals / t_mix,
@@ -689,31 +769,47 @@ def main():
         'higher_is_better': True,
         'scaling': 'weak',
         'vs_baseline': None,
-        'dtype': args.precision,
+        'dtype': 'f16+f32' if args.precision == 'f16r' else args.precision,
         'data': 'synthetic',
         'config': {'workload': (f'galaxy petal: {nt} GALAXY targets x {nz} z x {nb} basis, '
                                 if kind == 'GALAXY' else
                                 f'{nt} QSO targets x {nz} z x {nb} basis, Lyman-alpha IGM fused, ')
                                + '3 cameras 7958 px, 11-diagonal R',
                    'targets': nt, 'redshifts': nz, 'nbasis': nb, 'snr_per_pixel': args.snr,
+                   'precision': args.precision,
                    'solve': 'FP64 Cholesky',
+                   'chi2': ('explicit residual sum w (f - R T c)^2 at the fp16 coefficients, FP32 with '
+                            'FP64 partial sums' if args.precision == 'f16r' else
+                            "compensated FP64 f'Wf - 2c'y + c'Mc"),
                    'polish': (None if args.no_polish else
                               f'FP64 re-evaluation of the {_hip.RR_POLISH_WINDOW}-redshift windows within '
-                              f'{zscan.polish_tol(args.precision, True):g} relative of each '
-                              f'target minimum (in the timed step)'),
+                              f'{zscan.polish_tol(args.precision, True):g} of each target minimum '
+                              f'(relative to max(f\'Wf, |min|), in the timed step)'),
                    'accumulate': ('fp32 MFMA partial sums over 64 pixels, added in FP64'
                                   if args.precision == 'f16x3' else 'fp32 (MFMA), FP64 after the scan'),
                    'tile': dict(zip(('waves_items', 'waves_z', 'items_per_wave', 'z_per_wave'),
                                     table[tile]))},
         'autotune_ms': {'x'.join(map(str, table[t])): ms for t, ms in tile_ms.items()},
         'fp64': {'value': evals / t_f64, 'ms_per_step': t_f64 * 1e3, 'kernel_ms': k_f64,
-                 'roofline_frac': FLOP_PER_EVAL[kind] * evals / (k_f64 * 1e-3) / PEAK_FLOPS['f64']},
+                 'kernel': f'void zscan_f64r_kernel<{nb}, {12 if nb <= 10 else 8}>',
+                 'roofline_frac': flops / (k_f64 * 1e-3) / PEAK_FLOPS['f64'],
+                 'redshift_per_lane_kernel_ms': k_old,
+                 'redshift_per_lane_roofline_frac': flops / (k_old * 1e-3) / PEAK_FLOPS['f64']},
         'speedup_vs_fp64': t_f64 / t_mix,
         'best_z_agreement': agree,
         'max_rel_chi2_err': float(rel.max()),
         'max_rel_chi2_err_at_best_z': float(rel_best.max()),
         'max_chi2_err_over_fwf': float((np.abs(c_mix - c_f64) / fwf[:, None]).max()),
+        'median_rel_chi2_err': float(np.median(rel)),
+        'fp64_best_z_near_truth': zerr_ok,
         'setup_s': setup_s,
+        'step_with_packing': {'pack_ms': t_pack * 1e3, 'ms_per_step': (t_mix + t_pack) * 1e3,
+                              'value': evals / (t_mix + t_pack)},
+        'end_to_end': {'results_to_host_ms': t_d2h * 1e3,
+                       'ms_per_petal': (t_mix + t_pack + t_d2h) * 1e3,
+                       'value': evals / (t_mix + t_pack + t_d2h),
+                       'note': 'packing + scan + copy of zchi2, zcoeff, penalty, status to host numpy; '
+                               'template caches and target upload (setup_s) not included'},
         'roofline': {'bound': 'mfma', 'achieved': ach / 1e12, 'peak': PEAK_FLOPS[args.precision] / 1e12,
                      'unit': 'TFLOP/s', 'frac': ach / PEAK_FLOPS[args.precision],
                      'traffic': (measured_counter('traffic_bytes_per_launch', sig)
@@ -724,13 +820,18 @@ def main():
                      'kernel': sig,
                      'kernel_ms': k_mix,
                      'flop_per_eval': FLOP_PER_EVAL[kind], 'evals_per_launch': evals},
-        'median_rel_chi2_err': float(np.median(rel)),
+        'roofline_residual': (None if k_resid is None else {
+            'bound': 'fp32 vector ALU', 'kernel': f'zscan_resid_kernel<{nb}, ...>', 'kernel_ms': k_resid,
+            'flop_per_eval': resid_flop, 'achieved': resid_flop * evals / (k_resid * 1e-3) / 1e12,
+            'peak': PEAK_FLOPS['f32'] / 1e12, 'unit': 'TFLOP/s',
+            'frac': resid_flop * evals / (k_resid * 1e-3) / PEAK_FLOPS['f32'],
+            'note': 'peak counts packed FP32; the kernel issues unpacked FMAs (half that rate)'}),
         'polish': polish,
         'variants': variants,
+        'snr_sweep': sweep,
         'cpu_baseline': cpu,
     }
     print(json.dumps(out), flush=True)
-
 
 if __name__ == '__main__':
     main()

@@ -320,7 +320,9 @@ int launch_f64r_nw(const rr::ZArgs &a, hipStream_t s) {
 
 template <int NB>
 int launch_f64r(const rr::ZArgs &a, hipStream_t s) {
-    int nw = (a.tile > 0 && a.tile < kNumF64Tiles) ? kF64Tiles[a.tile] : (NB <= 6 ? 12 : 8);
+    // default: 3 waves per SIMD (a few spilled registers at nb 7..10 cost less
+    // than dropping to 2 waves); 2 waves from nb 11 on (the spills grow)
+    int nw = (a.tile > 0 && a.tile < kNumF64Tiles) ? kF64Tiles[a.tile] : (NB <= 10 ? 12 : 8);
     if (nw == 12) return launch_f64r_nw<NB, 12>(a, s);
     if (nw == 8) return launch_f64r_nw<NB, 8>(a, s);
     return launch_f64r_nw<NB, 8>(a, s);
