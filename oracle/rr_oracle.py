@@ -57,6 +57,10 @@ per_camera_coeff           zscan.py:245-348 over the pinned rebin, IGM and
                            (test_archetypes.py:31-57).  The nearest-
                            neighbour form without per_camera: parity
                            unpinned (the reference stops there).
+masked_ivar                hand-built resolution matrices with known row sums
+                           (just above, at and below 0.99), and the truncated
+                           edge rows of the reference's test/util.py recipe
+                           (tests/test_oracle.py)
 zfind_assemble             restatement of zfind.py:467-714; its pieces
                            (sort, delta chi2) pinned above, the nminima trim
                            and NODATA / LITTLE_COVERAGE bits by hand-built
@@ -283,6 +287,22 @@ def template_cache(template, dwave, myz=None, lines=None):
             continue
         data[k] *= T[:, :, None]
     return data
+
+
+# ---------------------------------------------------------------------------
+# Spectrum (targets.py)
+# ---------------------------------------------------------------------------
+MIN_RESOLUTION_INTEGRAL = 0.99              # constants.py:9
+
+
+def masked_ivar(R, ivar):
+    """targets.py:32-35: a copy of ivar, zero where the row of the resolution
+    matrix R (its integral over the output pixel's line spread) sums to less
+    than 0.99 -- pixels whose LSF leaves the camera."""
+    rows = np.asarray(scipy.sparse.csr_matrix(R).sum(axis=1)).ravel()
+    out = np.array(ivar, dtype=np.float64, copy=True)
+    out[rows < MIN_RESOLUTION_INTEGRAL] = 0.0
+    return out
 
 
 # ---------------------------------------------------------------------------

@@ -161,36 +161,84 @@ def chi2_at_redshift(z, flux, problem):
     return _FN.apply(z, flux, problem)
 
 
-def newton_refine(problem, z0, zlo, zhi, iters=4, dz_fd=1e-6):
-    """Newton iterations on dchi2/dz = 0 for every item, inside [zlo, zhi].
-
-    The curvature is the central difference of the analytic gradient.  A step
-    that would leave the bracket or meets non-positive curvature is halved
-    toward the lower-chi2 side.  Returns (z, chi2, dchi2/dz) as numpy arrays.
-    """
+def _subproblem(problem, item_target):
+    """The problem's targets and template with another item list (no new uploads)."""
     import torch
-    z = np.array(z0, dtype=np.float64)
+    sub = GradProblem.__new__(GradProblem)
+    sub.__dict__.update(problem.__dict__)
+    sub.item_target = np.asarray(item_target, dtype=np.int32)
+    sub.t_item_target = torch.as_tensor(sub.item_target, device='cuda')
+    return sub
+
+
+def newton_refine(problem, z0, zlo, zhi, iters=10, dz_fd=1e-6, rtol=1e-11, zref=None, chi2ref=None,
+                  info=None):
+    """Safeguarded Newton iterations on dchi2/dz = 0 for every item, inside [zlo, zhi].
+
+    One evaluation at z runs the FP64 forward at (z - h, z, z + h) -- chi2(z)
+    is the middle point -- and the adjoint gradient at the three; the
+    curvature is the central difference of the analytic gradient.  From the
+    current point the step is -g/H when H > 0, else a quarter of the bracket
+    downhill, clipped to the bracket.  A trial point whose chi2 exceeds the
+    current one is rejected and the step halved (backtracking); an accepted
+    trial becomes the current point, so chi2 never rises above chi2(z0).  An
+    item stops when the predicted decrease |g step| / 2 falls below
+    rtol * chi2, when its step falls below 1e-12 (1 + z), or after `iters`
+    evaluations.
+
+    zref, chi2ref: an optional reference point per item (the parabola fit,
+    fitz.py:259-261); an item whose final chi2 exceeds chi2ref returns zref
+    (its chi2 and gradient evaluated there).  info: a dict that receives
+    'converged' (bool per item), 'evaluations' and 'kept_reference' (bool
+    per item).  Returns (z, chi2, dchi2/dz) as numpy arrays.
+    """
+    z = np.clip(np.array(z0, dtype=np.float64), zlo, zhi)
     zlo = np.asarray(zlo, dtype=np.float64)
     zhi = np.asarray(zhi, dtype=np.float64)
     n = len(z)
-    # (z - h, z, z + h) of an item are consecutive, so a target's points form
-    # one multi-redshift scan item
-    sub = GradProblem.__new__(GradProblem)
-    sub.__dict__.update(problem.__dict__)
-    sub.item_target = np.repeat(problem.item_target, 3)
-    sub.t_item_target = torch.as_tensor(sub.item_target, device='cuda')
-    for _ in range(iters):
-        zz = np.stack([z - dz_fd, z, z + dz_fd], axis=1).ravel()
+
+    def eval3(idx, zc):
+        # (z - h, z, z + h) of an item are consecutive: one multi-redshift scan item
+        sub = _subproblem(problem, np.repeat(problem.item_target[idx], 3))
+        zz = np.stack([zc - dz_fd, zc, zc + dz_fd], axis=1).ravel()
         chi2, coeff = sub.forward(zz)
         g, _, _ = sub.grad(zz, coeff)
-        g = g.cpu().numpy().reshape(n, 3)
-        gm, g0, gp = g[:, 0], g[:, 1], g[:, 2]
-        h = (gp - gm) / (2 * dz_fd)
-        step = np.where(h > 0, -g0 / np.where(h > 0, h, 1.0), -np.sign(g0) * 0.25 * (zhi - zlo))
-        znew = z + step
-        bad = (znew < zlo) | (znew > zhi)
-        znew = np.where(bad, np.clip(z + 0.5 * step, zlo, zhi), znew)
-        z = znew
-    chi2, coeff = problem.forward(z)
-    g, _, _ = problem.grad(z, coeff)
-    return z, chi2.cpu().numpy(), g.cpu().numpy()
+        g = g.cpu().numpy().reshape(-1, 3)
+        c = chi2.cpu().numpy().reshape(-1, 3)
+        return c[:, 1], g[:, 1], (g[:, 2] - g[:, 0]) / (2 * dz_fd)
+
+    def newton_step(idx):
+        gi, hi_ = g[idx], H[idx]
+        s = np.where(hi_ > 0, -gi / np.where(hi_ > 0, hi_, 1.0), -np.sign(gi) * 0.25 * (zhi[idx] - zlo[idx]))
+        return np.clip(z[idx] + s, zlo[idx], zhi[idx]) - z[idx]
+
+    def done():
+        return ((np.abs(g * step) <= 2 * rtol * np.abs(f)) | (np.abs(step) <= 1e-12 * (1 + np.abs(z)))
+                | ~np.isfinite(step))
+
+    allidx = np.arange(n)
+    f, g, H = eval3(allidx, z)
+    step = newton_step(allidx)
+    evals = 1
+    conv = done()
+    while evals < iters and not conv.all():
+        idx = np.flatnonzero(~conv)
+        zt = np.clip(z[idx] + step[idx], zlo[idx], zhi[idx])
+        ft, gt, ht = eval3(idx, zt)
+        evals += 1
+        acc = ft <= f[idx]
+        ia, ir = idx[acc], idx[~acc]
+        z[ia], f[ia], g[ia], H[ia] = zt[acc], ft[acc], gt[acc], ht[acc]
+        step[ia] = newton_step(ia)
+        step[ir] *= 0.5
+        conv = done()
+    kept = np.zeros(n, dtype=bool)
+    if zref is not None and chi2ref is not None:
+        kept = np.asarray(chi2ref, dtype=np.float64) < f
+        if kept.any():
+            ik = np.flatnonzero(kept)
+            z[ik] = np.asarray(zref, dtype=np.float64)[ik]
+            f[ik], g[ik], _ = eval3(ik, z[ik])
+    if info is not None:
+        info.update(converged=conv.copy(), evaluations=evals, kept_reference=kept)
+    return z, f, g

@@ -544,10 +544,22 @@ class ScanPlan(object):
             _hip.check(rc, 'rr_zscan_polish')
         return self
 
-    def host(self):
-        """(zchi2, zcoeff, penalty, status) copied to numpy."""
-        return (self.zchi2.cpu().numpy(), self.zcoeff.cpu().numpy(),
-                self.penalty.cpu().numpy(), self.status.cpu().numpy())
+    def host(self, stream=None):
+        """(zchi2, zcoeff, penalty, status) copied to numpy.
+
+        The arrays live in page-locked host memory (torch's pinned allocator),
+        fresh for each call, filled by asynchronous copies on `stream`
+        (default: the current stream) that this call waits for.
+        """
+        import torch
+        outs = (self.zchi2, self.zcoeff, self.penalty, self.status)
+        hs = [torch.empty(t.shape, dtype=t.dtype, pin_memory=True) for t in outs]
+        s = torch.cuda.current_stream() if stream is None else stream
+        with torch.cuda.stream(s):
+            for h, t in zip(hs, outs):
+                h.copy_(t, non_blocking=True)
+        s.synchronize()
+        return tuple(h.numpy() for h in hs)
 
 
 # Smallest item count for which an FP64 plan of items sharing one redshift
@@ -594,6 +606,8 @@ def scan_targets(cache, targets, precision=None, oiisum=None, batches=None, poli
                         oiisum=oiisum, polish=polish, solve=solve or _method(cache))
         plan.run()
         c2, cf, pen, _ = plan.host()
+        if len(groups) == 1:                 # one layout: the plan's rows are the targets in order
+            return c2.reshape(n, nz), cf.reshape(n, nz, nb), pen.reshape(n, nz)
         zchi2[idx] = c2.reshape(n, nz)
         zcoeff[idx] = cf.reshape(n, nz, nb)
         penalty[idx] = pen.reshape(n, nz)

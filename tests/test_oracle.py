@@ -364,3 +364,35 @@ def test_oracle_nnls_in_the_scan_clamps_coefficients():
                                     algo='NMF')
     assert np.all(cf >= 0)
     assert np.all(c_nn >= c_pca * (1 - 1e-12))
+
+
+def test_masked_ivar_known_row_sums():
+    """targets.py:33-35 on resolution matrices whose row sums are known by
+    construction: rows below 0.99 (not at it) lose their ivar, in the oracle
+    and in the product's Spectrum alike."""
+    import scipy.sparse
+    from redrock_amd.targets import Spectrum
+    diag = np.array([1.0, 0.995, 0.99, 0.9899, 0.5, 1.2])
+    R = scipy.sparse.dia_matrix((diag[None, :], [0]), shape=(6, 6))
+    ivar = np.arange(1.0, 7.0)
+    want = ivar.copy()
+    want[[3, 4]] = 0.0
+    assert np.array_equal(orc.masked_ivar(R, ivar), want)
+    s = Spectrum(np.linspace(4000, 4005, 6), np.zeros(6), ivar.copy(), R)
+    assert np.array_equal(s.ivar, want)
+
+
+def test_masked_ivar_truncated_edges():
+    """The test recipe's Gaussian (sigma 2 px unsquared) loses 3.5 % of its
+    mass in the third row from each edge and 0.6 % in the fourth: rows 0-2 and
+    n-3..n-1 are masked, none else."""
+    from tests import refsim
+    from redrock_amd.targets import Spectrum
+    n = 100
+    R = refsim.resolution(n)
+    ivar = np.ones(n)
+    want = np.ones(n)
+    want[[0, 1, 2, n - 3, n - 2, n - 1]] = 0.0
+    assert np.array_equal(orc.masked_ivar(R, ivar), want)
+    s = Spectrum(np.arange(4000.0, 4000.0 + 5 * n, 5.0), np.zeros(n), ivar.copy(), R)
+    assert np.array_equal(s.ivar, want)
