@@ -171,7 +171,7 @@ def _subproblem(problem, item_target):
     return sub
 
 
-def newton_refine(problem, z0, zlo, zhi, iters=10, dz_fd=1e-6, rtol=1e-11, zref=None, chi2ref=None,
+def newton_refine(problem, z0, zlo, zhi, iters=12, dz_fd=1e-6, rtol=1e-15, zref=None, chi2ref=None,
                   info=None):
     """Safeguarded Newton iterations on dchi2/dz = 0 for every item, inside [zlo, zhi].
 

@@ -288,30 +288,34 @@ class TargetBatch(object):
 
     def __init__(self, targets=None, layout=None, arrays=None):
         import torch
+        dev = torch.device('cuda')
         if arrays is not None:
             layout, rband, w, f, wf = arrays
+            self.rband = torch.as_tensor(np.ascontiguousarray(rband), dtype=torch.float64, device=dev)
+            self.w = torch.as_tensor(np.ascontiguousarray(w), dtype=torch.float64, device=dev)
+            self.f = torch.as_tensor(np.ascontiguousarray(f), dtype=torch.float64, device=dev)
+            self.wf = torch.as_tensor(np.ascontiguousarray(wf), dtype=torch.float64, device=dev)
         else:
             layout = layout or PixelLayout.of(targets[0].spectra)
             nt, P = len(targets), layout.npix_pad
-            rband = np.zeros((nt, P, 11))
-            w = np.zeros((nt, P))
-            f = np.zeros((nt, P))
-            for i, tg in enumerate(targets):
+            for tg in targets:
                 if PixelLayout.of(tg.spectra).signature != layout.signature:
                     raise ValueError('targets of one batch must share their wavelength grids')
-                for s, off in zip(tg.spectra, layout.offsets):
-                    n = s.nwave
-                    rband[i, off:off + n] = s.rband()
-                    w[i, off:off + n] = s.ivar
-                    f[i, off:off + n] = s.flux
-            wf = w * f
+            self.rband = torch.zeros((nt, P, 11), dtype=torch.float64, device=dev)
+            w = np.zeros((nt, P))
+            f = np.zeros((nt, P))
+            for k, off in enumerate(layout.offsets):
+                spectra = [tg.spectra[k] for tg in targets]
+                n = spectra[0].nwave
+                _upload_bands(self.rband[:, off:off + n], spectra)
+                for i, sp in enumerate(spectra):
+                    w[i, off:off + n] = sp.ivar
+                    f[i, off:off + n] = sp.flux
+            self.w = torch.from_numpy(w).to(dev)
+            self.f = torch.from_numpy(f).to(dev)
+            self.wf = self.w * self.f
         self.layout = layout
-        self.nt = rband.shape[0]
-        dev = torch.device('cuda')
-        self.rband = torch.as_tensor(np.ascontiguousarray(rband), dtype=torch.float64, device=dev)
-        self.w = torch.as_tensor(np.ascontiguousarray(w), dtype=torch.float64, device=dev)
-        self.f = torch.as_tensor(np.ascontiguousarray(f), dtype=torch.float64, device=dev)
-        self.wf = torch.as_tensor(np.ascontiguousarray(wf), dtype=torch.float64, device=dev)
+        self.nt = self.rband.shape[0]
         self._mixed = dict()
 
     def lanes(self, precision, order=None):
@@ -358,6 +362,51 @@ class TargetBatch(object):
             _hip.check(rc, 'rr_pack_mixed')
             self._mixed[precision] = (rtile, sfcol, scales)
         return self._mixed[precision]
+
+
+def _upload_bands(out, spectra):
+    """Row bands (nt, n, 11) of one camera's resolution matrices into the device
+    view `out` (synchronous host-to-device copies from pageable staging).
+
+    Spectra whose R are dia_matrix with the same offsets and data shape (the
+    reference's layout, every DESI coadd) are uploaded as their diagonal data
+    and turned into row bands on the GPU, one slice per diagonal for all
+    targets: row p of the diagonal at offset `off` holds R[p, p + off] =
+    data[d, p + off] (column-indexed).  Others go one by one through
+    Spectrum.rband() on the host.
+    """
+    import scipy.sparse
+    import torch
+    from .targets import RBAND_HALF
+    Rs = [s.R for s in spectra]
+    first = Rs[0]
+    same = (all(scipy.sparse.issparse(R) and R.format == 'dia' for R in Rs)
+            and all(R.shape == first.shape and R.data.shape == first.data.shape
+                    and np.array_equal(R.offsets, first.offsets) for R in Rs))
+    nt, n = out.shape[0], out.shape[1]
+    if not same:
+        hn = np.empty((nt, n, 11))
+        for i, s in enumerate(spectra):
+            hn[i] = s.rband()
+        out.copy_(torch.from_numpy(hn))
+        return
+    nd, L = first.data.shape
+    hn = np.empty((nt, nd, L))
+    for i, R in enumerate(Rs):
+        hn[i] = R.data
+    d = torch.from_numpy(hn).to(out.device)
+    nrow, ncol = first.shape
+    L = min(ncol, L)
+    for di, o in enumerate(np.asarray(first.offsets).astype(np.int64)):
+        p0, p1 = max(0, -o), min(n, nrow, L - o)
+        if p1 <= p0:
+            continue
+        vals = d[:, di, p0 + o:p1 + o]
+        if abs(o) > RBAND_HALF:
+            if bool(torch.any(vals != 0)):
+                raise NotImplementedError('resolution matrix wider than 11 diagonals')
+            continue
+        out[:, p0:p1, o + RBAND_HALF] += vals
 
 
 def target_runs(item_target):
