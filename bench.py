@@ -636,6 +636,10 @@ def bench_petal(args):
     else:
         tile, tile_ms = args.tile, {}
         mix.desc.tile = tile
+    rtable = autotune.resid_table() if args.precision == 'f16r' else None
+    if rtable is not None:
+        log(f'residual kernel configuration {mix.desc.resid_tile}: {rtable[mix.desc.resid_tile]} '
+            f'(items tile, waves, items per wave, redshifts per wave)')
     for p, plan in plans.items():
         for _ in range(args.warmup):
             plan.run()
@@ -802,7 +806,10 @@ def bench_petal(args):
                    'accumulate': ('fp32 MFMA partial sums over 64 pixels, added in FP64'
                                   if args.precision == 'f16x3' else 'fp32 (MFMA), FP64 after the scan'),
                    'tile': dict(zip(('waves_items', 'waves_z', 'items_per_wave', 'z_per_wave'),
-                                    table[tile]))},
+                                    table[tile])),
+                   'residual_tile': (None if rtable is None else
+                                     dict(zip(('item_tiles', 'waves', 'items_per_wave', 'z_per_wave'),
+                                              rtable[mix.desc.resid_tile])))},
         'autotune_ms': {'x'.join(map(str, table[t])): ms for t, ms in tile_ms.items()},
         'fp64': {'value': evals / t_f64, 'ms_per_step': t_f64 * 1e3, 'kernel_ms': k_f64,
                  'kernel': f'void zscan_f64r_kernel<{nb}, {12 if nb <= 10 else 8}>',

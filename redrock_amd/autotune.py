@@ -1,11 +1,12 @@
-"""Tile-shape autotuner for the mixed-precision scan kernel.
+"""Tile-shape autotuner for the scan kernels.
 
-The kernel has several tile configurations (items x redshifts per
-wavefront, rr_zscan_tiles).  tune() times every configuration on a prepared
-ScanPlan — the main stage only, HIP events on the launch stream — and returns
-the fastest; choose() caches the winner per (nbasis, padded pixels,
-precision, item count bucket, uniform z) in a JSON file so later plans of the
-same shape reuse it without timing.
+The kernels have several tile configurations (items x redshifts per
+wavefront and workgroup, rr_zscan_tile_table; the explicit-residual kernel of
+'f16r', rr_zscan_resid_tiles).  tune() times every configuration on a
+prepared ScanPlan — the kernel's stage only, HIP events on the launch stream —
+and returns the fastest; choose() caches the winners per (nbasis, padded
+pixels, precision, item count bucket, uniform z) in a JSON file so later
+plans of the same shape reuse them without timing.
 
 Cache file: $RR_TUNE_CACHE, default ~/.cache/redrock_amd/tiles.json.
 """
@@ -33,6 +34,47 @@ def tile_table(precision=_hip.RR_PREC_F16):
     if k < 0:
         _hip.check(k, 'rr_zscan_tile_table')
     return [tuple(cfg[4 * i:4 * i + 4]) for i in range(min(k, n))]
+
+
+def resid_table():
+    """[(1, waves, 64, z_per_wave), ...] indexed by rr_zscan_desc.resid_tile
+    (the explicit-residual kernel of precision 'f16r'; entry 0 = default)."""
+    n = 32
+    cfg = (ctypes.c_int * (4 * n))()
+    k = _hip.lib().rr_zscan_resid_tiles(cfg, n)
+    if k < 0:
+        _hip.check(k, 'rr_zscan_resid_tiles')
+    return [tuple(cfg[4 * i:4 * i + 4]) for i in range(min(k, n))]
+
+
+def _time_stage(plan, stage, reps):
+    import torch
+    plan.run(stages=stage)   # warm-up (code load, caches)
+    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+           for _ in range(reps)]
+    for e0, e1 in evs:
+        e0.record()
+        plan.run(stages=stage)
+        e1.record()
+    torch.cuda.synchronize()
+    return min(e0.elapsed_time(e1) for e0, e1 in evs)
+
+
+def tune_resid(plan, reps=3):
+    """Time every residual-kernel configuration of an 'f16r' plan (after a full
+    run, so the coefficients exist); returns (best, {resid_tile: ms})."""
+    table = resid_table()
+    saved = plan.desc.resid_tile
+    plan.run()
+    timings = {t: _time_stage(_with(plan, 'resid_tile', t), _hip.RR_STAGE_RESID, reps)
+               for t in range(1, len(table))}
+    plan.desc.resid_tile = saved
+    return min(timings, key=timings.get), timings
+
+
+def _with(plan, field, value):
+    setattr(plan.desc, field, value)
+    return plan
 
 
 def _cache_path():
@@ -87,12 +129,19 @@ def choose(plan, reps=3, use_cache=True):
                 cache = json.load(fh)
         except (OSError, ValueError):
             cache = dict()
-    if key in cache:
+    resid = plan.desc.precision == _hip.RR_PREC_F16R
+    if key in cache and (not resid or 'resid_tile' in cache[key]):
         plan.desc.tile = int(cache[key]['tile'])
+        if resid:
+            plan.desc.resid_tile = int(cache[key]['resid_tile'])
         return plan.desc.tile, cache[key].get('ms', {})
     best, timings = tune(plan, reps)
     plan.desc.tile = best
     cache[key] = dict(tile=best, ms={str(k): v for k, v in timings.items()})
+    if resid:
+        rbest, rtimes = tune_resid(plan, reps)
+        plan.desc.resid_tile = rbest
+        cache[key].update(resid_tile=rbest, resid_ms={str(k): v for k, v in rtimes.items()})
     try:
         os.makedirs(os.path.dirname(path), exist_ok=True)
         with open(path + '.tmp', 'w') as fh:

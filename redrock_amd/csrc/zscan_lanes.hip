@@ -149,14 +149,19 @@ __global__ __launch_bounds__(64 * NW) void zscan_f64r_kernel(rr::ZArgs a, long l
 
     const int nblk = (int)(npl / kF64Block);
     Stage::load(group, 0, lds[0], wave, lane);
-    for (int j = 0; j < nblk; ++j) {
-        asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
-        __syncthreads();
-        if (j + 1 < nblk) Stage::load(group, j + 1, lds[(j + 1) & 1], wave, lane);
-        const double *L = lds[j & 1] + lane;
-#pragma unroll 1
-        for (int s = 0; s < kF64Block; ++s) {
-            const int p = j * kF64Block + s;
+    // one loop over pixels (the block turnover inside it): a nested block loop
+    // made the register allocator rotate the Gram sums through copies
+    const double *L = lds[0] + lane;
+    for (int p = 0, s = 0, j = 0; p < (int)npl; ++p, ++s) {
+        if (s == kF64Block) s = 0;
+        if (s == 0) {
+            asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+            __syncthreads();
+            if (j + 1 < nblk) Stage::load(group, j + 1, lds[(j + 1) & 1], wave, lane);
+            L = lds[j & 1] + lane;
+            ++j;
+        }
+        {
             const double *Lp = L + s * (kSlots64 * 64);
             double r[11];
 #pragma unroll
@@ -209,13 +214,19 @@ __global__ __launch_bounds__(64 * NW) void zscan_f64r_kernel(rr::ZArgs a, long l
 // stage's HUGE_CHI2.
 // ---------------------------------------------------------------------------
 
-template <int NB, int ZW, int NW, int WPE, int PB>
+// TL: the template rows of the block's redshifts are staged in LDS as well
+// (global_load_lds_dword, 4 bytes per lane, with the lane data), so their
+// latency hides behind the previous block; otherwise they are scalar loads.
+template <int NB, int ZW, int NW, int WPE, int PB, bool TL>
 __global__ __launch_bounds__(64 * NW, WPE) void zscan_resid_kernel(rr::ZArgs a, rr::TileMap map,
                                                                const float *__restrict__ T32,
                                                                const float *__restrict__ rlane) {
     static_assert(PB % 11 == 0, "the model window turns over every 11 pixels");
     using Stage = LaneStage<float, kSlots32, PB, NW>;
     __shared__ float lds[2][Stage::kElems];
+    constexpr int TROW = PB * NB;                          // floats of one redshift's block rows
+    constexpr int TINST = TL ? (NW * ZW * TROW + 63) / 64 : 0;
+    __shared__ float tl[TL ? 2 : 1][TL ? TINST * 64 : 1];
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     int ti, tz;
@@ -255,13 +266,35 @@ __global__ __launch_bounds__(64 * NW, WPE) void zscan_resid_kernel(rr::ZArgs a, 
 #pragma unroll
     for (int s = 0; s < ZW; ++s) acc[s] = 0.0;
 
+    // the template-row copy: lane slot e of the block -> redshift e / TROW of
+    // the workgroup, float e % TROW of its rows [j PB + 10, j PB + PB + 10)
+    // (row offsets from pixel -5); slots past the last redshift repeat it
+    auto load_rows = [&](int j, float *buf) {
+        const int zw0 = tz * NW * ZW;
+        for (int i = wave; i < TINST; i += NW) {
+            int e = i * 64 + lane;
+            if (e >= NW * ZW * TROW) e = NW * ZW * TROW - 1;
+            const int zi = e / TROW, off = e - zi * TROW;
+            const int iz = zw0 + zi < nzi ? zw0 + zi : nzi - 1;
+            const float *src = T32 + ((long long)(a.item_z0[0] + iz) * rows + (a.halo - 5)) * NB +
+                               (long long)(j * PB + 10) * NB + off;
+            __builtin_amdgcn_global_load_lds((const void *)src,
+                                             (void __attribute__((address_space(3))) *)(buf + i * 64), 4, 0, 0);
+        }
+    };
+
     const int nblk = (int)(npl / PB);               // the pack holds a multiple of 22 pixels
     Stage::load(group, 0, lds[0], wave, lane);
+    if constexpr (TL) load_rows(0, tl[0]);
     for (int j = 0; j < nblk; ++j) {
         asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
         __syncthreads();
-        if (j + 1 < nblk) Stage::load(group, j + 1, lds[(j + 1) & 1], wave, lane);
+        if (j + 1 < nblk) {
+            Stage::load(group, j + 1, lds[(j + 1) & 1], wave, lane);
+            if constexpr (TL) load_rows(j + 1, tl[(j + 1) & 1]);
+        }
         const float *L = lds[j & 1] + lane;
+        const float *TLb = tl[TL ? (j & 1) : 0] + (wave * ZW) * TROW;
         const long long p0 = (long long)j * PB;
         float part[ZW];
 #pragma unroll
@@ -276,7 +309,15 @@ __global__ __launch_bounds__(64 * NW, WPE) void zscan_resid_kernel(rr::ZArgs a, 
             const float sf = Lp[11 * 64];
 #pragma unroll
             for (int s = 0; s < ZW; ++s) {
-                U[s][(st + 10) % 11] = model(s, p + 10);           // u_{p+5}
+                if constexpr (TL) {
+                    const float *row = TLb + s * TROW + st * NB;      // row of u_{p+5}
+                    float u = row[0] * c[s][0];
+#pragma unroll
+                    for (int b = 1; b < NB; ++b) u = __builtin_fmaf(row[b], c[s][b], u);
+                    U[s][(st + 10) % 11] = u;
+                } else {
+                    U[s][(st + 10) % 11] = model(s, p + 10);       // u_{p+5}
+                }
                 float res = sf;
 #pragma unroll
                 for (int k = 0; k < 11; ++k) res = __builtin_fmaf(-r[k], U[s][(st + k) % 11], res);
@@ -334,13 +375,15 @@ int launch_f64r(const rr::ZArgs &a, hipStream_t s) {
 struct ResTile {
     int zw, nw, wpe, pb;
 };
-constexpr ResTile kResTiles[] = {{2, 16, 4, 0}, {2, 16, 4, 11}, {3, 16, 4, 11}, {2, 16, 4, 22},
-                                 {3, 16, 4, 22}, {4, 16, 4, 22}, {6, 8, 2, 22}};
+constexpr ResTile kResTiles[] = {{1, 16, 8, 0},  {2, 16, 4, 11}, {3, 16, 4, 11}, {2, 16, 4, 22},
+                                 {3, 16, 4, 22}, {4, 16, 4, 22}, {6, 8, 2, 22},  {1, 16, 8, 11},
+                                 {1, 16, 8, 22}, {1, 8, 8, 22},  {2, 8, 4, 22},  {1, 16, 8, 11},
+                                 {2, 16, 4, 11}, {3, 16, 4, 11}};
 constexpr int kNumResTiles = sizeof(kResTiles) / sizeof(kResTiles[0]);
 
-template <int NB, int ZW, int NW, int WPE, int PB>
+template <int NB, int ZW, int NW, int WPE, int PB, bool TL = false>
 int launch_resid_cfg(const rr::ZArgs &a, hipStream_t s) {
-    auto kernel = zscan_resid_kernel<NB, ZW, NW, WPE, PB>;
+    auto kernel = zscan_resid_kernel<NB, ZW, NW, WPE, PB, TL>;
     const rr::TileMap map = rr::make_tile_map((a.nitems + 63) / 64,
                                               (a.max_item_nz + NW * ZW - 1) / (NW * ZW),
                                               rr::resident_per_xcd(kernel, 64 * NW), 64.0 * kSlots32 * 4,
@@ -353,15 +396,23 @@ int launch_resid_cfg(const rr::ZArgs &a, hipStream_t s) {
 template <int NB>
 int launch_resid(const rr::ZArgs &a, hipStream_t s) {
     switch (a.resid_tile) {
-        case 0:   // default: 2 redshifts per wave; 22-pixel blocks for small bases
+        case 0:   // default: 2 redshifts per wave in 22-pixel blocks for small bases,
+                  // else one redshift per wave at 8 waves per SIMD
             if constexpr (NB <= 6) return launch_resid_cfg<NB, 2, 16, 4, 22>(a, s);
-            else return launch_resid_cfg<NB, 2, 16, 4, 11>(a, s);
+            else return launch_resid_cfg<NB, 1, 16, 8, 11>(a, s);
         case 1: return launch_resid_cfg<NB, 2, 16, 4, 11>(a, s);
         case 2: return launch_resid_cfg<NB, 3, 16, 4, 11>(a, s);
         case 3: return launch_resid_cfg<NB, 2, 16, 4, 22>(a, s);
         case 4: return launch_resid_cfg<NB, 3, 16, 4, 22>(a, s);
         case 5: return launch_resid_cfg<NB, 4, 16, 4, 22>(a, s);
         case 6: return launch_resid_cfg<NB, 6, 8, 2, 22>(a, s);
+        case 7: return launch_resid_cfg<NB, 1, 16, 8, 11>(a, s);
+        case 8: return launch_resid_cfg<NB, 1, 16, 8, 22>(a, s);
+        case 9: return launch_resid_cfg<NB, 1, 8, 8, 22>(a, s);
+        case 10: return launch_resid_cfg<NB, 2, 8, 4, 22>(a, s);
+        case 11: return launch_resid_cfg<NB, 1, 16, 8, 11, true>(a, s);
+        case 12: return launch_resid_cfg<NB, 2, 16, 4, 11, true>(a, s);
+        case 13: return launch_resid_cfg<NB, 3, 16, 4, 11, true>(a, s);
         default: return rr::fail(RR_EINVAL, "rr_zscan(residual): resid_tile %d outside 0..%d", a.resid_tile,
                                  kNumResTiles - 1);
     }
