@@ -405,10 +405,18 @@ typedef struct {
     const double *d_item_coeff; /* (nitems, nb) coefficients at that redshift       */
     double *d_dchi2_dz;         /* (nitems) out                                     */
     double *d_chi2;             /* (nitems) out, explicit FP64 residual, or NULL    */
-    double *d_dchi2_dy;         /* (nb, nx) accumulated (+=, FP64 atomics), or NULL */
+    double *d_dchi2_dy;         /* (nb, nx) accumulated (+=; items summed in item
+                                   order, no atomics), or NULL                       */
     const double *d_item_scale; /* (nitems) weight of each item's d_dchi2_dy term,
                                    or NULL for 1 (a loss sum_i s_i chi2_i)          */
+    void *d_work;               /* workspace of rr_grad_workspace_bytes() bytes
+                                   (only with d_dchi2_dy)                            */
+    long long work_bytes;
 } rr_grad_desc;
+
+/* Workspace rr_zscan_grad needs for this descriptor: 0 without d_dchi2_dy,
+ * else per item one double per pixel and one per template sample. */
+long long rr_grad_workspace_bytes(const rr_grad_desc *desc);
 
 /*
  * rr_zscan_grad — gradient of chi2(z) = min_c |W^1/2 (f - R T(z) c)|^2 with
@@ -419,6 +427,8 @@ typedef struct {
  * the Newton refinement that stand beside fitz's parabola fit
  * (py/redrock/fitz.py:259-261).  The chi2 output is the explicit residual
  * sum(w (f - R T c)^2) (zscan.py:777-780), a check on the forward pass.
+ * The template-flux gradient is summed in a fixed order (per item, per
+ * template sample, items in order): the same bits on every run.
  */
 int rr_zscan_grad(const rr_grad_desc *desc, void *stream);
 

@@ -26,7 +26,7 @@ EXPORTS = ('rr_abi_version', 'rr_last_error', 'rr_rebin', 'rr_zscan',
            'rr_zscan_workspace_bytes', 'rr_pack_mixed', 'rr_zscan_tiles', 'rr_zscan_tile_table',
            'rr_zscan_grad', 'rr_arch_prep', 'rr_arch_workspace_bytes', 'rr_arch_fit',
            'rr_zscan_polish', 'rr_pack_lanes', 'rr_pack_lanes_bytes', 'rr_zscan_resid_tiles',
-           'rr_zscan_noalloc')
+           'rr_zscan_noalloc', 'rr_grad_workspace_bytes')
 RR_NOALLOC_SCRATCH_BYTES = 64 << 20
 RR_ARCH_MAX_NA = 4
 RR_ARCH_MAX_NLEG = 12

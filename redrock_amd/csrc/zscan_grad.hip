@@ -18,8 +18,15 @@
 // One workgroup per item walks the target's pixel axis in chunks of 256
 // pixels: (1) m and dm/dz on the chunk plus a 10-pixel halo, (2) residual and
 // W r on the chunk plus 5, (3) g = R^T W r, the dz and chi2 sums and, if
-// requested, the template-flux scatter (FP64 atomics).
+// requested, v_q = -2 g_q t_q (item scale) per pixel into the workspace.
+// The template-flux gradient is then assembled without atomics, in a fixed
+// order (bit-identical from run to run):
+//   flux_rows_kernel: per item, every template sample k of the item's span
+//     sums v_q K_qk over the pixels q whose bin reaches it, in pixel order
+//     (one thread writes each sample), into the item's row P_i;
+//   flux_reduce_kernel: dchi2_dy[b][k] += sum_i c_ib P_i[k], items in order.
 #include "common.hpp"
+#include "zscan_args.hpp"
 
 #pragma clang fp contract(off)
 
@@ -36,8 +43,52 @@ struct GradArgs {
     const int *item_target;
     const double *item_z, *item_coeff, *item_scale;
     double *dchi2_dz, *chi2, *dchi2_dy;
+    double *v;        // workspace: (nitems, npix) scatter coefficients v_q
+    double *P;        // workspace: (nitems, nx) per-item template-flux rows
     rr_lyman_line lines[RR_MAX_LYMAN_LINES];
 };
+
+// Weight of template sample ks in the bin mean of [elo, ehi] at 1+z = s:
+// the coefficient bin_value's linear map gives y[ks] (the same expressions),
+// 0 when the bin does not reach it.
+__device__ double bin_weight(const GradArgs &a, double s, double elo, double ehi, int ks) {
+    const double *x = a.x;
+    const int nx = a.nx;
+    int lo = 0, hi = nx;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (x[mid] * s <= elo) lo = mid + 1;
+        else hi = mid;
+    }
+    const int j = lo < 1 ? 1 : (lo > nx - 1 ? nx - 1 : lo);
+    const double xj = x[j] * s, xjm1 = x[j - 1] * s;
+    const double width = ehi - elo;
+    double wsum = 0.0;
+    if (xj < ehi) {
+        int jend = j;
+        while (jend + 1 < nx - 1 && x[jend + 1] * s < ehi) ++jend;
+        if (ks < j - 1 || ks > jend + 1) return 0.0;
+        const double t = (elo - xjm1) / (xj - xjm1);
+        if (ks == j - 1) wsum += 0.5 * (xj - elo) * (1.0 - t);
+        if (ks == j) wsum += 0.5 * (xj - elo) * (1.0 + t);
+        double xprev = xj;
+        for (int jj = j + 1; jj <= jend; ++jj) {
+            const double xc = x[jj] * s;
+            if (ks == jj - 1) wsum += 0.5 * (xc - xprev);
+            if (ks == jj) wsum += 0.5 * (xc - xprev);
+            xprev = xc;
+        }
+        const double xe1 = x[jend + 1] * s;
+        const double t2 = (ehi - xprev) / (xe1 - xprev);
+        if (ks == jend) wsum += 0.5 * (ehi - xprev) * (2.0 - t2);
+        if (ks == jend + 1) wsum += 0.5 * (ehi - xprev) * t2;
+    } else {
+        if (ks != j - 1 && ks != j) return 0.0;
+        const double tm = (0.5 * (elo + ehi) - xjm1) / (xj - xjm1);
+        wsum = ks == j - 1 ? width * (1.0 - tm) : width * tm;
+    }
+    return wsum / width;
+}
 
 __device__ __forceinline__ double ycomb(const GradArgs &a, const double *c, int j) {
     double v = 0.0;
@@ -63,14 +114,9 @@ __device__ void bin_value(const GradArgs &a, const double *c, double s, double e
     const double xj = x[j] * s, xjm1 = x[j - 1] * s;
     const double width = ehi - elo;
     double r = 0.0;
-    // weights w_k of samples k contributing to the bin (before / width)
-    auto add = [&](int k, double wk) {
-        if (scatter) {
-            const double v = coef * wk / width;
-            for (int b = 0; b < a.nb; ++b)
-                atomicAdd(&a.dchi2_dy[(long long)b * nx + k], v * c[b]);
-        }
-    };
+    // weights w_k of samples k contributing to the bin (before / width): the
+    // linear map y -> bin mean whose transpose bin_weight() evaluates
+    auto add = [&](int k, double wk) { (void)k; (void)wk; (void)scatter; (void)coef; };
     if (xj < ehi) {
         int jend = j;
         while (jend + 1 < nx - 1 && x[jend + 1] * s < ehi) ++jend;
@@ -213,10 +259,8 @@ __global__ __launch_bounds__(256) void grad_kernel(GradArgs a) {
                 }
                 dz += gq * dmb[tid + 2 * HB];
                 if (want_dy) {
-                    const double elo = a.pe_lo[q], ehi = a.pe_hi[q];
                     const double tr = a.nlines ? igm_trans(a, s, a.pwave[q]) : 1.0;
-                    double mean, ye_lo, ye_hi;
-                    bin_value(a, cc, s, elo, ehi, mean, ye_lo, ye_hi, true, -2.0 * gq * tr * iscale);
+                    a.v[(long long)item * a.npix + q] = -2.0 * gq * tr * iscale;
                 }
             }
         }
@@ -230,7 +274,70 @@ __global__ __launch_bounds__(256) void grad_kernel(GradArgs a) {
     }
 }
 
+// One workgroup per item: P_i[k] = sum_q v_q K_qk for every template sample
+// k, in pixel order, written once (zero where no pixel reaches k).  The pixel
+// axis is the cameras (and exposures) one after another, each with increasing
+// bin edges: within a segment the pixels reaching k are those with
+// ehi_q > x_{k-1} s and elo_q < x_{k+1} s, found by binary search.
+constexpr int kMaxSeg = 64;
+
+__global__ __launch_bounds__(256) void flux_rows_kernel(GradArgs a) {
+    __shared__ int seg[kMaxSeg + 1];
+    __shared__ int nseg;
+    const int item = blockIdx.x;
+    const double s = 1. + a.item_z[item];
+    const double *v = a.v + (long long)item * a.npix;
+    double *P = a.P + (long long)item * a.nx;
+    if (threadIdx.x == 0) {                  // segment starts: where the lower edge falls back
+        int n = 0;
+        seg[n++] = 0;
+        for (int q = 1; q < a.npix && n < kMaxSeg; ++q)
+            if (a.pe_lo[q] < a.pe_lo[q - 1]) seg[n++] = q;
+        seg[n] = a.npix;
+        nseg = n;
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < a.nx; k += blockDim.x) {
+        const double xl = a.x[k > 0 ? k - 1 : 0] * s, xr = a.x[k + 1 < a.nx ? k + 1 : a.nx - 1] * s;
+        double acc = 0.0;
+        for (int g = 0; g < nseg; ++g) {
+            const int q0 = seg[g], q1 = seg[g + 1];
+            if (a.pe_hi[q1 - 1] <= xl || a.pe_lo[q0] >= xr) continue;
+            int lo = q0, hi = q1;            // first q of the segment with ehi_q > xl
+            while (lo < hi) {
+                const int mid = (lo + hi) >> 1;
+                if (a.pe_hi[mid] > xl) hi = mid;
+                else lo = mid + 1;
+            }
+            for (int q = lo; q < q1 && a.pe_lo[q] < xr; ++q)
+                acc += v[q] * bin_weight(a, s, a.pe_lo[q], a.pe_hi[q], k);
+        }
+        P[k] = acc;
+    }
+}
+
+// dchi2_dy[b][k] += sum_i c_ib P_i[k], one thread per sample, items in order.
+__global__ __launch_bounds__(256) void flux_reduce_kernel(GradArgs a) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= a.nx) return;
+    double acc[RR_MAX_NBASIS];
+    for (int b = 0; b < a.nb; ++b) acc[b] = 0.0;
+    for (int i = 0; i < a.nitems; ++i) {
+        const double p = a.P[(long long)i * a.nx + k];
+        const double *c = a.item_coeff + (long long)i * a.nb;
+        for (int b = 0; b < a.nb; ++b) acc[b] = fma(c[b], p, acc[b]);
+    }
+    for (int b = 0; b < a.nb; ++b) a.dchi2_dy[(long long)b * a.nx + k] += acc[b];
+}
+
 }  // namespace
+
+extern "C" long long rr_grad_workspace_bytes(const rr_grad_desc *d) {
+    if (!d || d->nitems < 0 || d->npix < 1 || d->nx < 2)
+        return rr::fail(RR_EINVAL, "rr_grad_workspace_bytes: bad descriptor");
+    if (!d->d_dchi2_dy) return 0;
+    return rr::align256((long long)d->nitems * d->npix * 8) + rr::align256((long long)d->nitems * d->nx * 8);
+}
 
 extern "C" int rr_zscan_grad(const rr_grad_desc *d, void *stream) {
     if (!d) return rr::fail(RR_EINVAL, "rr_zscan_grad: null descriptor");
@@ -249,7 +356,26 @@ extern "C" int rr_zscan_grad(const rr_grad_desc *d, void *stream) {
     a.item_target = d->d_item_target; a.item_z = d->d_item_z; a.item_coeff = d->d_item_coeff;
     a.item_scale = d->d_item_scale;
     a.dchi2_dz = d->d_dchi2_dz; a.chi2 = d->d_chi2; a.dchi2_dy = d->d_dchi2_dy;
+    a.v = nullptr;
+    a.P = nullptr;
+    if (d->d_dchi2_dy) {
+        const long long need = rr_grad_workspace_bytes(d);
+        if (!d->d_work || d->work_bytes < need)
+            return rr::fail(RR_EINVAL, "rr_zscan_grad: the template-flux gradient needs a workspace of %lld "
+                                       "bytes, %lld given", need, d->work_bytes);
+        a.v = static_cast<double *>(d->d_work);
+        a.P = reinterpret_cast<double *>(static_cast<char *>(d->d_work) +
+                                         rr::align256((long long)d->nitems * d->npix * 8));
+    }
     for (int l = 0; l < d->nlines; ++l) a.lines[l] = d->lines[l];
-    hipLaunchKernelGGL(grad_kernel, dim3(d->nitems), dim3(256), 0, static_cast<hipStream_t>(stream), a);
-    return rr::check_launch("rr_zscan_grad");
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    hipLaunchKernelGGL(grad_kernel, dim3(d->nitems), dim3(256), 0, st, a);
+    int rc = rr::check_launch("rr_zscan_grad");
+    if (rc || !d->d_dchi2_dy) return rc;
+    // the flux rows take at most kMaxSeg camera / exposure segments
+    hipLaunchKernelGGL(flux_rows_kernel, dim3(d->nitems), dim3(256), 0, st, a);
+    rc = rr::check_launch("rr_zscan_grad(flux rows)");
+    if (rc) return rc;
+    hipLaunchKernelGGL(flux_reduce_kernel, dim3((d->nx + 255) / 256), dim3(256), 0, st, a);
+    return rr::check_launch("rr_zscan_grad(flux reduce)");
 }

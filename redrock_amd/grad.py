@@ -31,6 +31,7 @@ class GradDesc(ctypes.Structure):
         ('d_rband', _vp), ('d_w', _vp), ('d_f', _vp),
         ('d_item_target', _vp), ('d_item_z', _vp), ('d_item_coeff', _vp),
         ('d_dchi2_dz', _vp), ('d_chi2', _vp), ('d_dchi2_dy', _vp), ('d_item_scale', _vp),
+        ('d_work', _vp), ('work_bytes', ctypes.c_longlong),
     ]
 
 
@@ -39,6 +40,8 @@ def _lib():
     if not hasattr(L, '_grad_ready'):
         L.rr_zscan_grad.restype = _i
         L.rr_zscan_grad.argtypes = [ctypes.POINTER(GradDesc), _vp]
+        L.rr_grad_workspace_bytes.restype = ctypes.c_longlong
+        L.rr_grad_workspace_bytes.argtypes = [ctypes.POINTER(GradDesc)]
         L._grad_ready = True
     return L
 
@@ -121,6 +124,13 @@ class GradProblem(object):
         d.d_dchi2_dz, d.d_chi2 = dz.data_ptr(), chi2.data_ptr()
         d.d_dchi2_dy = None if dy is None else dy.data_ptr()
         d.d_item_scale = None if sc is None else sc.data_ptr()
+        work = None
+        if dy is not None:
+            nbytes = _lib().rr_grad_workspace_bytes(ctypes.byref(d))
+            if nbytes < 0:
+                _hip.check(int(nbytes), 'rr_grad_workspace_bytes')
+            work = torch.empty(max(int(nbytes), 256), dtype=torch.uint8, device=dev)
+            d.d_work, d.work_bytes = work.data_ptr(), work.numel()
         rc = _lib().rr_zscan_grad(ctypes.byref(d), _hip.stream_handle())
         _hip.check(rc, 'rr_zscan_grad')
         return dz, chi2, dy

@@ -146,3 +146,15 @@ def test_newton_backtracks_and_keeps_reference(problem):
     info2 = {}
     zk, ck, gk = newton_refine(P3, lo, lo, hi, zref=lo, chi2ref=fake, info=info2)
     assert np.all(info2['kept_reference']) and np.allclose(zk, lo)
+
+
+def test_flux_gradient_run_to_run_identical(problem):
+    """d chi2 / d(template flux) is summed in a fixed order: two runs give the
+    same bits (no floating-point atomics)."""
+    tm, targets, zt, P = problem
+    z = np.array([0.3, 0.71, 0.95, 0.301, zt[1], zt[2] + 0.01])
+    chi2, coeff = P.forward(z)
+    out = [P.grad(z, coeff, want_dy=True, scale=np.linspace(0.5, 2.0, 6))[2].cpu().numpy()
+           for _ in range(3)]
+    assert np.any(out[0] != 0)
+    assert np.array_equal(out[0], out[1]) and np.array_equal(out[0], out[2])
