@@ -3,8 +3,9 @@
 * more than one GPU is refused with a message and exit status 2, before any
   device work;
 * the committed counter summary covers every tile configuration the
-  split-fp16 autotuner can choose on the galaxy petal, so the headline line's
-  roofline.traffic and roofline.mfma_busy are filled whichever tile wins;
+  autotuner can choose for the headline's fp16 normal-equation kernel on the
+  galaxy petal, so the line's roofline.traffic and roofline.mfma_busy are
+  filled whichever tile wins;
 * the algorithmic FLOPs per evaluation follow the reference-form count.
 """
 import os
@@ -26,10 +27,10 @@ def test_counters_cover_every_autotuned_tile():
     sys.path.insert(0, ROOT)
     import bench
     from redrock_amd import autotune
-    table = autotune.tile_table('f16x3')
+    table = autotune.tile_table('f16r')
     assert table
     for cfg in table:
-        sig = bench.kernel_signature('f16x3', 10, cfg)
+        sig = bench.kernel_signature('f16r', 10, cfg)
         assert bench.measured_counter('traffic_bytes_per_launch', sig) is not None, sig
         assert bench.measured_counter('mfma_util_percent', sig) is not None, sig
 
