@@ -7,7 +7,8 @@ to the redshift-per-lane FP64 kernel) through size-independent properties:
 the per-target best redshift (identical on >= 99.9 % of targets) and the
 largest |delta chi2| / chi2 over every (target, z).
 
-* galaxy petal: 500 GALAXY targets x 1977 z (nb 10), per-pixel S/N 1
+* galaxy petal: 500 GALAXY targets x 1977 z (nb 10), per-pixel S/N 1; the
+  f16r per-S/N bound |delta chi2| <= 1e-6 f'Wf at S/N 3 and 10
 * QSO petal: 500 QSO targets x 3795 z (nb 4), Lyman-alpha IGM fused
 * mixed batch: 5000 GALAXY / QSO / STAR targets, 0-30 % masked in runs of
   50-500 px, all three templates in one captured HIP graph, tiles autotuned
@@ -32,11 +33,11 @@ def _scan(dt, batch, nt, precision, **kw):
     return plan.zchi2.reshape(nt, nz).cpu().numpy(), plan
 
 
-def _petal(kind, nt=500):
+def _petal(kind, nt=500, snr=1.0):
     from redrock_amd import synth, zscan
     from redrock_amd.templates import DistTemplate
     tm = synth.galaxy_template() if kind == 'galaxy' else synth.qso_template()
-    targets, _ = synth.make_targets(tm, nt, seed=0)
+    targets, _ = synth.make_targets(tm, nt, seed=0, snr=snr)
     dwave = {s.wavehash: s.wave for s in targets[0].spectra}
     return tm, targets, dwave, DistTemplate(tm, dwave), zscan.TargetBatch(targets)
 
@@ -79,6 +80,28 @@ def test_galaxy_petal_mixed(galaxy_petal, precision):
     print(f'{precision}: max rel {rel.max():.2e}, best-z agreement {agree:.4f}')
     assert agree >= BESTZ_MIN
     assert rel.max() <= 1e-5
+
+
+@pytest.mark.parametrize('snr', [3.0, 10.0])
+def test_galaxy_petal_f16r_snr_bound(gpu, snr):
+    """Bright targets: every mixed mode's absolute chi2 error scales with
+    f'Wf, so the bound is stated per f'Wf: |delta chi2| <= 1e-6 f'Wf at every
+    (target, z), which is <= 1e-5 of chi2 wherever f'Wf <= 10 chi2; the best
+    redshift agrees on >= 99.9 % of targets."""
+    tm, targets, dwave, dt, batch = _petal('galaxy', snr=snr)
+    nt = len(targets)
+    print(f'case: galaxy petal f16r vs f64 at per-pixel S/N {snr}, {nt} x {dt.cache.nz}', flush=True)
+    c64, _ = _scan(dt, batch, nt, 'f64')
+    cm, _ = _scan(dt, batch, nt, 'f16r')
+    fwf = (batch.wf * batch.f).sum(dim=1).cpu().numpy()
+    err = np.abs(cm - c64)
+    agree = _agree(cm, c64)
+    print(f'S/N {snr}: max |dchi2|/fWf {(err / fwf[:, None]).max():.2e}, '
+          f'max rel {(err / c64).max():.2e}, best-z agreement {agree:.4f}')
+    assert agree >= BESTZ_MIN
+    assert np.all(err <= 1e-6 * fwf[:, None])
+    ok = fwf[:, None] <= 10 * c64
+    assert np.all(err[ok] <= 1e-5 * c64[ok])
 
 
 def test_qso_petal_full_grid(gpu):
