@@ -5,7 +5,8 @@
 // A wavefront holds 64 items at one redshift (or ZW redshifts), so every
 // template value it needs is wave-uniform: the template cache is laid out
 // [z][row][b] (basis fastest) and read with scalar loads straight into SGPRs,
-// which the vector FMAs take as their scalar operand.  Per-target data (R
+// which the vector FMAs take as their scalar operand (the residual kernel can
+// instead hold a block's rows in VGPRs and broadcast them with DPP).  Per-target data (R
 // band, weights, flux) are per lane; rr_pack_lanes stores them lane-major,
 // [target / 64][pixel][slot][target % 64], so each slot of a pixel is one
 // coalesced 512-byte (FP64) or 256-byte (FP32) load across the wavefront.
@@ -28,6 +29,7 @@
 // (c - c*)' M (c - c*): second order in the coefficient error, so the fp16
 // normal equations' first-order error (which the expansion f'Wf - 2c'y + c'Mc
 // carries, amplified by its cancellation) does not reach chi2.
+#include <utility>
 #include "common.hpp"
 #include "zscan_args.hpp"
 #include "xcd_map.hpp"
@@ -214,18 +216,85 @@ __global__ __launch_bounds__(64 * NW) void zscan_f64r_kernel(rr::ZArgs a, long l
 // stage's HUGE_CHI2.
 // ---------------------------------------------------------------------------
 
-// TL: the template rows of the block's redshifts are staged in LDS as well
-// (global_load_lds_dword, 4 bytes per lane, with the lane data), so their
-// latency hides behind the previous block; otherwise they are scalar loads.
-template <int NB, int ZW, int NW, int WPE, int PB, bool TL>
+// Template-row modes (TM):
+//   kRowsScalar: scalar loads into SGPRs, the FMAs' scalar operand;
+//   kRowsLds:    staged in LDS with the lane data (global_load_lds_dword, 4
+//                bytes per lane), read back as wave-uniform LDS loads;
+//   kRowsDpp:    a block's rows of each redshift in ceil(PB NB / 16) VGPRs,
+//                float f of the block in lane f % 16 of every 16-lane row of
+//                VGPR f / 16 (vector loads, one block ahead), and fed to the
+//                model FMAs through the DPP row broadcast (row_newbcast:
+//                every lane reads lane f % 16 of its own row), so the pixel
+//                loop issues no scalar loads at all.  Scalar and LDS loads
+//                share one counter that scalar loads decrement out of order,
+//                so a wave waiting for its LDS data also waits for every
+//                scalar load in flight; vector loads count separately.
+constexpr int kRowsScalar = 0, kRowsLds = 1, kRowsDpp = 2;
+
+// u = t[lane F of this row] * c  (the compiler folds the move into v_mul_f32_dpp)
+template <int F>
+__device__ __forceinline__ float bcast_mul(float t, float c) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, t), 0x150 + F,
+                                                              0xF, 0xF, true)) * c;
+}
+
+// u += t[lane F of this row] * c as one v_fmac_f32_dpp (the compiler does not
+// fold a DPP move into a tied-accumulator FMA); its hazard recognizer treats
+// the asm conservatively and pads VGPR write -> DPP read distances itself
+template <int F>
+__device__ __forceinline__ void bcast_fmac(float &u, float t, float c) {
+    asm("v_fmac_f32_dpp %0, %1, %2 row_newbcast:%3 row_mask:0xf bank_mask:0xf"
+        : "+v"(u)
+        : "v"(t), "v"(c), "n"(F));
+}
+
+// model u = sum_b T[row][b] c_b of block row ST, its floats ST NB .. ST NB + NB - 1
+template <int NB, int TV, int ST, int... B>
+__device__ __forceinline__ float model_dpp(const float (&t)[TV], const float (&c)[NB],
+                                           std::integer_sequence<int, B...>) {
+    float u = bcast_mul<(ST * NB) % 16>(t[(ST * NB) / 16], c[0]);
+    (bcast_fmac<(ST * NB + B + 1) % 16>(u, t[(ST * NB + B + 1) / 16], c[B + 1]), ...);
+    return u;
+}
+
+// one pixel (block row ST) of the residual pass for ZW redshifts
+template <int NB, int ZW, int TV, int ST>
+__device__ __forceinline__ void resid_pixel_dpp(const float *L, const float (&t)[ZW][TV],
+                                                const float (&c)[ZW][NB], float (&U)[ZW][11],
+                                                float (&part)[ZW]) {
+    const float *Lp = L + ST * (kSlots32 * 64);
+    float r[11];
+#pragma unroll
+    for (int k = 0; k < 11; ++k) r[k] = Lp[k * 64];
+    const float sf = Lp[11 * 64];
+#pragma unroll
+    for (int s = 0; s < ZW; ++s) {
+        U[s][(ST + 10) % 11] = model_dpp<NB, TV, ST>(t[s], c[s], std::make_integer_sequence<int, NB - 1>{});
+        float res = sf;
+#pragma unroll
+        for (int k = 0; k < 11; ++k) res = __builtin_fmaf(-r[k], U[s][(ST + k) % 11], res);
+        part[s] = __builtin_fmaf(res, res, part[s]);
+    }
+}
+
+template <int NB, int ZW, int TV, int... ST>
+__device__ __forceinline__ void resid_block_dpp(const float *L, const float (&t)[ZW][TV],
+                                                const float (&c)[ZW][NB], float (&U)[ZW][11],
+                                                float (&part)[ZW], std::integer_sequence<int, ST...>) {
+    (resid_pixel_dpp<NB, ZW, TV, ST>(L, t, c, U, part), ...);
+}
+
+template <int NB, int ZW, int NW, int WPE, int PB, int TM>
 __global__ __launch_bounds__(64 * NW, WPE) void zscan_resid_kernel(rr::ZArgs a, rr::TileMap map,
                                                                const float *__restrict__ T32,
                                                                const float *__restrict__ rlane) {
     static_assert(PB % 11 == 0, "the model window turns over every 11 pixels");
+    constexpr bool TL = TM == kRowsLds;
     using Stage = LaneStage<float, kSlots32, PB, NW>;
     __shared__ float lds[2][Stage::kElems];
     constexpr int TROW = PB * NB;                          // floats of one redshift's block rows
     constexpr int TINST = TL ? (NW * ZW * TROW + 63) / 64 : 0;
+    constexpr int TV = TM == kRowsDpp ? (TROW + 15) / 16 : 1;
     __shared__ float tl[TL ? 2 : 1][TL ? TINST * 64 : 1];
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -283,15 +352,34 @@ __global__ __launch_bounds__(64 * NW, WPE) void zscan_resid_kernel(rr::ZArgs a, 
         }
     };
 
+    // kRowsDpp: block j's rows of redshift s, float f at lane f % 16 of
+    // tv[s][f / 16] (the 4 rows of 16 lanes load the same 64 bytes); floats
+    // past the block repeat its last one
+    const int l16 = lane & 15;
+    auto load_tv = [&](int j, float (&dst)[ZW][TV]) {
+#pragma unroll
+        for (int s = 0; s < ZW; ++s) {
+            const float *base = Tz[s] + (long long)(j * PB + 10) * NB;
+#pragma unroll
+            for (int v = 0; v < TV; ++v) {
+                const int f = v * 16 + l16;
+                dst[s][v] = base[f < TROW ? f : TROW - 1];
+            }
+        }
+    };
+    float tcur[ZW][TV], tnext[ZW][TV];
+
     const int nblk = (int)(npl / PB);               // the pack holds a multiple of 22 pixels
     Stage::load(group, 0, lds[0], wave, lane);
     if constexpr (TL) load_rows(0, tl[0]);
+    if constexpr (TM == kRowsDpp) load_tv(0, tcur);
     for (int j = 0; j < nblk; ++j) {
         asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
         __syncthreads();
         if (j + 1 < nblk) {
             Stage::load(group, j + 1, lds[(j + 1) & 1], wave, lane);
             if constexpr (TL) load_rows(j + 1, tl[(j + 1) & 1]);
+            if constexpr (TM == kRowsDpp) load_tv(j + 1, tnext);
         }
         const float *L = lds[j & 1] + lane;
         const float *TLb = tl[TL ? (j & 1) : 0] + (wave * ZW) * TROW;
@@ -299,6 +387,16 @@ __global__ __launch_bounds__(64 * NW, WPE) void zscan_resid_kernel(rr::ZArgs a, 
         float part[ZW];
 #pragma unroll
         for (int s = 0; s < ZW; ++s) part[s] = 0.f;
+        if constexpr (TM == kRowsDpp) {
+            resid_block_dpp<NB, ZW, TV>(L, tcur, c, U, part, std::make_integer_sequence<int, PB>{});
+#pragma unroll
+            for (int s = 0; s < ZW; ++s) {
+                acc[s] += (double)part[s];
+#pragma unroll
+                for (int v = 0; v < TV; ++v) tcur[s][v] = tnext[s][v];
+            }
+            continue;
+        }
 #pragma unroll
         for (int st = 0; st < PB; ++st) {
             const long long p = p0 + st;
@@ -370,20 +468,24 @@ int launch_f64r(const rr::ZArgs &a, hipStream_t s) {
 }
 
 // Residual-kernel configurations (rr_zscan_resid_tiles): redshifts per wave,
-// waves per workgroup, waves per SIMD the registers must allow; entry 0 =
-// default.
+// waves per workgroup, waves per SIMD the registers must allow, pixels per
+// LDS block; entries 11-13 stage the template rows in LDS, 16-20 broadcast
+// them from VGPRs, the others read them with scalar loads; entry 0 = default
+// (by basis size).
 struct ResTile {
     int zw, nw, wpe, pb;
 };
-constexpr ResTile kResTiles[] = {{1, 16, 8, 0},  {2, 16, 4, 11}, {3, 16, 4, 11}, {2, 16, 4, 22},
+constexpr ResTile kResTiles[] = {{2, 16, 4, 0},  {2, 16, 4, 11}, {3, 16, 4, 11}, {2, 16, 4, 22},
                                  {3, 16, 4, 22}, {4, 16, 4, 22}, {6, 8, 2, 22},  {1, 16, 8, 11},
                                  {1, 16, 8, 22}, {1, 8, 8, 22},  {2, 8, 4, 22},  {1, 16, 8, 11},
-                                 {2, 16, 4, 11}, {3, 16, 4, 11}};
+                                 {2, 16, 4, 11}, {3, 16, 4, 11}, {2, 16, 8, 11}, {2, 12, 6, 11},
+                                 {1, 16, 8, 11}, {2, 16, 4, 11}, {2, 16, 4, 22}, {3, 16, 4, 11},
+                                 {2, 12, 6, 11}};
 constexpr int kNumResTiles = sizeof(kResTiles) / sizeof(kResTiles[0]);
 
-template <int NB, int ZW, int NW, int WPE, int PB, bool TL = false>
+template <int NB, int ZW, int NW, int WPE, int PB, int TM = kRowsScalar>
 int launch_resid_cfg(const rr::ZArgs &a, hipStream_t s) {
-    auto kernel = zscan_resid_kernel<NB, ZW, NW, WPE, PB, TL>;
+    auto kernel = zscan_resid_kernel<NB, ZW, NW, WPE, PB, TM>;
     const rr::TileMap map = rr::make_tile_map((a.nitems + 63) / 64,
                                               (a.max_item_nz + NW * ZW - 1) / (NW * ZW),
                                               rr::resident_per_xcd(kernel, 64 * NW), 64.0 * kSlots32 * 4,
@@ -396,10 +498,12 @@ int launch_resid_cfg(const rr::ZArgs &a, hipStream_t s) {
 template <int NB>
 int launch_resid(const rr::ZArgs &a, hipStream_t s) {
     switch (a.resid_tile) {
-        case 0:   // default: 2 redshifts per wave in 22-pixel blocks for small bases,
-                  // else one redshift per wave at 8 waves per SIMD
+        case 0:   // default: 2 redshifts per wave; small bases in 22-pixel blocks with
+                  // scalar template rows, larger ones in 11-pixel blocks with the rows
+                  // broadcast from VGPRs (the scalar loads of nb >= 7 rows stall the
+                  // LDS waits behind them)
             if constexpr (NB <= 6) return launch_resid_cfg<NB, 2, 16, 4, 22>(a, s);
-            else return launch_resid_cfg<NB, 1, 16, 8, 11>(a, s);
+            else return launch_resid_cfg<NB, 2, 16, 4, 11, kRowsDpp>(a, s);
         case 1: return launch_resid_cfg<NB, 2, 16, 4, 11>(a, s);
         case 2: return launch_resid_cfg<NB, 3, 16, 4, 11>(a, s);
         case 3: return launch_resid_cfg<NB, 2, 16, 4, 22>(a, s);
@@ -410,9 +514,16 @@ int launch_resid(const rr::ZArgs &a, hipStream_t s) {
         case 8: return launch_resid_cfg<NB, 1, 16, 8, 22>(a, s);
         case 9: return launch_resid_cfg<NB, 1, 8, 8, 22>(a, s);
         case 10: return launch_resid_cfg<NB, 2, 8, 4, 22>(a, s);
-        case 11: return launch_resid_cfg<NB, 1, 16, 8, 11, true>(a, s);
-        case 12: return launch_resid_cfg<NB, 2, 16, 4, 11, true>(a, s);
-        case 13: return launch_resid_cfg<NB, 3, 16, 4, 11, true>(a, s);
+        case 11: return launch_resid_cfg<NB, 1, 16, 8, 11, kRowsLds>(a, s);
+        case 12: return launch_resid_cfg<NB, 2, 16, 4, 11, kRowsLds>(a, s);
+        case 13: return launch_resid_cfg<NB, 3, 16, 4, 11, kRowsLds>(a, s);
+        case 14: return launch_resid_cfg<NB, 2, 16, 8, 11>(a, s);
+        case 15: return launch_resid_cfg<NB, 2, 12, 6, 11>(a, s);
+        case 16: return launch_resid_cfg<NB, 1, 16, 8, 11, kRowsDpp>(a, s);
+        case 17: return launch_resid_cfg<NB, 2, 16, 4, 11, kRowsDpp>(a, s);
+        case 18: return launch_resid_cfg<NB, 2, 16, 4, 22, kRowsDpp>(a, s);
+        case 19: return launch_resid_cfg<NB, 3, 16, 4, 11, kRowsDpp>(a, s);
+        case 20: return launch_resid_cfg<NB, 2, 12, 6, 11, kRowsDpp>(a, s);
         default: return rr::fail(RR_EINVAL, "rr_zscan(residual): resid_tile %d outside 0..%d", a.resid_tile,
                                  kNumResTiles - 1);
     }

@@ -126,3 +126,42 @@ def test_f16r_conventions_and_polish(galaxy70):
     assert rel.max() <= F16R_RTOL
     assert np.array_equal(chi2.argmin(axis=1), ref[0].argmin(axis=1))
     assert np.array_equal((chi2 + pen).argmin(axis=1), (ref[0] + ref[2]).argmin(axis=1))
+
+
+@pytest.mark.parametrize('kind', ['galaxy', 'qso'])
+def test_every_lane_configuration(galaxy70, kind):
+    """Every configuration the autotuner may pick: the FP64 lane kernel's
+    (waves per workgroup) give identical bits; the residual kernel's
+    (redshifts per wave, waves, pixels per LDS block) differ from the default
+    only by the FP32 partial sums' blocking and stay within the f16r bound of
+    the FP64 scan, HUGE_CHI2 rows kept."""
+    import torch
+    from redrock_amd import _hip, autotune
+    if kind == 'galaxy':
+        tm, targets, dwave, dt = galaxy70
+    else:
+        tm, targets, dwave, dt = _setup('qso', 70, seed=22, zsel=slice(0, None, 12))
+        for s in targets[5].spectra:
+            s.ivar[:] = 0.0
+    nz = dt.cache.nz
+    c64, _, _, p64 = _plan(dt, targets, 'f64', lanes=True)
+    for t in range(1, len(autotune.tile_table('f64'))):
+        p64.desc.tile = t
+        p64.run()
+        torch.cuda.synchronize()
+        print(f'case: {kind} f64 lane tile {t}', flush=True)
+        assert np.array_equal(p64.zchi2.cpu().numpy().reshape(70, nz), c64)
+    c0, _, _, plan = _plan(dt, targets, 'f16r', polish=False)
+    ok = np.arange(70) != 5
+    for t in range(1, len(autotune.resid_table())):
+        plan.desc.resid_tile = t
+        plan.run(stages=_hip.RR_STAGE_RESID)
+        torch.cuda.synchronize()
+        c = plan.zchi2.cpu().numpy().reshape(70, nz)
+        rel0 = np.abs(c[ok] - c0[ok]) / c0[ok]
+        rel = np.abs(c[ok] - c64[ok]) / c64[ok]
+        print(f'case: {kind} residual tile {t}: vs default {rel0.max():.2e}, vs f64 {rel.max():.2e}',
+              flush=True)
+        assert rel0.max() <= 1e-6       # FP32 partial sums over blocks of 11 or 22 pixels
+        assert rel.max() <= F16R_RTOL
+        assert np.all(c[5] == 9999999.0)
