@@ -4,8 +4,10 @@
 // One thread owns one (redshift, output bin) and loops over the basis, so the
 // sample search and the IGM factor are computed once per bin.  Threads are
 // laid out so that a wavefront's stores are contiguous: 256 bins of one
-// redshift per workgroup when bins are the fast output axis, 64 redshifts x 4
-// bins per workgroup when the output is z-fastest (the FP64 scan cache).  The bin mean
+// redshift per workgroup when bins are the fast output axis (the template
+// window those bins read is then staged in LDS, rebin_lds_kernel), 64
+// redshifts x 4 bins per workgroup when the output is z-fastest (the FP64
+// cache of the redshift-per-lane scan).  The bin mean
 // follows the serial sweep of py/redrock/rebin.py:126-176 expression by
 // expression, and this file is compiled with FP contraction off, so the FP64
 // result equals the sweep's double for double: the sample index j is the
@@ -54,19 +56,28 @@ __device__ __forceinline__ void store(const RebinArgs &a, long long off, double 
     }
 }
 
-template <int OUT, bool ZFAST>
-__global__ __launch_bounds__(256) void rebin_kernel(RebinArgs a, int nfast) {
-    int iz, ibin;
-    if (ZFAST) {   // lane -> redshift, wavefront -> bin; nfast = redshift blocks of 64
-        const int zb = blockIdx.x % nfast, bb = blockIdx.x / nfast;
-        iz = zb * 64 + (threadIdx.x & 63);
-        ibin = bb * 4 + (threadIdx.x >> 6);
-    } else {       // thread -> bin; nfast = bin blocks of 256
-        iz = blockIdx.x / nfast;
-        ibin = (blockIdx.x - iz * nfast) * blockDim.x + threadIdx.x;
+// IGM factor (igm.py:346-363): product over lines with lRest < line.
+__device__ __forceinline__ double igm_factor(const RebinArgs &a, int iz, int ibin, double opz) {
+    double T = 1.0;
+    if (a.nlines > 0) {
+        const double lobs = a.wave[ibin];
+        const double lrest = (1.0 / opz) * lobs;
+        for (int l = 0; l < a.nlines; ++l) {
+            const double line = a.lines[l].line;
+            if (lrest < line) {
+                const double zpix = lobs / line - 1.;
+                const double tau = a.lines[l].A * pow(1. + zpix, a.lines[l].B);
+                T *= exp(-tau);
+            }
+        }
     }
-    if (ibin >= a.nbin || iz >= a.nz) return;
+    return T;
+}
 
+// One (redshift, bin), template samples read from global memory.
+template <int OUT>
+__device__ void rebin_bin(const RebinArgs &a, int iz, int ibin) {
+    if (ibin >= a.nbin || iz >= a.nz) return;
     const double *x = a.x;
     const double opz = 1. + a.z[iz];
     const double elo = a.edges[ibin];
@@ -89,21 +100,7 @@ __global__ __launch_bounds__(256) void rebin_kernel(RebinArgs a, int nfast) {
         while (jend + 1 < nx - 1 && x[jend + 1] * opz < ehi) ++jend;
     }
 
-    // IGM factor (igm.py:346-363): product over lines with lRest < line.
-    double T = 1.0;
-    if (a.nlines > 0) {
-        const double lobs = a.wave[ibin];
-        const double lrest = (1.0 / opz) * lobs;
-        for (int l = 0; l < a.nlines; ++l) {
-            const double line = a.lines[l].line;
-            if (lrest < line) {
-                const double zpix = lobs / line - 1.;
-                const double tau = a.lines[l].A * pow(1. + zpix, a.lines[l].B);
-                T *= exp(-tau);
-            }
-        }
-    }
-
+    const double T = igm_factor(a, iz, ibin, opz);
     const double width = ehi - elo;
     const long long obase = (long long)iz * a.s_z + (long long)ibin * a.s_pix;
     for (int b = 0; b < a.nb; ++b) {
@@ -132,10 +129,132 @@ __global__ __launch_bounds__(256) void rebin_kernel(RebinArgs a, int nfast) {
     }
 }
 
+template <int OUT, bool ZFAST>
+__global__ __launch_bounds__(256) void rebin_kernel(RebinArgs a, int nfast) {
+    int iz, ibin;
+    if (ZFAST) {   // lane -> redshift, wavefront -> bin; nfast = redshift blocks of 64
+        const int zb = blockIdx.x % nfast, bb = blockIdx.x / nfast;
+        iz = zb * 64 + (threadIdx.x & 63);
+        ibin = bb * 4 + (threadIdx.x >> 6);
+    } else {       // thread -> bin; nfast = bin blocks of 256
+        iz = blockIdx.x / nfast;
+        ibin = (blockIdx.x - iz * nfast) * blockDim.x + threadIdx.x;
+    }
+    rebin_bin<OUT>(a, iz, ibin);
+}
+
+// Bins-fast layouts with the template window staged in LDS: a workgroup's
+// 256 consecutive bins of one redshift read one contiguous run of template
+// samples, [wlo, whi] (the first sample before the lower edge of its first bin
+// to the first sample beyond the upper edge of its last), so the workgroup
+// copies that run's wavelengths and, when they fit beside them, every basis
+// vector's values into LDS with coalesced loads (one wait for all of them);
+// each thread's binary search and interior sweep then read LDS.  The
+// expressions and their order are those of rebin_kernel (same doubles).  Values
+// of a window too long for all bases read global memory; a wavelength window
+// longer than half the LDS array (a template sampled far more finely than the
+// output bins) takes rebin_kernel's global path for that workgroup.
+constexpr int kLdsDoubles = 3072;   // 24 KB: 6 workgroups per CU
+
+// Bin mean of one basis vector: ys[k] is the sample of index wlo + k.
+template <typename YP>
+__device__ __forceinline__ double bin_mean(YP ys, const double *xs, int k, int kend, bool inside,
+                                           double elo, double ehi, double xj, double xjm1, double opz) {
+    const double width = ehi - elo;
+    double r = 0.0;
+    if (inside) {
+        const double yedge = ys[k - 1] + (elo - xjm1) * (ys[k] - ys[k - 1]) / (xj - xjm1);
+        r += 0.5 * (ys[k] + yedge) * (xj - elo);
+        double xprev = xj;
+        for (int kk = k + 1; kk <= kend; ++kk) {
+            const double xc = xs[kk] * opz;
+            r += 0.5 * (ys[kk] + ys[kk - 1]) * (xc - xprev);
+            xprev = xc;
+        }
+        const double xe1 = xs[kend + 1] * opz;
+        const double yedge2 = ys[kend] + (ehi - xprev) * (ys[kend + 1] - ys[kend]) / (xe1 - xprev);
+        r += 0.5 * (yedge2 + ys[kend]) * (ehi - xprev);
+    } else {
+        const double ylo = ys[k] + (elo - xj) * (ys[k] - ys[k - 1]) / (xj - xjm1);
+        const double yhi = ys[k] + (ehi - xj) * (ys[k] - ys[k - 1]) / (xj - xjm1);
+        r += 0.5 * (ylo + yhi) * width;
+    }
+    return r / width;
+}
+
+template <int OUT>
+__global__ __launch_bounds__(256) void rebin_lds_kernel(RebinArgs a, int nfast) {
+    __shared__ double lds[kLdsDoubles];
+    const int iz = blockIdx.x / nfast;
+    const int b0 = (blockIdx.x - iz * nfast) * 256;
+    const int ibin = b0 + threadIdx.x;
+    if (iz >= a.nz) return;                                 // uniform over the workgroup
+    const int b1 = b0 + 255 < a.nbin - 1 ? b0 + 255 : a.nbin - 1;
+    const double *x = a.x;
+    const int nx = a.nx;
+    const double opz = 1. + a.z[iz];
+    // the window: every sample index the bins [b0, b1] read (same clamps as
+    // rebin_kernel; whi >= jA keeps an out-of-range call inside the window)
+    int jA = upper_bound_scaled(x, 0, nx, opz, a.edges[b0]);
+    if (jA < 1) jA = 1;
+    if (jA > nx - 1) jA = nx - 1;
+    const int wlo = jA - 1;
+    int whi = upper_bound_scaled(x, 0, nx, opz, a.edges[b1 + 1]);
+    if (whi > nx - 1) whi = nx - 1;
+    if (whi < jA) whi = jA;
+    const int W = whi - wlo + 1;
+    if (2 * W > kLdsDoubles) {                              // uniform: global path
+        rebin_bin<OUT>(a, iz, ibin);
+        return;
+    }
+    double *xs = lds;
+    double *ys = lds + W;
+    const bool ystaged = (long long)W * (1 + a.nb) <= kLdsDoubles;
+    for (int i = threadIdx.x; i < W; i += 256) xs[i] = x[wlo + i];
+    if (ystaged) {
+        for (int b = 0; b < a.nb; ++b) {
+            const double *y = a.y + (long long)b * a.y_ld + wlo;
+            for (int i = threadIdx.x; i < W; i += 256) ys[b * W + i] = y[i];
+        }
+    }
+    __syncthreads();
+    if (ibin >= a.nbin) return;
+
+    const double elo = a.edges[ibin];
+    const double ehi = a.edges[ibin + 1];
+    // rebin.py:142, searched in the window: the answer lies in (wlo, whi + 1]
+    int lo = 1, hi = W;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (xs[mid] * opz <= elo) lo = mid + 1;
+        else hi = mid;
+    }
+    int j = wlo + lo;
+    if (j < 1) j = 1;
+    if (j > nx - 1) j = nx - 1;
+    const double xj = xs[j - wlo] * opz;
+    const double xjm1 = xs[j - 1 - wlo] * opz;
+    const bool inside = xj < ehi && j < nx - 1;
+    int jend = j;
+    if (inside) {
+        while (jend + 1 < nx - 1 && xs[jend + 1 - wlo] * opz < ehi) ++jend;
+    }
+    const double T = igm_factor(a, iz, ibin, opz);
+    const long long obase = (long long)iz * a.s_z + (long long)ibin * a.s_pix;
+    const int k = j - wlo, kend = jend - wlo;
+    for (int b = 0; b < a.nb; ++b) {
+        double r = ystaged ? bin_mean(ys + b * W, xs, k, kend, inside, elo, ehi, xj, xjm1, opz)
+                           : bin_mean(a.y + (long long)b * a.y_ld + wlo, xs, k, kend, inside, elo, ehi,
+                                      xj, xjm1, opz);
+        if (a.nlines > 0) r *= T;
+        store<OUT>(a, obase + (long long)b * a.s_b, r);
+    }
+}
+
 template <int OUT>
 void launch(bool zfast, dim3 grid, dim3 block, hipStream_t s, const RebinArgs &a, int nfast) {
     if (zfast) hipLaunchKernelGGL((rebin_kernel<OUT, true>), grid, block, 0, s, a, nfast);
-    else hipLaunchKernelGGL((rebin_kernel<OUT, false>), grid, block, 0, s, a, nfast);
+    else hipLaunchKernelGGL((rebin_lds_kernel<OUT>), grid, block, 0, s, a, nfast);
 }
 
 }  // namespace

@@ -205,7 +205,9 @@ def zfind(targets, templates, mp_procs=1, nminima=3, archetypes=None, priors=Non
         priors = Priors(priors)
     local = _local_targets(targets)
     dist = targets if hasattr(targets, 'local_target_ids') else _Local(local)
+    from . import zscan
     from .zscan import target_batches
+    zscan.unpolished.clear()
     batches = target_batches(local) if local else dict()
     if chi2_scan is None:                                    # zfind.py:289-292
         results = calc_zchi2_targets(dist, templates, mp_procs=mp_procs, use_gpu=use_gpu,
@@ -263,12 +265,15 @@ def zfind(targets, templates, mp_procs=1, nminima=3, archetypes=None, priors=Non
         results[tid].pop('meta', None)
     timings['assemble'] = clock() - ta
     timings['total'] = clock() - t0
+    timings['unpolished'] = dict(zscan.unpolished)
     last_timings.update(timings)
     return results, allzfit
 
 
 # wall-clock seconds of the phases of the last zfind call (scan, fitz per
-# template, assemble, total): host timings for profiling, not part of the API
+# template, assemble, total): host timings for profiling, not part of the API;
+# 'unpolished' maps each template whose mixed-precision scan ran without the
+# FP64 polish to the reason
 last_timings = dict()
 
 

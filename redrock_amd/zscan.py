@@ -36,6 +36,7 @@ from __future__ import annotations
 import ctypes
 import math
 import os
+import warnings
 
 import numpy as np
 
@@ -547,16 +548,18 @@ class ScanPlan(object):
         self.desc = d
         self.precision = precision
         self.polish = None
+        # why a mixed-precision scan runs without its FP64 polish (None: it has
+        # one, or the scan is FP64 or the polish was turned off)
+        self.unpolished = None
         tol = polish_tol(precision, polish)
         if tol is not None and d.solve != _hip.RR_SOLVE_PCA:
-            if polish not in (None, False):
-                raise ValueError('the FP64 polish re-solves with M c = y: not for NNLS templates')
+            self.unpolished = 'the FP64 polish re-solves M c = y: not for bounded (NNLS / BVLS) solves'
+        elif tol is not None and self.nitems > 0 and int(item_nz.max()) > _hip.RR_POLISH_MAX_NZ:
+            self.unpolished = f'items of more than {_hip.RR_POLISH_MAX_NZ} redshifts'
+        if self.unpolished is not None:
+            if polish is not None:           # asked for explicitly
+                warnings.warn(f'ScanPlan: no FP64 polish ({self.unpolished})', stacklevel=2)
             tol = None
-        if tol is not None and self.nitems > 0 and int(item_nz.max()) > _hip.RR_POLISH_MAX_NZ:
-            if polish is not None:
-                raise ValueError(f'the FP64 polish handles items of at most '
-                                 f'{_hip.RR_POLISH_MAX_NZ} redshifts')
-            tol = None                       # default polish: left out for such long items
         if tol is not None and self.nitems > 0:
             if not 1 <= polish_windows <= _hip.RR_POLISH_MAX_WINDOWS:
                 raise ValueError(f'polish_windows must be in 1..{_hip.RR_POLISH_MAX_WINDOWS}')
@@ -629,6 +632,11 @@ def _oiisum(template):
     return None
 
 
+# template full_type -> why its last mixed-precision scan ran without the FP64
+# polish (scan_targets fills it; zfind reports it in last_timings)
+unpolished = dict()
+
+
 def scan_targets(cache, targets, precision=None, oiisum=None, batches=None, polish=None,
                  solve=None):
     """Scan every target over every redshift of `cache`.
@@ -653,6 +661,9 @@ def scan_targets(cache, targets, precision=None, oiisum=None, batches=None, poli
         n = len(idx)
         plan = ScanPlan(cache, batch, np.arange(n), np.zeros(n), np.full(n, nz), precision,
                         oiisum=oiisum, polish=polish, solve=solve or _method(cache))
+        if plan.unpolished is not None:
+            tm = getattr(cache, 'template', None)
+            unpolished[getattr(tm, 'full_type', '?')] = plan.unpolished
         plan.run()
         c2, cf, pen, _ = plan.host()
         if len(groups) == 1:                 # one layout: the plan's rows are the targets in order

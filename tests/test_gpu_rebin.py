@@ -112,3 +112,40 @@ def test_cache_layouts_agree(gpu):
     b = t16[:n16].view(torch.float16).reshape(len(zsel), tm.nbasis, lay.cache_rows).double() / cache.tscale
     assert torch.all(a[:, :, :lay.halo] == 0) and torch.all(b[:, :, :lay.halo] == 0)
     assert torch.allclose(a, b, rtol=2 ** -10, atol=1e-6 * float(a.abs().max()))
+
+
+@pytest.mark.parametrize('step', [0.5, 0.02])
+def test_lds_window_equals_global_path(gpu, step):
+    """Bins-fast outputs stage each workgroup's template window in LDS; the
+    z-fastest output reads the samples from global memory: the same doubles.
+    A 0.02 A template sampling gives windows of more than 2048 samples at low z,
+    which take the global path inside the bins-fast launch; both equal the
+    oracle's sweep, IGM factor included."""
+    import torch
+    from redrock_amd import rebin, synth
+    rng = np.random.default_rng(5)
+    wave = np.arange(3000.0, 9000.0 + step / 2, step)
+    flux = 1.0 + 0.3 * np.sin(wave[None, :] / np.array([[37.0], [91.0], [13.0]])) \
+        + 0.05 * rng.standard_normal((3, wave.size))
+    obs = synth.camera_grids()['b']
+    edges = rebin.centers2edges(obs)
+    z = np.linspace(0.0, 0.19, 37)                # rest frame inside [3000, 9000] A
+    lyman = [(4500.0, 0.002, 3.0)]              # a line inside the window, so the factor varies
+    d = lambda a: torch.as_tensor(np.ascontiguousarray(a), dtype=torch.float64, device='cuda')
+    x, y, e, zz, w = d(wave), d(flux), d(edges), d(z), d(obs)
+    nz, nbin, nb = len(z), len(obs), flux.shape[0]
+    print(f'case: rebin LDS window vs global, step {step}, {wave.size} samples', flush=True)
+    a = torch.empty((nz, nbin, nb), dtype=torch.float64, device='cuda')     # bins-fast: LDS
+    rebin.rebin_into(a, x, y, e, zz, nbin * nb, nb, 1, lyman=lyman, wave=w)
+    b = torch.empty((nb, nbin, nz), dtype=torch.float64, device='cuda')     # z-fastest: global
+    rebin.rebin_into(b, x, y, e, zz, 1, nz, nbin * nz, lyman=lyman, wave=w)
+    a = a.cpu().numpy()
+    b = b.permute(2, 1, 0).cpu().numpy()
+    assert np.array_equal(a, b)
+    ref = orc.trapz_rebin(wave, flux, xnew=obs, myz=z)
+    T = np.ones((nz, nbin))
+    for i, zi in enumerate(z):
+        lrest = obs / (1 + zi)
+        m = lrest < 4500.0
+        T[i, m] = np.exp(-0.002 * (obs[m] / 4500.0) ** 3.0)
+    assert np.allclose(a, ref * T[:, :, None], rtol=1e-14, atol=0)

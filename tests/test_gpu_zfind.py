@@ -111,6 +111,35 @@ def test_zfind_nmf_template_matches_oracle(gpu):
         assert np.all(allresults[t.id][tn.full_type]['zcoeff'] >= 0)
 
 
+@pytest.mark.parametrize('precision', ['f16x3', 'f16r'])
+def test_zfind_polish_skips_nmf_template(gpu, precision):
+    """polish=True with a PCA and an NMF template: the NMF (bounded) scan runs
+    without the FP64 polish, with a warning, and zfind records it; the PCA
+    template is polished; the table equals the FP64 run's in z and spectype."""
+    from redrock_amd import synth, zfind as zf
+    from redrock_amd.templates import Template, DistTemplate
+    from redrock_amd.targets import DistTargetsCopy
+    g = synth.galaxy_template()
+    tg = Template(spectype='GALAXY', redshifts=g.redshifts[600:900], wave=g.wave, flux=g.flux,
+                  igm_model=g.igm_model)
+    tn = Template(spectype='GALAXY', subtype='NMF', redshifts=g.redshifts[600:900], wave=g.wave,
+                  flux=np.abs(g.flux), method='NMF', igm_model=g.igm_model)
+    targets, _ = synth.make_targets(g, 3, seed=31, zrange=(0.38, 0.58), snr=0.7)
+    dwave = {sp.wavehash: sp.wave for sp in targets[0].spectra}
+    dts = [DistTemplate(t, dwave) for t in (tg, tn)]
+    print(f'case: zfind PCA + NMF, {precision}, polish=True', flush=True)
+    with pytest.warns(UserWarning, match='no FP64 polish'):
+        allresults, got = zf.zfind(DistTargetsCopy(targets), dts, nminima=3, precision=precision,
+                                   polish=True)
+    assert set(zf.last_timings['unpolished']) == {tn.full_type}
+    _, want = zf.zfind(DistTargetsCopy(targets), dts, nminima=3, precision='f64')
+    assert zf.last_timings['unpolished'] == {}
+    assert np.array_equal(got['spectype'].astype(str), want['spectype'].astype(str))
+    assert np.allclose(got['z'], want['z'], rtol=1e-9, atol=1e-12)
+    for t in targets:
+        assert np.all(allresults[t.id][tn.full_type]['zcoeff'] >= 0)
+
+
 def test_zfind_mixed_precision_best_z(gpu):
     """The split-fp16 scan feeds the same fits: identical best redshift per target."""
     from redrock_amd import zfind as zf
