@@ -20,7 +20,7 @@
 extern "C" {
 #endif
 
-#define RR_ABI_VERSION 9
+#define RR_ABI_VERSION 10
 
 /* Error codes. */
 #define RR_OK 0
@@ -209,16 +209,25 @@ typedef struct {
      *                  templates (scipy.optimize.nnls(M, y), zscan.py:1153-1172),
      *                  Lawson-Hanson active sets; no solution within 3 nb
      *                  iterations -> HUGE_CHI2, status RR_STATUS_SINGULAR
-     *                  (the reference's LinAlgError, zscan.py:769-771)         */
+     *                  (the reference's LinAlgError, zscan.py:769-771)
+     *   RR_SOLVE_BVLS: min |M c - y|^2 over d_bounds[0][j] <= c_j <=
+     *                  d_bounds[1][j] (scipy lsq_linear(M, y, bounds, 'bvls'),
+     *                  the reference's BVLS branch, zscan.py:1175-1201, for
+     *                  any bounds), bounded-variable active sets; the same
+     *                  failure convention                                      */
     int solve;
     /* RR_SOLVE_NNLS: bit j set = coefficient j >= 0, the others free.  Every
      * bit for NMF / NNLS templates; the archetype bounds of the reference's
      * BVLS calls (zscan.py:289-294: archetype coefficients >= 0, Legendre
      * terms free) otherwise.  Ignored by RR_SOLVE_PCA.                       */
     int nonneg_mask;
+    /* RR_SOLVE_BVLS: (2, nb) lower bounds then upper bounds, +-inf allowed,
+     * lower < upper for every coefficient; NULL otherwise.                  */
+    const double *d_bounds;
 } rr_zscan_desc;
 #define RR_SOLVE_PCA 0
 #define RR_SOLVE_NNLS 1
+#define RR_SOLVE_BVLS 2
 
 #define RR_STAGE_STATS 1    /* per-item f'Wf and sum(w)                     */
 #define RR_STAGE_MAIN 2     /* normal equations (the scan kernel)           */

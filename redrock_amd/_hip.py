@@ -12,7 +12,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 # RR_LIB_PATH: load the library from another path (a build kept elsewhere)
 LIB_PATH = os.environ.get('RR_LIB_PATH') or os.path.join(HERE, 'libredrock_hip.so')
 
-RR_ABI_VERSION = 9
+RR_ABI_VERSION = 10
 RR_F64, RR_F32, RR_F16, RR_BF16 = 0, 1, 2, 3
 RR_PREC_F64, RR_PREC_F16, RR_PREC_BF16, RR_PREC_F32, RR_PREC_F16X3, RR_PREC_F16R = 0, 1, 2, 3, 4, 5
 RR_STATUS_OK, RR_STATUS_SINGULAR = 0, 1
@@ -55,12 +55,12 @@ class ZscanDesc(ctypes.Structure):
         ('d_zchi2', _vp), ('d_zcoeff', _vp), ('d_penalty', _vp), ('d_status', _vp),
         ('max_item_nz', _i), ('total_rows', _ll),
         ('d_work', _vp), ('work_bytes', _ll),
-        ('stages', _i), ('solve', _i), ('nonneg_mask', _i),
+        ('stages', _i), ('solve', _i), ('nonneg_mask', _i), ('d_bounds', _vp),
     ]
 
 
 RR_STAGE_STATS, RR_STAGE_MAIN, RR_STAGE_FINISH, RR_STAGE_RESID = 1, 2, 4, 8
-RR_SOLVE_PCA, RR_SOLVE_NNLS = 0, 1
+RR_SOLVE_PCA, RR_SOLVE_NNLS, RR_SOLVE_BVLS = 0, 1, 2
 
 RR_POLISH_WINDOW = 4
 RR_POLISH_MAX_WINDOWS = 64

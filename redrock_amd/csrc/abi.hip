@@ -80,8 +80,10 @@ static int scan_args(const rr_zscan_desc *d, rr::ZArgs &a) {
     } else {
         return rr::fail(RR_EINVAL, "rr_zscan: unknown precision %d", d->precision);
     }
-    if (d->solve != RR_SOLVE_PCA && d->solve != RR_SOLVE_NNLS)
+    if (d->solve != RR_SOLVE_PCA && d->solve != RR_SOLVE_NNLS && d->solve != RR_SOLVE_BVLS)
         return rr::fail(RR_EINVAL, "rr_zscan: unknown solve %d", d->solve);
+    if (d->solve == RR_SOLVE_BVLS && !d->d_bounds)
+        return rr::fail(RR_EINVAL, "rr_zscan: RR_SOLVE_BVLS needs d_bounds");
 
     a.nitems = d->nitems; a.npix = d->npix; a.npix_pad = d->npix_pad; a.nb = d->nb;
     a.precision = d->precision; a.halo = d->halo;
@@ -100,6 +102,7 @@ static int scan_args(const rr_zscan_desc *d, rr::ZArgs &a) {
     a.work = d->d_work; a.total_rows = d->total_rows;
     a.solve = d->solve;
     a.nonneg_mask = d->nonneg_mask;
+    a.bounds = d->d_bounds;
     return RR_OK;
 }
 
@@ -196,7 +199,8 @@ extern "C" int rr_zscan_polish(const rr_zscan_desc *d, const rr_polish_desc *p, 
     if (v) return v < 0 ? v : RR_OK;
     if (d->precision == RR_PREC_F64) return RR_OK;
     if (d->solve != RR_SOLVE_PCA)
-        return rr::fail(RR_EUNSUPPORTED, "rr_zscan_polish: re-solves M c = y; NNLS scans are not polished");
+        return rr::fail(RR_EUNSUPPORTED,
+                        "rr_zscan_polish: re-solves M c = y; bounded (NNLS, BVLS) scans are not polished");
     if (!p) return rr::fail(RR_EINVAL, "rr_zscan_polish: null polish descriptor");
     if (!p->d_tcache64 || !p->d_windows || !p->d_nwindows || !p->d_queue)
         return rr::fail(RR_EINVAL, "rr_zscan_polish: null required pointer");

@@ -295,6 +295,177 @@ __device__ inline bool nnls_solve(const double (&M)[NB][NB], const double (&y)[N
     return false;
 }
 
+// Bounded-variable least squares min |M x - y|^2 over lo <= x <= hi (either
+// bound may be infinite): scipy lsq_linear(M, y, bounds, 'bvls'), the
+// reference's BVLS branch (zscan.py:1175-1201) for any bounds.  The active-set
+// method of Stark and Parker (Bounded-variable least squares, Computational
+// Statistics 10, 1995), in the form of nnls_solve above with two-sided
+// bounds: a coefficient with a finite bound starts at it (the lower one when
+// both are finite), the unbounded ones stay free throughout; the bound
+// coefficient whose gradient points most into the box enters the free set;
+// an infeasible free-set solution is cut back to the first bound crossed.
+// Each free-set subproblem is ls_free on y - M_B x_B.  lo[j] < hi[j] for every
+// j (the host checks).  Returns false when 3 NB inner iterations did not
+// settle (the reference's LinAlgError, zscan.py:769-771).
+template <int NB>
+__device__ inline bool bvls_solve(const double (&M)[NB][NB], const double (&y)[NB], double (&x)[NB],
+                                  const double (&lo)[NB], const double (&hi)[NB]) {
+    double scale = 0.0, ymax = 0.0;
+#pragma unroll
+    for (int j = 0; j < NB; ++j) {
+        double cs = 0.0;
+#pragma unroll
+        for (int i = 0; i < NB; ++i) cs += fabs(M[i][j]);
+        scale = fmax(scale, cs);
+        ymax = fmax(ymax, fabs(y[j]));
+    }
+    const double tol = 10.0 * NB * 2.220446049250313e-16 * scale * ymax;
+    unsigned F = 0u, U = 0u;                 // free set; at the upper bound (others: lower)
+#pragma unroll
+    for (int j = 0; j < NB; ++j) {
+        const bool hl = lo[j] > -INFINITY, hh = hi[j] < INFINITY;
+        if (!hl && !hh) {
+            F |= 1u << j;
+            x[j] = 0.0;
+        } else if (hl) {
+            x[j] = lo[j];
+        } else {
+            x[j] = hi[j];
+            U |= 1u << j;
+        }
+    }
+    // s = argmin over the free set F with the others held at x
+    auto solve_free = [&](unsigned P, double (&s)[NB]) {
+        double rb[NB];
+#pragma unroll
+        for (int i = 0; i < NB; ++i) {
+            double v = y[i];
+#pragma unroll
+            for (int j = 0; j < NB; ++j)
+                if (!((P >> j) & 1u)) v = fma(-M[i][j], x[j], v);
+            rb[i] = v;
+        }
+        ls_free<NB>(M, rb, P, s);
+#pragma unroll
+        for (int j = 0; j < NB; ++j)
+            if (!((P >> j) & 1u)) s[j] = x[j];
+    };
+    double w[NB];                            // M^T (y - M x), the negative gradient
+    auto gradient = [&]() {
+        double r[NB];
+#pragma unroll
+        for (int i = 0; i < NB; ++i) {
+            double v = y[i];
+#pragma unroll
+            for (int j = 0; j < NB; ++j) v = fma(-M[i][j], x[j], v);
+            r[i] = v;
+        }
+#pragma unroll
+        for (int j = 0; j < NB; ++j) {
+            double v = 0.0;
+#pragma unroll
+            for (int i = 0; i < NB; ++i) v = fma(M[i][j], r[i], v);
+            w[j] = v;
+        }
+    };
+    if (F) {
+        double s[NB];
+        solve_free(F, s);
+#pragma unroll
+        for (int j = 0; j < NB; ++j) x[j] = s[j];
+    }
+    gradient();
+    unsigned rejected = 0u;
+    int iter = 0;
+    for (int outer = 0; outer < 4 * NB; ++outer) {
+        int t = -1;
+        double wt = tol;
+#pragma unroll
+        for (int j = 0; j < NB; ++j) {
+            if (((F | rejected) >> j) & 1u) continue;
+            const double cand = ((U >> j) & 1u) ? -w[j] : w[j];
+            if (cand > wt) {
+                wt = cand;
+                t = j;
+            }
+        }
+        if (t < 0) return true;              // Kuhn-Tucker conditions hold
+        const unsigned F0 = F, U0 = U;
+        const bool from_upper = (U >> t) & 1u;
+        F |= 1u << t;
+        U &= ~(1u << t);
+        double s[NB];
+        solve_free(F, s);
+        bool moves = false;
+#pragma unroll
+        for (int j = 0; j < NB; ++j)
+            if (j == t) moves = from_upper ? s[j] < x[j] : s[j] > x[j];
+        if (!moves) {                        // t would not move into the box
+            F = F0;
+            U = U0;
+            rejected |= 1u << t;
+            continue;
+        }
+        for (;;) {
+            bool feasible = true;
+#pragma unroll
+            for (int j = 0; j < NB; ++j)
+                if (((F >> j) & 1u) && !(s[j] > lo[j] && s[j] < hi[j])) feasible = false;
+            if (feasible) {
+#pragma unroll
+                for (int j = 0; j < NB; ++j) x[j] = s[j];
+                break;
+            }
+            if (++iter > 3 * NB) return false;
+            // move from x towards s until the first free coefficient reaches a bound
+            double alpha = 2.0;
+            int jmin = -1;
+            bool jup = false;
+#pragma unroll
+            for (int j = 0; j < NB; ++j) {
+                if (!((F >> j) & 1u)) continue;
+                if (!(s[j] > lo[j])) {
+                    const double d = x[j] - s[j];
+                    const double a = d > 0.0 ? (x[j] - lo[j]) / d : 0.0;
+                    if (a < alpha) {
+                        alpha = a;
+                        jmin = j;
+                        jup = false;
+                    }
+                } else if (!(s[j] < hi[j])) {
+                    const double d = s[j] - x[j];
+                    const double a = d > 0.0 ? (hi[j] - x[j]) / d : 0.0;
+                    if (a < alpha) {
+                        alpha = a;
+                        jmin = j;
+                        jup = true;
+                    }
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < NB; ++j) {
+                if (!((F >> j) & 1u)) continue;
+                x[j] = fma(alpha, s[j] - x[j], x[j]);
+                const bool at_lo = (j == jmin && !jup) || !(x[j] > lo[j]);
+                const bool at_hi = (j == jmin && jup) || !(x[j] < hi[j]);
+                if (at_lo) {
+                    F &= ~(1u << j);
+                    x[j] = lo[j];
+                } else if (at_hi) {
+                    F &= ~(1u << j);
+                    U |= 1u << j;
+                    x[j] = hi[j];
+                }
+            }
+            if (F == 0u) break;
+            solve_free(F, s);
+        }
+        rejected = 0u;
+        gradient();
+    }
+    return false;
+}
+
 // Error-free transformations for the compensated chi2 = f'Wf - c'y.
 __device__ __forceinline__ void two_sum(double a, double b, double &s, double &e) {
     s = a + b;

@@ -27,8 +27,9 @@ struct ZArgs {
     int max_item_nz;
     void *work;            // workspace: item stats then normal equations
     long long total_rows;  // rows of the output arrays (>= max item_out + nz)
-    int solve;             // RR_SOLVE_PCA or RR_SOLVE_NNLS
+    int solve;             // RR_SOLVE_PCA, RR_SOLVE_NNLS or RR_SOLVE_BVLS
     int nonneg_mask;       // RR_SOLVE_NNLS: bit j set = coefficient j >= 0
+    const double *bounds;  // RR_SOLVE_BVLS: (2, nb) lower then upper bounds
 };
 
 // Kernel-argument form of rr_polish_desc.

@@ -14,6 +14,8 @@
 //              over c >= 0 (scipy.optimize.nnls(M, y), zscan.py:1153-1172),
 //              or over c_j >= 0 for the coefficients of nonneg_mask only
 //              (the reference's BVLS archetype bounds, zscan.py:289-294).
+//   BVLS     : min |Mc - y|^2 over lo <= c <= hi for any bounds (scipy
+//              lsq_linear(M, y, bounds, 'bvls'), zscan.py:1175-1201).
 // chi2 = f'Wf - 2 c'y + c'Mc with the unregularised M (compensated sum): the
 // reference's explicit residual sum(w (f - A c)^2) (:777-780) expanded, which
 // stays exact with a prior and is stationary in c.
@@ -50,7 +52,7 @@ __global__ __launch_bounds__(64) void stats_kernel(rr::ZArgs a) {
     }
 }
 
-template <int NB, bool MIXED, bool NNLS>
+template <int NB, bool MIXED, int SOLVE>
 __global__ __launch_bounds__(64) void finish_kernel(rr::ZArgs a, long long ne_off) {
     const int item = blockIdx.y;
     const int iz = blockIdx.x * 64 + threadIdx.x;
@@ -106,8 +108,16 @@ __global__ __launch_bounds__(64) void finish_kernel(rr::ZArgs a, long long ne_of
 #pragma unroll
     for (int i = 0; i < NB; ++i) c[i] = y[i];
     bool ok;
-    if constexpr (NNLS) {
+    if constexpr (SOLVE == RR_SOLVE_NNLS) {
         ok = rr::nnls_solve<NB>(M, y, c, (unsigned)a.nonneg_mask);
+    } else if constexpr (SOLVE == RR_SOLVE_BVLS) {
+        double lo[NB], hi[NB];
+#pragma unroll
+        for (int j = 0; j < NB; ++j) {
+            lo[j] = a.bounds[j];
+            hi[j] = a.bounds[NB + j];
+        }
+        ok = rr::bvls_solve<NB>(M, y, c, lo, hi);
     } else if constexpr (MIXED) {
         double L[NB][NB];
 #pragma unroll
@@ -165,17 +175,22 @@ int launch_finish(const rr::ZArgs &a, hipStream_t s) {
     const bool mixed = a.precision != RR_PREC_F64;
     const long long ne_off = rr::align256((long long)a.nitems * 2 * 8);
     dim3 grid((a.max_item_nz + 63) / 64, a.nitems);
-    // the NNLS solve is its own instantiation: its active-set loop would
-    // raise the register use (and spill) of the plain solves
+    // the bounded solves are their own instantiations: their active-set loops
+    // would raise the register use (and spill) of the plain solves
     if (a.solve == RR_SOLVE_NNLS) {
         if (mixed)
-            hipLaunchKernelGGL((finish_kernel<NB, true, true>), grid, dim3(64), 0, s, a, ne_off);
+            hipLaunchKernelGGL((finish_kernel<NB, true, RR_SOLVE_NNLS>), grid, dim3(64), 0, s, a, ne_off);
         else
-            hipLaunchKernelGGL((finish_kernel<NB, false, true>), grid, dim3(64), 0, s, a, ne_off);
+            hipLaunchKernelGGL((finish_kernel<NB, false, RR_SOLVE_NNLS>), grid, dim3(64), 0, s, a, ne_off);
+    } else if (a.solve == RR_SOLVE_BVLS) {
+        if (mixed)
+            hipLaunchKernelGGL((finish_kernel<NB, true, RR_SOLVE_BVLS>), grid, dim3(64), 0, s, a, ne_off);
+        else
+            hipLaunchKernelGGL((finish_kernel<NB, false, RR_SOLVE_BVLS>), grid, dim3(64), 0, s, a, ne_off);
     } else if (mixed) {
-        hipLaunchKernelGGL((finish_kernel<NB, true, false>), grid, dim3(64), 0, s, a, ne_off);
+        hipLaunchKernelGGL((finish_kernel<NB, true, RR_SOLVE_PCA>), grid, dim3(64), 0, s, a, ne_off);
     } else {
-        hipLaunchKernelGGL((finish_kernel<NB, false, false>), grid, dim3(64), 0, s, a, ne_off);
+        hipLaunchKernelGGL((finish_kernel<NB, false, RR_SOLVE_PCA>), grid, dim3(64), 0, s, a, ne_off);
     }
     return rr::check_launch("rr_zscan(finish)");
 }

@@ -70,29 +70,33 @@ SOLVES = ('PCA', 'NMF', 'NNLS', 'BVLS')
 
 
 def solve_spec(method, nb, bounds=None):
-    """(RR_SOLVE_*, non-negativity mask) of a solve_matrices algorithm
+    """(RR_SOLVE_*, non-negativity mask, bounds) of a solve_matrices algorithm
     (zscan.py:1107-1205): PCA -> M c = y; NMF / NNLS -> nnls(M, y), every
     coefficient >= 0; BVLS -> lsq_linear(M, y, bounds, 'bvls') for bounds
-    (2, nb) or (2,) whose lower bounds are 0 or -inf and upper bounds +inf (the
-    archetype bounds of zscan.py:289-294; None: no bound at all)."""
+    (2, nb) or (2,) (None: no bound at all).  Bounds whose lower bounds are 0
+    or -inf and upper bounds +inf (the archetype bounds of zscan.py:289-294)
+    run as the masked NNLS solve; any others as RR_SOLVE_BVLS with the (2, nb)
+    float64 bounds array returned third (None for the other solves).  As
+    lsq_linear, every lower bound must be strictly below its upper bound."""
     key = 'PCA' if method is None else str(method).upper()
     if key == 'PCA':
-        return _hip.RR_SOLVE_PCA, 0
+        return _hip.RR_SOLVE_PCA, 0, None
     if key in ('NMF', 'NNLS'):
-        return _hip.RR_SOLVE_NNLS, (1 << nb) - 1
+        return _hip.RR_SOLVE_NNLS, (1 << nb) - 1, None
     if key == 'BVLS':
         if bounds is None:
-            return _hip.RR_SOLVE_NNLS, 0
+            return _hip.RR_SOLVE_NNLS, 0, None
         b = np.asarray(bounds, dtype=np.float64)
         if b.shape == (2,):                      # scalar bounds, broadcast as lsq_linear does
             b = np.repeat(b[:, None], nb, axis=1)
         if b.shape != (2, nb):
             raise ValueError(f'BVLS bounds must have shape (2, {nb}) or (2,), got {b.shape}')
         lo, hi = b
-        if not (np.all(np.isposinf(hi)) and np.all((lo == 0.0) | np.isneginf(lo))):
-            raise NotImplementedError('BVLS bounds other than c >= 0 or free per coefficient '
-                                      '(the archetype bounds) are not available in the scan')
-        return _hip.RR_SOLVE_NNLS, int(sum(1 << j for j in range(nb) if lo[j] == 0.0))
+        if np.any(np.isnan(b)) or not np.all(lo < hi):
+            raise ValueError('Each lower bound must be strictly less than each upper bound.')
+        if np.all(np.isposinf(hi)) and np.all((lo == 0.0) | np.isneginf(lo)):
+            return _hip.RR_SOLVE_NNLS, int(sum(1 << j for j in range(nb) if lo[j] == 0.0)), None
+        return _hip.RR_SOLVE_BVLS, 0, np.ascontiguousarray(b)
     raise NotImplementedError(f'solve_matrices algorithm {method!r} is not available in the scan')
 
 
@@ -538,7 +542,10 @@ class ScanPlan(object):
         d.d_status = self.status.data_ptr()
         d.max_item_nz = int(item_nz.max()) if self.nitems else 0
         d.total_rows = rows
-        d.solve, d.nonneg_mask = solve_spec(solve, nb, bounds)
+        d.solve, d.nonneg_mask, bnd = solve_spec(solve, nb, bounds)
+        if bnd is not None:
+            self._bounds = torch.as_tensor(bnd, dtype=torch.float64, device=dev).contiguous()
+            d.d_bounds = self._bounds.data_ptr()
         nbytes = _hip.lib().rr_zscan_workspace_bytes(ctypes.byref(d))
         if nbytes < 0:
             _hip.check(int(nbytes), 'rr_zscan_workspace_bytes')
@@ -770,7 +777,8 @@ def calc_zchi2_batch(spectra, tdata, weights, flux, wflux, nz, nbasis, solve_mat
     polish: FP64 polish of a mixed-precision scan (ScanPlan).
     solve_matrices_algorithm: 'PCA' (default), 'NMF' / 'NNLS' (coefficients
     >= 0, scipy.optimize.nnls(M, y) in the reference, zscan.py:1153-1172) or
-    'BVLS' with solver_args['bounds'] of the archetype form (solve_spec).
+    'BVLS' with solver_args['bounds'] (any bounds, scipy lsq_linear(M, y,
+    bounds, 'bvls') in the reference, zscan.py:1175-1201; solve_spec).
     Returns numpy (zchi2 (nz,), zcoeff (nz, nbasis)).
     """
     import torch

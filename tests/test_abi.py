@@ -112,18 +112,24 @@ def test_solve_argument():
     assert zscan.solve_code(None) == _hip.RR_SOLVE_PCA
     assert zscan.solve_code('pca') == _hip.RR_SOLVE_PCA
     assert zscan.solve_code('NMF') == zscan.solve_code('NNLS') == _hip.RR_SOLVE_NNLS
-    assert zscan.solve_spec('NMF', 4) == (_hip.RR_SOLVE_NNLS, 0b1111)
+    assert zscan.solve_spec('NMF', 4) == (_hip.RR_SOLVE_NNLS, 0b1111, None)
     # BVLS with the archetype bounds (zscan.py:289-294): archetype >= 0,
-    # Legendre terms free; no bounds at all = every coefficient free
+    # Legendre terms free, the masked NNLS solve; no bounds at all = every
+    # coefficient free
     inf = np.inf
-    assert zscan.solve_spec('BVLS', 4, [[0, 0, -inf, -inf], [inf] * 4]) == (_hip.RR_SOLVE_NNLS, 0b0011)
-    assert zscan.solve_spec('bvls', 3) == (_hip.RR_SOLVE_NNLS, 0)
-    assert zscan.solve_spec('BVLS', 3, (0, inf)) == (_hip.RR_SOLVE_NNLS, 0b111)
-    assert zscan.solve_spec('BVLS', 3, (-inf, inf)) == (_hip.RR_SOLVE_NNLS, 0)
-    with pytest.raises(NotImplementedError):          # a finite upper bound
-        zscan.solve_spec('BVLS', 2, [[0, 0], [1, inf]])
-    with pytest.raises(NotImplementedError):          # a non-zero lower bound
-        zscan.solve_spec('BVLS', 2, [[-1, 0], [inf, inf]])
+    assert zscan.solve_spec('BVLS', 4, [[0, 0, -inf, -inf], [inf] * 4]) == (_hip.RR_SOLVE_NNLS, 0b0011, None)
+    assert zscan.solve_spec('bvls', 3) == (_hip.RR_SOLVE_NNLS, 0, None)
+    assert zscan.solve_spec('BVLS', 3, (0, inf)) == (_hip.RR_SOLVE_NNLS, 0b111, None)
+    assert zscan.solve_spec('BVLS', 3, (-inf, inf)) == (_hip.RR_SOLVE_NNLS, 0, None)
+    # any other bounds: the two-sided BVLS solve with the bounds array
+    code, mask, b = zscan.solve_spec('BVLS', 2, [[0, 0], [1, inf]])
+    assert code == _hip.RR_SOLVE_BVLS and b.shape == (2, 2) and b.dtype == np.float64
+    assert np.array_equal(b, [[0, 0], [1, inf]])
+    code, _, b = zscan.solve_spec('BVLS', 3, (-1.0, 2.0))        # broadcast as lsq_linear
+    assert code == _hip.RR_SOLVE_BVLS and np.array_equal(b, [[-1.0] * 3, [2.0] * 3])
+    for bad in ([[0, 1], [0, 2]], [[0, np.nan], [1, 2]], [[2, 0], [1, 1]]):
+        with pytest.raises(ValueError, match='strictly less'):   # lsq_linear's rule
+            zscan.solve_spec('BVLS', 2, bad)
     with pytest.raises(ValueError):
         zscan.solve_spec('BVLS', 3, [[0, 0], [inf, inf]])
     with pytest.raises(NotImplementedError):
@@ -138,6 +144,9 @@ def test_solve_argument():
     d.solve = 7
     assert lib.rr_zscan(ctypes.byref(d), None) == -1
     assert b'unknown solve' in lib.rr_last_error()
+    d.solve = _hip.RR_SOLVE_BVLS                       # BVLS without its bounds
+    assert lib.rr_zscan(ctypes.byref(d), None) == -1
+    assert b'd_bounds' in lib.rr_last_error()
 
 
 def test_polish_tolerance_argument():

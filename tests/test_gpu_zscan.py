@@ -172,11 +172,55 @@ def test_calc_zchi2_batch_bvls_archetype_bounds(gpu):
     chi2, _ = zscan.calc_zchi2_batch(spectra, tdata, w, f, wf, len(zsel), tm.nbasis, 'BVLS')
     pchi2, _ = zscan.calc_zchi2_batch(spectra, tdata, w, f, wf, len(zsel), tm.nbasis, 'PCA')
     assert np.max(np.abs(chi2 - pchi2) / pchi2) <= F64_RTOL
-    bad = bounds.copy()
-    bad[1][0] = 1.0
-    with pytest.raises(NotImplementedError):
-        zscan.calc_zchi2_batch(spectra, tdata, w, f, wf, len(zsel), tm.nbasis, 'BVLS',
-                               solver_args={'bounds': bad})
+
+
+
+def test_calc_zchi2_batch_bvls_general_bounds(gpu):
+    """'BVLS' with two-sided and one-sided finite bounds on every kind of
+    coefficient (lsq_linear(M, y, bounds, 'bvls'), zscan.py:1175-1201): FP64
+    within 1e-9 of the oracle, coefficients inside the box with lower and upper
+    bounds active, with and without a prior; split fp16 within 1e-5 of f'Wf."""
+    from redrock_amd import synth, zscan
+    tm = synth.galaxy_template()
+    tg, _ = synth.make_targets(tm, 1, seed=29)
+    spectra = tg[0].spectra
+    dwave = {s.wavehash: s.wave for s in spectra}
+    zsel = tm.redshifts[400:480]
+    tdata = orc.template_cache(tm, dwave, myz=zsel)
+    w, f, wf = orc.spectral_data(spectra)
+    _, c0 = orc.calc_zchi2_batch(spectra, tdata, w, f, wf, len(zsel), tm.nbasis)
+    a = np.abs(c0).mean(axis=0)                  # typical size of each coefficient
+    nb = tm.nbasis
+    lo = np.full(nb, -np.inf)
+    hi = np.full(nb, np.inf)
+    lo[0:3], hi[0:3] = -0.3 * a[0:3], 0.4 * a[0:3]     # boxes
+    lo[3:5] = 0.2 * a[3:5]                             # lower only, above zero
+    hi[5:7] = -0.1 * a[5:7]                            # upper only, below zero
+    bounds = np.array([lo, hi])
+    rchi2, rcoeff = orc.calc_zchi2_batch(spectra, tdata, w, f, wf, len(zsel), nb, bounds=bounds)
+    at_lo = np.isclose(rcoeff, lo, rtol=0, atol=1e-12 * a.max())
+    at_hi = np.isclose(rcoeff, hi, rtol=0, atol=1e-12 * a.max())
+    print(f'case: BVLS general bounds, {len(zsel)} z; at lower {at_lo.mean():.2f}, '
+          f'at upper {at_hi.mean():.2f}', flush=True)
+    assert at_lo.any() and at_hi.any()
+    args = {'bounds': bounds}
+    chi2, coeff = zscan.calc_zchi2_batch(spectra, tdata, w, f, wf, len(zsel), nb, 'BVLS',
+                                         solver_args=args)
+    print('max rel chi2 err', np.max(np.abs(chi2 - rchi2) / rchi2))
+    assert np.max(np.abs(chi2 - rchi2) / rchi2) <= F64_RTOL
+    assert np.all(coeff >= lo) and np.all(coeff <= hi)
+    assert np.allclose(coeff, rcoeff, rtol=1e-6, atol=1e-8 * np.abs(rcoeff).max())
+    prior = np.diag(np.full(nb, 30.0))
+    rchi2p, rcoeffp = orc.calc_zchi2_batch(spectra, tdata, w, f, wf, len(zsel), nb, prior=prior,
+                                           bounds=bounds)
+    chi2p, coeffp = zscan.calc_zchi2_batch(spectra, tdata, w, f, wf, len(zsel), nb, 'BVLS',
+                                           solver_args=args, prior=prior)
+    assert np.max(np.abs(chi2p - rchi2p) / rchi2p) <= F64_RTOL
+    assert np.allclose(coeffp, rcoeffp, rtol=1e-6, atol=1e-8 * np.abs(rcoeffp).max())
+    chi2m, coeffm = zscan.calc_zchi2_batch(spectra, tdata, w, f, wf, len(zsel), nb, 'BVLS',
+                                           solver_args=args, precision='f16x3')
+    assert np.max(np.abs(chi2m - rchi2)) / np.sum(wf * f) <= 1e-5
+    assert np.all(coeffm >= lo) and np.all(coeffm <= hi)
 
 
 def test_huge_chi2_conventions(gpu):
