@@ -128,7 +128,7 @@ class _Refiner(object):
         self.batches = dict()
         self.slot = dict()
         for i, tg in enumerate(targets):
-            sig = PixelLayout.of(tg.spectra).signature
+            sig = PixelLayout.signature_of(tg.spectra)
             self.batches.setdefault(sig, []).append(i)
         if batch is not None and len(self.batches) == 1 and batch.nt == len(targets):
             self._tb = {sig: batch for sig in self.batches}       # one layout: reuse it
@@ -313,9 +313,9 @@ def _finalize(template, target, redshifts, nminima, cands, get):
     return results
 
 
-def _stack(results, target):
+def _stack(results, target, npix=None):
     """Sort candidates by chi2 and stack them into the reference's dict of arrays
-    (fitz.py:359-395)."""
+    (fitz.py:359-395); npix: the target's good-pixel count if known."""
     ii = np.argsort([tmp['chi2'] for tmp in results])
     results = [results[i] for i in ii]
     for r in results:
@@ -326,7 +326,8 @@ def _stack(results, target):
         xstr = np.zeros((0, 1), dtype=str)
         return dict(z=xfloat, zerr=xfloat, zwarn=xint, chi2=xfloat, zz=xfloat, zzchi2=xfloat,
                     coeff=xfloat, fitmethod=xstr, npixels=xint)
-    npix = sum(int((s.ivar > 0.).sum()) for s in target.spectra)
+    if npix is None:
+        npix = sum(int((s.ivar > 0.).sum()) for s in target.spectra)
     for r in results:
         r['npixels'] = npix
     return {k: _vstack([r[k] for r in results]) for k in results[0].keys()}
@@ -350,7 +351,7 @@ def _vstack(vals):
 
 def fitz_targets(zchi2, redshifts, targets, template, nminima=3, zminfit_npoints=15,
                  precision='f64', speculate=2, newton=False, archetype=None, deg_legendre=None,
-                 per_camera=False, n_nearest=None, prior_sigma=None, batch=None):
+                 per_camera=False, n_nearest=None, prior_sigma=None, batch=None, npixels=None):
     """fitz for every target of one template (zchi2: (nt, nz) incl. penalty).
 
     Returns a list with the reference's per-target result dict.  With
@@ -362,11 +363,12 @@ def fitz_targets(zchi2, redshifts, targets, template, nminima=3, zminfit_npoints
     redshift (fitz.py:329-357): deg_legendre Legendre polynomials (per
     camera with per_camera), prior_sigma on their coefficients, n_nearest
     archetypes combined.  batch: the TargetBatch of `targets` (one layout),
-    reused instead of uploading them again.
+    reused instead of uploading them again.  npixels: each target's number of
+    pixels with ivar > 0, when the caller has it already.
     """
     out = _fitz_targets(zchi2, redshifts, targets, template, nminima, zminfit_npoints,
                         precision, speculate, archetype, deg_legendre, per_camera, n_nearest,
-                        prior_sigma, batch)
+                        prior_sigma, batch, npixels)
     if newton:
         _add_newton(out, targets, template)
     return out
@@ -399,7 +401,7 @@ class _NeedMore(Exception):
 
 def _fitz_targets(zchi2, redshifts, targets, template, nminima, zminfit_npoints, precision,
                   speculate, archetype=None, deg_legendre=None, per_camera=False, n_nearest=None,
-                  prior_sigma=None, batch=None):
+                  prior_sigma=None, batch=None, npixels=None):
     zchi2 = np.atleast_2d(np.asarray(zchi2, dtype=np.float64))
     assert zchi2.shape == (len(targets), len(redshifts))
     nz = 15 if zminfit_npoints is None else zminfit_npoints
@@ -443,7 +445,9 @@ def _fitz_targets(zchi2, redshifts, targets, template, nminima, zminfit_npoints,
     if archetype is not None:
         _archetype_pass(out, targets, template, archetype, ref, deg_legendre, per_camera, n_nearest,
                         prior_sigma)
-    return [_stack(r, tg) for r, tg in zip(out, targets)]
+    if npixels is None:
+        npixels = [None] * len(targets)
+    return [_stack(r, tg, n) for r, tg, n in zip(out, targets, npixels)]
 
 
 def _archetype_pass(rows, targets, template, archetype, ref, deg_legendre, per_camera, n_nearest,

@@ -18,7 +18,6 @@ import numpy as np
 from . import constants
 from .fitz import fitz_targets, get_dv
 from .templates import parse_fulltype
-from .zscan import calc_zchi2_targets
 from .zwarning import ZWarningMask as ZW
 from .zwarning import badfit_mask
 
@@ -209,9 +208,10 @@ def zfind(targets, templates, mp_procs=1, nminima=3, archetypes=None, priors=Non
     from .zscan import target_batches
     zscan.unpolished.clear()
     batches = target_batches(local) if local else dict()
+    full = dict()                     # full_type -> (nt, nz) zchi2 + penalty of `local`
     if chi2_scan is None:                                    # zfind.py:289-292
-        results = calc_zchi2_targets(dist, templates, mp_procs=mp_procs, use_gpu=use_gpu,
-                                     precision=precision, batches=batches, polish=polish)
+        results, full = zscan._scan_templates(dist, templates, precision=precision,
+                                              batches=batches, polish=polish)
     else:
         from .results import read_zscan_redrock
         saved = read_zscan_redrock(chi2_scan)
@@ -224,18 +224,26 @@ def zfind(targets, templates, mp_procs=1, nminima=3, archetypes=None, priors=Non
 
     timings['scan'] = clock() - t0
     if priors is not None:                                   # zfind.py:318-322
+        full = dict()
         for tid in results:
             for ft in results[tid]:
                 results[tid][ft]['zchi2'] = (results[tid][ft]['zchi2']
                                              + priors.eval(tid, results[tid][ft]['redshifts']))
 
+    # good pixels per target (the npixels column, fitz.py:388-390), once for every template
+    if len(batches) == 1:
+        npix = [int(n) for n in next(iter(batches.values())).good_pixels]
+    else:
+        npix = [sum(int(np.count_nonzero(s.ivar > 0.)) for s in tg.spectra) for tg in local]
     order = np.array([t.template.full_type for t in templates]).argsort()
     for t in np.array(list(templates), dtype=object)[order]:      # zfind.py:330-367
         ft = t.template.full_type
         if not local:
             continue
-        chi2 = np.stack([results[tg.id][ft]['zchi2'] + results[tg.id][ft]['penalty']
-                         for tg in local])
+        chi2 = full.get(ft)
+        if chi2 is None:
+            chi2 = np.stack([results[tg.id][ft]['zchi2'] + results[tg.id][ft]['penalty']
+                             for tg in local])
         tf = clock()
         archetype = None
         if archetypes and t.template.template_type in archetypes:
@@ -244,7 +252,8 @@ def zfind(targets, templates, mp_procs=1, nminima=3, archetypes=None, priors=Non
                             zminfit_npoints=zminfit_npoints, newton=newton, archetype=archetype,
                             deg_legendre=deg_legendre, per_camera=bool(per_camera),
                             n_nearest=n_nearest, prior_sigma=prior_sigma,
-                            batch=next(iter(batches.values())) if len(batches) == 1 else None)
+                            batch=next(iter(batches.values())) if len(batches) == 1 else None,
+                            npixels=npix)
         for tg, zfit in zip(local, fits):
             results[tg.id][ft]['zfit'] = zfit
         timings['fitz'][ft] = clock() - tf
@@ -260,7 +269,10 @@ def zfind(targets, templates, mp_procs=1, nminima=3, archetypes=None, priors=Non
                     deg_legendre=0 if deg_legendre is None else int(deg_legendre),
                     n_nearest=n_nearest, ncamera=ncamera)
     ta = clock()
-    allzfit = _assemble(results, _all_ids(targets), templates, nminima, arch=arch)
+    if arch is None:
+        allzfit = _assemble_table(results, _all_ids(targets), templates, nminima)
+    else:
+        allzfit = _assemble(results, _all_ids(targets), templates, nminima, arch=arch)
     for tid in results:
         results[tid].pop('meta', None)
     timings['assemble'] = clock() - ta
@@ -285,6 +297,146 @@ def _pad_cols(val, width):
     if val.shape[1] < width:
         val = np.pad(val, ((0, 0), (0, width - val.shape[1])), constant_values=0)
     return val[:, :width]
+
+
+def _assemble_table(allresults, target_ids, templates, nminima):
+    """_assemble without archetypes, with every target's rows handled at once:
+    the same table, row order and dtypes (the per-target sorts, the nminima
+    per spectype trim and calc_deltachi2 become sorts keyed by target and
+    padded (target, candidate, candidate) arrays)."""
+    cols = dict()            # key -> list of (rows, ...) arrays in (target, template) order
+    tpos, spec, subt, ncf, mvd = [], [], [], [], []
+    keys = None
+    for ti, tid in enumerate(target_ids):
+        for fulltype in allresults[tid]:
+            if fulltype == 'meta':
+                continue
+            tmp = allresults[tid][fulltype].pop('zfit')
+            nmin = len(tmp['chi2'])
+            if nmin == 0:
+                print(f'WARNING: no {fulltype} chi2 vs. z minima for targetid {tid}')
+                continue
+            spectype, subtype = parse_fulltype(fulltype)
+            if keys is None:
+                keys = list(tmp.keys())
+            for k in keys:
+                cols.setdefault(k, []).append(tmp[k])
+            tpos.append(np.full(nmin, ti))
+            spec.append(np.full(nmin, spectype))
+            subt.append(np.full(nmin, subtype))
+            ncf.append(np.full(nmin, tmp['coeff'].shape[1]))
+            mvd.append(np.full(nmin, constants.max_velo_diff_star if spectype.upper() == 'STAR'
+                               else constants.max_velo_diff))
+    if keys is None:
+        return dict()
+    maxnc = max(c.shape[1] for c in cols['coeff'])
+    nrows = [len(c) for c in cols['coeff']]
+    coeff = np.zeros((sum(nrows), maxnc), dtype=np.result_type(*cols['coeff']))
+    r0 = 0
+    for c, nr in zip(cols['coeff'], nrows):
+        coeff[r0:r0 + nr, :c.shape[1]] = c
+        r0 += nr
+    tab = dict(coeff=coeff)
+    for k in keys:
+        if k == 'coeff':
+            continue
+        v = np.vstack(cols[k])
+        tab[k] = v.flatten() if (k != 'coeff' and v.shape[1] == 1) else v
+    tab['spectype'] = np.concatenate(spec)
+    tab['subtype'] = np.concatenate(subt)
+    tab['ncoeff'] = np.concatenate(ncf)
+    mv = np.concatenate(mvd)
+    tp = np.concatenate(tpos)
+    zw = tab['zwarn']
+    zw[tab['npixels'] == 0] |= ZW.NODATA
+    zw[tab['npixels'] < 10 * tab['ncoeff']] |= ZW.LITTLE_COVERAGE
+    n = len(tp)
+    # per target: sort by (bad fit, chi2), stable (sort_zfit_dict)
+    bad = (zw & badfit_mask) != 0
+    first = np.lexsort((tab['chi2'], bad, tp))
+    rank_sorted = np.empty(n, dtype=np.int64)
+    rank_sorted[first] = np.arange(n)
+    # nminima per spectype: rank of each row among its (target, spectype) in that order
+    scode = np.unique(tab['spectype'], return_inverse=True)[1].reshape(-1)
+    o = np.lexsort((rank_sorted, scode, tp))
+    grp = np.r_[True, (tp[o][1:] != tp[o][:-1]) | (scode[o][1:] != scode[o][:-1])]
+    start = np.maximum.accumulate(np.where(grp, np.arange(n), 0))
+    within = np.empty(n, dtype=np.int64)
+    within[o] = np.arange(n) - start
+    keep = within < nminima
+    # a target that lost rows is re-sorted from the spectype-grouped order, so
+    # (bad fit, chi2) ties then follow spectype (the reference's second sort)
+    dropped = np.zeros(len(target_ids), dtype=bool)
+    dropped[tp[~keep]] = True
+    tie = np.where(dropped[tp], scode, 0)
+    idx = np.flatnonzero(keep)
+    idx = idx[np.lexsort((rank_sorted[idx], tie[idx], tab['chi2'][idx], bad[idx], tp[idx]))]
+    for k in tab:
+        tab[k] = tab[k][idx]
+    tp, mv = tp[idx], mv[idx]
+    n = len(tp)
+    # znum and calc_deltachi2 per target over padded (target, row) arrays
+    gstart = np.r_[True, tp[1:] != tp[:-1]]
+    g = np.cumsum(gstart) - 1
+    first_row = np.flatnonzero(gstart)
+    znum = np.arange(n) - first_row[g]
+    ng, G = len(first_row), int(znum.max()) + 1
+    valid = np.zeros((ng, G), dtype=bool)
+    valid[g, znum] = True
+    C = np.zeros((ng, G))
+    Z = np.zeros((ng, G))
+    L = np.zeros((ng, G))
+    OK = np.zeros((ng, G), dtype=bool)
+    C[g, znum], Z[g, znum], L[g, znum] = tab['chi2'], tab['z'], mv
+    OK[g, znum] = (tab['zwarn'] & badfit_mask) == 0
+    dv = get_dv(Z[:, None, :], Z[:, :, None])            # [t, i, k] = dv of k relative to i
+    lim = np.minimum(L[:, :, None], L[:, None, :])
+    far = np.abs(dv) > lim
+    dchi2 = C[:, None, :] - C[:, :, None]
+    okk = (OK & valid)[:, None, :]
+    later = np.triu(np.ones((G, G), dtype=bool), k=1)[None]
+    cand = later & far & okk
+    has = cand.any(axis=2)
+    dmin = np.where(cand, dchi2, np.inf).min(axis=2)
+    deltachi2 = np.where(has, dmin, 0.0)[g, znum]
+    other = ~np.eye(G, dtype=bool)[None]
+    close = other & okk & (np.abs(dchi2) < constants.min_deltachi2) & (np.abs(dv) >= lim)
+    setzwarn = close.any(axis=2)[g, znum]
+    tab['targetid'] = np.asarray(target_ids)[tp]
+    tab['znum'] = znum
+    tab['deltachi2'] = deltachi2
+    tab['zwarn'][setzwarn] |= ZW.SMALL_DELTA_CHI2
+    order = keys + ['spectype', 'subtype', 'ncoeff', 'targetid', 'znum', 'deltachi2']
+    names = ['targetid'] + [k for k in order if k != 'targetid' and 'pca' not in k.lower()] + \
+        [k for k in order if 'pca' in k.lower()]
+    allzfit = {k: tab[k] for k in names}
+    return _finish_table(allzfit, allresults, templates, None)
+
+
+def _finish_table(allzfit, allresults, templates, arch):
+    """Metadata columns, string widths and the COEFF width (zfind.py:598-693)."""
+    meta0 = allresults[allzfit['targetid'][0]].get('meta', dict())
+    metakeys = sorted(k for k in meta0 if not k.endswith('_datatype'))
+    for mk in metakeys:
+        dt = meta0.get(mk + '_datatype')
+        vals = [allresults[t]['meta'][mk] for t in allzfit['targetid']]
+        allzfit[mk] = np.array(vals, dtype=dt) if dt is not None else np.array(vals)
+    allzfit['subtype'] = allzfit['subtype'].astype(
+        f"<U{max(20, max((len(s) for s in allzfit['subtype']), default=0))}")
+    allzfit['spectype'] = allzfit['spectype'].astype('<U6')
+    maxcoeff = max(t.template.nbasis for t in templates)
+    if arch is not None and arch['per_camera']:                # zfind.py:677-683
+        maxcoeff = max(maxcoeff, arch['ncamera'] * arch['deg_legendre'] + (arch['n_nearest'] or 1))
+    coeff = allzfit['coeff']
+    if coeff.shape[1] != maxcoeff:
+        c = np.zeros((coeff.shape[0], maxcoeff), dtype=coeff.dtype)
+        c[:, :coeff.shape[1]] = coeff
+        coeff = c
+    nc = np.unique(allzfit['ncoeff'])
+    if len(nc) == 1:
+        coeff = coeff[:, :int(nc[0])]
+    allzfit['coeff'] = coeff
+    return allzfit
 
 
 def _assemble(allresults, target_ids, templates, nminima, arch=None):
@@ -395,28 +547,7 @@ def _assemble(allresults, target_ids, templates, nminima, arch=None):
     names = ['targetid'] + [k for k in keys if 'pca' not in k.lower()] + [k for k in keys if 'pca' in k.lower()]
     allzfit = {k: np.concatenate([r[k] for r in rows]) for k in names}
     # target metadata columns (zfind.py:598-636): '<key>_datatype' entries give dtypes
-    meta0 = allresults[allzfit['targetid'][0]].get('meta', dict())
-    metakeys = sorted(k for k in meta0 if not k.endswith('_datatype'))
-    for mk in metakeys:
-        dt = meta0.get(mk + '_datatype')
-        vals = [allresults[t]['meta'][mk] for t in allzfit['targetid']]
-        allzfit[mk] = np.array(vals, dtype=dt) if dt is not None else np.array(vals)
-    allzfit['subtype'] = allzfit['subtype'].astype(
-        f"<U{max(20, max((len(s) for s in allzfit['subtype']), default=0))}")
-    allzfit['spectype'] = allzfit['spectype'].astype('<U6')
-    maxcoeff = max(t.template.nbasis for t in templates)
-    if arch is not None and arch['per_camera']:                # zfind.py:677-683
-        maxcoeff = max(maxcoeff, arch['ncamera'] * arch['deg_legendre'] + (arch['n_nearest'] or 1))
-    coeff = allzfit['coeff']
-    if coeff.shape[1] != maxcoeff:
-        c = np.zeros((coeff.shape[0], maxcoeff), dtype=coeff.dtype)
-        c[:, :coeff.shape[1]] = coeff
-        coeff = c
-    nc = np.unique(allzfit['ncoeff'])
-    if len(nc) == 1:
-        coeff = coeff[:, :int(nc[0])]
-    allzfit['coeff'] = coeff
-    return allzfit
+    return _finish_table(allzfit, allresults, templates, arch)
 
 
 ZBEST_COLUMNS = ('TARGETID', 'Z', 'ZERR', 'ZWARN', 'CHI2', 'COEFF', 'NPIXELS', 'SPECTYPE',

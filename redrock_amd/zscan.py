@@ -159,6 +159,11 @@ class PixelLayout(object):
     def of(spectra):
         return PixelLayout([s.wavehash for s in spectra], [s.nwave for s in spectra])
 
+    @staticmethod
+    def signature_of(spectra):
+        """PixelLayout.of(spectra).signature without building the layout."""
+        return (tuple(s.wavehash for s in spectra), tuple(int(s.nwave) for s in spectra))
+
 
 class TemplateCache(object):
     """A template rebinned to a redshift grid on every wavelength grid, in HBM.
@@ -288,7 +293,8 @@ class TargetBatch(object):
 
     rband (nt, npix_pad, 11), w, f, wf (nt, npix_pad) float64; pixels past
     npix have zero weight.  w and f are the concatenated ivar and flux
-    (spectral_data, zscan.py:192-209); wf = w*f.
+    (spectral_data, zscan.py:192-209); wf = w*f.  good_pixels: host int
+    array, each target's number of pixels with ivar > 0.
     """
 
     def __init__(self, targets=None, layout=None, arrays=None):
@@ -300,11 +306,12 @@ class TargetBatch(object):
             self.w = torch.as_tensor(np.ascontiguousarray(w), dtype=torch.float64, device=dev)
             self.f = torch.as_tensor(np.ascontiguousarray(f), dtype=torch.float64, device=dev)
             self.wf = torch.as_tensor(np.ascontiguousarray(wf), dtype=torch.float64, device=dev)
+            self.good_pixels = np.count_nonzero(np.asarray(w) > 0, axis=1)
         else:
             layout = layout or PixelLayout.of(targets[0].spectra)
             nt, P = len(targets), layout.npix_pad
             for tg in targets:
-                if PixelLayout.of(tg.spectra).signature != layout.signature:
+                if PixelLayout.signature_of(tg.spectra) != layout.signature:
                     raise ValueError('targets of one batch must share their wavelength grids')
             self.rband = torch.zeros((nt, P, 11), dtype=torch.float64, device=dev)
             w = np.zeros((nt, P))
@@ -316,6 +323,7 @@ class TargetBatch(object):
                 for i, sp in enumerate(spectra):
                     w[i, off:off + n] = sp.ivar
                     f[i, off:off + n] = sp.flux
+            self.good_pixels = np.count_nonzero(w > 0, axis=1)
             self.w = torch.from_numpy(w).to(dev)
             self.f = torch.from_numpy(f).to(dev)
             self.wf = self.w * self.f
@@ -660,7 +668,7 @@ def scan_targets(cache, targets, precision=None, oiisum=None, batches=None, poli
     penalty = np.zeros((nt, nz))
     groups = dict()
     for i, tg in enumerate(targets):
-        groups.setdefault(PixelLayout.of(tg.spectra).signature, []).append(i)
+        groups.setdefault(PixelLayout.signature_of(tg.spectra), []).append(i)
     for sig, idx in groups.items():
         batch = None if batches is None else batches.get(sig)
         if batch is None:
@@ -887,7 +895,7 @@ def target_batches(targets):
     (the grouping scan_targets uses)."""
     groups = dict()
     for tg in targets:
-        groups.setdefault(PixelLayout.of(tg.spectra).signature, []).append(tg)
+        groups.setdefault(PixelLayout.signature_of(tg.spectra), []).append(tg)
     return {sig: TargetBatch(tgs) for sig, tgs in groups.items()}
 
 
@@ -899,7 +907,15 @@ def calc_zchi2_targets(targets, templates, mp_procs=1, use_gpu=False, precision=
     once for all templates).
     Returns results[targetid][fulltype] = dict(redshifts, zchi2, penalty, zcoeff).
     """
+    return _scan_templates(targets, templates, precision, batches, polish)[0]
+
+
+def _scan_templates(targets, templates, precision=None, batches=None, polish=None):
+    """calc_zchi2_targets' results and, per template full_type, the (nt, nz)
+    arrays zchi2 + penalty of the local targets in order (what zfind's fine
+    fits start from), without splitting and re-stacking them per target."""
     results = {tid: dict() for tid in targets.local_target_ids()}
+    full = dict()
     local = targets.local()
     if batches is None and local:
         batches = target_batches(local)
@@ -907,7 +923,8 @@ def calc_zchi2_targets(targets, templates, mp_procs=1, use_gpu=False, precision=
         ft = t.template.full_type
         zchi2, zcoeff, penalty = calc_zchi2(targets.local_target_ids(), local, t,
                                             precision=precision, batches=batches, polish=polish)
+        full[ft] = zchi2 + penalty
         for i, tg in enumerate(local):
             results[tg.id][ft] = dict(redshifts=t.template.redshifts, zchi2=zchi2[i],
                                       penalty=penalty[i], zcoeff=zcoeff[i])
-    return results
+    return results, full

@@ -134,6 +134,31 @@ def test_assemble_matches_oracle(seed):
             assert np.array_equal(got[k].astype(str), want[k].astype(str)), k
 
 
+@pytest.mark.parametrize('seed', range(12))
+def test_assemble_table_equals_per_target_assembly(seed):
+    """The all-targets-at-once assembly (zfind without archetypes) gives the
+    per-target assembly's table: same columns in the same order, dtypes, rows
+    and row order, also with exact chi2 ties (HUGE_CHI2 rows) across templates
+    and candidates, and for every nminima."""
+    import copy
+    res, tids, templates = _fake_results(seed)
+    rng = np.random.default_rng(100 + seed)
+    if seed % 2:
+        for tid in tids:
+            for ft in res[tid]:
+                c = res[tid][ft]['zfit']['chi2']
+                c[rng.random(c.shape) < 0.4] = 9999999.0
+                c[rng.random(c.shape) < 0.2] = 7050.0
+    for nmin in (1, 2, 3):
+        a = zf._assemble(copy.deepcopy(res), tids, templates, nminima=nmin)
+        b = zf._assemble_table(copy.deepcopy(res), tids, templates, nminima=nmin)
+        assert list(a.keys()) == list(b.keys())
+        for k in a:
+            assert a[k].dtype == b[k].dtype, k
+            assert a[k].shape == b[k].shape, k
+            assert np.array_equal(a[k], b[k]), k
+
+
 def test_assemble_known_answer():
     """Hand-computed: nminima trim per spectype, bad fits last, flags."""
     def zfit(zs, chi2s, zwarn, nb, npix):
