@@ -169,6 +169,45 @@ class _Refiner(object):
                  for i in range(0, nitem, chunk)]
         return tuple(np.concatenate([p[k] for p in parts]) for k in range(3))
 
+    def _scan_blocks(self, zblocks, block, owners):
+        """chi2/coeff/penalty of item j: target owners[j] at the redshifts
+        zblocks[block[j]] (fine grids: candidates at the same coarse minimum
+        share one grid).  The template cache holds each distinct grid once and
+        items run ordered by grid, so items of one grid read the same cache
+        rows one after another.  Falls back to _scan when the distinct grids
+        exceed one cache, or the grids are not the packed 5-16 point kind."""
+        nblk, per_item = zblocks.shape
+        nitem = len(owners)
+        nb = self.template.nbasis
+        npix = sum(len(w) for w in self.dwave.values()) + 64
+        fits = nblk <= max(1, int(self._cache_budget() // (nb * npix * 8 * per_item)))
+        if nitem == 0 or not fits or not (4 < per_item <= 16) or self.precision != 'f64':
+            return self._scan(zblocks[block].reshape(-1), owners, per_item)
+        from .zscan import TemplateCache, ScanPlan
+        chi2 = np.zeros((nitem, per_item))
+        coeff = np.zeros((nitem, per_item, nb))
+        pen = np.zeros((nitem, per_item))
+        dw = dict()
+        for tg in self.targets:
+            for s in tg.spectra:
+                dw.setdefault(s.wavehash, s.wave)
+        cache = TemplateCache(self.template, dw, redshifts=zblocks.reshape(-1), check_range=False)
+        order = np.lexsort((np.arange(nitem), block))            # by grid, stable
+        for sig in self.batches:
+            sel = np.array([j for j in order if self.slot[owners[j]][0] == sig], dtype=np.int64)
+            if sel.size == 0:
+                continue
+            slots = np.array([self.slot[owners[j]][1] for j in sel])
+            plan = ScanPlan(cache, self._tb[sig], slots, block[sel] * per_item,
+                            np.full(sel.size, per_item), self.precision, oiisum=self.oii,
+                            solve=self.template.solve_matrices_algorithm)
+            plan.run()
+            c2, cf, pn, _ = plan.host()
+            chi2[sel] = c2.reshape(sel.size, per_item)
+            coeff[sel] = cf.reshape(sel.size, per_item, nb)
+            pen[sel] = pn.reshape(sel.size, per_item)
+        return chi2, coeff, pen
+
     def _scan_chunk(self, zlist, owners, per_item):
         from .zscan import TemplateCache, ScanPlan, target_runs
         nitem = len(owners)
@@ -236,7 +275,10 @@ class _Refiner(object):
         for j in np.where(~ok)[0]:
             out[j] = dict(imin=pairs[j][1], error=ValueError('edges must be within input x range'))
         good = np.where(ok)[0]
-        zzchi2, _, zzpen = self._scan(zz_rows[good].reshape(-1), [pairs[j][0] for j in good], nz)
+        # candidates at the same coarse minimum have the same fine grid
+        ublk, first, inv = np.unique(imins[good], return_index=True, return_inverse=True)
+        zzchi2, _, zzpen = self._scan_blocks(zz_rows[good][first], inv.reshape(-1),
+                                             [pairs[j][0] for j in good])
         c2 = zzchi2[:len(good)].reshape(len(good), nz).copy()
         if self.oii is not None:
             c2 += zzpen[:len(good)].reshape(len(good), nz)
