@@ -38,6 +38,9 @@ METRIC = 'zchi2 evals/sec on 1 GPU: mixed-precision vs FP64 plain at same best-z
 FLOP_PER_EVAL = {'GALAXY': 3763200, 'QSO': 1137348, 'STAR': 1495337}
 PEAK_FLOPS = {'f16': 2.5e15, 'bf16': 2.5e15, 'f16x3': 2.5e15, 'f16r': 2.5e15, 'f64': 78.6e12,
               'f32': 157.3e12}
+# fields of rr_zscan_resid_tiles entries by kind (1: FP32 lane kernel, 2: matrix cores)
+RESID_FIELDS = {1: ('kind', 'waves', 'items_per_wave', 'z_per_wave'),
+                2: ('kind', 'waves', 'items_per_workgroup', 'z_per_workgroup')}
 # the GPU box allots this many host cores to a one-GPU job
 CPU_SHARE = 16
 
@@ -638,8 +641,11 @@ def bench_petal(args):
         mix.desc.tile = tile
     rtable = autotune.resid_table() if args.precision == 'f16r' else None
     if rtable is not None:
-        log(f'residual kernel configuration {mix.desc.resid_tile}: {rtable[mix.desc.resid_tile]} '
-            f'(items tile, waves, items per wave, redshifts per wave)')
+        # resid_tile 0 with the matrix-core inputs is that kernel's first entry
+        rentry = (rtable[mix.desc.resid_tile] if mix.desc.resid_tile or not mix.desc.d_tcache_u
+                  else next(r for r in rtable if r[0] == 2))
+        rdesc = (RESID_FIELDS[rentry[0]], rentry)
+        log(f'residual kernel configuration {mix.desc.resid_tile}: {dict(zip(*rdesc))}')
     for p, plan in plans.items():
         for _ in range(args.warmup):
             plan.run()
@@ -687,7 +693,10 @@ def bench_petal(args):
         batch._mixed.clear()
         batch.mixed(args.precision)
         if args.precision == 'f16r':
-            batch.lanes('f16r')
+            if mix.resid in ('mfma', 'both'):
+                batch.mixed('f16x3')       # the matrix-core residual's hi + lo R tiles
+            if mix.resid in ('lanes', 'both'):
+                batch.lanes('f16r')
     torch.cuda.synchronize()
     t_pack = (time.perf_counter() - t) / args.steps
     # final full runs so the outputs hold complete results
@@ -776,6 +785,12 @@ def bench_petal(args):
     ach = flops / (k_mix * 1e-3)
     npix = batch.layout.npix
     resid_flop = 2 * npix * (nb + 11 + 1)          # u = T c, R-apply, square-accumulate per eval
+    resid_mfma = bool(args.precision == 'f16r' and mix.desc.d_tcache_u
+                      and (mix.desc.resid_tile == 0 or rtable[mix.desc.resid_tile][0] == 2))
+    # matrix-core residual, executed MFMA work per eval: per 16-pixel tile 3
+    # 16x16x32 MFMAs over 16 redshifts (R-apply) and nmf over 16 targets (u = T c)
+    nmf = (_hip.resid_u_slots(nb)[0] // 32) if resid_mfma else 0
+    resid_exec = (batch.layout.npix_pad // 16) * (3 + nmf) * 16384 // 16 if resid_mfma else None
     out = {
         'metric': METRIC,
         'value': evals / t_mix,
@@ -796,8 +811,9 @@ def bench_petal(args):
                    'targets': nt, 'redshifts': nz, 'nbasis': nb, 'snr_per_pixel': args.snr,
                    'precision': args.precision,
                    'solve': 'FP64 Cholesky',
-                   'chi2': ('explicit residual sum w (f - R T c)^2 at the fp16 coefficients, FP32 with '
-                            'FP64 partial sums' if args.precision == 'f16r' else
+                   'chi2': ('explicit residual sum w (f - R T c)^2 at the fp16 coefficients: u = T c and '
+                            'the R-apply as split-fp16 MFMAs (hi + lo operands, fp32 accumulation), '
+                            'squares in fp32 per 32 pixels, FP64 sums' if args.precision == 'f16r' else
                             "compensated FP64 f'Wf - 2c'y + c'Mc"),
                    'polish': (None if args.no_polish else
                               f'FP64 re-evaluation of the {_hip.RR_POLISH_WINDOW}-redshift windows within '
@@ -807,9 +823,7 @@ def bench_petal(args):
                                   if args.precision == 'f16x3' else 'fp32 (MFMA), FP64 after the scan'),
                    'tile': dict(zip(('waves_items', 'waves_z', 'items_per_wave', 'z_per_wave'),
                                     table[tile])),
-                   'residual_tile': (None if rtable is None else
-                                     dict(zip(('item_tiles', 'waves', 'items_per_wave', 'z_per_wave'),
-                                              rtable[mix.desc.resid_tile])))},
+                   'residual_tile': None if rtable is None else dict(zip(*rdesc))},
         'autotune_ms': {'x'.join(map(str, table[t])): ms for t, ms in tile_ms.items()},
         'fp64': {'value': evals / t_f64, 'ms_per_step': t_f64 * 1e3, 'kernel_ms': k_f64,
                  'kernel': f'void zscan_f64r_kernel<{nb}, {12 if nb <= 10 else 8}>',
@@ -842,6 +856,15 @@ def bench_petal(args):
                      'kernel_ms': k_mix,
                      'flop_per_eval': FLOP_PER_EVAL[kind], 'evals_per_launch': evals},
         'roofline_residual': (None if k_resid is None else {
+            'bound': 'mfma', 'kernel': f'zscan_resid_mfma_kernel<{nb}, ...>', 'kernel_ms': k_resid,
+            'executed_flop_per_eval': resid_exec,
+            'achieved': resid_exec * evals / (k_resid * 1e-3) / 1e12, 'peak': PEAK_FLOPS['f16'] / 1e12,
+            'unit': 'TFLOP/s', 'frac': resid_exec * evals / (k_resid * 1e-3) / PEAK_FLOPS['f16'],
+            'flop_per_eval': resid_flop,
+            'algorithmic_tflops': resid_flop * evals / (k_resid * 1e-3) / 1e12,
+            'note': ('split-fp16 MFMA work executed (u = T c over 16 targets, R-apply over 16 '
+                     'redshifts, hi/lo products); flop_per_eval is the FP32 form (nb + 12 FMAs '
+                     'per pixel)')} if resid_mfma else {
             'bound': 'fp32 vector ALU', 'kernel': f'zscan_resid_kernel<{nb}, ...>', 'kernel_ms': k_resid,
             'flop_per_eval': resid_flop, 'achieved': resid_flop * evals / (k_resid * 1e-3) / 1e12,
             'peak': PEAK_FLOPS['f32'] / 1e12, 'unit': 'TFLOP/s',

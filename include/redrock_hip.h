@@ -20,7 +20,7 @@
 extern "C" {
 #endif
 
-#define RR_ABI_VERSION 10
+#define RR_ABI_VERSION 11
 
 /* Error codes. */
 #define RR_OK 0
@@ -224,6 +224,21 @@ typedef struct {
     /* RR_SOLVE_BVLS: (2, nb) lower bounds then upper bounds, +-inf allowed,
      * lower < upper for every coefficient; NULL otherwise.                  */
     const double *d_bounds;
+
+    /* F16R explicit residual on the matrix cores (the configurations of
+     * kind 2 in rr_zscan_resid_tiles, and the default when all three are
+     * set; NULL = not used).  Needs uniform_z and halo == 8.
+     *  d_tcache_u: fp16 template operand, element (zc, row, s) at
+     *         (zc*cache_rows + row)*KS + s, KS = rr_resid_u_slots(nb, &np):
+     *         s < nb: hi(T[s]); nb <= s < 2nb: lo(T[s-nb]); 2nb <= s < 3nb:
+     *         hi(T[s-2nb]); 3nb <= s < 4nb when np == 4: lo(T[s-3nb]); other
+     *         slots 0.  T = template value * tscale, hi = fp16(T),
+     *         lo = fp16(T - hi), round to nearest even.
+     *  d_rtile3, d_scales3: rr_pack_mixed outputs for RR_PREC_F16X3 (hi and
+     *         lo tiles of sqrt(w) R, scales)                                 */
+    const void *d_tcache_u;
+    const void *d_rtile3;
+    const double *d_scales3;
 } rr_zscan_desc;
 #define RR_SOLVE_PCA 0
 #define RR_SOLVE_NNLS 1
@@ -335,11 +350,24 @@ int rr_zscan_tiles(int *cfg, int max);
 int rr_zscan_tile_table(int precision, int *cfg, int max);
 
 /*
- * rr_zscan_resid_tiles — configurations of the explicit-residual kernel of
- * RR_PREC_F16R (rr_zscan_desc.resid_tile): entry i = (1, waves per
- * workgroup, 64 items per wave, redshifts per wave); entry 0 is the default.
+ * rr_zscan_resid_tiles — configurations of the explicit-residual stage of
+ * RR_PREC_F16R (rr_zscan_desc.resid_tile): first the FP32 lane kernel's,
+ * entry i = (1, waves per workgroup, 64 items per wave, redshifts per wave),
+ * then the matrix-core kernel's, entry i = (2, waves per workgroup, 16 items
+ * per workgroup, redshifts per workgroup).  Entry 0 is the default: the
+ * matrix-core kernel when d_tcache_u, d_rtile3 and d_scales3 are set, else
+ * the lane kernel.  Returns the number of entries.
  */
 int rr_zscan_resid_tiles(int *cfg, int max);
+
+/*
+ * rr_resid_u_slots — row length KS (fp16 elements per (z, row)) of the
+ * d_tcache_u layout for a basis of nb vectors, and in *nprod the number of
+ * hi/lo products it holds (3 or 4).  Returns a negative code for nb outside
+ * 1..RR_MAX_NBASIS.  No reference counterpart: the layout of this build's
+ * matrix-core residual kernel.
+ */
+int rr_resid_u_slots(int nb, int *nprod);
 
 int rr_pack_mixed(const double *d_rband, const double *d_w, const double *d_f, int ntarget,
                   int npix_pad, int precision, void *d_rtile, void *d_sfcol,

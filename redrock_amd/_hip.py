@@ -12,7 +12,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 # RR_LIB_PATH: load the library from another path (a build kept elsewhere)
 LIB_PATH = os.environ.get('RR_LIB_PATH') or os.path.join(HERE, 'libredrock_hip.so')
 
-RR_ABI_VERSION = 10
+RR_ABI_VERSION = 11
 RR_F64, RR_F32, RR_F16, RR_BF16 = 0, 1, 2, 3
 RR_PREC_F64, RR_PREC_F16, RR_PREC_BF16, RR_PREC_F32, RR_PREC_F16X3, RR_PREC_F16R = 0, 1, 2, 3, 4, 5
 RR_STATUS_OK, RR_STATUS_SINGULAR = 0, 1
@@ -26,7 +26,7 @@ EXPORTS = ('rr_abi_version', 'rr_last_error', 'rr_rebin', 'rr_zscan',
            'rr_zscan_workspace_bytes', 'rr_pack_mixed', 'rr_zscan_tiles', 'rr_zscan_tile_table',
            'rr_zscan_grad', 'rr_arch_prep', 'rr_arch_workspace_bytes', 'rr_arch_fit',
            'rr_zscan_polish', 'rr_pack_lanes', 'rr_pack_lanes_bytes', 'rr_zscan_resid_tiles',
-           'rr_zscan_noalloc', 'rr_grad_workspace_bytes')
+           'rr_zscan_noalloc', 'rr_grad_workspace_bytes', 'rr_resid_u_slots')
 RR_NOALLOC_SCRATCH_BYTES = 64 << 20
 RR_ARCH_MAX_NA = 4
 RR_ARCH_MAX_NLEG = 12
@@ -56,6 +56,7 @@ class ZscanDesc(ctypes.Structure):
         ('max_item_nz', _i), ('total_rows', _ll),
         ('d_work', _vp), ('work_bytes', _ll),
         ('stages', _i), ('solve', _i), ('nonneg_mask', _i), ('d_bounds', _vp),
+        ('d_tcache_u', _vp), ('d_rtile3', _vp), ('d_scales3', _vp),
     ]
 
 
@@ -127,6 +128,8 @@ def lib():
     L.rr_pack_lanes.argtypes = [_vp, _vp, _vp, _vp, _i, _i, _i, _vp, _vp]
     L.rr_zscan_resid_tiles.restype = _i
     L.rr_zscan_resid_tiles.argtypes = [ctypes.POINTER(_i), _i]
+    L.rr_resid_u_slots.restype = _i
+    L.rr_resid_u_slots.argtypes = [_i, ctypes.POINTER(_i)]
     L.rr_zscan_tiles.restype = _i
     L.rr_zscan_tiles.argtypes = [ctypes.POINTER(_i), _i]
     L.rr_zscan_tile_table.restype = _i
@@ -142,6 +145,16 @@ def lib():
         raise HipError(f'libredrock_hip ABI {v}, expected {RR_ABI_VERSION}')
     _lib = L
     return L
+
+
+def resid_u_slots(nb):
+    """(KS, nprod) of the matrix-core residual's template operand cache:
+    fp16 elements per (z, row) and the hi/lo products they hold (3 or 4)."""
+    n = _i(0)
+    ks = lib().rr_resid_u_slots(int(nb), ctypes.byref(n))
+    if ks < 0:
+        check(ks, 'rr_resid_u_slots')
+    return ks, n.value
 
 
 def lane_pixels(npix_pad):

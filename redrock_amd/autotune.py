@@ -37,8 +37,10 @@ def tile_table(precision=_hip.RR_PREC_F16):
 
 
 def resid_table():
-    """[(1, waves, 64, z_per_wave), ...] indexed by rr_zscan_desc.resid_tile
-    (the explicit-residual kernel of precision 'f16r'; entry 0 = default)."""
+    """Configurations of the explicit-residual stage of precision 'f16r',
+    indexed by rr_zscan_desc.resid_tile (entry 0 = default): the FP32 lane
+    kernel's (1, waves, 64 items per wave, redshifts per wave), then the
+    matrix-core kernel's (2, waves, 16 items, redshifts per workgroup)."""
     n = 32
     cfg = (ctypes.c_int * (4 * n))()
     k = _hip.lib().rr_zscan_resid_tiles(cfg, n)
@@ -66,8 +68,10 @@ def tune_resid(plan, reps=3):
     table = resid_table()
     saved = plan.desc.resid_tile
     plan.run()
+    # configurations whose inputs the plan holds (ScanPlan resid=)
+    have = {1: bool(plan.desc.d_rlane), 2: bool(plan.desc.d_tcache_u)}
     timings = {t: _time_stage(_with(plan, 'resid_tile', t), _hip.RR_STAGE_RESID, reps)
-               for t in range(1, len(table))}
+               for t in range(1, len(table)) if have.get(table[t][0], False)}
     plan.desc.resid_tile = saved
     return min(timings, key=timings.get), timings
 

@@ -31,6 +31,14 @@ static bool lane_inputs(const rr_zscan_desc *d) {
            d->cache_rows >= d->halo + rr::lane_pixels(d->npix_pad) + 5;
 }
 
+// Inputs of the matrix-core explicit residual of F16R.
+static bool resid_mfma_inputs(const rr_zscan_desc *d) {
+    return d->d_tcache_u && d->d_rtile3 && d->d_scales3 && d->uniform_z && d->halo == 8;
+}
+
+// Default configuration of the matrix-core residual (index into its table).
+constexpr int kResidMfmaDefault = 0;
+
 extern "C" long long rr_zscan_workspace_bytes(const rr_zscan_desc *d) {
     if (!d || d->nitems < 0 || d->nb < 1 || d->nb > RR_MAX_NBASIS || d->total_rows < 0)
         return rr::fail(RR_EINVAL, "rr_zscan_workspace_bytes: bad descriptor");
@@ -70,9 +78,10 @@ static int scan_args(const rr_zscan_desc *d, rr::ZArgs &a) {
         if (!lane_inputs(d) && d->cache_ldz < d->cache_nz)
             return rr::fail(RR_EINVAL, "rr_zscan: cache_ldz < cache_nz");
     } else if (mixed) {
-        if (d->precision == RR_PREC_F16R && !lane_inputs(d))
-            return rr::fail(RR_EINVAL, "rr_zscan: F16R needs d_tcache_r and d_rlane, uniform items "
-                                       "and cache_rows >= halo + rr_lane_pixels(npix_pad) + 5");
+        if (d->precision == RR_PREC_F16R && !lane_inputs(d) && !resid_mfma_inputs(d))
+            return rr::fail(RR_EINVAL, "rr_zscan: F16R needs uniform items and either d_tcache_u, "
+                                       "d_rtile3 and d_scales3 (matrix-core residual) or d_tcache_r and "
+                                       "d_rlane with cache_rows >= halo + rr_lane_pixels(npix_pad) + 5");
         if (!d->d_rtile || !d->d_scales || (d->precision == RR_PREC_F32 && !d->d_sfcol))
             return rr::fail(RR_EINVAL, "rr_zscan: mixed precision needs rr_pack_mixed outputs");
         if (d->halo != 8) return rr::fail(RR_EINVAL, "rr_zscan: mixed precision needs halo == 8");
@@ -103,6 +112,9 @@ static int scan_args(const rr_zscan_desc *d, rr::ZArgs &a) {
     a.solve = d->solve;
     a.nonneg_mask = d->nonneg_mask;
     a.bounds = d->d_bounds;
+    a.tcache_u = d->d_tcache_u;
+    a.rtile3 = d->d_rtile3;
+    a.scales3 = d->d_scales3;
     return RR_OK;
 }
 
@@ -134,7 +146,25 @@ extern "C" int rr_zscan(const rr_zscan_desc *d, void *stream) {
     if (rc) return rc;
     if (st & RR_STAGE_FINISH) rc = rr::zscan_finish(a, s);
     if (rc) return rc;
-    if ((st & RR_STAGE_RESID) && d->precision == RR_PREC_F16R) rc = rr::zscan_resid(a, s);
+    if ((st & RR_STAGE_RESID) && d->precision == RR_PREC_F16R) {
+        // resid_tile: the lane kernels' table, then the matrix-core kernel's
+        const int nlane = rr::zscan_resid_tiles(nullptr, 0);
+        const int nmfma = rr::zscan_resid_mfma_tiles(nullptr, 0);
+        int t = d->resid_tile;
+        if (t < 0 || t >= nlane + nmfma)
+            return rr::fail(RR_EINVAL, "rr_zscan(residual): resid_tile %d outside 0..%d", t, nlane + nmfma - 1);
+        if (t == 0 && resid_mfma_inputs(d)) t = nlane + kResidMfmaDefault;
+        if (t >= nlane) {
+            if (!resid_mfma_inputs(d))
+                return rr::fail(RR_EINVAL, "rr_zscan(residual): resid_tile %d needs d_tcache_u, d_rtile3, "
+                                           "d_scales3, uniform items and halo 8", t);
+            rc = rr::zscan_resid_mfma(a, t - nlane, s);
+        } else {
+            if (!lane_inputs(d))
+                return rr::fail(RR_EINVAL, "rr_zscan(residual): resid_tile %d needs d_tcache_r and d_rlane", t);
+            rc = rr::zscan_resid(a, s);
+        }
+    }
     return rc;
 }
 
@@ -235,7 +265,19 @@ extern "C" int rr_zscan_tile_table(int precision, int *cfg, int max) {
     return 0;
 }
 
+// Entries: the lane kernel's configurations (kind 1), then the matrix-core
+// kernel's (kind 2).
 extern "C" int rr_zscan_resid_tiles(int *cfg, int max) {
     if (!cfg || max < 0) return rr::fail(RR_EINVAL, "rr_zscan_resid_tiles: bad arguments");
-    return rr::zscan_resid_tiles(cfg, max);
+    const int nlane = rr::zscan_resid_tiles(cfg, max);
+    const int room = max > nlane ? max - nlane : 0;
+    const int nmfma = rr::zscan_resid_mfma_tiles(cfg + 4 * (max < nlane ? max : nlane), room);
+    (void)nmfma;
+    return nlane + rr::zscan_resid_mfma_tiles(nullptr, 0);
+}
+
+extern "C" int rr_resid_u_slots(int nb, int *nprod) {
+    if (nb < 1 || nb > RR_MAX_NBASIS) return rr::fail(RR_EINVAL, "rr_resid_u_slots: nb=%d outside 1..%d", nb, RR_MAX_NBASIS);
+    if (nprod) *nprod = rr::resid_u_nprod(nb);
+    return rr::resid_u_slots(nb);
 }

@@ -28,14 +28,19 @@ struct TileMap {
 
 // resident: workgroups one XCD holds at once; bytes_item / bytes_z: bytes one
 // item tile / one z tile reads per pixel block (they set the best aspect).
-inline TileMap make_tile_map(int nti, int ntz, int resident, double bytes_item, double bytes_z) {
+// fit: the super-tile holds at most `resident` workgroups (all of it runs at
+// once); otherwise at least `resident`.
+inline TileMap make_tile_map(int nti, int ntz, int resident, double bytes_item, double bytes_z,
+                             bool fit = false) {
     TileMap m;
     m.nti = nti;
     m.ntz = ntz;
     // minimise bytes per workgroup: bytes_item / sz + bytes_z / si at si*sz = resident
     int si = (int)std::lround(std::sqrt(resident * bytes_z / bytes_item));
     si = si < 1 ? 1 : (si > nti ? nti : si);
-    int sz = (resident + si - 1) / si;
+    if (fit)   // a divisor of resident: super-tiles then start and run in lockstep batches
+        while (si > 1 && resident % si) --si;
+    int sz = fit ? resident / si : (resident + si - 1) / si;
     sz = sz < 1 ? 1 : (sz > ntz ? ntz : sz);
     m.si = si;
     m.sz = sz;

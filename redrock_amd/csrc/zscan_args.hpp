@@ -30,7 +30,16 @@ struct ZArgs {
     int solve;             // RR_SOLVE_PCA, RR_SOLVE_NNLS or RR_SOLVE_BVLS
     int nonneg_mask;       // RR_SOLVE_NNLS: bit j set = coefficient j >= 0
     const double *bounds;  // RR_SOLVE_BVLS: (2, nb) lower then upper bounds
+    const void *tcache_u;  // RR_PREC_F16R, matrix-core residual: template operand cache
+    const void *rtile3;    //   hi and lo R tiles (rr_pack_mixed, F16X3)
+    const double *scales3; //   and their scales
 };
+
+// Hi/lo products of the matrix-core residual's u = T c (Th.Ch, Tl.Ch, Th.Cl
+// and, when it still fits the same number of 32-slot MFMAs or needs two
+// anyway, Tl.Cl) and the slots per (z, row) of its template operand cache.
+__host__ __device__ constexpr int resid_u_nprod(int nb) { return 4 * nb <= 32 ? 4 : (3 * nb <= 32 ? 3 : 4); }
+__host__ __device__ constexpr int resid_u_slots(int nb) { return 32 * ((resid_u_nprod(nb) * nb + 31) / 32); }
 
 // Kernel-argument form of rr_polish_desc.
 struct PolishArgs {
@@ -74,6 +83,8 @@ int zscan_f64r(const ZArgs &a, hipStream_t s);
 int zscan_f64r_tiles(int *cfg, int max);
 int zscan_resid(const ZArgs &a, hipStream_t s);
 int zscan_resid_tiles(int *cfg, int max);
+int zscan_resid_mfma(const ZArgs &a, int cfg, hipStream_t s);
+int zscan_resid_mfma_tiles(int *cfg, int max);
 // pixels of the lane-major per-target pack (a multiple of 22, >= npix_pad)
 __host__ __device__ constexpr long long lane_pixels(long long npix_pad) { return rr_lane_pixels(npix_pad); }
 int zscan_mfma(const ZArgs &a, hipStream_t s);

@@ -230,6 +230,9 @@ class TemplateCache(object):
               unscaled: the caches of the lane-per-item kernels
         f16/bf16/f32: [nz][nb][cache_rows] (pixel fastest), values * tscale
         f16x3: fp16 [nz][hi, lo][nb][cache_rows], hi + lo = values * tscale
+        f16u: fp16 [nz][cache_rows][KS], the template operand of the 'f16r'
+              matrix-core residual: slots hi | lo | hi [| lo] of values *
+              tscale per basis vector (_hip.resid_u_slots)
         Returns (tensor, ldz).
         """
         import torch
@@ -252,6 +255,22 @@ class TemplateCache(object):
             for k, off in zip(layout.keys, layout.offsets):
                 rebin_into(buf[(layout.halo + off) * ldz:], d['x'], d['y'], d['edges'][k], d['z'],
                            1, ldz, rows * ldz, _hip.RR_F64, 1.0, self.lyman(k), d['wave'][k])
+        elif precision == 'f16u':
+            ldz = 0
+            ks, nprod = _hip.resid_u_slots(nb)
+            full = torch.zeros(nz * rows * nb, dtype=torch.float32, device='cuda')
+            for k, off in zip(layout.keys, layout.offsets):
+                rebin_into(full[(layout.halo + off) * nb:], d['x'], d['y'], d['edges'][k], d['z'],
+                           rows * nb, nb, 1, _hip.RR_F32, self.tscale, self.lyman(k), d['wave'][k])
+            full = full.reshape(nz, rows, nb)
+            hi = full.half()
+            lo = (full - hi.float()).half()
+            del full
+            buf = torch.zeros((nz, rows, ks), dtype=torch.float16, device='cuda')
+            for j, part in enumerate((hi, lo, hi, lo)[:nprod]):
+                buf[:, :, j * nb:(j + 1) * nb] = part
+            del hi, lo
+            buf = buf.reshape(-1)
         elif precision == 'f16x3':
             ldz = 0
             full = torch.zeros(nz * nb * rows, dtype=torch.float32, device='cuda')
@@ -447,14 +466,17 @@ class ScanPlan(object):
 
     def __init__(self, cache, batch, item_target, item_z0, item_nz, precision='f64',
                  prior=None, oiisum=None, tile=0, polish=None, polish_windows=POLISH_WINDOWS,
-                 solve='PCA', bounds=None, lanes=None):
+                 solve='PCA', bounds=None, lanes=None, resid=None):
         """polish: FP64 re-evaluation of the rows near each item's minimum after a
         mixed-precision scan (rr_zscan_polish), see polish_tol(); it needs the
         FP64 template cache as well.  solve: the template's method, 'PCA' or
         'NMF' / 'NNLS' (non-negative coefficients, no polish), or 'BVLS' with
         `bounds` (solve_spec).  lanes: FP64 only, scan with one item per lane
         (items sharing one redshift range); None = when there are at least
-        LANE_MIN_ITEMS such items ('f16r' always needs them)."""
+        LANE_MIN_ITEMS such items ('f16r' always needs them).  resid ('f16r'):
+        inputs of the explicit-residual stage to build, 'mfma' (the
+        matrix-core kernel, the default), 'lanes' (the FP32 lane kernel) or
+        'both' (every configuration of rr_zscan_resid_tiles can run)."""
         import torch
         if precision not in _hip.PRECISIONS:
             raise ValueError(f'unknown precision {precision!r}')
@@ -530,7 +552,17 @@ class ScanPlan(object):
             d.d_rtile, d.d_scales = rtile.data_ptr(), scales.data_ptr()
             d.d_sfcol = None if sfcol is None else sfcol.data_ptr()
             self._keep += [rtile, sfcol, scales]
-        if self.lanes and self.nitems > 0:
+        resid = RESID_DEFAULT if resid is None else resid
+        if resid not in ('mfma', 'lanes', 'both'):
+            raise ValueError(f"resid must be 'mfma', 'lanes' or 'both', not {resid!r}")
+        self.resid = resid if precision == 'f16r' else None
+        if precision == 'f16r' and resid in ('mfma', 'both') and self.nitems > 0:
+            tu, _ = cache.get(layout, 'f16u')
+            rt3, _, sc3 = batch.mixed('f16x3')
+            d.d_tcache_u, d.d_rtile3, d.d_scales3 = tu.data_ptr(), rt3.data_ptr(), sc3.data_ptr()
+            self._keep += [tu, rt3, sc3]
+        lane_in = self.lanes and (precision == 'f64' or resid in ('lanes', 'both'))
+        if lane_in and self.nitems > 0:
             tr, _ = cache.get(layout, 'f64r' if precision == 'f64' else 'f32r')
             identity = np.array_equal(item_target, np.arange(len(item_target)))
             rl = batch.lanes(precision, None if identity else item_target)
@@ -628,6 +660,10 @@ class ScanPlan(object):
         s.synchronize()
         return tuple(h.numpy() for h in hs)
 
+
+# Inputs a 'f16r' plan builds for its explicit-residual stage (ScanPlan resid=):
+# the matrix-core kernel's (split-fp16 template operand cache and R tiles)
+RESID_DEFAULT = 'mfma'
 
 # Smallest item count for which an FP64 plan of items sharing one redshift
 # range takes the lane-per-item kernel (fewer items would leave lanes idle;

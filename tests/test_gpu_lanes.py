@@ -131,10 +131,11 @@ def test_f16r_conventions_and_polish(galaxy70):
 @pytest.mark.parametrize('kind', ['galaxy', 'qso'])
 def test_every_lane_configuration(galaxy70, kind):
     """Every configuration the autotuner may pick: the FP64 lane kernel's
-    (waves per workgroup) give identical bits; the residual kernel's
-    (redshifts per wave, waves, pixels per LDS block) differ from the default
-    only by the FP32 partial sums' blocking and stay within the f16r bound of
-    the FP64 scan, HUGE_CHI2 rows kept."""
+    (waves per workgroup) give identical bits; the residual stage's -- the
+    matrix-core kernel's (default; redshift blocks and waves per workgroup,
+    identical bits to each other) and the FP32 lane kernel's (redshifts per
+    wave, waves, pixels per LDS block) -- differ only by rounding and stay
+    within the f16r bound of the FP64 scan, HUGE_CHI2 rows kept."""
     import torch
     from redrock_amd import _hip, autotune
     if kind == 'galaxy':
@@ -151,17 +152,21 @@ def test_every_lane_configuration(galaxy70, kind):
         torch.cuda.synchronize()
         print(f'case: {kind} f64 lane tile {t}', flush=True)
         assert np.array_equal(p64.zchi2.cpu().numpy().reshape(70, nz), c64)
-    c0, _, _, plan = _plan(dt, targets, 'f16r', polish=False)
+    c0, _, _, plan = _plan(dt, targets, 'f16r', polish=False, resid='both')
     ok = np.arange(70) != 5
-    for t in range(1, len(autotune.resid_table())):
+    table = autotune.resid_table()
+    assert table[-1][0] == 2
+    for t in range(1, len(table)):
         plan.desc.resid_tile = t
         plan.run(stages=_hip.RR_STAGE_RESID)
         torch.cuda.synchronize()
         c = plan.zchi2.cpu().numpy().reshape(70, nz)
         rel0 = np.abs(c[ok] - c0[ok]) / c0[ok]
         rel = np.abs(c[ok] - c64[ok]) / c64[ok]
-        print(f'case: {kind} residual tile {t}: vs default {rel0.max():.2e}, vs f64 {rel.max():.2e}',
-              flush=True)
-        assert rel0.max() <= 1e-6       # FP32 partial sums over blocks of 11 or 22 pixels
+        print(f'case: {kind} residual tile {t} {table[t]}: vs default {rel0.max():.2e}, '
+              f'vs f64 {rel.max():.2e}', flush=True)
+        if table[t][0] == 2:            # matrix-core kernel: same arithmetic as the default
+            assert np.array_equal(c, c0)
+        assert rel0.max() <= 1e-6       # FP32 lane kernel: its own rounding
         assert rel.max() <= F16R_RTOL
         assert np.all(c[5] == 9999999.0)
