@@ -280,7 +280,7 @@ int launch_tile(const rr::ZArgs &a, hipStream_t s) {
     auto kernel = zscan_mfma_kernel<NB, TY, WA, WB, TW, ZW>;
     const rr::TileMap map = rr::make_tile_map((a.nitems + TT - 1) / TT, (a.max_item_nz + ZR - 1) / ZR,
                                               rr::resident_per_xcd(kernel, 256),
-                                              TT * 2.0 * 64 * 16, ZR * NB * 6.0 * 16);
+                                              TT * 2.0 * 64 * 16, ZR * NB * 6.0 * 16, true);
     const long long ne_off = rr::align256((long long)a.nitems * 2 * 8);
     hipLaunchKernelGGL(kernel, dim3((unsigned)map.blocks), dim3(256), 0, s, a, ne_off, map);
     return rr::check_launch("rr_zscan(mixed)");
