@@ -9,9 +9,9 @@ already resident in HBM (template cache built once, as DistTemplate does).
 
 The headline value is the 'f16r' scan: fp16 normal equations on MFMA (fp32
 accumulation) for the coefficients, FP64 Cholesky, then chi2 as the
-reference's explicit residual sum w (f - R T c)^2 at those coefficients in
-FP32 with FP64 partial sums, plus the FP64 polish of the rows near each
-minimum.  The plain FP64 scan (one target per lane) runs on the same inputs
+reference's explicit residual sum w (f - R T c)^2 at those coefficients
+(u = T c and the R-apply as split-fp16 MFMAs, fp32 accumulation, FP64 sums),
+plus the FP64 polish of the rows near each minimum.  The plain FP64 scan (one target per lane) runs on the same inputs
 for the speedup and the best-z agreement; the split-fp16 (f16x3) and the
 single-pass fp16 / bf16 / fp32-MFMA modes are reported beside it as
 variants.  The CPU baseline is the float64 oracle (a numpy restatement of

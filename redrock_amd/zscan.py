@@ -15,8 +15,8 @@ The ``use_gpu`` and ``fullprecision`` arguments are accepted for signature
 compatibility (the scan is always on the GPU and always deterministic).
 ``precision`` selects 'f64' (default, the reference's arithmetic) or a
 mixed-precision MFMA form: 'f16r' (fp16 normal equations for the
-coefficients, chi2 as the explicit FP32 residual at them: within ~1e-7 of
-FP64), 'f16x3' (split fp16, chi2 within ~1e-5 of FP64 at per-pixel S/N 1),
+coefficients, chi2 as the explicit residual at them, evaluated with
+split-fp16 MFMA products: within ~1e-6 of FP64), 'f16x3' (split fp16, chi2 within ~1e-5 of FP64 at per-pixel S/N 1),
 'f16', 'bf16' or 'f32'; the default can be changed with the environment
 variable RR_PRECISION.
 
