@@ -257,20 +257,11 @@ class TemplateCache(object):
                            1, ldz, rows * ldz, _hip.RR_F64, 1.0, self.lyman(k), d['wave'][k])
         elif precision == 'f16u':
             ldz = 0
-            ks, nprod = _hip.resid_u_slots(nb)
             full = torch.zeros(nz * rows * nb, dtype=torch.float32, device='cuda')
             for k, off in zip(layout.keys, layout.offsets):
                 rebin_into(full[(layout.halo + off) * nb:], d['x'], d['y'], d['edges'][k], d['z'],
                            rows * nb, nb, 1, _hip.RR_F32, self.tscale, self.lyman(k), d['wave'][k])
-            full = full.reshape(nz, rows, nb)
-            hi = full.half()
-            lo = (full - hi.float()).half()
-            del full
-            buf = torch.zeros((nz, rows, ks), dtype=torch.float16, device='cuda')
-            for j, part in enumerate((hi, lo, hi, lo)[:nprod]):
-                buf[:, :, j * nb:(j + 1) * nb] = part
-            del hi, lo
-            buf = buf.reshape(-1)
+            buf = _u_operand(full.reshape(nz, rows, nb))
         elif precision == 'f16x3':
             ldz = 0
             full = torch.zeros(nz * nb * rows, dtype=torch.float32, device='cuda')
@@ -558,6 +549,11 @@ class ScanPlan(object):
         self.resid = resid if precision == 'f16r' else None
         if precision == 'f16r' and resid in ('mfma', 'both') and self.nitems > 0:
             tu, _ = cache.get(layout, 'f16u')
+            # the kernel reads (cache_nz, cache_rows, KS) fp16 elements
+            need = cache.nz * layout.cache_rows * _hip.resid_u_slots(nb)[0]
+            if tu.dtype != torch.float16 or tu.numel() != need:
+                raise ValueError(f'template operand cache holds {tu.numel()} {tu.dtype} elements, '
+                                 f'the residual kernel reads {need} fp16')
             rt3, _, sc3 = batch.mixed('f16x3')
             d.d_tcache_u, d.d_rtile3, d.d_scales3 = tu.data_ptr(), rt3.data_ptr(), sc3.data_ptr()
             self._keep += [tu, rt3, sc3]
@@ -859,6 +855,21 @@ def calc_zchi2_batch(spectra, tdata, weights, flux, wflux, nz, nbasis, solve_mat
     return c2, cf
 
 
+def _u_operand(full):
+    """The template operand cache of the matrix-core residual from a float32
+    (nz, cache_rows, nb) cache (values * tscale): flat fp16 (nz, cache_rows,
+    KS), K slots hi | lo | hi [| lo] per basis vector (rr_resid_u_slots)."""
+    import torch
+    nz, rows, nb = full.shape
+    ks, nprod = _hip.resid_u_slots(nb)
+    hi = full.half()
+    lo = (full - hi.float()).half()
+    buf = torch.zeros((nz, rows, ks), dtype=torch.float16, device=full.device)
+    for j, part in enumerate((hi, lo, hi, lo)[:nprod]):
+        buf[:, :, j * nb:(j + 1) * nb] = part
+    return buf.reshape(-1)
+
+
 def _host(a):
     if hasattr(a, 'detach'):
         return a.detach().cpu().numpy()
@@ -892,7 +903,14 @@ class _ExplicitCache(object):
         if key in self._bufs:
             return self._bufs[key]
         nb, nz, rows = self.nb, self.nz, layout.cache_rows
-        if precision in ('f64r', 'f32r'):
+        if precision == 'f16u':
+            ldz = 0
+            full = torch.zeros((nz, rows, nb), dtype=torch.float32, device='cuda')
+            for k, off, n in zip(layout.keys, layout.offsets, layout.sizes):
+                t = torch.as_tensor(self.tdata[k], dtype=torch.float64, device='cuda')
+                full[:, layout.halo + off:layout.halo + off + n, :] = (t * self.tscale).float()
+            buf = _u_operand(full)
+        elif precision in ('f64r', 'f32r'):
             ldz = 0
             dt = torch.float64 if precision == 'f64r' else torch.float32
             buf = torch.zeros((nz, rows, nb), dtype=dt, device='cuda')
@@ -918,10 +936,12 @@ class _ExplicitCache(object):
                 buf = torch.cat([buf.reshape(-1), torch.zeros(64, dtype=torch.float32, device='cuda')])
             elif precision == 'f16x3':
                 buf = _split_planes(buf)
-            else:
+            elif precision in ('f16', 'bf16'):
                 dt = torch.float16 if precision == 'f16' else torch.bfloat16
                 buf = torch.cat([buf.to(dt).view(torch.int16).reshape(-1),
                                  torch.zeros(64, dtype=torch.int16, device='cuda')])
+            else:
+                raise ValueError(f'no template cache layout {precision!r}')
         self._bufs[key] = (buf, ldz)
         return self._bufs[key]
 

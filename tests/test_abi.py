@@ -191,3 +191,30 @@ def test_workspace_size_formula():
     d.precision = _hip.RR_PREC_F16
     ne = 10 * 12 - 45
     assert lib.rr_zscan_workspace_bytes(ctypes.byref(d)) == 256 + ((ne * 1000 * 8 + 255) // 256) * 256
+
+
+def test_resid_u_operand_layout():
+    """The matrix-core residual's template operand (zscan._u_operand, read by
+    zscan_resid_mfma.hip): per (z, row) KS fp16 slots hi | lo | hi [| lo] of
+    each basis value, zero beyond, hi + lo within 2^-22 of the float32 value
+    (and 2^-25 absolute where lo is an fp16 subnormal);
+    KS and the product count as rr_resid_u_slots states for every nb."""
+    import torch
+    from redrock_amd import _hip, zscan
+    rng = np.random.default_rng(3)
+    for nb in range(1, _hip.RR_MAX_NBASIS + 1):
+        ks, nprod = _hip.resid_u_slots(nb)
+        assert nprod == (4 if 4 * nb <= 32 else 3 if 3 * nb <= 32 else 4)
+        assert ks == 32 * ((nprod * nb + 31) // 32)
+        full = torch.from_numpy(rng.normal(size=(3, 7, nb)).astype(np.float32) * 100)
+        buf = zscan._u_operand(full).reshape(3, 7, ks).double().numpy()
+        f = full.numpy().astype(np.float64)
+        hi, lo = buf[..., :nb], buf[..., nb:2 * nb]
+        # 2^-22 relative, and half the fp16 subnormal spacing (2^-25) where lo is subnormal
+        assert np.all(np.abs(hi + lo - f) <= 2.0 ** -22 * np.abs(f) + 2.0 ** -25)
+        assert np.array_equal(buf[..., 2 * nb:3 * nb], hi)
+        if nprod == 4:
+            assert np.array_equal(buf[..., 3 * nb:4 * nb], lo)
+        assert np.all(buf[..., nprod * nb:] == 0)
+    with pytest.raises(Exception):
+        _hip.resid_u_slots(0)

@@ -170,3 +170,50 @@ def test_every_lane_configuration(galaxy70, kind):
         assert rel0.max() <= 1e-6       # FP32 lane kernel: its own rounding
         assert rel.max() <= F16R_RTOL
         assert np.all(c[5] == 9999999.0)
+
+
+def _basis(nb):
+    """A GALAXY-like basis of nb vectors: the synthetic galaxy template's first
+    nb, or for nb > 10 those 10 plus extra emission-line components."""
+    from redrock_amd import synth
+    from redrock_amd.templates import Template
+    g = synth.galaxy_template()
+    if nb <= g.nbasis:
+        flux = g.flux[:nb]
+    else:
+        rng = np.random.default_rng(nb)
+        extra = [synth._gauss(g.wave, mu, 2.0) for mu in (3869.0, 6717.0)][:nb - g.nbasis]
+        extra = synth._with_texture(extra, g.wave, rng, 0.3)
+        flux = synth._orthonormalise(list(g.flux) + list(extra), rng, mix=False)
+    return Template(spectype='GALAXY', redshifts=g.redshifts[700:745], wave=g.wave, flux=flux,
+                    igm_model=g.igm_model)
+
+
+@pytest.mark.parametrize('nb', [1, 2, 3, 7, 8, 9, 11, 12])
+def test_f16r_every_basis_size(gpu, nb):
+    """The matrix-core residual at every basis size it instantiates: the
+    products u = T c take 3 or 4 hi/lo terms in one or two MFMAs per tile
+    (rr_resid_u_slots), items in 16-wide tiles (34 targets: the last tile
+    partial), 45 redshifts (the last 32-redshift block partial); within the
+    f16r bound of the FP64 scan (itself within 1e-9 of the oracle) and of the
+    oracle on three targets, same best redshift."""
+    from redrock_amd import _hip, synth
+    from redrock_amd.templates import DistTemplate
+    tm = _basis(nb)
+    assert tm.nbasis == nb
+    targets, _ = synth.make_targets(synth.galaxy_template(), 34, seed=60 + nb, zrange=(0.45, 0.6))
+    dwave = {s.wavehash: s.wave for s in targets[0].spectra}
+    dt = DistTemplate(tm, dwave)
+    ks, nprod = _hip.resid_u_slots(nb)
+    print(f'case: f16r nb {nb} (K slots {ks}, {nprod} products), 34 targets x 45 z', flush=True)
+    chi2, _, _, plan = _plan(dt, targets, 'f16r', polish=False)
+    assert plan.desc.d_tcache_u
+    c64 = _plan(dt, targets, 'f64')[0]
+    rel = np.abs(chi2 - c64) / c64
+    print(f'nb {nb}: f16r vs f64 max rel {rel.max():.2e}')
+    assert rel.max() <= F16R_RTOL
+    assert np.array_equal(chi2.argmin(axis=1), c64.argmin(axis=1))
+    sel = [0, 17, 33]
+    rchi2, _, _ = orc.calc_zchi2([targets[i] for i in sel], orc.template_cache(tm, dwave), nb)
+    assert (np.abs(chi2[sel] - rchi2) / rchi2).max() <= F16R_RTOL
+    assert np.all(chi2 >= c64 * (1 - 2e-7))

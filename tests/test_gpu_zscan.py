@@ -125,6 +125,13 @@ def test_calc_zchi2_batch_with_prior(gpu):
                                          prior=prior, precision='bf16', polish=10.0)
     assert np.max(np.abs(chi2 - rchi2) / rchi2) <= F64_RTOL
     assert np.allclose(coeff, rcoeff, rtol=1e-7, atol=1e-9 * np.abs(rcoeff).max())
+    # f16r: coefficients from the fp16 normal equations with the prior, chi2
+    # the explicit residual (which, as the reference's, leaves the prior out)
+    print('case: f16r, prior, multi-exposure, 60 z', flush=True)
+    chi2, coeff = zscan.calc_zchi2_batch(spectra, tdata, w, f, wf, len(zsel), tm.nbasis, 'PCA',
+                                         prior=prior, precision='f16r', polish=False)
+    assert np.max(np.abs(chi2 - rchi2) / rchi2) <= 1e-6
+    assert np.array_equal(chi2.argmin(), rchi2.argmin())
 
 
 def test_calc_zchi2_batch_bvls_archetype_bounds(gpu):
