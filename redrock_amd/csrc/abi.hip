@@ -153,7 +153,9 @@ extern "C" int rr_zscan(const rr_zscan_desc *d, void *stream) {
         int t = d->resid_tile;
         if (t < 0 || t >= nlane + nmfma)
             return rr::fail(RR_EINVAL, "rr_zscan(residual): resid_tile %d outside 0..%d", t, nlane + nmfma - 1);
-        if (t == 0 && resid_mfma_inputs(d)) t = nlane + kResidMfmaDefault;
+        // default: 16 waves (4 per SIMD, 128 VGPRs) up to nb 10, where it
+        // fits without spills; 8 waves beyond
+        if (t == 0 && resid_mfma_inputs(d)) t = nlane + (d->nb <= 10 ? kResidMfmaDefault : 1);
         if (t >= nlane) {
             if (!resid_mfma_inputs(d))
                 return rr::fail(RR_EINVAL, "rr_zscan(residual): resid_tile %d needs d_tcache_u, d_rtile3, "

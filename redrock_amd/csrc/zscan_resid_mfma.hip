@@ -37,7 +37,8 @@
 // Per 32-pixel chunk: phase A (the wave's ZB / NW redshifts, 2 pixel tiles)
 // writes U of pixels [32c + 8, 32c + 40) into a 64-pixel ring in LDS; barrier;
 // phase C (the wave's 16 / NW targets x NZB redshift blocks, 2 output tiles)
-// reads U of pixels [32c - 8, 32c + 40); barrier.  Template rows and R tiles
+// reads U of pixels [32c - 8, 32c + 40) into registers; barrier; phase C's
+// MFMAs run beside the next chunk's phase A.  Template rows and R tiles
 // stream from global memory (L2) into registers one chunk ahead.
 //
 // Reference path replaced: zscan.py:752-780 (chi2 of each (target, z)).
@@ -240,31 +241,37 @@ __global__ __launch_bounds__(64 * NW) void zscan_resid_mfma_kernel(rr::ZArgs a, 
 #pragma unroll
         for (int k = 0; k < NZB; ++k) acc[j][k] = 0.0;
 
-    // phase C of chunk c: output tiles 2c, 2c + 1.  Per target the U operands
-    // of both tiles and every redshift block are read first, then the MFMA
-    // chains run (independent chains interleave).
-    auto phase_c = [&](int c, const u16x8 (&rt)[TPW][2][2]) {
+    // phase C of chunk c: output tiles 2c, 2c + 1.  read_c takes every U
+    // operand of the wave (its targets, both tiles, every redshift block) into
+    // registers; once every wave has done so (a barrier) the ring may be
+    // rewritten, and mfma_c runs the chains beside the next chunk's phase A.
+    typedef u32x4 UOps[TPW][NZB][2];
+    auto read_c = [&](int c, UOps &uh, UOps &ul) {
 #pragma unroll
         for (int j = 0; j < TPW; ++j) {
-            // slots past the last item compute on a repeated target (no
-            // branch: every path waits for the same loads) and write nothing
             const int tl = wave * TPW + j;
-            u32x4 uh[NZB][2], ul[NZB][2];
 #pragma unroll
             for (int k = 0; k < NZB; ++k)
 #pragma unroll
                 for (int qq = 0; qq < 2; ++qq) {
                     const int q = 2 * c + qq;
                     const int off = tl * TSH + (16 * k + i16) * kRowH + ((16 * q + 8 * g) & (kRing - 1));
-                    uh[k][qq] = *reinterpret_cast<const u32x4 *>(Uh + off);
-                    ul[k][qq] = *reinterpret_cast<const u32x4 *>(Ul + off);
+                    uh[j][k][qq] = *reinterpret_cast<const u32x4 *>(Uh + off);
+                    ul[j][k][qq] = *reinterpret_cast<const u32x4 *>(Ul + off);
                 }
+        }
+    };
+    // slots past the last item compute on a repeated target (no branch: every
+    // path waits for the same loads) and write nothing
+    auto mfma_c = [&](const u16x8 (&rt)[TPW][2][2], const UOps &uh, const UOps &ul) {
+#pragma unroll
+        for (int j = 0; j < TPW; ++j) {
 #pragma unroll
             for (int k = 0; k < NZB; ++k) {
                 float part = 0.f;
 #pragma unroll
                 for (int qq = 0; qq < 2; ++qq) {
-                    u32x4 h = uh[k][qq], l = ul[k][qq];
+                    u32x4 h = uh[j][k][qq], l = ul[j][k][qq];
                     h.x = g == 0 ? negb[j][k].x : h.x;
                     l.x = g == 0 ? 0u : l.x;
                     const u16x8 bh = __builtin_bit_cast(u16x8, h), bl = __builtin_bit_cast(u16x8, l);
@@ -309,10 +316,12 @@ __global__ __launch_bounds__(64 * NW) void zscan_resid_mfma_kernel(rr::ZArgs a, 
     load_t(24, ta[1]);
     load_r(0, ra);
     // step c: loads of chunk c + 1, phase A of chunk c (pixels [32c + 8,
-    // 32c + 40)), barrier, phase C of chunk c, barrier.  The loads are issued
-    // on every step (the last one re-reads its own chunk) so that every path
-    // reaches phase A with the same loads outstanding and the wait for the
-    // current operands leaves the next chunk's loads in flight.
+    // 32c + 40)), barrier, the U operands of chunk c into registers, barrier,
+    // the phase C chains of chunk c (which the next step's phase A follows
+    // without a barrier).  The loads are issued on every step (the last one
+    // re-reads its own chunk) so that every path reaches phase A with the same
+    // loads outstanding and the wait for the current operands leaves the next
+    // chunk's loads in flight.
     auto step = [&](int c, u16x8 (&tcur)[2][ZPW][NMF], u16x8 (&tnxt)[2][ZPW][NMF],
                     u16x8 (&rcur)[TPW][2][2], u16x8 (&rnxt)[TPW][2][2]) {
         const int cn = c + 1 < nchunk ? c + 1 : c;
@@ -322,8 +331,10 @@ __global__ __launch_bounds__(64 * NW) void zscan_resid_mfma_kernel(rr::ZArgs a, 
         phase_a(32 * c + 8, tcur[0]);
         phase_a(32 * c + 24, tcur[1]);
         __syncthreads();
-        phase_c(c, rcur);
+        UOps uh, ul;
+        read_c(c, uh, ul);
         __syncthreads();
+        mfma_c(rcur, uh, ul);
     };
     // pairs of steps without a condition inside (a load whose only use sat in
     // a conditional block would be sunk into it, next to its use), then the
