@@ -691,12 +691,9 @@ def bench_petal(args):
     t = time.perf_counter()
     for _ in range(args.steps):
         batch._mixed.clear()
-        batch.mixed(args.precision)
-        if args.precision == 'f16r':
-            if mix.resid in ('mfma', 'both'):
-                batch.mixed('f16x3')       # the matrix-core residual's hi + lo R tiles
-            if mix.resid in ('lanes', 'both'):
-                batch.lanes('f16r')
+        batch.mixed(args.precision)        # f16r: the split-fp16 tiles of both its stages
+        if args.precision == 'f16r' and mix.resid in ('lanes', 'both'):
+            batch.lanes('f16r')
     torch.cuda.synchronize()
     t_pack = (time.perf_counter() - t) / args.steps
     # final full runs so the outputs hold complete results

@@ -20,7 +20,7 @@
 extern "C" {
 #endif
 
-#define RR_ABI_VERSION 11
+#define RR_ABI_VERSION 12
 
 /* Error codes. */
 #define RR_OK 0
@@ -148,9 +148,10 @@ typedef struct {
     /* Mixed precision modes only (NULL otherwise), written by rr_pack_mixed: */
     const void *d_rtile;     /* (ntarget, npix_pad/16, 64, 8) MFMA A-operand
                                 tiles of sqrt(w) R (16-bit: K slots 0, 1 of
-                                each row hold sqrt(w) f split hi + lo; F16X3:
-                                (ntarget, npix_pad/16, 2, 64, 8), hi and lo
-                                tiles)                                          */
+                                each row hold sqrt(w) f split hi + lo; F16X3
+                                and F16R: the F16X3 tiles, (ntarget, 2,
+                                npix_pad/16, 64, 8), hi then lo plane; F16R's
+                                fp16 normal equations read the hi plane)        */
     const void *d_sfcol;     /* RR_PREC_F32 only (NULL allowed otherwise):
                                 (ntarget, npix_pad/32, 4, 8) fp32 sqrt(w) f in
                                 MFMA operand order                              */
@@ -226,19 +227,15 @@ typedef struct {
     const double *d_bounds;
 
     /* F16R explicit residual on the matrix cores (the configurations of
-     * kind 2 in rr_zscan_resid_tiles, and the default when all three are
-     * set; NULL = not used).  Needs uniform_z and halo == 8.
+     * kind 2 in rr_zscan_resid_tiles, and the default when set; NULL = not
+     * used).  Needs uniform_z and halo == 8; reads d_rtile (F16X3 tiles).
      *  d_tcache_u: fp16 template operand, element (zc, row, s) at
      *         (zc*cache_rows + row)*KS + s, KS = rr_resid_u_slots(nb, &np):
      *         s < nb: hi(T[s]); nb <= s < 2nb: lo(T[s-nb]); 2nb <= s < 3nb:
      *         hi(T[s-2nb]); 3nb <= s < 4nb when np == 4: lo(T[s-3nb]); other
      *         slots 0.  T = template value * tscale, hi = fp16(T),
-     *         lo = fp16(T - hi), round to nearest even.
-     *  d_rtile3, d_scales3: rr_pack_mixed outputs for RR_PREC_F16X3 (hi and
-     *         lo tiles of sqrt(w) R, scales)                                 */
+     *         lo = fp16(T - hi), round to nearest even.                     */
     const void *d_tcache_u;
-    const void *d_rtile3;
-    const double *d_scales3;
 } rr_zscan_desc;
 #define RR_SOLVE_PCA 0
 #define RR_SOLVE_NNLS 1
@@ -323,7 +320,9 @@ int rr_pack_lanes(const double *d_rband, const double *d_w, const double *d_f, c
  * npix_pad[, 11]) arrays of rr_zscan_desc; precision is RR_PREC_F16 or
  * RR_PREC_BF16 (16-bit tiles; sqrt(w) f split hi + lo into the two K slots of
  * each tile row that lie outside the band; d_sfcol is not used and may be
- * NULL), RR_PREC_F16X3 (hi and lo fp16 tiles, twice the 16-bit size;
+ * NULL), RR_PREC_F16X3 (hi and lo fp16 tiles, twice the 16-bit size, in
+ * planes (ntarget, 2, npix_pad/16, 64, 8); the hi plane equals the
+ * RR_PREC_F16 tiles value for value; also the tiles of RR_PREC_F16R;
  * d_sfcol not used) or RR_PREC_F32 (fp32 tiles of twice the 16-bit size,
  * rtile (ntarget, npix_pad/16, 64, 8) floats, and sqrt(w) f in d_sfcol
  * (ntarget, npix_pad/32, 4, 8) floats).
@@ -355,7 +354,7 @@ int rr_zscan_tile_table(int precision, int *cfg, int max);
  * entry i = (1, waves per workgroup, 64 items per wave, redshifts per wave),
  * then the matrix-core kernel's, entry i = (2, waves per workgroup, 16 items
  * per workgroup, redshifts per workgroup).  Entry 0 is the default: the
- * matrix-core kernel when d_tcache_u, d_rtile3 and d_scales3 are set, else
+ * matrix-core kernel when d_tcache_u is set, else
  * the lane kernel.  Returns the number of entries.
  */
 int rr_zscan_resid_tiles(int *cfg, int max);

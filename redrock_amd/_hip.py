@@ -12,7 +12,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 # RR_LIB_PATH: load the library from another path (a build kept elsewhere)
 LIB_PATH = os.environ.get('RR_LIB_PATH') or os.path.join(HERE, 'libredrock_hip.so')
 
-RR_ABI_VERSION = 11
+RR_ABI_VERSION = 12
 RR_F64, RR_F32, RR_F16, RR_BF16 = 0, 1, 2, 3
 RR_PREC_F64, RR_PREC_F16, RR_PREC_BF16, RR_PREC_F32, RR_PREC_F16X3, RR_PREC_F16R = 0, 1, 2, 3, 4, 5
 RR_STATUS_OK, RR_STATUS_SINGULAR = 0, 1
@@ -56,7 +56,7 @@ class ZscanDesc(ctypes.Structure):
         ('max_item_nz', _i), ('total_rows', _ll),
         ('d_work', _vp), ('work_bytes', _ll),
         ('stages', _i), ('solve', _i), ('nonneg_mask', _i), ('d_bounds', _vp),
-        ('d_tcache_u', _vp), ('d_rtile3', _vp), ('d_scales3', _vp),
+        ('d_tcache_u', _vp),
     ]
 
 

@@ -33,7 +33,7 @@ static bool lane_inputs(const rr_zscan_desc *d) {
 
 // Inputs of the matrix-core explicit residual of F16R.
 static bool resid_mfma_inputs(const rr_zscan_desc *d) {
-    return d->d_tcache_u && d->d_rtile3 && d->d_scales3 && d->uniform_z && d->halo == 8;
+    return d->d_tcache_u && d->uniform_z && d->halo == 8;
 }
 
 // Default configuration of the matrix-core residual (index into its table).
@@ -79,9 +79,9 @@ static int scan_args(const rr_zscan_desc *d, rr::ZArgs &a) {
             return rr::fail(RR_EINVAL, "rr_zscan: cache_ldz < cache_nz");
     } else if (mixed) {
         if (d->precision == RR_PREC_F16R && !lane_inputs(d) && !resid_mfma_inputs(d))
-            return rr::fail(RR_EINVAL, "rr_zscan: F16R needs uniform items and either d_tcache_u, "
-                                       "d_rtile3 and d_scales3 (matrix-core residual) or d_tcache_r and "
-                                       "d_rlane with cache_rows >= halo + rr_lane_pixels(npix_pad) + 5");
+            return rr::fail(RR_EINVAL, "rr_zscan: F16R needs uniform items and either d_tcache_u "
+                                       "(matrix-core residual) or d_tcache_r and d_rlane with "
+                                       "cache_rows >= halo + rr_lane_pixels(npix_pad) + 5");
         if (!d->d_rtile || !d->d_scales || (d->precision == RR_PREC_F32 && !d->d_sfcol))
             return rr::fail(RR_EINVAL, "rr_zscan: mixed precision needs rr_pack_mixed outputs");
         if (d->halo != 8) return rr::fail(RR_EINVAL, "rr_zscan: mixed precision needs halo == 8");
@@ -113,8 +113,7 @@ static int scan_args(const rr_zscan_desc *d, rr::ZArgs &a) {
     a.nonneg_mask = d->nonneg_mask;
     a.bounds = d->d_bounds;
     a.tcache_u = d->d_tcache_u;
-    a.rtile3 = d->d_rtile3;
-    a.scales3 = d->d_scales3;
+    a.rtile_planes = (d->precision == RR_PREC_F16X3 || d->precision == RR_PREC_F16R) ? 2 : 1;
     return RR_OK;
 }
 
@@ -158,8 +157,8 @@ extern "C" int rr_zscan(const rr_zscan_desc *d, void *stream) {
         if (t == 0 && resid_mfma_inputs(d)) t = nlane + (d->nb <= 10 ? kResidMfmaDefault : 1);
         if (t >= nlane) {
             if (!resid_mfma_inputs(d))
-                return rr::fail(RR_EINVAL, "rr_zscan(residual): resid_tile %d needs d_tcache_u, d_rtile3, "
-                                           "d_scales3, uniform items and halo 8", t);
+                return rr::fail(RR_EINVAL, "rr_zscan(residual): resid_tile %d needs d_tcache_u, uniform "
+                                           "items and halo 8", t);
             rc = rr::zscan_resid_mfma(a, t - nlane, s);
         } else {
             if (!lane_inputs(d))

@@ -31,8 +31,7 @@ struct ZArgs {
     int nonneg_mask;       // RR_SOLVE_NNLS: bit j set = coefficient j >= 0
     const double *bounds;  // RR_SOLVE_BVLS: (2, nb) lower then upper bounds
     const void *tcache_u;  // RR_PREC_F16R, matrix-core residual: template operand cache
-    const void *rtile3;    //   hi and lo R tiles (rr_pack_mixed, F16X3)
-    const double *scales3; //   and their scales
+    int rtile_planes;      // R tile planes per target in rtile: 2 (hi, lo) for F16X3 / F16R
 };
 
 // Hi/lo products of the matrix-core residual's u = T c (Th.Ch, Tl.Ch, Th.Cl

@@ -158,7 +158,7 @@ __global__ __launch_bounds__(256) void zscan_mfma_kernel(rr::ZArgs a, long long 
             const int r = c - TCH, u = r >> 7, h = (r >> 6) & 1, l = r & 63;
             const int it = itemb + u < a.nitems ? itemb + u : a.nitems - 1;
             const int t = a.item_target[it];
-            src[k] = RT + (((long long)t * ntile + h) * 64 + l) * 8;
+            src[k] = RT + (((long long)t * ntile * a.rtile_planes + h) * 64 + l) * 8;
             step[k] = 2 * 64 * 8;
         }
     }

@@ -31,20 +31,21 @@ namespace {
 
 constexpr int kFlushPairs = 2;   // fp32 partial sums span at most this many tile pairs
 
-// rtile (nt, ntile, 2, 64, 8) fp16: [tile q][hi, lo][lane l][slot v]; lane
-// (i = l%16, g = l/16) slot v -> sqrt(w_p) rscale R[p][8g+v-i-3], p = 16q+i,
-// split hi + lo.  K slots 0, 1 of group g = 0 of the hi tile hold
-// sqrt(w_p) f_p fscale split hi + lo (zero in the lo tile).
-__global__ __launch_bounds__(64) void pack3_kernel(const double *__restrict__ rband,
-                                                   const double *__restrict__ w,
-                                                   const double *__restrict__ f, int npix_pad,
-                                                   unsigned short *__restrict__ rtile,
-                                                   double *__restrict__ scales) {
+// rtile (nt, 2, ntile, 64, 8) fp16: [hi, lo plane][tile q][lane l][slot v];
+// lane (i = l%16, g = l/16) slot v -> sqrt(w_p) rscale R[p][8g+v-i-3], p =
+// 16q+i, split hi + lo.  K slots 0, 1 of group g = 0 of the hi tile hold
+// sqrt(w_p) f_p fscale split hi + lo (zero in the lo tile).  The hi plane of a
+// target is, value for value, the fp16 tile set of rr_pack_mixed(F16) (same
+// scales, same rounding), so the fp16 normal equations of F16R read it.
+// Packing: the per-target power-of-two scales (one wave per target), then
+// every tile of every target in parallel (one wave per tile).
+__global__ __launch_bounds__(64) void scales3_kernel(const double *__restrict__ w,
+                                                     const double *__restrict__ f, int npix_pad,
+                                                     double *__restrict__ scales) {
     const int t = blockIdx.x;
     const int lane = threadIdx.x;
     const double *W = w + (long long)t * npix_pad;
     const double *F = f + (long long)t * npix_pad;
-    const double *Rb = rband + (long long)t * npix_pad * 11;
     double mw = 0.0, mf = 0.0;
     for (int p = lane; p < npix_pad; p += 64) {
         const double sw = sqrt(W[p]);
@@ -53,36 +54,49 @@ __global__ __launch_bounds__(64) void pack3_kernel(const double *__restrict__ rb
     }
     mw = wave_max(mw);
     mf = wave_max(mf);
-    const double rscale = (mw > 0.0) ? exp2(-ceil(log2(mw))) : 1.0;
-    const double fscale = (mf > 0.0) ? exp2(8.0 - ceil(log2(mf))) : 1.0;
     if (lane == 0) {
-        scales[2 * t] = rscale;
-        scales[2 * t + 1] = fscale;
+        scales[2 * t] = (mw > 0.0) ? exp2(-ceil(log2(mw))) : 1.0;
+        scales[2 * t + 1] = (mf > 0.0) ? exp2(8.0 - ceil(log2(mf))) : 1.0;
     }
+}
+
+constexpr int kPackTiles = 4;   // tiles (waves) per packing workgroup
+
+__global__ __launch_bounds__(64 * kPackTiles) void pack3_kernel(const double *__restrict__ rband,
+                                                                const double *__restrict__ w,
+                                                                const double *__restrict__ f,
+                                                                int npix_pad,
+                                                                unsigned short *__restrict__ rtile,
+                                                                const double *__restrict__ scales) {
+    const int t = blockIdx.y;
     const int ntile = npix_pad / 16;
-    unsigned short *RT = rtile + (long long)t * ntile * 2 * 64 * 8;
+    const int q = blockIdx.x * kPackTiles + (threadIdx.x >> 6);
+    if (q >= ntile) return;
+    const int lane = threadIdx.x & 63;
+    const double rscale = scales[2 * t], fscale = scales[2 * t + 1];
     const int i = lane & 15, g = lane >> 4;
-    for (int q = 0; q < ntile; ++q) {
-        const int p = 16 * q + i;
-        const double sw = sqrt(W[p]);
-        u16x8 hi8, lo8;
+    const int p = 16 * q + i;
+    const long long tp = (long long)t * npix_pad + p;
+    const double sw = sqrt(w[tp]);
+    const double *Rb = rband + tp * 11;
+    u16x8 hi8, lo8;
 #pragma unroll
-        for (int v = 0; v < 8; ++v) {
-            const int k = 8 * g + v - i - 3;     // < 0 for g = 0, v <= 1
-            const float r = (k >= 0 && k <= 10) ? (float)(Rb[(long long)p * 11 + k] * sw * rscale) : 0.f;
-            const unsigned short h = to16<RR_PREC_F16>(r);
-            hi8[v] = h;
-            lo8[v] = to16<RR_PREC_F16>(r - from16<RR_PREC_F16>(h));
-        }
-        if (g == 0) {
-            const float sf = (float)(sw * F[p] * fscale);
-            const unsigned short h = to16<RR_PREC_F16>(sf);
-            hi8[0] = h;
-            hi8[1] = to16<RR_PREC_F16>(sf - from16<RR_PREC_F16>(h));
-        }
-        *reinterpret_cast<u16x8 *>(RT + (((long long)q * 2 + 0) * 64 + lane) * 8) = hi8;
-        *reinterpret_cast<u16x8 *>(RT + (((long long)q * 2 + 1) * 64 + lane) * 8) = lo8;
+    for (int v = 0; v < 8; ++v) {
+        const int k = 8 * g + v - i - 3;     // < 0 for g = 0, v <= 1
+        const float r = (k >= 0 && k <= 10) ? (float)(Rb[k] * sw * rscale) : 0.f;
+        const unsigned short h = to16<RR_PREC_F16>(r);
+        hi8[v] = h;
+        lo8[v] = to16<RR_PREC_F16>(r - from16<RR_PREC_F16>(h));
     }
+    if (g == 0) {
+        const float sf = (float)(sw * f[tp] * fscale);
+        const unsigned short h = to16<RR_PREC_F16>(sf);
+        hi8[0] = h;
+        hi8[1] = to16<RR_PREC_F16>(sf - from16<RR_PREC_F16>(h));
+    }
+    unsigned short *RT = rtile + (long long)t * 2 * ntile * 512;
+    *reinterpret_cast<u16x8 *>(RT + ((long long)q * 64 + lane) * 8) = hi8;
+    *reinterpret_cast<u16x8 *>(RT + (((long long)ntile + q) * 64 + lane) * 8) = lo8;
 }
 
 typedef _Float16 half2v __attribute__((ext_vector_type(2)));
@@ -155,8 +169,8 @@ __global__ __launch_bounds__(256) void zscan_mfma3_kernel(rr::ZArgs a, long long
             const int r = c - TCH, u = r >> 8, t = (r >> 7) & 1, h = (r >> 6) & 1, l = r & 63;
             const int it = itemb + u < a.nitems ? itemb + u : a.nitems - 1;
             const int tg = a.item_target[it];
-            src[k] = RT + ((((long long)tg * ntile + t) * 2 + h) * 64 + l) * 8;
-            step[k] = 2 * 2 * 64 * 8;
+            src[k] = RT + ((((long long)tg * 2 + h) * ntile + t) * 64 + l) * 8;
+            step[k] = 2 * 64 * 8;
         }
     }
     auto load = [&](int qp, int buf) {
@@ -352,7 +366,10 @@ int zscan_mfma3(const ZArgs &a, hipStream_t s) {
 
 int pack_mixed3(const double *d_rband, const double *d_w, const double *d_f, int ntarget, int npix_pad,
                 void *d_rtile, double *d_scales, hipStream_t s) {
-    hipLaunchKernelGGL(pack3_kernel, dim3(ntarget), dim3(64), 0, s, d_rband, d_w, d_f, npix_pad,
+    hipLaunchKernelGGL(scales3_kernel, dim3(ntarget), dim3(64), 0, s, d_w, d_f, npix_pad, d_scales);
+    const int ntile = npix_pad / 16;
+    hipLaunchKernelGGL(pack3_kernel, dim3((ntile + kPackTiles - 1) / kPackTiles, ntarget),
+                       dim3(64 * kPackTiles), 0, s, d_rband, d_w, d_f, npix_pad,
                        static_cast<unsigned short *>(d_rtile), d_scales);
     return check_launch("rr_pack_mixed(f16x3)");
 }

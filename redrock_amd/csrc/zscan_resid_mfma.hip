@@ -20,7 +20,8 @@
 //     redshift), split into Uh = fp16(U) and Ul = fp16(U - Uh) on the way
 //   phase C, per target t, 16 output pixels x 16 redshifts:
 //     D[p][z] = Rh Ul + Rl Uh + Rh Uh       A = sqrt(w) R tile (rr_pack_mixed
-//     F16X3 tiles, input pixels [16q-8, 16q+24)), B = U_t
+//     F16X3 tiles, the ones F16R scans with; input pixels [16q-8, 16q+24)),
+//     B = U_t
 //
 // sqrt(w) f rides in K slots 0, 1 of the hi R tile (input pixels 16q-8 and
 // 16q-7, outside every row's band); the U operand carries -beta there instead
@@ -290,7 +291,7 @@ __global__ __launch_bounds__(64 * NW) void zscan_resid_mfma_kernel(rr::ZArgs a, 
     const char *rz[TPW];
 #pragma unroll
     for (int j = 0; j < TPW; ++j)
-        rz[j] = reinterpret_cast<const char *>(R3 + (long long)tgtC[j] * ntile * 1024);
+        rz[j] = reinterpret_cast<const char *>(R3 + (long long)tgtC[j] * 2 * ntile * 512);
     const unsigned r_lane = (unsigned)lane * 16u;
     auto load_r = [&](int c, u16x8 (&rt)[TPW][2][2]) {
 #pragma unroll
@@ -300,7 +301,7 @@ __global__ __launch_bounds__(64 * NW) void zscan_resid_mfma_kernel(rr::ZArgs a, 
 #pragma unroll
                 for (int h = 0; h < 2; ++h)
                     rt[j][qq][h] = *reinterpret_cast<const u16x8 *>(
-                        rz[j] + __builtin_amdgcn_readfirstlane(((2 * c + qq) * 2 + h) * 1024) + r_lane);
+                        rz[j] + __builtin_amdgcn_readfirstlane((h * ntile + 2 * c + qq) * 1024) + r_lane);
     };
 
     // operands of two chunks in flight: the loads for chunk c + 1 are issued
@@ -384,7 +385,7 @@ int launch_cfg(const rr::ZArgs &a, hipStream_t s) {
                                               ZB * 2048.0, true);
     hipLaunchKernelGGL(kernel, dim3((unsigned)map.blocks), dim3(64 * NW), 0, s, a, map,
                        static_cast<const unsigned short *>(a.tcache_u),
-                       static_cast<const unsigned short *>(a.rtile3), a.scales3);
+                       static_cast<const unsigned short *>(a.rtile), a.scales);
     return rr::check_launch("rr_zscan(residual, MFMA)");
 }
 

@@ -369,7 +369,7 @@ class TargetBatch(object):
         """(rtile, sfcol, scales) for a mixed precision (rr_pack_mixed), cached."""
         import torch
         if precision == 'f16r':
-            precision = 'f16'                # the fp16 normal equations use the fp16 tiles
+            precision = 'f16x3'              # hi planes: the fp16 normal equations; both: the residual
         if precision not in self._mixed:
             P = self.layout.npix_pad
             dt = torch.float32 if precision == 'f32' else torch.int16
@@ -554,9 +554,8 @@ class ScanPlan(object):
             if tu.dtype != torch.float16 or tu.numel() != need:
                 raise ValueError(f'template operand cache holds {tu.numel()} {tu.dtype} elements, '
                                  f'the residual kernel reads {need} fp16')
-            rt3, _, sc3 = batch.mixed('f16x3')
-            d.d_tcache_u, d.d_rtile3, d.d_scales3 = tu.data_ptr(), rt3.data_ptr(), sc3.data_ptr()
-            self._keep += [tu, rt3, sc3]
+            d.d_tcache_u = tu.data_ptr()
+            self._keep.append(tu)
         lane_in = self.lanes and (precision == 'f64' or resid in ('lanes', 'both'))
         if lane_in and self.nitems > 0:
             tr, _ = cache.get(layout, 'f64r' if precision == 'f64' else 'f32r')
