@@ -2,10 +2,14 @@
 //
 // Work decomposition: one wavefront = one item (a target scanned over a z
 // range) x 64 consecutive redshifts, one lane per redshift (items of at most
-// 16 or 4 redshifts: 4 or 16 items per wavefront, see zscan_f64_kernel).  A
-// workgroup's 4 waves are 4 items on the same 64-z block, so their
-// template-cache reads hit
-// the same lines in L1/L2.  Per lane and per block of P pixels:
+// 16 or 4 redshifts: 4 or 16 items per wavefront, see zscan_f64_kernel).  For
+// full grids a workgroup's 4 waves are 4 items on the same 64-z block, so
+// their template-cache reads hit the same lines in L1/L2.  For short items
+// (fine grids, Newton steps, one redshift per candidate) the 4 waves share
+// one wavefront's items and take a quarter of the pixels each: a lane's
+// dependent chain of loads is a quarter as long and 4x as many waves are in
+// flight (these launches hold few items, and one wave per item left most
+// SIMDs idle behind a full-length chain).  Per lane and per block of P pixels:
 //   A[p][b] = sum_k R[p][k] T[b][p+k-5][z]      (banded R-apply, 11 taps)
 //   M[b1][b2] += w_p A[p][b1] A[p][b2],  y[b] += A[p][b] (w f)_p
 // With one item per wave R, w and w*f are wave-uniform and come through
@@ -13,7 +17,10 @@
 // row is one coalesced 512-byte line run (64 / G consecutive redshifts per
 // item with G items per wave).  The kernel writes the
 // per-redshift normal equations [M | y] to the workspace; the FP64 solve and
-// chi2 follow in zscan_finish.hip.
+// chi2 follow in zscan_finish.hip.  Split workgroups add their quarters
+// through LDS in the fixed order m0 + (m1 + (m2 + m3)): deterministic, and
+// the same doubles for 4 and 16 items per wave; full-grid rows (and the lane
+// kernel, same order as the full-grid form) differ from them by rounding.
 //
 // Reference path replaced: zscan.py:670-737 (GPU) / :752-780 (CPU).
 #include "common.hpp"
@@ -24,11 +31,12 @@ namespace {
 // G items per wavefront, 64 / G redshifts each (G = 1 for full redshift
 // grids; 4 or 16 for the short items of fine-grid fits and Newton steps, whose
 // redshifts would leave most lanes of a one-item wave idle).  Every lane runs
-// the same per-pixel arithmetic in the same order whatever G is, so the
+// the same per-pixel arithmetic in the same order for G = 4 and 16, so their
 // results do not depend on G.
 template <int NB, int P, int G>
 __global__ __launch_bounds__(256) void zscan_f64_kernel(rr::ZArgs a, long long ne_off) {
     constexpr int LZ = 64 / G;           // lanes (redshifts) per item
+    constexpr bool kSplit = G > 1;       // the 4 waves: one pixel quarter each
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     int item, zl;
@@ -38,8 +46,8 @@ __global__ __launch_bounds__(256) void zscan_f64_kernel(rr::ZArgs a, long long n
         if (item >= a.nitems) return;
         zl = lane;
     } else {
-        const int first = (blockIdx.y * 4 + wave) * G;
-        if (first >= a.nitems) return;   // uniform over the wave
+        const int first = blockIdx.y * G;
+        if (first >= a.nitems) return;   // uniform over the workgroup
         item = first + lane / LZ;
         zl = lane % LZ;
         if (item >= a.nitems) {          // idle lanes repeat the last item, write nothing
@@ -77,7 +85,13 @@ __global__ __launch_bounds__(256) void zscan_f64_kernel(rr::ZArgs a, long long n
     // row of pixel p in the cache = p + halo; window of block p0 starts at p0-5
     const double *Tz = T + (long long)(a.halo - 5) * ldz + zc;
 
-    for (int p0 = 0; p0 < a.npix_pad; p0 += P) {
+    int pbeg = 0, pend = a.npix_pad;
+    if constexpr (kSplit) {
+        const int nblk = a.npix_pad / P;
+        pbeg = (wave * nblk / 4) * P;
+        pend = ((wave + 1) * nblk / 4) * P;
+    }
+    for (int p0 = pbeg; p0 < pend; p0 += P) {
         double r[P][11];
 #pragma unroll
         for (int i = 0; i < P; ++i)
@@ -116,6 +130,24 @@ __global__ __launch_bounds__(256) void zscan_f64_kernel(rr::ZArgs a, long long n
         }
     }
 
+    if constexpr (kSplit) {
+        // quarters summed in the fixed order m0 + (m1 + (m2 + m3))
+        __shared__ double red[NM + NB][64];
+        for (int s = 3; s >= 1; --s) {
+            if (wave == s) {
+#pragma unroll
+                for (int i = 0; i < NM; ++i) red[i][lane] = s < 3 ? m[i] + red[i][lane] : m[i];
+#pragma unroll
+                for (int i = 0; i < NB; ++i) red[NM + i][lane] = s < 3 ? y[i] + red[NM + i][lane] : y[i];
+            }
+            __syncthreads();
+        }
+        if (wave != 0) return;
+#pragma unroll
+        for (int i = 0; i < NM; ++i) m[i] += red[i][lane];
+#pragma unroll
+        for (int i = 0; i < NB; ++i) y[i] += red[NM + i][lane];
+    }
     if (!live) return;
     // normal equations of this (item, z) -> workspace, k-major (coalesced)
     const long long row = a.item_out[item] + iz;
@@ -136,7 +168,9 @@ __global__ __launch_bounds__(256) void zscan_f64_kernel(rr::ZArgs a, long long n
 template <int NB, int G>
 int launch_g(const rr::ZArgs &a, hipStream_t s) {
     constexpr int LZ = 64 / G;
-    dim3 grid((a.max_item_nz + LZ - 1) / LZ, (a.nitems + 4 * G - 1) / (4 * G));
+    // full grids: 4 items per workgroup; short items: G items shared by its 4 waves
+    const int per_wg = G == 1 ? 4 : G;
+    dim3 grid((a.max_item_nz + LZ - 1) / LZ, (a.nitems + per_wg - 1) / per_wg);
     const long long ne_off = rr::align256((long long)a.nitems * 2 * 8);
     hipLaunchKernelGGL((zscan_f64_kernel<NB, 2, G>), grid, dim3(256), 0, s, a, ne_off);
     return rr::check_launch("rr_zscan(f64)");
