@@ -282,8 +282,8 @@ def bench_grad(args):
     t_grad_dy, _ = timed(lambda: prob.grad(z0, coeff, want_dy=True), k)
     # reference method: 15-point fine grid over z[i-1..i+1] and a parabola through the minimum
     ref = fz._Refiner(tm, targets, zg, 15, 'f64', batch=batch)
-    t_par, rr = timed(lambda: ref.refine(list(zip(owner.tolist(), imin.tolist())), c2), 1)
-    zpar = np.array([r.get('zmin', np.nan) for r in rr])
+    t_par, rr = timed(lambda: ref.refine(owner, imin), 1)
+    zpar = np.where(rr['err'], np.nan, rr['zmin'])
     # Newton starts at the parabola vertex where it lies inside the +-5 step bracket
     ok = np.isfinite(zpar) & (zpar >= zlo) & (zpar <= zhi)
     zstart = np.where(ok, zpar, z0)

@@ -12,9 +12,12 @@ targets of one template at once:
    coefficients at each parabola minimum (fitz.py:265-293);
 3. the reference's sequential control flow — skipping minima within
    max_velo_diff of accepted ones, the parabola fit, zwarn bits, dedup and
-   sort — is replayed per target on the host from those results.  If a
-   target skips more candidates than were precomputed, another pass
-   refines the next ones, so the outcome is the reference's.
+   sort — is replayed on the host from those results for every target at
+   once (select_minima: one numpy step per candidate position over all
+   targets still choosing).  A target that reaches a candidate not refined
+   yet waits; the next candidates of all waiting targets are refined
+   together and they resume where they stopped, so the outcome is the
+   reference's.
 """
 from __future__ import annotations
 
@@ -125,19 +128,26 @@ class _Refiner(object):
         from .rebin import centers2edges
         self.edges = [centers2edges(w) for w in self.dwave.values()]
         self.oii = _oiisum(template)
-        self.batches = dict()
-        self.slot = dict()
-        for i, tg in enumerate(targets):
-            sig = PixelLayout.signature_of(tg.spectra)
-            self.batches.setdefault(sig, []).append(i)
-        if batch is not None and len(self.batches) == 1 and batch.nt == len(targets):
-            self._tb = {sig: batch for sig in self.batches}       # one layout: reuse it
+        nt = len(targets)
+        if batch is not None and batch.nt == nt:
+            # the batch of these targets (its constructor checked that they
+            # share one layout): reuse it
+            self.batches = {batch.layout.signature: list(range(nt))}
+            self._tb = {batch.layout.signature: batch}
         else:
+            self.batches = dict()
+            for i, tg in enumerate(targets):
+                self.batches.setdefault(PixelLayout.signature_of(tg.spectra), []).append(i)
             self._tb = {sig: TargetBatch([targets[i] for i in idx])
                         for sig, idx in self.batches.items()}
-        for sig, idx in self.batches.items():
-            for k, i in enumerate(idx):
-                self.slot[i] = (sig, k)
+        # target -> (layout number, slot in that layout's batch)
+        self.sigs = list(self.batches)
+        self.tsig = np.zeros(nt, dtype=np.int64)
+        self.tslot = np.zeros(nt, dtype=np.int64)
+        for g, sig in enumerate(self.sigs):
+            idx = np.asarray(self.batches[sig], dtype=np.int64)
+            self.tsig[idx] = g
+            self.tslot[idx] = np.arange(idx.size)
 
     def batch_of(self, targets):
         """The TargetBatch of all targets when they share one layout, else None."""
@@ -192,12 +202,14 @@ class _Refiner(object):
             for s in tg.spectra:
                 dw.setdefault(s.wavehash, s.wave)
         cache = TemplateCache(self.template, dw, redshifts=zblocks.reshape(-1), check_range=False)
+        owners = np.asarray(owners, dtype=np.int64)
         order = np.lexsort((np.arange(nitem), block))            # by grid, stable
-        for sig in self.batches:
-            sel = np.array([j for j in order if self.slot[owners[j]][0] == sig], dtype=np.int64)
+        osig = self.tsig[owners[order]]
+        for g, sig in enumerate(self.sigs):
+            sel = order[osig == g]
             if sel.size == 0:
                 continue
-            slots = np.array([self.slot[owners[j]][1] for j in sel])
+            slots = self.tslot[owners[sel]]
             plan = ScanPlan(cache, self._tb[sig], slots, block[sel] * per_item,
                             np.full(sel.size, per_item), self.precision, oiisum=self.oii,
                             solve=self.template.solve_matrices_algorithm)
@@ -221,12 +233,13 @@ class _Refiner(object):
             for s in tg.spectra:
                 dw.setdefault(s.wavehash, s.wave)
         cache = TemplateCache(self.template, dw, redshifts=np.asarray(zlist), check_range=False)
-        for sig, idx in self.batches.items():
-            sel = [j for j, t in enumerate(owners) if self.slot[t][0] == sig]
-            if not sel:
+        owners = np.asarray(owners, dtype=np.int64)
+        for g, sig in enumerate(self.sigs):
+            sel = np.flatnonzero(self.tsig[owners] == g)
+            if sel.size == 0:
                 continue
-            slots = np.array([self.slot[owners[j]][1] for j in sel])
-            if len(sel) == nitem and self.precision == 'f64' and (per_item > 16 or per_item <= 4):
+            slots = self.tslot[owners[sel]]
+            if sel.size == nitem and self.precision == 'f64' and (per_item > 16 or per_item <= 4):
                 # consecutive candidates of one target share a scan item (their
                 # fine grids are consecutive in zlist): more redshifts per
                 # wavefront.  Grids of 5-16 points stay one item each: the FP64
@@ -235,14 +248,14 @@ class _Refiner(object):
                 plan = ScanPlan(cache, self._tb[sig], tr, zr, nr, self.precision, oiisum=self.oii,
                                 solve=self.template.solve_matrices_algorithm)
             else:
-                plan = ScanPlan(cache, self._tb[sig], slots, [j * per_item for j in sel],
-                                [per_item] * len(sel), self.precision, oiisum=self.oii,
+                plan = ScanPlan(cache, self._tb[sig], slots, sel * per_item,
+                                np.full(sel.size, per_item), self.precision, oiisum=self.oii,
                                 solve=self.template.solve_matrices_algorithm)
             plan.run()
             c2, cf, pn, _ = plan.host()
-            chi2[sel] = c2.reshape(len(sel), per_item)
-            coeff[sel] = cf.reshape(len(sel), per_item, nb)
-            pen[sel] = pn.reshape(len(sel), per_item)
+            chi2[sel] = c2.reshape(sel.size, per_item)
+            coeff[sel] = cf.reshape(sel.size, per_item, nb)
+            pen[sel] = pn.reshape(sel.size, per_item)
         return chi2, coeff, pen
 
     def _edges_ok_many(self, zlo, zhi):
@@ -256,103 +269,194 @@ class _Refiner(object):
             ok &= ~((e[0] < xmin) | (e[-1] > xmax))
         return ok
 
-    def refine(self, pairs, zchi2):
-        """Refine candidates pairs = [(target, imin)]: returns per-pair dicts.
+    def refine(self, pt, pimin):
+        """Refine the candidates (target pt[j], coarse minimum pimin[j]).
 
-        A candidate whose fine grid leaves the template range gets
-        {'imin', 'error'}: the reference raises ValueError there
-        (fitz.py:226-231), but only if its control flow reaches the candidate.
+        Returns a dict of arrays over the candidates: zz, zzchi2 (n, nz_fine)
+        (the fine grid, chi2 + penalty on it), zmin, sigma, chi2min, zwarn
+        (the parabola through the 3 points about the fine-grid minimum,
+        fitz.py:226-263), coeff (n, nbasis) and zigm (the coefficients at zmin
+        and the redshift of their IGM transmission, fitz.py:265-304), and err:
+        the candidate's fine grid, or its zmin inside the redshift range,
+        leaves the template's wavelength range — the reference raises
+        ValueError('edges must be within input x range') there (fitz.py:226-
+        231, rebin.py:332-341), if its control flow reaches the candidate.
         """
         nz = self.nz_fine
         zs = self.redshifts
-        out = [None] * len(pairs)
-        imins = np.array([imin for _, imin in pairs], dtype=np.int64).reshape(-1)
-        lo = zs[np.maximum(imins - 1, 0)]
-        hi = zs[np.minimum(imins + 1, len(zs) - 1)]
+        pt = np.asarray(pt, dtype=np.int64).reshape(-1)
+        pimin = np.asarray(pimin, dtype=np.int64).reshape(-1)
+        n = pt.size
+        nb = self.template.nbasis
+        lo = zs[np.maximum(pimin - 1, 0)]
+        hi = zs[np.minimum(pimin + 1, len(zs) - 1)]
         # row by row what np.linspace(zs[ilo], zs[ihi], nz) gives (fitz.py:226-231)
-        zz_rows = np.linspace(lo, hi, nz, axis=1)
+        zz_rows = np.linspace(lo, hi, nz, axis=1) if n else np.zeros((0, nz))
+        res = dict(zz=zz_rows, zzchi2=np.zeros((n, nz)), zmin=np.zeros(n), sigma=np.zeros(n),
+                   chi2min=np.zeros(n), zwarn=np.zeros(n, dtype=np.int64), coeff=np.zeros((n, nb)),
+                   zigm=np.full(n, np.nan), err=np.zeros(n, dtype=bool))
+        if n == 0:
+            return res
         ok = self._edges_ok_many(np.minimum(lo, hi), np.maximum(lo, hi))
-        for j in np.where(~ok)[0]:
-            out[j] = dict(imin=pairs[j][1], error=ValueError('edges must be within input x range'))
-        good = np.where(ok)[0]
+        res['err'][~ok] = True
+        good = np.flatnonzero(ok)
+        if good.size == 0:
+            return res
         # candidates at the same coarse minimum have the same fine grid
-        ublk, first, inv = np.unique(imins[good], return_index=True, return_inverse=True)
-        zzchi2, _, zzpen = self._scan_blocks(zz_rows[good][first], inv.reshape(-1),
-                                             [pairs[j][0] for j in good])
-        c2 = zzchi2[:len(good)].reshape(len(good), nz).copy()
+        _, first, inv = np.unique(pimin[good], return_index=True, return_inverse=True)
+        zzchi2, _, zzpen = self._scan_blocks(zz_rows[good][first], inv.reshape(-1), pt[good])
+        c2 = zzchi2[:good.size].reshape(good.size, nz).copy()
         if self.oii is not None:
-            c2 += zzpen[:len(good)].reshape(len(good), nz)
+            c2 += zzpen[:good.size].reshape(good.size, nz)
+        res['zzchi2'][good] = c2
         # the 3 points about each fine-grid minimum (fitz.py:233-250)
-        i = np.clip(np.argmin(c2, axis=1), 1, nz - 2) if len(good) else np.zeros(0, np.int64)
-        r3 = np.arange(len(good))[:, None]
+        i = np.clip(np.argmin(c2, axis=1), 1, nz - 2)
+        r3 = np.arange(good.size)[:, None]
         c3 = i[:, None] + np.arange(-1, 2)[None, :]
         zmin, sigma, chi2min, zwarn = minfit_batch(zz_rows[good][r3, c3], c2[r3, c3])
-        for k, j in enumerate(good):
-            out[j] = dict(imin=pairs[j][1], zz=zz_rows[j], zzchi2=c2[k], zmin=float(zmin[k]),
-                          sigma=float(sigma[k]), chi2min=float(chi2min[k]), zwarn=int(zwarn[k]))
+        res['zmin'][good], res['sigma'][good] = zmin, sigma
+        res['chi2min'][good], res['zwarn'][good] = chi2min, zwarn
         # coefficients at each parabola minimum (fitz.py:265-304)
-        zv = np.array([out[j]['zmin'] for j in good], dtype=np.float64)
-        sel = [j for j, o in zip(good, self._edges_ok_many(zv, zv)) if o]
-        _, cf, _ = self._scan(np.array([out[j]['zmin'] for j in sel], dtype=np.float64),
-                              [pairs[j][0] for j in sel], 1)
-        for k, j in enumerate(sel):
-            out[j]['coeff'] = cf[k, 0]
-            out[j]['zigm'] = out[j]['zmin']          # transmission at zmin (fitz.py:266-284)
-        for j in good:
-            if 'coeff' in out[j]:
-                continue
-            zmin = out[j]['zmin']
-            if zmin < zs[0] or zs[-1] < zmin:
-                # beyond the redshift range: zero coefficients (fitz.py:295-301)
-                out[j]['coeff'] = np.zeros(self.template.nbasis)
-                out[j]['zwarn'] |= ZW.Z_FITLIMIT
-                out[j]['zwarn'] |= ZW.BAD_MINFIT
-                out[j]['zigm'] = np.nan                   # no transmission computed there
-            else:
-                out[j]['error'] = ValueError('edges must be within input x range')
-        return out
+        eok = self._edges_ok_many(zmin, zmin)
+        sel = good[eok]
+        _, cf, _ = self._scan(res['zmin'][sel], pt[sel], 1)
+        res['coeff'][sel] = cf[:, 0]
+        res['zigm'][sel] = res['zmin'][sel]         # transmission at zmin (fitz.py:266-284)
+        rest = good[~eok]
+        beyond = (res['zmin'][rest] < zs[0]) | (zs[-1] < res['zmin'][rest])
+        # beyond the redshift range: zero coefficients (fitz.py:295-301), no transmission
+        res['zwarn'][rest[beyond]] |= ZW.Z_FITLIMIT | ZW.BAD_MINFIT
+        res['err'][rest[~beyond]] = True
+        return res
 
 
-def _finalize(template, target, redshifts, nminima, cands, get):
-    """Sequential control flow of fitz.py:198-395 over precomputed candidates.
+def select_minima(zchi2, redshifts, nminima, max_velo_diff, refine, speculate=2):
+    """The candidate loop of fitz.py:198-327 for every row of zchi2 at once.
 
-    get(k) returns the refined dict of the k-th candidate (computing it on
-    demand).  Returns the reference's dict of stacked arrays.
+    Per row (target) t the reference walks find_minima(zchi2[t]) in order:
+    stop at nminima accepted; skip a minimum within max_velo_diff of an
+    accepted redshift; else refine it (fine grid, parabola), flag the
+    parabola vertex outside the fine grid (BAD_MINFIT, the grid's lowest
+    point instead) or near the scan's ends (Z_FITLIMIT), and skip it if the
+    refined redshift is within max_velo_diff of an accepted one.  Here one
+    numpy step takes the next candidate of every target still choosing.
+    refine(pt, pimin) returns the arrays of _Refiner.refine for candidates
+    (pt[j], pimin[j]); the first nminima + speculate candidates of every
+    target are refined up front, and a target that reaches an unrefined
+    candidate waits: the next nminima + speculate candidates of all waiting
+    targets are refined together and they resume where they stopped.
+
+    Returns (nacc, acc): nacc[t] accepted minima of target t, and acc a dict
+    of (nt, nminima) arrays in acceptance order — z, zerr, zwarn, chi2 and
+    row, the row of the candidate in store, the concatenation of every
+    refine() result (a dict of arrays, returned as acc['store']).
     """
-    max_velo_diff = (constants.max_velo_diff_star if template.template_type == 'STAR'
-                     else constants.max_velo_diff)
-    results = list()
-    for k, imin in enumerate(cands):
-        if len(results) == nminima:
+    zchi2 = np.atleast_2d(np.asarray(zchi2, dtype=np.float64))
+    zs = np.asarray(redshifts, dtype=np.float64)
+    nt = zchi2.shape[0]
+    cands = [find_minima(zchi2[t]) for t in range(nt)]
+    ncand = np.array([c.size for c in cands], dtype=np.int64)
+    W = max(1, nminima + speculate)
+    # candidate positions -> coarse index (listed up to filled[t]), and ->
+    # store row once refined
+    width = int(min(W, ncand.max())) if nt else 0
+    cmat = np.full((nt, max(width, 1)), -1, dtype=np.int64)
+    rowmat = np.full(cmat.shape, -1, dtype=np.int64)
+    filled = np.minimum(ncand, width)
+    for t, c in enumerate(cands):
+        cmat[t, :filled[t]] = c[:filled[t]]
+
+    def fill(ts, upto):
+        nonlocal cmat, rowmat
+        upto = np.minimum(upto, ncand[ts])
+        if upto.size and int(upto.max()) > cmat.shape[1]:
+            grow = int(upto.max()) - cmat.shape[1] + W
+            cmat = np.pad(cmat, ((0, 0), (0, grow)), constant_values=-1)
+            rowmat = np.pad(rowmat, ((0, 0), (0, grow)), constant_values=-1)
+        for t, e in zip(ts, upto):
+            if e > filled[t]:
+                cmat[t, filled[t]:e] = cands[t][filled[t]:e]
+                filled[t] = e
+
+    parts, nstore = [], 0
+    kpos = np.zeros(nt, dtype=np.int64)
+    nacc = np.zeros(nt, dtype=np.int64)
+    acc = dict(z=np.full((nt, max(nminima, 1)), np.nan), zerr=np.zeros((nt, max(nminima, 1))),
+               zwarn=np.zeros((nt, max(nminima, 1)), dtype=np.int64),
+               chi2=np.zeros((nt, max(nminima, 1))), row=np.full((nt, max(nminima, 1)), -1))
+    lo_lim, hi_lim = zs[min(1, zs.size - 1)], zs[max(zs.size - 2, 0)]
+
+    # first request: the first W candidates of every target
+    tt = np.repeat(np.arange(nt), np.minimum(ncand, W))
+    kk = np.concatenate([np.arange(min(c, W)) for c in ncand]) if nt else np.zeros(0, np.int64)
+    store = None
+    while True:
+        if tt.size:
+            res = refine(tt, cmat[tt, kk])
+            rowmat[tt, kk] = nstore + np.arange(tt.size)
+            nstore += tt.size
+            parts.append(res)
+            store = {k: np.concatenate([p[k] for p in parts]) for k in parts[0]}
+            parts = [store]
+        blocked = np.zeros(nt, dtype=bool)
+        while True:
+            act = np.flatnonzero((nacc < nminima) & (kpos < ncand) & ~blocked)
+            if act.size == 0:
+                break
+            k = kpos[act]
+            short = k >= filled[act]
+            if np.any(short):
+                fill(act[short], k[short] + W)
+            imin = cmat[act, k]
+            zprev = acc['z'][act]                 # NaN past the accepted ones: never close
+            with np.errstate(invalid='ignore'):
+                skip1 = np.any(np.abs(get_dv(zs[imin][:, None], zprev)) < max_velo_diff, axis=1)
+            r = rowmat[act, k]
+            miss = ~skip1 & (r < 0)
+            blocked[act[miss]] = True
+            go = ~skip1 & ~miss
+            ga, gr = act[go], r[go]
+            if gr.size:
+                if np.any(store['err'][gr]):
+                    raise ValueError('edges must be within input x range')
+                zmin = store['zmin'][gr]
+                zwarn = store['zwarn'][gr].copy()
+                chi2min = store['chi2min'][gr].copy()
+                zz = store['zz'][gr]
+                zbest = zmin.copy()
+                out = (zbest < zz[:, 0]) | (zbest > zz[:, -1])
+                if np.any(out):
+                    zwarn[out] |= ZW.BAD_MINFIT
+                    zzc = store['zzchi2'][gr[out]]
+                    i = np.argmin(zzc, axis=1)
+                    zbest[out] = zz[out, i]
+                    chi2min[out] = zzc[np.arange(i.size), i]
+                lim = (zbest < lo_lim) | (zbest > hi_lim) | (zmin < lo_lim) | (zmin > hi_lim)
+                zwarn[lim] |= ZW.Z_FITLIMIT
+                with np.errstate(invalid='ignore'):
+                    skip2 = np.any(np.abs(get_dv(zbest[:, None], acc['z'][ga])) < max_velo_diff,
+                                   axis=1)
+                a = ~skip2
+                at, slot = ga[a], nacc[ga[a]]
+                acc['z'][at, slot] = zbest[a]
+                acc['zerr'][at, slot] = store['sigma'][gr[a]]
+                acc['zwarn'][at, slot] = zwarn[a]
+                acc['chi2'][at, slot] = chi2min[a]
+                acc['row'][at, slot] = gr[a]
+                nacc[at] += 1
+            kpos[act[~miss]] += 1
+        wait = np.flatnonzero(blocked)
+        if wait.size == 0:
             break
-        zprev = np.array([tmp['z'] for tmp in results])
-        dv = get_dv(z=redshifts[imin], zref=zprev)
-        if np.any(np.abs(dv) < max_velo_diff):
-            continue
-        r = get(k)
-        if 'error' in r:
-            raise r['error']
-        zz, zzchi2 = r['zz'], r['zzchi2']
-        zmin, zwarn, chi2min = r['zmin'], r['zwarn'], r['chi2min']
-        zbest = zmin
-        zerr = r['sigma']
-        if zbest < zz[0] or zbest > zz[-1]:
-            zwarn |= ZW.BAD_MINFIT
-            i = np.where(zzchi2 == np.min(zzchi2))[0][0]
-            zbest = zz[i]
-            chi2min = zzchi2[i]
-        if zbest < redshifts[1] or zbest > redshifts[-2]:
-            zwarn |= ZW.Z_FITLIMIT
-        if zmin < redshifts[1] or zmin > redshifts[-2]:
-            zwarn |= ZW.Z_FITLIMIT
-        zprev = np.array([tmp['z'] for tmp in results])
-        dv = get_dv(z=zbest, zref=zprev)
-        if np.any(np.abs(dv) < max_velo_diff):
-            continue
-        results.append(dict(z=zbest, zerr=zerr, zwarn=zwarn, chi2=chi2min, zz=zz,
-                            zzchi2=zzchi2, coeff=r['coeff'], fitmethod=template.method,
-                            _zigm=r.get('zigm', np.nan)))
-    return results
+        # the next W candidates of every waiting target
+        need = np.minimum(kpos[wait] + W, ncand[wait])
+        fill(wait, need)
+        tt = np.repeat(wait, need - kpos[wait])
+        kk = np.concatenate([np.arange(b, e) for b, e in zip(kpos[wait], need)])
+        unref = rowmat[tt, kk] < 0
+        tt, kk = tt[unref], kk[unref]
+    acc['store'] = store
+    return nacc, acc
 
 
 def _stack(results, target, npix=None):
@@ -437,59 +541,61 @@ def _add_newton(out, targets, template):
         out[t]['chi2newton'][k, 0] = cv
 
 
-class _NeedMore(Exception):
-    """A target's sequential fitz logic needs a candidate not refined yet."""
-
-
 def _fitz_targets(zchi2, redshifts, targets, template, nminima, zminfit_npoints, precision,
                   speculate, archetype=None, deg_legendre=None, per_camera=False, n_nearest=None,
                   prior_sigma=None, batch=None, npixels=None):
     zchi2 = np.atleast_2d(np.asarray(zchi2, dtype=np.float64))
-    assert zchi2.shape == (len(targets), len(redshifts))
+    nt = len(targets)
+    assert zchi2.shape == (nt, len(redshifts))
     nz = 15 if zminfit_npoints is None else zminfit_npoints
     ref = _Refiner(template, targets, redshifts, nz, precision, batch=batch)
-    cands = [find_minima(zchi2[t]) for t in range(len(targets))]
-    done = [dict() for _ in targets]
-
-    def run(pairs):
-        if not pairs:
-            return
-        for (t, imin), r in zip(pairs, ref.refine(pairs, zchi2)):
-            done[t][imin] = r
-
-    first = [(t, int(c)) for t in range(len(targets))
-             for c in cands[t][:nminima + speculate]]
-    run(first)
-    # The sequential logic of each target may reach past its speculative set
-    # (minima skipped within max_velo_diff): such targets stop, their next
-    # candidates are refined together in one batch, and they start over.
-    out = [None] * len(targets)
-    pending = list(range(len(targets)))
-    while pending:
-        need, again = [], []
-        for t in pending:
-            missing = []
-
-            def get(k, t=t, missing=missing):
-                imin = int(cands[t][k])
-                if imin not in done[t]:
-                    missing.extend(int(c) for c in cands[t][k:k + nminima + speculate]
-                                   if int(c) not in done[t])
-                    raise _NeedMore()
-                return done[t][imin]
-            try:
-                out[t] = _finalize(template, targets[t], redshifts, nminima, cands[t], get)
-            except _NeedMore:
-                again.append(t)
-                need.extend((t, c) for c in missing)
-        run(need)
-        pending = again
+    mvd = (constants.max_velo_diff_star if template.template_type == 'STAR'
+           else constants.max_velo_diff)
+    nacc, acc = select_minima(zchi2, ref.redshifts, nminima, mvd, ref.refine, speculate)
+    st = acc['store']
+    if npixels is None:
+        npixels = [sum(int((s.ivar > 0.).sum()) for s in tg.spectra) for tg in targets]
     if archetype is not None:
+        # per-candidate dicts in acceptance order, refit by the archetypes, sorted
+        out = [[dict(z=acc['z'][t, k], zerr=acc['zerr'][t, k], zwarn=int(acc['zwarn'][t, k]),
+                     chi2=acc['chi2'][t, k], zz=st['zz'][acc['row'][t, k]],
+                     zzchi2=st['zzchi2'][acc['row'][t, k]], coeff=st['coeff'][acc['row'][t, k]],
+                     fitmethod=template.method, _zigm=st['zigm'][acc['row'][t, k]])
+                for k in range(nacc[t])] for t in range(nt)]
         _archetype_pass(out, targets, template, archetype, ref, deg_legendre, per_camera, n_nearest,
                         prior_sigma)
-    if npixels is None:
-        npixels = [None] * len(targets)
-    return [_stack(r, tg, n) for r, tg, n in zip(out, targets, npixels)]
+        return [_stack(r, tg, n) for r, tg, n in zip(out, targets, npixels)]
+    # every target's accepted minima sorted by chi2 (fitz.py:359-362), one array per column
+    nm = acc['z'].shape[1]
+    valid = np.arange(nm)[None, :] < nacc[:, None]
+    o = np.argsort(np.where(valid, acc['chi2'], np.inf), axis=1, kind='stable')
+    sel = np.take_along_axis(valid, o, axis=1)
+    col = {k: np.take_along_axis(acc[k], o, axis=1)[sel] for k in ('z', 'zerr', 'zwarn', 'chi2', 'row')}
+    rows = col.pop('row')
+    ntot = rows.size
+    nbasis = template.nbasis
+    zz = st['zz'][rows] if ntot else np.zeros((0, nz))
+    zzchi2 = st['zzchi2'][rows] if ntot else np.zeros((0, nz))
+    coeff = st['coeff'][rows] if ntot else np.zeros((0, nbasis))
+    z1, zerr1, zwarn1, chi21 = (col[k].reshape(ntot, 1) for k in ('z', 'zerr', 'zwarn', 'chi2'))
+    fitmethod = np.full((ntot, 1), template.method)
+    npix = np.repeat(np.asarray(npixels, dtype=np.int64), nacc).reshape(ntot, 1)
+    bounds = np.concatenate([[0], np.cumsum(nacc)])
+    xfloat = np.zeros((0, 1), dtype=float)
+    xint = np.zeros((0, 1), dtype=int)
+    xstr = np.zeros((0, 1), dtype=str)
+    empty = dict(z=xfloat, zerr=xfloat, zwarn=xint, chi2=xfloat, zz=xfloat, zzchi2=xfloat,
+                 coeff=xfloat, fitmethod=xstr, npixels=xint)
+    res = []
+    for t in range(nt):
+        a, b = bounds[t], bounds[t + 1]
+        if a == b:
+            res.append(dict(empty))
+            continue
+        res.append(dict(z=z1[a:b], zerr=zerr1[a:b], zwarn=zwarn1[a:b], chi2=chi21[a:b], zz=zz[a:b],
+                        zzchi2=zzchi2[a:b], coeff=coeff[a:b], fitmethod=fitmethod[a:b],
+                        npixels=npix[a:b]))
+    return res
 
 
 def _archetype_pass(rows, targets, template, archetype, ref, deg_legendre, per_camera, n_nearest,
