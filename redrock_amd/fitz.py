@@ -35,7 +35,10 @@ def get_dv(z, zref):
 def find_minima(x):
     """Indices of local minima of x including edges, sorted by x (fitz.py:46-67)."""
     x = np.asarray(x)
-    ii = np.where(np.r_[True, x[1:] <= x[:-1]] & np.r_[x[:-1] <= x[1:], True])[0]
+    m = np.ones(x.shape, dtype=bool)
+    m[1:] = x[1:] <= x[:-1]
+    m[:-1] &= x[:-1] <= x[1:]
+    ii = np.flatnonzero(m)
     jj = np.argsort(x[ii])
     return ii[jj]
 
@@ -544,6 +547,9 @@ def _add_newton(out, targets, template):
 def _fitz_targets(zchi2, redshifts, targets, template, nminima, zminfit_npoints, precision,
                   speculate, archetype=None, deg_legendre=None, per_camera=False, n_nearest=None,
                   prior_sigma=None, batch=None, npixels=None):
+    if archetype is None:
+        return _as_dicts(*fitz_columns(zchi2, redshifts, targets, template, nminima, zminfit_npoints,
+                                       precision, speculate, batch, npixels))
     zchi2 = np.atleast_2d(np.asarray(zchi2, dtype=np.float64))
     nt = len(targets)
     assert zchi2.shape == (nt, len(redshifts))
@@ -555,17 +561,21 @@ def _fitz_targets(zchi2, redshifts, targets, template, nminima, zminfit_npoints,
     st = acc['store']
     if npixels is None:
         npixels = [sum(int((s.ivar > 0.).sum()) for s in tg.spectra) for tg in targets]
-    if archetype is not None:
-        # per-candidate dicts in acceptance order, refit by the archetypes, sorted
-        out = [[dict(z=acc['z'][t, k], zerr=acc['zerr'][t, k], zwarn=int(acc['zwarn'][t, k]),
-                     chi2=acc['chi2'][t, k], zz=st['zz'][acc['row'][t, k]],
-                     zzchi2=st['zzchi2'][acc['row'][t, k]], coeff=st['coeff'][acc['row'][t, k]],
-                     fitmethod=template.method, _zigm=st['zigm'][acc['row'][t, k]])
-                for k in range(nacc[t])] for t in range(nt)]
-        _archetype_pass(out, targets, template, archetype, ref, deg_legendre, per_camera, n_nearest,
-                        prior_sigma)
-        return [_stack(r, tg, n) for r, tg, n in zip(out, targets, npixels)]
-    # every target's accepted minima sorted by chi2 (fitz.py:359-362), one array per column
+    # per-candidate dicts in acceptance order, refit by the archetypes, sorted
+    out = [[dict(z=acc['z'][t, k], zerr=acc['zerr'][t, k], zwarn=int(acc['zwarn'][t, k]),
+                 chi2=acc['chi2'][t, k], zz=st['zz'][acc['row'][t, k]],
+                 zzchi2=st['zzchi2'][acc['row'][t, k]], coeff=st['coeff'][acc['row'][t, k]],
+                 fitmethod=template.method, _zigm=st['zigm'][acc['row'][t, k]])
+            for k in range(nacc[t])] for t in range(nt)]
+    _archetype_pass(out, targets, template, archetype, ref, deg_legendre, per_camera, n_nearest,
+                    prior_sigma)
+    return [_stack(r, tg, n) for r, tg, n in zip(out, targets, npixels)]
+
+
+def _columns(acc, nacc, st, template, npixels, nz):
+    """Every target's accepted minima sorted by chi2 (fitz.py:359-362) as flat
+    columns, target by target: z, zerr, zwarn, chi2, fitmethod, npixels (n,),
+    zz, zzchi2 (n, nz), coeff (n, nbasis)."""
     nm = acc['z'].shape[1]
     valid = np.arange(nm)[None, :] < nacc[:, None]
     o = np.argsort(np.where(valid, acc['chi2'], np.inf), axis=1, kind='stable')
@@ -574,28 +584,52 @@ def _fitz_targets(zchi2, redshifts, targets, template, nminima, zminfit_npoints,
     rows = col.pop('row')
     ntot = rows.size
     nbasis = template.nbasis
-    zz = st['zz'][rows] if ntot else np.zeros((0, nz))
-    zzchi2 = st['zzchi2'][rows] if ntot else np.zeros((0, nz))
-    coeff = st['coeff'][rows] if ntot else np.zeros((0, nbasis))
-    z1, zerr1, zwarn1, chi21 = (col[k].reshape(ntot, 1) for k in ('z', 'zerr', 'zwarn', 'chi2'))
-    fitmethod = np.full((ntot, 1), template.method)
-    npix = np.repeat(np.asarray(npixels, dtype=np.int64), nacc).reshape(ntot, 1)
+    cols = dict(z=col['z'], zerr=col['zerr'], zwarn=col['zwarn'], chi2=col['chi2'],
+                zz=st['zz'][rows] if ntot else np.zeros((0, nz)),
+                zzchi2=st['zzchi2'][rows] if ntot else np.zeros((0, nz)),
+                coeff=st['coeff'][rows] if ntot else np.zeros((0, nbasis)),
+                fitmethod=np.full(ntot, template.method),
+                npixels=np.repeat(np.asarray(npixels, dtype=np.int64), nacc))
+    return nacc, cols
+
+
+def _as_dicts(nacc, cols):
+    """Per-target result dicts (fitz.py:359-395) viewing the flat columns."""
+    ntot = cols['z'].size
+    c2 = {k: (v.reshape(ntot, 1) if v.ndim == 1 else v) for k, v in cols.items()}
     bounds = np.concatenate([[0], np.cumsum(nacc)])
     xfloat = np.zeros((0, 1), dtype=float)
     xint = np.zeros((0, 1), dtype=int)
     xstr = np.zeros((0, 1), dtype=str)
     empty = dict(z=xfloat, zerr=xfloat, zwarn=xint, chi2=xfloat, zz=xfloat, zzchi2=xfloat,
                  coeff=xfloat, fitmethod=xstr, npixels=xint)
+    keys = ('z', 'zerr', 'zwarn', 'chi2', 'zz', 'zzchi2', 'coeff', 'fitmethod', 'npixels')
     res = []
-    for t in range(nt):
+    for t in range(len(nacc)):
         a, b = bounds[t], bounds[t + 1]
-        if a == b:
-            res.append(dict(empty))
-            continue
-        res.append(dict(z=z1[a:b], zerr=zerr1[a:b], zwarn=zwarn1[a:b], chi2=chi21[a:b], zz=zz[a:b],
-                        zzchi2=zzchi2[a:b], coeff=coeff[a:b], fitmethod=fitmethod[a:b],
-                        npixels=npix[a:b]))
+        res.append(dict(empty) if a == b else {k: c2[k][a:b] for k in keys})
     return res
+
+
+def fitz_columns(zchi2, redshifts, targets, template, nminima=3, zminfit_npoints=15,
+                 precision='f64', speculate=2, batch=None, npixels=None):
+    """fitz_targets without archetypes, as flat columns: (nacc, cols) with
+    nacc[t] the number of minima of target t and cols the dict of their
+    z, zerr, zwarn, chi2, fitmethod, npixels (n,), zz, zzchi2 (n, nz_fine)
+    and coeff (n, nbasis), target by target, each target's rows sorted by
+    chi2 — the per-target dicts of fitz_targets stacked (zfind builds its
+    table from them)."""
+    zchi2 = np.atleast_2d(np.asarray(zchi2, dtype=np.float64))
+    nt = len(targets)
+    assert zchi2.shape == (nt, len(redshifts))
+    nz = 15 if zminfit_npoints is None else zminfit_npoints
+    ref = _Refiner(template, targets, redshifts, nz, precision, batch=batch)
+    mvd = (constants.max_velo_diff_star if template.template_type == 'STAR'
+           else constants.max_velo_diff)
+    nacc, acc = select_minima(zchi2, ref.redshifts, nminima, mvd, ref.refine, speculate)
+    if npixels is None:
+        npixels = [sum(int((s.ivar > 0.).sum()) for s in tg.spectra) for tg in targets]
+    return _columns(acc, nacc, acc['store'], template, npixels, nz)
 
 
 def _archetype_pass(rows, targets, template, archetype, ref, deg_legendre, per_camera, n_nearest,

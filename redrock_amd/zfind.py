@@ -16,7 +16,7 @@ from __future__ import annotations
 import numpy as np
 
 from . import constants
-from .fitz import fitz_targets, get_dv
+from .fitz import fitz_columns, fitz_targets, get_dv
 from .templates import parse_fulltype
 from .zwarning import ZWarningMask as ZW
 from .zwarning import badfit_mask
@@ -236,6 +236,9 @@ def zfind(targets, templates, mp_procs=1, nminima=3, archetypes=None, priors=Non
     else:
         npix = [sum(int(np.count_nonzero(s.ivar > 0.)) for s in tg.spectra) for tg in local]
     order = np.array([t.template.full_type for t in templates]).argsort()
+    # without archetypes or Newton columns the fits go to the table as flat
+    # columns (fitz_columns) instead of per-target dicts
+    blocks = dict() if not archetypes and not newton else None
     for t in np.array(list(templates), dtype=object)[order]:      # zfind.py:330-367
         ft = t.template.full_type
         if not local:
@@ -248,14 +251,19 @@ def zfind(targets, templates, mp_procs=1, nminima=3, archetypes=None, priors=Non
         archetype = None
         if archetypes and t.template.template_type in archetypes:
             archetype = archetypes[t.template.template_type]
-        fits = fitz_targets(chi2, t.template.redshifts, local, t.template, nminima=nminima,
-                            zminfit_npoints=zminfit_npoints, newton=newton, archetype=archetype,
-                            deg_legendre=deg_legendre, per_camera=bool(per_camera),
-                            n_nearest=n_nearest, prior_sigma=prior_sigma,
-                            batch=next(iter(batches.values())) if len(batches) == 1 else None,
-                            npixels=npix)
-        for tg, zfit in zip(local, fits):
-            results[tg.id][ft]['zfit'] = zfit
+        one = next(iter(batches.values())) if len(batches) == 1 else None
+        if blocks is not None:
+            blocks[ft] = ([tg.id for tg in local],) + fitz_columns(
+                chi2, t.template.redshifts, local, t.template, nminima=nminima,
+                zminfit_npoints=zminfit_npoints, batch=one, npixels=npix)
+        else:
+            fits = fitz_targets(chi2, t.template.redshifts, local, t.template, nminima=nminima,
+                                zminfit_npoints=zminfit_npoints, newton=newton, archetype=archetype,
+                                deg_legendre=deg_legendre, per_camera=bool(per_camera),
+                                n_nearest=n_nearest, prior_sigma=prior_sigma, batch=one,
+                                npixels=npix)
+            for tg, zfit in zip(local, fits):
+                results[tg.id][ft]['zfit'] = zfit
         timings['fitz'][ft] = clock() - tf
 
     for tg in local:
@@ -270,7 +278,7 @@ def zfind(targets, templates, mp_procs=1, nminima=3, archetypes=None, priors=Non
                     n_nearest=n_nearest, ncamera=ncamera)
     ta = clock()
     if arch is None:
-        allzfit = _assemble_table(results, _all_ids(targets), templates, nminima)
+        allzfit = _assemble_table(results, _all_ids(targets), templates, nminima, blocks)
     else:
         allzfit = _assemble(results, _all_ids(targets), templates, nminima, arch=arch)
     for tid in results:
@@ -299,54 +307,78 @@ def _pad_cols(val, width):
     return val[:, :width]
 
 
-def _assemble_table(allresults, target_ids, templates, nminima):
+def _assemble_table(allresults, target_ids, templates, nminima, blocks=None):
     """_assemble without archetypes, with every target's rows handled at once:
     the same table, row order and dtypes (the per-target sorts, the nminima
     per spectype trim and calc_deltachi2 become sorts keyed by target and
-    padded (target, candidate, candidate) arrays)."""
-    cols = dict()            # key -> list of (rows, ...) arrays in (target, template) order
-    tpos, spec, subt, ncf, mvd = [], [], [], [], []
+    padded (target, candidate, candidate) arrays).  blocks: {fulltype:
+    (target ids, nacc, columns)} from fitz_columns; templates without a block
+    come from the per-target 'zfit' dicts."""
+    blocks = dict(blocks or {})
+    pos = {tid: i for i, tid in enumerate(target_ids)}
+    fts = []
+    for tid in target_ids:
+        for ft in allresults[tid]:
+            if ft != 'meta' and ft not in fts:
+                fts.append(ft)
+    for ft in fts:                        # per-target dicts -> the same flat columns
+        if ft in blocks:
+            continue
+        ids = [tid for tid in target_ids if ft in allresults[tid]]
+        per = [allresults[tid][ft].pop('zfit') for tid in ids]
+        nacc = np.array([len(z['chi2']) for z in per], dtype=np.int64)
+        full = [z for z in per if len(z['chi2'])]
+        cols = {k: (lambda v: v.reshape(-1) if v.ndim == 2 and v.shape[1] == 1 and k != 'coeff' else v)(
+            np.vstack([z[k] for z in full])) for k in (full[0] if full else [])}
+        blocks[ft] = (ids, nacc, cols)
     keys = None
-    for ti, tid in enumerate(target_ids):
-        for fulltype in allresults[tid]:
-            if fulltype == 'meta':
-                continue
-            tmp = allresults[tid][fulltype].pop('zfit')
-            nmin = len(tmp['chi2'])
-            if nmin == 0:
-                print(f'WARNING: no {fulltype} chi2 vs. z minima for targetid {tid}')
-                continue
-            spectype, subtype = parse_fulltype(fulltype)
-            if keys is None:
-                keys = list(tmp.keys())
-            for k in keys:
-                cols.setdefault(k, []).append(tmp[k])
-            tpos.append(np.full(nmin, ti))
-            spec.append(np.full(nmin, spectype))
-            subt.append(np.full(nmin, subtype))
-            ncf.append(np.full(nmin, tmp['coeff'].shape[1]))
-            mvd.append(np.full(nmin, constants.max_velo_diff_star if spectype.upper() == 'STAR'
-                               else constants.max_velo_diff))
+    parts = []                            # (target position, template rank, candidate, ft, cols)
+    for ft in fts:
+        ids, nacc, cols = blocks[ft]
+        for tid, n in zip(ids, nacc):
+            if n == 0:
+                print(f'WARNING: no {ft} chi2 vs. z minima for targetid {tid}')
+        if int(np.sum(nacc)) == 0:
+            continue
+        if keys is None:
+            keys = list(cols.keys())
+        tpos_b = np.repeat(np.array([pos[tid] for tid in ids], dtype=np.int64), nacc)
+        rank = np.array([list(k for k in allresults[tid] if k != 'meta').index(ft) for tid in ids])
+        cand = np.arange(tpos_b.size) - np.repeat(np.cumsum(nacc) - nacc, nacc)
+        parts.append((tpos_b, np.repeat(rank, nacc), cand, ft, cols))
     if keys is None:
         return dict()
-    maxnc = max(c.shape[1] for c in cols['coeff'])
-    nrows = [len(c) for c in cols['coeff']]
-    coeff = np.zeros((sum(nrows), maxnc), dtype=np.result_type(*cols['coeff']))
+    # rows in (target, its template order, candidate) order, as the per-target loop appends them
+    tp_all = np.concatenate([p[0] for p in parts])
+    orig = np.lexsort((np.concatenate([p[2] for p in parts]), np.concatenate([p[1] for p in parts]),
+                       tp_all))
+    maxnc = max(p[4]['coeff'].shape[1] for p in parts)
+    nrow = tp_all.size
+    coeff = np.zeros((nrow, maxnc), dtype=np.result_type(*[p[4]['coeff'] for p in parts]))
     r0 = 0
-    for c, nr in zip(cols['coeff'], nrows):
-        coeff[r0:r0 + nr, :c.shape[1]] = c
-        r0 += nr
-    tab = dict(coeff=coeff)
+    for p in parts:
+        c = p[4]['coeff']
+        coeff[r0:r0 + c.shape[0], :c.shape[1]] = c
+        r0 += c.shape[0]
+    tab = dict(coeff=coeff[orig])
     for k in keys:
         if k == 'coeff':
             continue
-        v = np.vstack(cols[k])
-        tab[k] = v.flatten() if (k != 'coeff' and v.shape[1] == 1) else v
-    tab['spectype'] = np.concatenate(spec)
-    tab['subtype'] = np.concatenate(subt)
-    tab['ncoeff'] = np.concatenate(ncf)
-    mv = np.concatenate(mvd)
-    tp = np.concatenate(tpos)
+        tab[k] = np.concatenate([p[4][k] for p in parts])[orig]
+    spec, subt, ncf, mvd = [], [], [], []
+    for p in parts:
+        spectype, subtype = parse_fulltype(p[3])
+        n = p[0].size
+        spec.append(np.full(n, spectype))
+        subt.append(np.full(n, subtype))
+        ncf.append(np.full(n, p[4]['coeff'].shape[1]))
+        mvd.append(np.full(n, constants.max_velo_diff_star if spectype.upper() == 'STAR'
+                           else constants.max_velo_diff))
+    tab['spectype'] = np.concatenate(spec)[orig]
+    tab['subtype'] = np.concatenate(subt)[orig]
+    tab['ncoeff'] = np.concatenate(ncf)[orig]
+    mv = np.concatenate(mvd)[orig]
+    tp = tp_all[orig]
     zw = tab['zwarn']
     zw[tab['npixels'] == 0] |= ZW.NODATA
     zw[tab['npixels'] < 10 * tab['ncoeff']] |= ZW.LITTLE_COVERAGE

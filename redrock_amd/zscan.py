@@ -36,6 +36,7 @@ from __future__ import annotations
 import ctypes
 import math
 import os
+import threading
 import warnings
 
 import numpy as np
@@ -330,12 +331,14 @@ class TargetBatch(object):
                 spectra = [tg.spectra[k] for tg in targets]
                 n = spectra[0].nwave
                 _upload_bands(self.rband[:, off:off + n], spectra)
-                for i, sp in enumerate(spectra):
-                    w[i, off:off + n] = sp.ivar
-                    f[i, off:off + n] = sp.flux
+                if nt:
+                    w[:, off:off + n] = [sp.ivar for sp in spectra]
+                    f[:, off:off + n] = [sp.flux for sp in spectra]
             self.good_pixels = np.count_nonzero(w > 0, axis=1)
-            self.w = torch.from_numpy(w).to(dev)
-            self.f = torch.from_numpy(f).to(dev)
+            self.w = torch.empty((nt, P), dtype=torch.float64, device=dev)
+            self.f = torch.empty((nt, P), dtype=torch.float64, device=dev)
+            _upload_rows(self.w, w)
+            _upload_rows(self.f, f)
             self.wf = self.w * self.f
         self.layout = layout
         self.nt = self.rband.shape[0]
@@ -387,9 +390,75 @@ class TargetBatch(object):
         return self._mixed[precision]
 
 
+# Host-to-device uploads of per-target rows go through two pinned staging
+# buffers kept for the process: a small thread pool fills one (numpy copies
+# release the GIL) while the asynchronous copy of the other runs.
+STAGE_BYTES = 64 << 20
+_STAGING = []
+_POOL = []
+_STAGE_LOCK = threading.Lock()        # one upload at a time uses the staging buffers
+
+
+def _staging(k):
+    import torch
+    while len(_STAGING) <= k:
+        _STAGING.append((torch.empty(STAGE_BYTES, dtype=torch.uint8, pin_memory=True),
+                         torch.cuda.Event()))
+    return _STAGING[k]
+
+
+def _copy_pool():
+    if not _POOL:
+        import concurrent.futures
+        _POOL.append(concurrent.futures.ThreadPoolExecutor(
+            max_workers=max(1, min(8, len(os.sched_getaffinity(0))))))
+    return _POOL[0]
+
+
+def _upload_rows(dst, rows):
+    """dst[i] = rows[i] for a contiguous float64 device tensor dst (n, ...) and
+    n host arrays of shape dst.shape[1:] (or an (n, ...) host array), in
+    chunks of STAGE_BYTES through the pinned staging buffers; the copies are
+    ordered on the current stream."""
+    import torch
+    n = dst.shape[0]
+    if n == 0:
+        return
+    shape = tuple(dst.shape[1:])
+    row = int(np.prod(shape, dtype=np.int64)) * 8
+    if row > STAGE_BYTES:
+        for i in range(n):
+            dst[i].copy_(torch.from_numpy(np.ascontiguousarray(rows[i], dtype=np.float64)))
+        return
+    per = max(1, STAGE_BYTES // row)
+    pool = _copy_pool()
+    nthr = pool._max_workers
+    stream = torch.cuda.current_stream()
+    with _STAGE_LOCK:
+        _upload_chunks(dst, rows, n, shape, row, per, pool, nthr, stream)
+
+
+def _upload_chunks(dst, rows, n, shape, row, per, pool, nthr, stream):
+    import torch
+    for j, c0 in enumerate(range(0, n, per)):
+        c1 = min(n, c0 + per)
+        buf, ev = _staging(j % 2)
+        ev.synchronize()                  # the copy that last read this buffer is done
+        src = buf[:(c1 - c0) * row].view(torch.float64).view((c1 - c0,) + shape)
+        host = src.numpy()
+
+        def fill(a, b, host=host, c0=c0):
+            for i in range(a, b):
+                host[i - c0] = rows[i]
+        cuts = np.linspace(c0, c1, min(nthr, c1 - c0) + 1).astype(np.int64)
+        list(pool.map(lambda ab: fill(*ab), zip(cuts[:-1], cuts[1:])))
+        dst[c0:c1].copy_(src, non_blocking=True)
+        ev.record(stream)
+
+
 def _upload_bands(out, spectra):
     """Row bands (nt, n, 11) of one camera's resolution matrices into the device
-    view `out` (synchronous host-to-device copies from pageable staging).
+    view `out`.
 
     Spectra whose R are dia_matrix with the same offsets and data shape (the
     reference's layout, every DESI coadd) are uploaded as their diagonal data
@@ -414,10 +483,8 @@ def _upload_bands(out, spectra):
         out.copy_(torch.from_numpy(hn))
         return
     nd, L = first.data.shape
-    hn = np.empty((nt, nd, L))
-    for i, R in enumerate(Rs):
-        hn[i] = R.data
-    d = torch.from_numpy(hn).to(out.device)
+    d = torch.empty((nt, nd, L), dtype=torch.float64, device=out.device)
+    _upload_rows(d, [R.data for R in Rs])
     nrow, ncol = first.shape
     L = min(ncol, L)
     for di, o in enumerate(np.asarray(first.offsets).astype(np.int64)):
@@ -430,6 +497,17 @@ def _upload_bands(out, spectra):
                 raise NotImplementedError('resolution matrix wider than 11 diagonals')
             continue
         out[:, p0:p1, o + RBAND_HALF] += vals
+
+
+def _dev(a, dtype):
+    """A host array as a new device tensor, copied without waiting for the
+    stream's earlier work: staged in page-locked memory (torch's pinned
+    allocator keeps the block until the copy has run)."""
+    import torch
+    t = torch.from_numpy(np.ascontiguousarray(a, dtype=dtype))
+    if t.numel() == 0:
+        return t.to('cuda')
+    return t.pin_memory().to('cuda', non_blocking=True)
 
 
 def target_runs(item_target):
@@ -510,18 +588,16 @@ class ScanPlan(object):
         self.item_out = item_out
         self.item_nz = item_nz
         self._keep = [tbuf, batch]
-        self.t_item_target = torch.as_tensor(item_target, device=dev)
-        self.t_item_z0 = torch.as_tensor(item_z0, device=dev)
-        self.t_item_nz = torch.as_tensor(item_nz, device=dev)
-        self.t_item_out = torch.as_tensor(item_out, dtype=torch.int64, device=dev)
+        self.t_item_target = _dev(item_target, np.int32)
+        self.t_item_z0 = _dev(item_z0, np.int32)
+        self.t_item_nz = _dev(item_nz, np.int32)
+        self.t_item_out = _dev(item_out, np.int64)
         self.zchi2 = torch.empty(rows, dtype=torch.float64, device=dev)
         self.zcoeff = torch.empty((rows, nb), dtype=torch.float64, device=dev)
         self.penalty = torch.empty(rows, dtype=torch.float64, device=dev)
         self.status = torch.empty(rows, dtype=torch.uint8, device=dev)
-        self.prior = None if prior is None else torch.as_tensor(
-            np.ascontiguousarray(prior, dtype=np.float64), device=dev)
-        self.oiisum = None if oiisum is None else torch.as_tensor(
-            np.ascontiguousarray(oiisum, dtype=np.float64), device=dev)
+        self.prior = None if prior is None else _dev(prior, np.float64)
+        self.oiisum = None if oiisum is None else _dev(oiisum, np.float64)
         d = _hip.ZscanDesc()
         d.nitems = self.nitems
         d.npix = layout.npix
@@ -579,7 +655,7 @@ class ScanPlan(object):
         d.total_rows = rows
         d.solve, d.nonneg_mask, bnd = solve_spec(solve, nb, bounds)
         if bnd is not None:
-            self._bounds = torch.as_tensor(bnd, dtype=torch.float64, device=dev).contiguous()
+            self._bounds = _dev(bnd, np.float64)
             d.d_bounds = self._bounds.data_ptr()
         nbytes = _hip.lib().rr_zscan_workspace_bytes(ctypes.byref(d))
         if nbytes < 0:
@@ -968,12 +1044,20 @@ def calc_zchi2_targets(targets, templates, mp_procs=1, use_gpu=False, precision=
 def _scan_templates(targets, templates, precision=None, batches=None, polish=None):
     """calc_zchi2_targets' results and, per template full_type, the (nt, nz)
     arrays zchi2 + penalty of the local targets in order (what zfind's fine
-    fits start from), without splitting and re-stacking them per target."""
+    fits start from), without splitting and re-stacking them per target.
+
+    With one wavelength layout every template's scan is enqueued first; each
+    scan's results are copied to page-locked host memory on a second stream
+    as soon as its kernels finish and are unpacked on the host while the
+    later templates' scans run."""
     results = {tid: dict() for tid in targets.local_target_ids()}
     full = dict()
     local = targets.local()
     if batches is None and local:
         batches = target_batches(local)
+    if local and len(batches) == 1:
+        return _scan_templates_pipelined(results, local, templates, precision,
+                                         next(iter(batches.values())), polish)
     for t in templates:
         ft = t.template.full_type
         zchi2, zcoeff, penalty = calc_zchi2(targets.local_target_ids(), local, t,
@@ -982,4 +1066,45 @@ def _scan_templates(targets, templates, precision=None, batches=None, polish=Non
         for i, tg in enumerate(local):
             results[tg.id][ft] = dict(redshifts=t.template.redshifts, zchi2=zchi2[i],
                                       penalty=penalty[i], zcoeff=zcoeff[i])
+    return results, full
+
+
+def _scan_templates_pipelined(results, local, templates, precision, batch, polish):
+    import torch
+    precision = precision or default_precision()
+    nt = len(local)
+    main = torch.cuda.current_stream()
+    copy = torch.cuda.Stream()
+    pending = []
+    for t in templates:
+        cache = t.cache
+        nz = cache.nz
+        plan = ScanPlan(cache, batch, np.arange(nt), np.zeros(nt), np.full(nt, nz), precision,
+                        oiisum=_oiisum(t.template), polish=polish, solve=_method(cache))
+        if plan.unpolished is not None:
+            unpolished[t.template.full_type] = plan.unpolished
+        plan.run(main)
+        ran = torch.cuda.Event()
+        ran.record(main)
+        outs = (plan.zchi2, plan.zcoeff, plan.penalty)
+        hs = [torch.empty(o.shape, dtype=o.dtype, pin_memory=True) for o in outs]
+        copy.wait_event(ran)
+        with torch.cuda.stream(copy):
+            for h, o in zip(hs, outs):
+                h.copy_(o, non_blocking=True)
+        done = torch.cuda.Event()
+        done.record(copy)
+        pending.append((t, plan, hs, done, nz))
+    full = dict()
+    for t, plan, hs, done, nz in pending:
+        done.synchronize()
+        nb = plan.nb
+        zchi2, zcoeff, penalty = (h.numpy() for h in hs)
+        zchi2, zcoeff, penalty = zchi2.reshape(nt, nz), zcoeff.reshape(nt, nz, nb), penalty.reshape(nt, nz)
+        ft = t.template.full_type
+        full[ft] = zchi2 + penalty
+        for i, tg in enumerate(local):
+            results[tg.id][ft] = dict(redshifts=t.template.redshifts, zchi2=zchi2[i],
+                                      penalty=penalty[i], zcoeff=zcoeff[i])
+    main.wait_stream(copy)
     return results, full
