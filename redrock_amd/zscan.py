@@ -317,7 +317,7 @@ class TargetBatch(object):
             self.w = torch.as_tensor(np.ascontiguousarray(w), dtype=torch.float64, device=dev)
             self.f = torch.as_tensor(np.ascontiguousarray(f), dtype=torch.float64, device=dev)
             self.wf = torch.as_tensor(np.ascontiguousarray(wf), dtype=torch.float64, device=dev)
-            self.good_pixels = np.count_nonzero(np.asarray(w) > 0, axis=1)
+            self._good = np.count_nonzero(np.asarray(w) > 0, axis=1)
         else:
             layout = layout or PixelLayout.of(targets[0].spectra)
             nt, P = len(targets), layout.npix_pad
@@ -325,24 +325,29 @@ class TargetBatch(object):
                 if PixelLayout.signature_of(tg.spectra) != layout.signature:
                     raise ValueError('targets of one batch must share their wavelength grids')
             self.rband = torch.zeros((nt, P, 11), dtype=torch.float64, device=dev)
-            w = np.zeros((nt, P))
-            f = np.zeros((nt, P))
+            self.w = torch.zeros((nt, P), dtype=torch.float64, device=dev)
+            self.f = torch.zeros((nt, P), dtype=torch.float64, device=dev)
             for k, off in enumerate(layout.offsets):
                 spectra = [tg.spectra[k] for tg in targets]
                 n = spectra[0].nwave
                 _upload_bands(self.rband[:, off:off + n], spectra)
-                if nt:
-                    w[:, off:off + n] = [sp.ivar for sp in spectra]
-                    f[:, off:off + n] = [sp.flux for sp in spectra]
-            self.good_pixels = np.count_nonzero(w > 0, axis=1)
-            self.w = torch.empty((nt, P), dtype=torch.float64, device=dev)
-            self.f = torch.empty((nt, P), dtype=torch.float64, device=dev)
-            _upload_rows(self.w, w)
-            _upload_rows(self.f, f)
+                seg = torch.empty((2, nt, n), dtype=torch.float64, device=dev)
+                _upload_rows(seg[0], [sp.ivar for sp in spectra])
+                _upload_rows(seg[1], [sp.flux for sp in spectra])
+                self.w[:, off:off + n] = seg[0]
+                self.f[:, off:off + n] = seg[1]
+            self._good = None
             self.wf = self.w * self.f
         self.layout = layout
         self.nt = self.rband.shape[0]
         self._mixed = dict()
+
+    @property
+    def good_pixels(self):
+        """Host int array: each target's number of pixels with ivar > 0."""
+        if self._good is None:
+            self._good = (self.w > 0).sum(dim=1).cpu().numpy()
+        return self._good
 
     def lanes(self, precision, order=None):
         """Lane-major pack of the lane-per-item kernels (rr_pack_lanes), 'f64' or
@@ -394,6 +399,7 @@ class TargetBatch(object):
 # buffers kept for the process: a small thread pool fills one (numpy copies
 # release the GIL) while the asynchronous copy of the other runs.
 STAGE_BYTES = 64 << 20
+UPLOAD_THREADS = 2
 _STAGING = []
 _POOL = []
 _STAGE_LOCK = threading.Lock()        # one upload at a time uses the staging buffers
@@ -411,7 +417,7 @@ def _copy_pool():
     if not _POOL:
         import concurrent.futures
         _POOL.append(concurrent.futures.ThreadPoolExecutor(
-            max_workers=max(1, min(8, len(os.sched_getaffinity(0))))))
+            max_workers=max(1, min(UPLOAD_THREADS, len(os.sched_getaffinity(0))))))
     return _POOL[0]
 
 
@@ -448,8 +454,10 @@ def _upload_chunks(dst, rows, n, shape, row, per, pool, nthr, stream):
         host = src.numpy()
 
         def fill(a, b, host=host, c0=c0):
-            for i in range(a, b):
-                host[i - c0] = rows[i]
+            # one call per slice: numpy copies without the GIL, and few
+            # Python steps keep the workers from contending for it
+            np.concatenate([np.reshape(r, (1,) + shape) for r in rows[a:b]], axis=0,
+                           out=host[a - c0:b - c0], casting='unsafe')
         cuts = np.linspace(c0, c1, min(nthr, c1 - c0) + 1).astype(np.int64)
         list(pool.map(lambda ab: fill(*ab), zip(cuts[:-1], cuts[1:])))
         dst[c0:c1].copy_(src, non_blocking=True)
@@ -473,8 +481,9 @@ def _upload_bands(out, spectra):
     Rs = [s.R for s in spectra]
     first = Rs[0]
     same = (all(scipy.sparse.issparse(R) and R.format == 'dia' for R in Rs)
-            and all(R.shape == first.shape and R.data.shape == first.data.shape
-                    and np.array_equal(R.offsets, first.offsets) for R in Rs))
+            and all(R.shape == first.shape and R.data.shape == first.data.shape for R in Rs))
+    if same:            # every offsets array as one (nt, ndiag) comparison
+        same = bool(np.all(np.array([R.offsets for R in Rs]) == np.asarray(first.offsets)[None, :]))
     nt, n = out.shape[0], out.shape[1]
     if not same:
         hn = np.empty((nt, n, 11))
