@@ -1078,6 +1078,14 @@ def _scan_templates(targets, templates, precision=None, batches=None, polish=Non
     return results, full
 
 
+def _plan_bytes(rows, nb):
+    """Device bytes a full-grid ScanPlan of `rows` (target, z) rows holds:
+    chi2, coefficients, penalty, status and the normal-equation workspace
+    (upper bound: the mixed-precision column count)."""
+    ne = nb * (nb + 2) - nb * (nb - 1) // 2
+    return int(rows) * (8 * (nb + 2) + 1 + 8 * ne)
+
+
 def _scan_templates_pipelined(results, local, templates, precision, batch, polish):
     import torch
     precision = precision or default_precision()
@@ -1085,9 +1093,28 @@ def _scan_templates_pipelined(results, local, templates, precision, batch, polis
     main = torch.cuda.current_stream()
     copy = torch.cuda.Stream()
     pending = []
+    full = dict()
+
+    def unpack(entry):
+        t, plan, hs, done, nz = entry
+        done.synchronize()
+        nb = plan.nb
+        zchi2, zcoeff, penalty = (h.numpy() for h in hs)
+        zchi2, zcoeff, penalty = zchi2.reshape(nt, nz), zcoeff.reshape(nt, nz, nb), penalty.reshape(nt, nz)
+        ft = t.template.full_type
+        full[ft] = zchi2 + penalty
+        for i, tg in enumerate(local):
+            results[tg.id][ft] = dict(redshifts=t.template.redshifts, zchi2=zchi2[i],
+                                      penalty=penalty[i], zcoeff=zcoeff[i])
+
     for t in templates:
         cache = t.cache
         nz = cache.nz
+        # a plan holds its outputs and normal-equation workspace until it is
+        # unpacked: unpack the pending ones first when the next would not fit
+        need = _plan_bytes(nt * nz, cache.nb)
+        while pending and torch.cuda.mem_get_info()[0] < 2 * need:
+            unpack(pending.pop(0))
         plan = ScanPlan(cache, batch, np.arange(nt), np.zeros(nt), np.full(nt, nz), precision,
                         oiisum=_oiisum(t.template), polish=polish, solve=_method(cache))
         if plan.unpolished is not None:
@@ -1104,16 +1131,7 @@ def _scan_templates_pipelined(results, local, templates, precision, batch, polis
         done = torch.cuda.Event()
         done.record(copy)
         pending.append((t, plan, hs, done, nz))
-    full = dict()
-    for t, plan, hs, done, nz in pending:
-        done.synchronize()
-        nb = plan.nb
-        zchi2, zcoeff, penalty = (h.numpy() for h in hs)
-        zchi2, zcoeff, penalty = zchi2.reshape(nt, nz), zcoeff.reshape(nt, nz, nb), penalty.reshape(nt, nz)
-        ft = t.template.full_type
-        full[ft] = zchi2 + penalty
-        for i, tg in enumerate(local):
-            results[tg.id][ft] = dict(redshifts=t.template.redshifts, zchi2=zchi2[i],
-                                      penalty=penalty[i], zcoeff=zcoeff[i])
+    for entry in pending:
+        unpack(entry)
     main.wait_stream(copy)
     return results, full

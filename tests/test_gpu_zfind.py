@@ -191,3 +191,20 @@ def test_zfind_resumes_from_details_file(gpu, tmp_path):
                 if key != 'redshifts':
                     d1 = d1.astype(np.float32)
                 assert np.array_equal(d1, zscan2[tid][ft][key]), (tid, ft, key)
+
+
+def test_scan_templates_pipelined_under_memory_pressure(gpu, monkeypatch):
+    """The pipelined scan of every template unpacks pending scans first when
+    the next plan would not fit; the results are the same either way."""
+    from redrock_amd import zscan
+    targets, dts, dwave, dist = _setup()
+    print('case: _scan_templates with and without early unpacking', flush=True)
+    res_a, full_a = zscan._scan_templates(dist, dts, precision='f16r')
+    monkeypatch.setattr(zscan, '_plan_bytes', lambda rows, nb: 1 << 62)
+    res_b, full_b = zscan._scan_templates(dist, dts, precision='f16r')
+    assert list(full_a) == list(full_b)
+    for ft in full_a:
+        assert np.array_equal(full_a[ft], full_b[ft])
+        for t in targets:
+            for k in ('zchi2', 'penalty', 'zcoeff'):
+                assert np.array_equal(res_a[t.id][ft][k], res_b[t.id][ft][k])
