@@ -208,50 +208,23 @@ def zfind(targets, templates, mp_procs=1, nminima=3, archetypes=None, priors=Non
     from .zscan import target_batches
     zscan.unpolished.clear()
     batches = target_batches(local) if local else dict()
-    full = dict()                     # full_type -> (nt, nz) zchi2 + penalty of `local`
-    if chi2_scan is None:                                    # zfind.py:289-292
-        results, full = zscan._scan_templates(dist, templates, precision=precision,
-                                              batches=batches, polish=polish)
-    else:
-        from .results import read_zscan_redrock
-        saved = read_zscan_redrock(chi2_scan)
-        results = dict()
-        for tg in local:
-            if tg.id not in saved:
-                raise KeyError(f'{chi2_scan}: no scan for target {tg.id}')
-            results[tg.id] = {ft: {k: np.asarray(v, dtype=np.float64) for k, v in d.items()}
-                              for ft, d in saved[tg.id].items()}
-
-    timings['scan'] = clock() - t0
-    if priors is not None:                                   # zfind.py:318-322
-        full = dict()
-        for tid in results:
-            for ft in results[tid]:
-                results[tid][ft]['zchi2'] = (results[tid][ft]['zchi2']
-                                             + priors.eval(tid, results[tid][ft]['redshifts']))
-
+    one = next(iter(batches.values())) if len(batches) == 1 else None
     # good pixels per target (the npixels column, fitz.py:388-390), once for every template
-    if len(batches) == 1:
-        npix = [int(n) for n in next(iter(batches.values())).good_pixels]
+    if one is not None:
+        npix = [int(n) for n in one.good_pixels]
     else:
         npix = [sum(int(np.count_nonzero(s.ivar > 0.)) for s in tg.spectra) for tg in local]
-    order = np.array([t.template.full_type for t in templates]).argsort()
     # without archetypes or Newton columns the fits go to the table as flat
     # columns (fitz_columns) instead of per-target dicts
     blocks = dict() if not archetypes and not newton else None
-    for t in np.array(list(templates), dtype=object)[order]:      # zfind.py:330-367
+
+    def fit(t, chi2):
+        """The fine fits of every target for template t (zfind.py:330-367)."""
         ft = t.template.full_type
-        if not local:
-            continue
-        chi2 = full.get(ft)
-        if chi2 is None:
-            chi2 = np.stack([results[tg.id][ft]['zchi2'] + results[tg.id][ft]['penalty']
-                             for tg in local])
         tf = clock()
         archetype = None
         if archetypes and t.template.template_type in archetypes:
             archetype = archetypes[t.template.template_type]
-        one = next(iter(batches.values())) if len(batches) == 1 else None
         if blocks is not None:
             blocks[ft] = ([tg.id for tg in local],) + fitz_columns(
                 chi2, t.template.redshifts, local, t.template, nminima=nminima,
@@ -265,6 +238,63 @@ def zfind(targets, templates, mp_procs=1, nminima=3, archetypes=None, priors=Non
             for tg, zfit in zip(local, fits):
                 results[tg.id][ft]['zfit'] = zfit
         timings['fitz'][ft] = clock() - tf
+
+    def with_priors(ft, redshifts):
+        """zchi2 += the prior (zfind.py:318-322); the (nt, nz) zchi2 + penalty."""
+        for tid in results:
+            results[tid][ft]['zchi2'] = results[tid][ft]['zchi2'] + priors.eval(tid, redshifts)
+        return np.stack([results[tg.id][ft]['zchi2'] + results[tg.id][ft]['penalty']
+                         for tg in local])
+
+    if chi2_scan is None and one is not None:                # zfind.py:289-292
+        # each template's fine fits start on a second stream as soon as its
+        # scan is on the host, while the later templates' scans run
+        import torch
+        side = torch.cuda.Stream()
+        side.wait_stream(torch.cuda.current_stream())
+        results = {tid: dict() for tid in dist.local_target_ids()}
+        for t, zchi2, zcoeff, penalty in zscan.scan_templates_iter(local, templates, precision, one,
+                                                                   polish):
+            ft = t.template.full_type
+            for i, tg in enumerate(local):
+                results[tg.id][ft] = dict(redshifts=t.template.redshifts, zchi2=zchi2[i],
+                                          penalty=penalty[i], zcoeff=zcoeff[i])
+            chi2 = zchi2 + penalty if priors is None else with_priors(ft, t.template.redshifts)
+            timings['scan'] = clock() - t0
+            with torch.cuda.stream(side):
+                fit(t, chi2)
+        torch.cuda.current_stream().wait_stream(side)
+    else:
+        full = dict()                 # full_type -> (nt, nz) zchi2 + penalty of `local`
+        if chi2_scan is None:
+            results, full = zscan._scan_templates(dist, templates, precision=precision,
+                                                  batches=batches, polish=polish)
+        else:
+            from .results import read_zscan_redrock
+            saved = read_zscan_redrock(chi2_scan)
+            results = dict()
+            for tg in local:
+                if tg.id not in saved:
+                    raise KeyError(f'{chi2_scan}: no scan for target {tg.id}')
+                results[tg.id] = {ft: {k: np.asarray(v, dtype=np.float64) for k, v in d.items()}
+                                  for ft, d in saved[tg.id].items()}
+        timings['scan'] = clock() - t0
+        if priors is not None:                               # zfind.py:318-322
+            full = dict()
+            for tid in results:
+                for ft in results[tid]:
+                    results[tid][ft]['zchi2'] = (results[tid][ft]['zchi2']
+                                                 + priors.eval(tid, results[tid][ft]['redshifts']))
+        order = np.array([t.template.full_type for t in templates]).argsort()
+        for t in np.array(list(templates), dtype=object)[order]:      # zfind.py:330-367
+            ft = t.template.full_type
+            if not local:
+                continue
+            chi2 = full.get(ft)
+            if chi2 is None:
+                chi2 = np.stack([results[tg.id][ft]['zchi2'] + results[tg.id][ft]['penalty']
+                                 for tg in local])
+            fit(t, chi2)
 
     for tg in local:
         results[tg.id]['meta'] = getattr(tg, 'meta', dict())
@@ -292,6 +322,8 @@ def zfind(targets, templates, mp_procs=1, nminima=3, archetypes=None, priors=Non
 
 # wall-clock seconds of the phases of the last zfind call (scan, fitz per
 # template, assemble, total): host timings for profiling, not part of the API;
+# with one wavelength layout a template's fits overlap the later scans, and
+# 'scan' is the time until the last scan result reached the host;
 # 'unpolished' maps each template whose mixed-precision scan ran without the
 # FP64 polish to the reason
 last_timings = dict()
