@@ -414,10 +414,11 @@ def _staging(k):
 
 
 def _copy_pool():
+    """(thread pool, its number of threads) for the staging copies."""
     if not _POOL:
         import concurrent.futures
-        _POOL.append(concurrent.futures.ThreadPoolExecutor(
-            max_workers=max(1, min(UPLOAD_THREADS, len(os.sched_getaffinity(0))))))
+        n = max(1, min(UPLOAD_THREADS, len(os.sched_getaffinity(0))))
+        _POOL.append((concurrent.futures.ThreadPoolExecutor(max_workers=n), n))
     return _POOL[0]
 
 
@@ -437,8 +438,7 @@ def _upload_rows(dst, rows):
             dst[i].copy_(torch.from_numpy(np.ascontiguousarray(rows[i], dtype=np.float64)))
         return
     per = max(1, STAGE_BYTES // row)
-    pool = _copy_pool()
-    nthr = pool._max_workers
+    pool, nthr = _copy_pool()
     stream = torch.cuda.current_stream()
     with _STAGE_LOCK:
         _upload_chunks(dst, rows, n, shape, row, per, pool, nthr, stream)
@@ -482,8 +482,7 @@ def download(tensors, stream=None):
         chunks += [(ti, off, min(STAGE_BYTES, nbytes - off)) for off in range(0, nbytes, STAGE_BYTES)]
     if not chunks:
         return outs
-    pool = _copy_pool()
-    nthr = pool._max_workers
+    pool, nthr = _copy_pool()
 
     def finish(j, ti, off, nb):
         buf, ev = _staging(j % 2)
@@ -1091,10 +1090,9 @@ def _scan_templates(targets, templates, precision=None, batches=None, polish=Non
     arrays zchi2 + penalty of the local targets in order (what zfind's fine
     fits start from), without splitting and re-stacking them per target.
 
-    With one wavelength layout every template's scan is enqueued first; each
-    scan's results are copied to page-locked host memory on a second stream
-    as soon as its kernels finish and are unpacked on the host while the
-    later templates' scans run."""
+    With one wavelength layout every template's scan is enqueued first and
+    each one's results are copied to the host while the later scans run
+    (scan_templates_iter)."""
     results = {tid: dict() for tid in targets.local_target_ids()}
     full = dict()
     local = targets.local()
