@@ -250,7 +250,7 @@ def zfind(targets, templates, mp_procs=1, nminima=3, archetypes=None, priors=Non
         # each template's fine fits start on a second stream as soon as its
         # scan is on the host, while the later templates' scans run
         import torch
-        side = torch.cuda.Stream()
+        side = zscan.pipeline_streams()[2]
         side.wait_stream(torch.cuda.current_stream())
         results = {tid: dict() for tid in dist.local_target_ids()}
         for t, zchi2, zcoeff, penalty in zscan.scan_templates_iter(local, templates, precision, one,

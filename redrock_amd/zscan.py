@@ -1112,6 +1112,22 @@ def _scan_templates(targets, templates, precision=None, batches=None, polish=Non
     return results, full
 
 
+_PIPE = []
+
+
+def pipeline_streams():
+    """(scan, copy, fit) streams of the pipelined template scans and zfind's
+    fine fits, taken once per process.  Streams share a few hardware queues,
+    and work on two streams of one queue runs in order: three consecutive
+    streams of torch's pool are on three queues (measured), so results are
+    copied and fine fits run beside the scans.  Taken per call instead, a
+    stream can land on the queue of the default stream or of another."""
+    import torch
+    if not _PIPE:
+        _PIPE.extend(torch.cuda.Stream() for _ in range(3))
+    return tuple(_PIPE)
+
+
 def _plan_bytes(rows, nb):
     """Device bytes a full-grid ScanPlan of `rows` (target, z) rows holds:
     chi2, coefficients, penalty, status and the normal-equation workspace
@@ -1136,16 +1152,19 @@ def scan_templates_iter(local, templates, precision, batch, polish=None):
     template; yields (template, zchi2 (nt, nz), zcoeff (nt, nz, nb),
     penalty (nt, nz)) in template order, as numpy arrays (download()).
 
-    Every template's scan is enqueued on the current stream before the first
-    result is handed out (a template's plan is unpacked earlier only when the
-    next one would not fit in free HBM); a scan's results are copied to the
-    host on a second stream when they are handed out, while the later
-    templates' scans run."""
+    Every template's scan is enqueued on the scan stream of
+    pipeline_streams() (after the work already on the current stream) before
+    the first result is handed out (a template's plan is unpacked earlier
+    only when the next one would not fit in free HBM); a scan's results are
+    copied to the host on the copy stream when they are handed out, while the
+    later templates' scans run.  When the iterator ends the current stream
+    waits for both."""
     import torch
     precision = precision or default_precision()
     nt = len(local)
-    main = torch.cuda.current_stream()
-    copy = torch.cuda.Stream()
+    caller = torch.cuda.current_stream()
+    main, copy, _ = pipeline_streams()
+    main.wait_stream(caller)          # the target upload and anything else before
     pending = []
 
     def unpack(entry):
@@ -1173,4 +1192,5 @@ def scan_templates_iter(local, templates, precision, batch, polish=None):
         pending.append((t, plan, ran, nz))
     while pending:
         yield unpack(pending.pop(0))
-    main.wait_stream(copy)
+    caller.wait_stream(main)
+    caller.wait_stream(copy)
