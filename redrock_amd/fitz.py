@@ -333,7 +333,7 @@ class _Refiner(object):
         return res
 
 
-def select_minima(zchi2, redshifts, nminima, max_velo_diff, refine, speculate=2):
+def select_minima(zchi2, redshifts, nminima, max_velo_diff, refine, speculate=1):
     """The candidate loop of fitz.py:198-327 for every row of zchi2 at once.
 
     Per row (target) t the reference walks find_minima(zchi2[t]) in order:
@@ -499,7 +499,7 @@ def _vstack(vals):
 
 
 def fitz_targets(zchi2, redshifts, targets, template, nminima=3, zminfit_npoints=15,
-                 precision='f64', speculate=2, newton=False, archetype=None, deg_legendre=None,
+                 precision='f64', speculate=1, newton=False, archetype=None, deg_legendre=None,
                  per_camera=False, n_nearest=None, prior_sigma=None, batch=None, npixels=None):
     """fitz for every target of one template (zchi2: (nt, nz) incl. penalty).
 
@@ -612,7 +612,7 @@ def _as_dicts(nacc, cols):
 
 
 def fitz_columns(zchi2, redshifts, targets, template, nminima=3, zminfit_npoints=15,
-                 precision='f64', speculate=2, batch=None, npixels=None):
+                 precision='f64', speculate=1, batch=None, npixels=None):
     """fitz_targets without archetypes, as flat columns: (nacc, cols) with
     nacc[t] the number of minima of target t and cols the dict of their
     z, zerr, zwarn, chi2, fitmethod, npixels (n,), zz, zzchi2 (n, nz_fine)
