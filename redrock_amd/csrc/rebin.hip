@@ -6,8 +6,9 @@
 // laid out so that a wavefront's stores are contiguous: 256 bins of one
 // redshift per workgroup when bins are the fast output axis (the template
 // window those bins read is then staged in LDS, rebin_lds_kernel), 64
-// redshifts x 4 bins per workgroup when the output is z-fastest (the FP64
-// cache of the redshift-per-lane scan).  The bin mean
+// redshifts x 4 runs of 8 consecutive bins per workgroup when the output is
+// z-fastest (the FP64 cache of the redshift-per-lane scan; a thread searches
+// for its first bin's samples and steps forward for the next ones).  The bin mean
 // follows the serial sweep of py/redrock/rebin.py:126-176 expression by
 // expression, and this file is compiled with FP contraction off, so the FP64
 // result equals the sweep's double for double: the sample index j is the
@@ -74,9 +75,13 @@ __device__ __forceinline__ double igm_factor(const RebinArgs &a, int iz, int ibi
     return T;
 }
 
-// One (redshift, bin), template samples read from global memory.
+// One (redshift, bin), template samples read from global memory.  jraw: on
+// entry a lower bound of the first sample beyond the lower edge (0, or that
+// index for an earlier bin of the same redshift: the edges increase), on
+// exit that index for this bin.  From 0 it is found by binary search, else
+// by stepping forward (a bin holds a few samples); both give the same index.
 template <int OUT>
-__device__ void rebin_bin(const RebinArgs &a, int iz, int ibin) {
+__device__ void rebin_bin(const RebinArgs &a, int iz, int ibin, int &jraw) {
     if (ibin >= a.nbin || iz >= a.nz) return;
     const double *x = a.x;
     const double opz = 1. + a.z[iz];
@@ -86,7 +91,12 @@ __device__ void rebin_bin(const RebinArgs &a, int iz, int ibin) {
 
     // rebin.py:142 — first sample beyond the lower edge; clamped so that an
     // out-of-range call stays in bounds (the host checks the range first).
-    int j = upper_bound_scaled(x, 0, nx, opz, elo);
+    if (jraw == 0) {
+        jraw = upper_bound_scaled(x, 0, nx, opz, elo);
+    } else {
+        while (jraw < nx && x[jraw] * opz <= elo) ++jraw;
+    }
+    int j = jraw;
     if (j < 1) j = 1;
     if (j > nx - 1) j = nx - 1;
     const double xj = x[j] * opz;
@@ -129,18 +139,22 @@ __device__ void rebin_bin(const RebinArgs &a, int iz, int ibin) {
     }
 }
 
+// z-fastest outputs: kZBins consecutive bins per thread, one binary search
+constexpr int kZBins = 8;
+
 template <int OUT, bool ZFAST>
 __global__ __launch_bounds__(256) void rebin_kernel(RebinArgs a, int nfast) {
-    int iz, ibin;
-    if (ZFAST) {   // lane -> redshift, wavefront -> bin; nfast = redshift blocks of 64
+    int jraw = 0;
+    if (ZFAST) {   // lane -> redshift, wavefront -> kZBins bins; nfast = redshift blocks of 64
         const int zb = blockIdx.x % nfast, bb = blockIdx.x / nfast;
-        iz = zb * 64 + (threadIdx.x & 63);
-        ibin = bb * 4 + (threadIdx.x >> 6);
+        const int iz = zb * 64 + (threadIdx.x & 63);
+        const int ibin0 = (bb * 4 + (threadIdx.x >> 6)) * kZBins;
+        for (int t = 0; t < kZBins; ++t) rebin_bin<OUT>(a, iz, ibin0 + t, jraw);
     } else {       // thread -> bin; nfast = bin blocks of 256
-        iz = blockIdx.x / nfast;
-        ibin = (blockIdx.x - iz * nfast) * blockDim.x + threadIdx.x;
+        const int iz = blockIdx.x / nfast;
+        const int ibin = (blockIdx.x - iz * nfast) * blockDim.x + threadIdx.x;
+        rebin_bin<OUT>(a, iz, ibin, jraw);
     }
-    rebin_bin<OUT>(a, iz, ibin);
 }
 
 // Bins-fast layouts with the template window staged in LDS: a workgroup's
@@ -204,7 +218,8 @@ __global__ __launch_bounds__(256) void rebin_lds_kernel(RebinArgs a, int nfast) 
     if (whi < jA) whi = jA;
     const int W = whi - wlo + 1;
     if (2 * W > kLdsDoubles) {                              // uniform: global path
-        rebin_bin<OUT>(a, iz, ibin);
+        int jraw = 0;
+        rebin_bin<OUT>(a, iz, ibin, jraw);
         return;
     }
     double *xs = lds;
@@ -279,7 +294,8 @@ extern "C" int rr_rebin(const double *d_x, int nx, const double *d_y, int nb, lo
     for (int l = 0; l < nlines; ++l) a.lines[l] = lines[l];
     const bool zfast = s_z == 1 && s_pix != 1;
     const int nfast = zfast ? (nz + 63) / 64 : (nbin + 255) / 256;
-    const long long nblocks = zfast ? (long long)nfast * ((nbin + 3) / 4) : (long long)nfast * nz;
+    const long long nblocks = zfast ? (long long)nfast * ((nbin + 4 * kZBins - 1) / (4 * kZBins))
+                                    : (long long)nfast * nz;
     if (nblocks > 0x7fffffffLL) return rr::fail(RR_EINVAL, "rr_rebin: grid too large");
     dim3 block(256), grid((unsigned)nblocks);
     hipStream_t s = static_cast<hipStream_t>(stream);
