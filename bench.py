@@ -700,9 +700,11 @@ def bench_petal(args):
     mix.run()
     f64.run()
     torch.cuda.synchronize()
+    host = mix.host()                      # new host arrays on every call, as for every petal
     t = time.perf_counter()
-    host = mix.host()
-    t_d2h = time.perf_counter() - t
+    for _ in range(args.steps):
+        host = mix.host()
+    t_d2h = (time.perf_counter() - t) / args.steps
 
     c_mix = host[0].reshape(nt, nz)
     c_f64 = f64.zchi2.reshape(nt, nz).cpu().numpy()
@@ -840,7 +842,8 @@ def bench_petal(args):
         'end_to_end': {'results_to_host_ms': t_d2h * 1e3,
                        'ms_per_petal': (t_mix + t_pack + t_d2h) * 1e3,
                        'value': evals / (t_mix + t_pack + t_d2h),
-                       'note': 'packing + scan + copy of zchi2, zcoeff, penalty, status to host numpy; '
+                       'note': 'packing + scan + copy of zchi2, zcoeff, penalty, status to new host numpy '
+                               'arrays (mean of several petals); '
                                'template caches and target upload (setup_s) not included'},
         'roofline': {'bound': 'mfma', 'achieved': ach / 1e12, 'peak': PEAK_FLOPS[args.precision] / 1e12,
                      'unit': 'TFLOP/s', 'frac': ach / PEAK_FLOPS[args.precision],

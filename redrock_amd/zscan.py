@@ -399,7 +399,8 @@ class TargetBatch(object):
 # buffers kept for the process: a small thread pool fills one (numpy copies
 # release the GIL) while the asynchronous copy of the other runs.
 STAGE_BYTES = 64 << 20
-UPLOAD_THREADS = 2
+UPLOAD_THREADS = 2        # filling staging from many small host arrays: more contend (measured)
+DOWNLOAD_THREADS = 4      # copying staging out into new arrays: 5.1 -> 4.0 ms for 96 MB
 _STAGING = []
 _POOL = []
 _STAGE_LOCK = threading.Lock()        # one upload at a time uses the staging buffers
@@ -413,13 +414,16 @@ def _staging(k):
     return _STAGING[k]
 
 
-def _copy_pool():
-    """(thread pool, its number of threads) for the staging copies."""
-    if not _POOL:
-        import concurrent.futures
-        n = max(1, min(UPLOAD_THREADS, len(os.sched_getaffinity(0))))
-        _POOL.append((concurrent.futures.ThreadPoolExecutor(max_workers=n), n))
-    return _POOL[0]
+def _copy_pool(nthreads=None):
+    """(thread pool, its number of threads) for the staging copies: one pool
+    per thread count (UPLOAD_THREADS by default)."""
+    n = max(1, min(UPLOAD_THREADS if nthreads is None else nthreads, len(os.sched_getaffinity(0))))
+    for pool, m in _POOL:
+        if m == n:
+            return pool, m
+    import concurrent.futures
+    _POOL.append((concurrent.futures.ThreadPoolExecutor(max_workers=n), n))
+    return _POOL[-1]
 
 
 def _upload_rows(dst, rows):
@@ -482,7 +486,7 @@ def download(tensors, stream=None):
         chunks += [(ti, off, min(STAGE_BYTES, nbytes - off)) for off in range(0, nbytes, STAGE_BYTES)]
     if not chunks:
         return outs
-    pool, nthr = _copy_pool()
+    pool, nthr = _copy_pool(DOWNLOAD_THREADS)
 
     def finish(j, ti, off, nb):
         buf, ev = _staging(j % 2)
