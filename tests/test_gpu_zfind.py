@@ -208,3 +208,22 @@ def test_scan_templates_pipelined_under_memory_pressure(gpu, monkeypatch):
         for t in targets:
             for k in ('zchi2', 'penalty', 'zcoeff'):
                 assert np.array_equal(res_a[t.id][ft][k], res_b[t.id][ft][k])
+
+
+def test_zfind_two_layouts_matches_oracle(gpu):
+    """Targets of two wavelength layouts (three cameras, and two of them): one
+    target batch per layout, scanned and refined per layout (the path beside
+    the pipelined one-layout scan), against the oracle pipeline."""
+    from redrock_amd import synth, zfind as zf
+    from redrock_amd.targets import Target, DistTargetsCopy
+    targets, dts, dwave, _ = _setup()
+    more, _ = synth.make_targets(synth.galaxy_template(), 2, seed=32, zrange=(0.40, 0.55), snr=0.7)
+    two = [Target(100 + i, t.spectra[:2]) for i, t in enumerate(more)]
+    mixed = [targets[0], two[0], targets[1], two[1], targets[2]]
+    print('case: zfind, 5 targets in two wavelength layouts', flush=True)
+    _, got = zf.zfind(DistTargetsCopy(mixed), dts, nminima=3, precision='f64')
+    want = _oracle_zfind(mixed, dts, dwave, 3)
+    for k in ('targetid', 'spectype', 'subtype', 'znum', 'npixels', 'zwarn'):
+        assert np.array_equal(got[k].astype(str), want[k].astype(str)), k
+    assert np.allclose(got['z'], want['z'], rtol=1e-9, atol=1e-12)
+    assert np.allclose(got['chi2'], want['chi2'], rtol=1e-9, atol=0)
