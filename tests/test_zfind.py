@@ -159,6 +159,37 @@ def test_assemble_table_equals_per_target_assembly(seed):
             assert np.array_equal(a[k], b[k]), k
 
 
+@pytest.mark.parametrize('seed', range(6))
+def test_assemble_table_from_columns(seed):
+    """zfind hands the fine fits to the table as flat columns per template
+    (fitz.fitz_columns: ids, per-target counts, columns target by target);
+    the table equals the per-target assembly of the same fits."""
+    import copy
+    res, tids, templates = _fake_results(seed)
+    rng = np.random.default_rng(200 + seed)
+    if seed % 2:
+        for tid in tids:
+            for ft in res[tid]:
+                c = res[tid][ft]['zfit']['chi2']
+                c[rng.random(c.shape) < 0.4] = 9999999.0
+    for nmin in (1, 3):
+        want = zf._assemble(copy.deepcopy(res), tids, templates, nminima=nmin)
+        blocks = dict()
+        cols_res = copy.deepcopy(res)
+        for ft in res[tids[0]]:
+            per = [cols_res[tid][ft].pop('zfit') for tid in tids]
+            nacc = np.array([len(z['chi2']) for z in per], dtype=np.int64)
+            cols = {k: np.concatenate([z[k] for z in per]) for k in per[0]}
+            cols = {k: (v.reshape(-1) if k != 'coeff' and v.ndim == 2 and v.shape[1] == 1 else v)
+                    for k, v in cols.items()}
+            blocks[ft] = (list(tids), nacc, cols)
+        got = zf._assemble_table(cols_res, tids, templates, nminima=nmin, blocks=blocks)
+        assert list(got.keys()) == list(want.keys())
+        for k in want:
+            assert got[k].dtype == want[k].dtype, k
+            assert np.array_equal(got[k], want[k]), k
+
+
 def test_assemble_known_answer():
     """Hand-computed: nminima trim per spectype, bad fits last, flags."""
     def zfit(zs, chi2s, zwarn, nb, npix):
