@@ -127,3 +127,33 @@ def test_select_minima_no_targets():
     zs = np.linspace(0.1, 1.0, 10)
     nacc, acc = select_minima(np.zeros((0, 10)), zs, 3, constants.max_velo_diff, _Refine(zs), 2)
     assert nacc.size == 0 and acc['store'] is None
+
+
+def test_columns_and_dicts_equal_stacked_results():
+    """fitz_targets' per-target dicts (flat columns viewed per target) equal
+    the reference's stacking of each target's accepted minima (fitz.py:359-395:
+    sorted by chi2, (n, 1) columns, the empty-result dict)."""
+    from types import SimpleNamespace
+    from redrock_amd.fitz import _as_dicts, _columns, _stack
+    zs, zchi2 = _problem(nt=12)
+    zchi2[7] = 0.0 * zchi2[7] - np.arange(zs.size)          # one minimum at the last redshift
+    nacc, acc = select_minima(zchi2, zs, 3, constants.max_velo_diff, _Refine(zs), 1)
+    st = acc['store']
+    tm = SimpleNamespace(nbasis=3, method='PCA')
+    npix = list(range(100, 112))
+    got = _as_dicts(*_columns(acc, nacc, st, tm, npix, NZF))
+    for t in range(zchi2.shape[0]):
+        rows = [dict(z=acc['z'][t, k], zerr=acc['zerr'][t, k], zwarn=int(acc['zwarn'][t, k]),
+                     chi2=acc['chi2'][t, k], zz=st['zz'][acc['row'][t, k]],
+                     zzchi2=st['zzchi2'][acc['row'][t, k]], coeff=st['coeff'][acc['row'][t, k]],
+                     fitmethod='PCA') for k in range(nacc[t])]
+        want = _stack(rows, None, npix[t])
+        assert list(got[t].keys()) == list(want.keys())
+        for k in want:
+            assert got[t][k].shape == want[k].shape, (t, k)
+            assert got[t][k].dtype == want[k].dtype, (t, k)
+            assert np.array_equal(got[t][k], want[k]), (t, k)
+    # a target without minima gets the reference's empty dict
+    empty = _as_dicts(np.zeros(1, dtype=np.int64), {k: v[:0] for k, v in
+                                                    _columns(acc, nacc, st, tm, npix, NZF)[1].items()})[0]
+    assert empty['z'].shape == (0, 1) and empty['fitmethod'].dtype.kind == 'U'
