@@ -451,7 +451,9 @@ typedef struct {
 } rr_grad_desc;
 
 /* Workspace rr_zscan_grad needs for this descriptor: 0 without d_dchi2_dy,
- * else per item one double per pixel and one per template sample. */
+ * else per item one double per pixel and one per template sample, and
+ * min(nitems, 64) x nbasis doubles per template sample (the partial sums of
+ * the fixed-order reduction over items). */
 long long rr_grad_workspace_bytes(const rr_grad_desc *desc);
 
 /*

@@ -24,7 +24,8 @@
 //   flux_rows_kernel: per item, every template sample k of the item's span
 //     sums v_q K_qk over the pixels q whose bin reaches it, in pixel order
 //     (one thread writes each sample), into the item's row P_i;
-//   flux_reduce_kernel: dchi2_dy[b][k] += sum_i c_ib P_i[k], items in order.
+//   flux_partial_kernel, flux_sum_kernel: dchi2_dy[b][k] += sum_i c_ib P_i[k],
+//     items in 64 chunks summed in item order, the chunk sums in chunk order.
 #include "common.hpp"
 #include "zscan_args.hpp"
 
@@ -316,18 +317,40 @@ __global__ __launch_bounds__(256) void flux_rows_kernel(GradArgs a) {
     }
 }
 
-// dchi2_dy[b][k] += sum_i c_ib P_i[k], one thread per sample, items in order.
-__global__ __launch_bounds__(256) void flux_reduce_kernel(GradArgs a) {
+// dchi2_dy[b][k] += sum_i c_ib P_i[k] in two passes with a fixed order: the
+// items in kRedChunks consecutive chunks, each summed in item order by its own
+// workgroups (one thread per sample), then the chunks' sums added in chunk
+// order.  (One pass over all items had ~80 workgroups for the chip, each
+// thread a chain of nitems dependent loads.)
+constexpr int kRedChunks = 64;
+
+__host__ __device__ inline int red_chunks(int nitems) {
+    return nitems < kRedChunks ? (nitems > 0 ? nitems : 1) : kRedChunks;
+}
+
+__global__ __launch_bounds__(256) void flux_partial_kernel(GradArgs a, double *part) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int S = red_chunks(a.nitems), sc = blockIdx.y;
     if (k >= a.nx) return;
+    const int i0 = (int)((long long)sc * a.nitems / S), i1 = (int)((long long)(sc + 1) * a.nitems / S);
     double acc[RR_MAX_NBASIS];
     for (int b = 0; b < a.nb; ++b) acc[b] = 0.0;
-    for (int i = 0; i < a.nitems; ++i) {
+    for (int i = i0; i < i1; ++i) {
         const double p = a.P[(long long)i * a.nx + k];
         const double *c = a.item_coeff + (long long)i * a.nb;
         for (int b = 0; b < a.nb; ++b) acc[b] = fma(c[b], p, acc[b]);
     }
-    for (int b = 0; b < a.nb; ++b) a.dchi2_dy[(long long)b * a.nx + k] += acc[b];
+    for (int b = 0; b < a.nb; ++b) part[((long long)sc * a.nb + b) * a.nx + k] = acc[b];
+}
+
+__global__ __launch_bounds__(256) void flux_sum_kernel(GradArgs a, const double *part) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int b = blockIdx.y;
+    if (k >= a.nx) return;
+    const int S = red_chunks(a.nitems);
+    double acc = 0.0;
+    for (int sc = 0; sc < S; ++sc) acc += part[((long long)sc * a.nb + b) * a.nx + k];
+    a.dchi2_dy[(long long)b * a.nx + k] += acc;
 }
 
 }  // namespace
@@ -336,7 +359,8 @@ extern "C" long long rr_grad_workspace_bytes(const rr_grad_desc *d) {
     if (!d || d->nitems < 0 || d->npix < 1 || d->nx < 2)
         return rr::fail(RR_EINVAL, "rr_grad_workspace_bytes: bad descriptor");
     if (!d->d_dchi2_dy) return 0;
-    return rr::align256((long long)d->nitems * d->npix * 8) + rr::align256((long long)d->nitems * d->nx * 8);
+    return rr::align256((long long)d->nitems * d->npix * 8) + rr::align256((long long)d->nitems * d->nx * 8) +
+           rr::align256((long long)red_chunks(d->nitems) * d->nb * d->nx * 8);
 }
 
 extern "C" int rr_zscan_grad(const rr_grad_desc *d, void *stream) {
@@ -376,6 +400,14 @@ extern "C" int rr_zscan_grad(const rr_grad_desc *d, void *stream) {
     hipLaunchKernelGGL(flux_rows_kernel, dim3(d->nitems), dim3(256), 0, st, a);
     rc = rr::check_launch("rr_zscan_grad(flux rows)");
     if (rc) return rc;
-    hipLaunchKernelGGL(flux_reduce_kernel, dim3((d->nx + 255) / 256), dim3(256), 0, st, a);
-    return rr::check_launch("rr_zscan_grad(flux reduce)");
+    double *part = reinterpret_cast<double *>(
+        static_cast<char *>(d->d_work) + rr::align256((long long)d->nitems * d->npix * 8) +
+        rr::align256((long long)d->nitems * d->nx * 8));
+    hipLaunchKernelGGL(flux_partial_kernel, dim3((d->nx + 255) / 256, red_chunks(d->nitems)), dim3(256), 0,
+                       st, a, part);
+    rc = rr::check_launch("rr_zscan_grad(flux partial sums)");
+    if (rc) return rc;
+    hipLaunchKernelGGL(flux_sum_kernel, dim3((d->nx + 255) / 256, d->nb), dim3(256), 0, st, a,
+                       static_cast<const double *>(part));
+    return rr::check_launch("rr_zscan_grad(flux sums)");
 }
