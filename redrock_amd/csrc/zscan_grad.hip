@@ -51,16 +51,15 @@ struct GradArgs {
 
 // Weight of template sample ks in the bin mean of [elo, ehi] at 1+z = s:
 // the coefficient bin_value's linear map gives y[ks] (the same expressions),
-// 0 when the bin does not reach it.
+// 0 when the bin does not reach it.  The caller passes only bins with
+// elo < x[min(ks + 1, nx - 1)] s, so the first sample beyond the lower edge
+// (rebin.py:142) is at most that index and is found walking down from it
+// (a bin holds a few samples) — the index a binary search returns.
 __device__ double bin_weight(const GradArgs &a, double s, double elo, double ehi, int ks) {
     const double *x = a.x;
     const int nx = a.nx;
-    int lo = 0, hi = nx;
-    while (lo < hi) {
-        const int mid = (lo + hi) >> 1;
-        if (x[mid] * s <= elo) lo = mid + 1;
-        else hi = mid;
-    }
+    int lo = ks + 1 < nx - 1 ? ks + 1 : nx - 1;
+    while (lo > 0 && x[lo - 1] * s > elo) --lo;
     const int j = lo < 1 ? 1 : (lo > nx - 1 ? nx - 1 : lo);
     const double xj = x[j] * s, xjm1 = x[j - 1] * s;
     const double width = ehi - elo;
