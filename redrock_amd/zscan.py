@@ -252,7 +252,14 @@ class TemplateCache(object):
                            rows * nb, nb, 1, ot, 1.0, self.lyman(k), d['wave'][k])
         elif precision == 'f64':
             ldz = _roundup(max(nz, 1), 64)
-            buf = torch.zeros(nb * rows * ldz, dtype=torch.float64, device='cuda')
+            # zeros only where the rebin writes nothing (halo rows, rows after
+            # the pixels, redshift padding): a fine-grid cache is written once
+            buf = torch.empty(nb * rows * ldz, dtype=torch.float64, device='cuda')
+            v = buf.view(nb, rows, ldz)
+            h, e = layout.halo, layout.halo + layout.npix
+            v[:, :h].zero_()
+            v[:, e:].zero_()
+            v[:, h:e, nz:].zero_()
             for k, off in zip(layout.keys, layout.offsets):
                 rebin_into(buf[(layout.halo + off) * ldz:], d['x'], d['y'], d['edges'][k], d['z'],
                            1, ldz, rows * ldz, _hip.RR_F64, 1.0, self.lyman(k), d['wave'][k])

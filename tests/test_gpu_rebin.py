@@ -104,9 +104,16 @@ def test_cache_layouts_agree(gpu):
     zsel = tm.redshifts[100:140]
     cache = TemplateCache(tm, grids, redshifts=zsel)
     lay = cache.layout(list(grids))
+    # the f64 cache zeroes only what the rebin does not write: hand it NaN memory
+    ldz = 64
+    junk = torch.full((tm.nbasis * lay.cache_rows * ldz,), float('nan'), dtype=torch.float64, device='cuda')
+    del junk
     t64, ldz = cache.get(lay, 'f64')
     t16, _ = cache.get(lay, 'f16')
-    a = t64.reshape(tm.nbasis, lay.cache_rows, ldz)[:, :, :len(zsel)].permute(2, 0, 1)
+    full = t64.reshape(tm.nbasis, lay.cache_rows, ldz)
+    assert not torch.any(torch.isnan(full))
+    assert torch.all(full[:, lay.halo + lay.npix:] == 0) and torch.all(full[:, :, len(zsel):] == 0)
+    a = full[:, :, :len(zsel)].permute(2, 0, 1)
     n16 = len(zsel) * tm.nbasis * lay.cache_rows
     assert torch.all(t16[n16:] == 0) and t16.numel() >= n16 + 64   # zero block after the cache
     b = t16[:n16].view(torch.float16).reshape(len(zsel), tm.nbasis, lay.cache_rows).double() / cache.tscale
