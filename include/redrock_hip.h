@@ -20,7 +20,7 @@
 extern "C" {
 #endif
 
-#define RR_ABI_VERSION 12
+#define RR_ABI_VERSION 13
 
 /* Error codes. */
 #define RR_OK 0
@@ -548,6 +548,18 @@ long long rr_arch_workspace_bytes(const rr_arch_desc *desc);
  * rebin.py:332-341) for the fused path; searches are clamped in bounds.
  */
 int rr_arch_fit(const rr_arch_desc *desc, void *stream);
+
+/*
+ * rr_minima_head — host function (no device work): the first k entries of
+ * find_minima(x[t]) (fitz.py:46-67: local minima, edges included, sorted by
+ * value) for every row t of the nt x nz array x (row stride ldx doubles),
+ * the candidates the fine-fit loop walks first (fitz.py:198-210).
+ * head: nt x k, -1 past a row's candidates; ncand[t]: the row's number of
+ * candidates; flag[t] = 1 where ties or non-finite values could leave the
+ * order to the sort's tie handling (the caller takes find_minima there).
+ */
+int rr_minima_head(const double *x, long long nt, long long nz, long long ldx, int k,
+                   long long *head, long long *ncand, unsigned char *flag);
 
 #ifdef __cplusplus
 }

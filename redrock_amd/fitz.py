@@ -43,6 +43,38 @@ def find_minima(x):
     return ii[jj]
 
 
+def minima_head(x, k):
+    """The first k candidates of find_minima for every row of a 2-D x:
+    (head, ncand) with head[t, :min(k, ncand[t])] == find_minima(x[t])[:k]
+    (-1 after) and ncand[t] = find_minima(x[t]).size.  One host pass per row
+    in the library (rr_minima_head); rows where ties or non-finite values
+    could leave the order to np.argsort's tie handling take find_minima."""
+    from . import _hip
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    nt, nz = x.shape
+    k = int(k)
+    head = np.empty((nt, k), dtype=np.int64)
+    ncand = np.empty(nt, dtype=np.int64)
+    flag = np.empty(nt, dtype=np.uint8)
+    if nt:
+        rc = _hip.lib().rr_minima_head(x.ctypes.data, nt, nz, nz, k, head.ctypes.data,
+                                       ncand.ctypes.data, flag.ctypes.data)
+        _hip.check(rc, 'rr_minima_head')
+    for t in np.flatnonzero(flag):
+        c = find_minima(x[t])[:k]
+        head[t] = -1
+        head[t, :c.size] = c
+    return head, ncand
+
+
+def _ranges(b, n):
+    """(i, k): for every i, k runs over b[i], ..., b[i] + n[i] - 1, concatenated."""
+    n = np.asarray(n, dtype=np.int64)
+    i = np.repeat(np.arange(n.size), n)
+    k = np.arange(i.size, dtype=np.int64) - np.repeat(np.cumsum(n) - n, n) + np.repeat(b, n)
+    return i, k
+
+
 def minfit_batch(x, y):
     """minfit of many 3-point sets at once: x, y (K, 3), x increasing in each
     row.  Returns arrays (x0, xerr, y0, zwarn) of length K.
@@ -357,17 +389,15 @@ def select_minima(zchi2, redshifts, nminima, max_velo_diff, refine, speculate=1)
     zchi2 = np.atleast_2d(np.asarray(zchi2, dtype=np.float64))
     zs = np.asarray(redshifts, dtype=np.float64)
     nt = zchi2.shape[0]
-    cands = [find_minima(zchi2[t]) for t in range(nt)]
-    ncand = np.array([c.size for c in cands], dtype=np.int64)
     W = max(1, nminima + speculate)
+    head, ncand = minima_head(zchi2, W)
     # candidate positions -> coarse index (listed up to filled[t]), and ->
     # store row once refined
     width = int(min(W, ncand.max())) if nt else 0
     cmat = np.full((nt, max(width, 1)), -1, dtype=np.int64)
+    cmat[:, :width] = head[:, :width]
     rowmat = np.full(cmat.shape, -1, dtype=np.int64)
     filled = np.minimum(ncand, width)
-    for t, c in enumerate(cands):
-        cmat[t, :filled[t]] = c[:filled[t]]
 
     def fill(ts, upto):
         nonlocal cmat, rowmat
@@ -376,10 +406,14 @@ def select_minima(zchi2, redshifts, nminima, max_velo_diff, refine, speculate=1)
             grow = int(upto.max()) - cmat.shape[1] + W
             cmat = np.pad(cmat, ((0, 0), (0, grow)), constant_values=-1)
             rowmat = np.pad(rowmat, ((0, 0), (0, grow)), constant_values=-1)
-        for t, e in zip(ts, upto):
-            if e > filled[t]:
-                cmat[t, filled[t]:e] = cands[t][filled[t]:e]
-                filled[t] = e
+        more = upto > filled[ts]
+        ts, upto = ts[more], upto[more]
+        if ts.size == 0:
+            return
+        h, _ = minima_head(zchi2[ts], int(upto.max()))
+        i, k = _ranges(filled[ts], upto - filled[ts])
+        cmat[ts[i], k] = h[i, k]
+        filled[ts] = upto
 
     parts, nstore = [], 0
     kpos = np.zeros(nt, dtype=np.int64)
@@ -390,8 +424,7 @@ def select_minima(zchi2, redshifts, nminima, max_velo_diff, refine, speculate=1)
     lo_lim, hi_lim = zs[min(1, zs.size - 1)], zs[max(zs.size - 2, 0)]
 
     # first request: the first W candidates of every target
-    tt = np.repeat(np.arange(nt), np.minimum(ncand, W))
-    kk = np.concatenate([np.arange(min(c, W)) for c in ncand]) if nt else np.zeros(0, np.int64)
+    tt, kk = _ranges(np.zeros(nt, dtype=np.int64), np.minimum(ncand, W))
     store = None
     while True:
         if tt.size:
@@ -454,8 +487,8 @@ def select_minima(zchi2, redshifts, nminima, max_velo_diff, refine, speculate=1)
         # the next W candidates of every waiting target
         need = np.minimum(kpos[wait] + W, ncand[wait])
         fill(wait, need)
-        tt = np.repeat(wait, need - kpos[wait])
-        kk = np.concatenate([np.arange(b, e) for b, e in zip(kpos[wait], need)])
+        i, kk = _ranges(kpos[wait], need - kpos[wait])
+        tt = wait[i]
         unref = rowmat[tt, kk] < 0
         tt, kk = tt[unref], kk[unref]
     acc['store'] = store

@@ -157,3 +157,36 @@ def test_columns_and_dicts_equal_stacked_results():
     empty = _as_dicts(np.zeros(1, dtype=np.int64), {k: v[:0] for k, v in
                                                     _columns(acc, nacc, st, tm, npix, NZF)[1].items()})[0]
     assert empty['z'].shape == (0, 1) and empty['fitmethod'].dtype.kind == 'U'
+
+
+@pytest.mark.parametrize('nz', [1, 2, 7, 300])
+def test_minima_head_equals_find_minima(nz):
+    """The first k candidates of every row (rr_minima_head, a host function
+    of the library) equal find_minima's, order included: distinct values,
+    ties (flat rows, repeated values: np.argsort's order), NaN and infinite
+    values, k beyond a row's candidates and beyond the library's 64."""
+    from redrock_amd.fitz import minima_head
+    rng = np.random.default_rng(nz)
+    x = rng.normal(size=(60, nz)).cumsum(axis=1)
+    x[1] = 3.0                                        # all tied
+    x[2] = np.round(x[2])                             # many ties
+    x[3, ::3] = np.nan
+    x[4, ::5] = np.inf
+    x[5] = -x[5]
+    x[6, ::2] = -np.inf
+    x[7] = np.round(rng.normal(size=nz) * 2) * 0.5    # ties between separate minima
+    if nz >= 40:                                      # minima 5, 1, 3, 2, 3 between peaks:
+        x[8] = 10.0                                   # a tie at the cut of a 3-long head
+        x[8, 4:40:8] = [5.0, 1.0, 3.0, 2.0, 3.0]
+        x[9, 17:33] = x[9, 17:33].min() - 1.0         # a flat run holding the lowest values
+    for k in (0, 1, 3, 4, 9, 70):
+        head, ncand = minima_head(x, k)
+        assert head.shape == (60, k)
+        for t in range(x.shape[0]):
+            want = find_minima(x[t])
+            assert ncand[t] == want.size, (k, t)
+            n = min(k, want.size)
+            assert np.array_equal(head[t, :n], want[:k]), (k, t)
+            assert np.all(head[t, n:] == -1), (k, t)
+    h0, n0 = minima_head(np.zeros((0, nz)), 4)
+    assert h0.shape == (0, 4) and n0.size == 0
