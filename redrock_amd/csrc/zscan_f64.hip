@@ -33,8 +33,10 @@ namespace {
 // redshifts would leave most lanes of a one-item wave idle).  Every lane runs
 // the same per-pixel arithmetic in the same order for G = 4 and 16, so their
 // results do not depend on G.
-template <int NB, int P, int G>
-__global__ __launch_bounds__(256) void zscan_f64_kernel(rr::ZArgs a, long long ne_off) {
+// PF: load every basis vector's template window of a pixel block before the
+// first is used (launches too small to fill the chip; more registers)
+template <int NB, int P, int G, bool PF>
+__device__ __forceinline__ void zscan_f64_body(const rr::ZArgs &a, long long ne_off) {
     constexpr int LZ = 64 / G;           // lanes (redshifts) per item
     constexpr bool kSplit = G > 1;       // the 4 waves: one pixel quarter each
     const int lane = threadIdx.x & 63;
@@ -98,18 +100,39 @@ __global__ __launch_bounds__(256) void zscan_f64_kernel(rr::ZArgs a, long long n
 #pragma unroll
             for (int k = 0; k < 11; ++k) r[i][k] = Rt[(long long)(p0 + i) * 11 + k];
         double A[P][NB];
+        if constexpr (PF) {
+            // every basis vector's window loaded before the first is used
+            double win[NB][P + 10];
 #pragma unroll
-        for (int b = 0; b < NB; ++b) {
-            const double *col = Tz + b * bstride + (long long)p0 * ldz;
-            double win[P + 10];
+            for (int b = 0; b < NB; ++b) {
+                const double *col = Tz + b * bstride + (long long)p0 * ldz;
 #pragma unroll
-            for (int i = 0; i < P + 10; ++i) win[i] = col[i * ldz];
+                for (int i = 0; i < P + 10; ++i) win[b][i] = col[i * ldz];
+            }
 #pragma unroll
-            for (int i = 0; i < P; ++i) {
-                double s = r[i][0] * win[i];
+            for (int b = 0; b < NB; ++b) {
 #pragma unroll
-                for (int k = 1; k < 11; ++k) s = fma(r[i][k], win[i + k], s);
-                A[i][b] = s;
+                for (int i = 0; i < P; ++i) {
+                    double s = r[i][0] * win[b][i];
+#pragma unroll
+                    for (int k = 1; k < 11; ++k) s = fma(r[i][k], win[b][i + k], s);
+                    A[i][b] = s;
+                }
+            }
+        } else {
+#pragma unroll
+            for (int b = 0; b < NB; ++b) {
+                const double *col = Tz + b * bstride + (long long)p0 * ldz;
+                double win[P + 10];
+#pragma unroll
+                for (int i = 0; i < P + 10; ++i) win[i] = col[i * ldz];
+#pragma unroll
+                for (int i = 0; i < P; ++i) {
+                    double s = r[i][0] * win[i];
+#pragma unroll
+                    for (int k = 1; k < 11; ++k) s = fma(r[i][k], win[i + k], s);
+                    A[i][b] = s;
+                }
             }
         }
 #pragma unroll
@@ -165,6 +188,24 @@ __global__ __launch_bounds__(256) void zscan_f64_kernel(rr::ZArgs a, long long n
     }
 }
 
+template <int NB, int P, int G>
+__global__ __launch_bounds__(256) void zscan_f64_kernel(rr::ZArgs a, long long ne_off) {
+    zscan_f64_body<NB, P, G, false>(a, ne_off);
+}
+
+// The same arithmetic for launches too small to fill the chip (fewer
+// workgroups than CUs): one wave per SIMD, every basis vector's window loads
+// of a pixel block in flight together instead of one round trip per basis
+// vector.  Same operations in the same order: the same doubles.
+template <int NB, int P, int G>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1)))
+void zscan_f64_small_kernel(rr::ZArgs a, long long ne_off) {
+    zscan_f64_body<NB, P, G, true>(a, ne_off);
+}
+
+// launches of fewer workgroups than this take zscan_f64_small_kernel
+constexpr long long kSmallLaunch = 256;
+
 template <int NB, int G>
 int launch_g(const rr::ZArgs &a, hipStream_t s) {
     constexpr int LZ = 64 / G;
@@ -172,7 +213,10 @@ int launch_g(const rr::ZArgs &a, hipStream_t s) {
     const int per_wg = G == 1 ? 4 : G;
     dim3 grid((a.max_item_nz + LZ - 1) / LZ, (a.nitems + per_wg - 1) / per_wg);
     const long long ne_off = rr::align256((long long)a.nitems * 2 * 8);
-    hipLaunchKernelGGL((zscan_f64_kernel<NB, 2, G>), grid, dim3(256), 0, s, a, ne_off);
+    if ((long long)grid.x * grid.y < kSmallLaunch)
+        hipLaunchKernelGGL((zscan_f64_small_kernel<NB, 2, G>), grid, dim3(256), 0, s, a, ne_off);
+    else
+        hipLaunchKernelGGL((zscan_f64_kernel<NB, 2, G>), grid, dim3(256), 0, s, a, ne_off);
     return rr::check_launch("rr_zscan(f64)");
 }
 

@@ -86,6 +86,27 @@ def test_f64_short_items_match_full_grid(galaxy):
     assert np.array_equal(rows['g4'][1][:n16], rows['g16'][1][:n16])
 
 
+def test_f64_small_launch_equals_large(galaxy):
+    """Short-item launches of fewer than 256 workgroups take a variant that
+    loads every basis vector's template window before using the first; rows
+    equal the large launch's bit for bit (16 and 4 items per wavefront)."""
+    from redrock_amd import zscan
+    tm, targets, dt, *_ = galaxy
+    batch = zscan.TargetBatch(targets)
+    nt, nz = len(targets), dt.cache.nz
+    rng = np.random.default_rng(11)
+    for lo, hi, n in ((1, 5, 4200), (5, 17, 1100)):     # >= 256 workgroups of 16 / 4 items
+        it = rng.integers(0, nt, size=n)
+        L = rng.integers(lo, hi, size=n)
+        z0 = rng.integers(0, nz - L + 1)
+        print(f'case: f64 short items of {lo}-{hi - 1} redshifts, {n} items vs the first 100', flush=True)
+        big = zscan.ScanPlan(dt.cache, batch, it, z0, L, 'f64').run()
+        small = zscan.ScanPlan(dt.cache, batch, it[:100], z0[:100], L[:100], 'f64').run()
+        m = int(L[:100].sum())
+        assert np.array_equal(small.zchi2.cpu().numpy(), big.zchi2.cpu().numpy()[:m])
+        assert np.array_equal(small.zcoeff.cpu().numpy(), big.zcoeff.cpu().numpy()[:m])
+
+
 def test_f64_deterministic(galaxy):
     """Run-to-run bit identity (the reference checks mp_procs=1 vs 2, test_zscan.py:113-140)."""
     from redrock_amd import zscan
