@@ -256,9 +256,9 @@ def zfind(targets, templates, mp_procs=1, nminima=3, archetypes=None, priors=Non
         for t, zchi2, zcoeff, penalty in zscan.scan_templates_iter(local, templates, precision, one,
                                                                    polish):
             ft = t.template.full_type
-            for i, tg in enumerate(local):
-                results[tg.id][ft] = dict(redshifts=t.template.redshifts, zchi2=zchi2[i],
-                                          penalty=penalty[i], zcoeff=zcoeff[i])
+            zs = t.template.redshifts
+            for tg, c2, pn, cf in zip(local, list(zchi2), list(penalty), list(zcoeff)):
+                results[tg.id][ft] = dict(redshifts=zs, zchi2=c2, penalty=pn, zcoeff=cf)
             chi2 = zchi2 + penalty if priors is None else with_priors(ft, t.template.redshifts)
             timings['scan'] = clock() - t0
             with torch.cuda.stream(side):

@@ -315,7 +315,9 @@ class TargetBatch(object):
     array, each target's number of pixels with ivar > 0.
     """
 
-    def __init__(self, targets=None, layout=None, arrays=None):
+    def __init__(self, targets=None, layout=None, arrays=None, grouped=False):
+        """grouped: the targets are known to share `layout` (target_batches
+        grouped them by its signature), so they are not checked again."""
         import torch
         dev = torch.device('cuda')
         if arrays is not None:
@@ -328,7 +330,7 @@ class TargetBatch(object):
         else:
             layout = layout or PixelLayout.of(targets[0].spectra)
             nt, P = len(targets), layout.npix_pad
-            for tg in targets:
+            for tg in ([] if grouped else targets):
                 if PixelLayout.signature_of(tg.spectra) != layout.signature:
                     raise ValueError('targets of one batch must share their wavelength grids')
             self.rband = torch.zeros((nt, P, 11), dtype=torch.float64, device=dev)
@@ -1082,7 +1084,7 @@ def target_batches(targets):
     groups = dict()
     for tg in targets:
         groups.setdefault(PixelLayout.signature_of(tg.spectra), []).append(tg)
-    return {sig: TargetBatch(tgs) for sig, tgs in groups.items()}
+    return {sig: TargetBatch(tgs, layout=PixelLayout(*sig), grouped=True) for sig, tgs in groups.items()}
 
 
 def calc_zchi2_targets(targets, templates, mp_procs=1, use_gpu=False, precision=None, batches=None,
