@@ -20,7 +20,7 @@
 extern "C" {
 #endif
 
-#define RR_ABI_VERSION 13
+#define RR_ABI_VERSION 14
 
 /* Error codes. */
 #define RR_OK 0
@@ -560,6 +560,15 @@ int rr_arch_fit(const rr_arch_desc *desc, void *stream);
  */
 int rr_minima_head(const double *x, long long nt, long long nz, long long ldx, int k,
                    long long *head, long long *ncand, unsigned char *flag);
+
+/*
+ * rr_gather_rows — host function (no device work): dst[i * row_bytes ..] =
+ * the row_bytes bytes at src[i], for i < n, on nthreads threads.  The host
+ * half of a batch upload (spectral_data, zscan.py:192-209, for every target
+ * at once): per-target arrays into one pinned staging buffer before the DMA.
+ */
+int rr_gather_rows(const unsigned long long *src, long long n, long long row_bytes, void *dst,
+                   int nthreads);
 
 #ifdef __cplusplus
 }

@@ -12,7 +12,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 # RR_LIB_PATH: load the library from another path (a build kept elsewhere)
 LIB_PATH = os.environ.get('RR_LIB_PATH') or os.path.join(HERE, 'libredrock_hip.so')
 
-RR_ABI_VERSION = 13
+RR_ABI_VERSION = 14
 RR_F64, RR_F32, RR_F16, RR_BF16 = 0, 1, 2, 3
 RR_PREC_F64, RR_PREC_F16, RR_PREC_BF16, RR_PREC_F32, RR_PREC_F16X3, RR_PREC_F16R = 0, 1, 2, 3, 4, 5
 RR_STATUS_OK, RR_STATUS_SINGULAR = 0, 1
@@ -26,7 +26,8 @@ EXPORTS = ('rr_abi_version', 'rr_last_error', 'rr_rebin', 'rr_zscan',
            'rr_zscan_workspace_bytes', 'rr_pack_mixed', 'rr_zscan_tiles', 'rr_zscan_tile_table',
            'rr_zscan_grad', 'rr_arch_prep', 'rr_arch_workspace_bytes', 'rr_arch_fit',
            'rr_zscan_polish', 'rr_pack_lanes', 'rr_pack_lanes_bytes', 'rr_zscan_resid_tiles',
-           'rr_zscan_noalloc', 'rr_grad_workspace_bytes', 'rr_resid_u_slots', 'rr_minima_head')
+           'rr_zscan_noalloc', 'rr_grad_workspace_bytes', 'rr_resid_u_slots', 'rr_minima_head',
+           'rr_gather_rows')
 RR_NOALLOC_SCRATCH_BYTES = 64 << 20
 RR_ARCH_MAX_NA = 4
 RR_ARCH_MAX_NLEG = 12
@@ -142,6 +143,8 @@ def lib():
     L.rr_arch_fit.argtypes = [ctypes.POINTER(ArchDesc), _vp]
     L.rr_minima_head.restype = _i
     L.rr_minima_head.argtypes = [_vp, _ll, _ll, _ll, _i, _vp, _vp, _vp]
+    L.rr_gather_rows.restype = _i
+    L.rr_gather_rows.argtypes = [_vp, _ll, _ll, _vp, _i]
     v = L.rr_abi_version()
     if v != RR_ABI_VERSION:
         raise HipError(f'libredrock_hip ABI {v}, expected {RR_ABI_VERSION}')

@@ -218,3 +218,26 @@ def test_resid_u_operand_layout():
         assert np.all(buf[..., nprod * nb:] == 0)
     with pytest.raises(Exception):
         _hip.resid_u_slots(0)
+
+
+def test_gather_rows_copies_every_row():
+    """rr_gather_rows (host code, no GPU): rows from separate arrays land in
+    order in one buffer, for thread counts below, at and above the row count."""
+    import ctypes
+    import numpy as np
+    from redrock_amd import _hip
+    rng = np.random.default_rng(4)
+    rows = [rng.normal(size=(3, 37)) for _ in range(23)]
+    ptrs = np.array([r.ctypes.data for r in rows], dtype=np.uint64)
+    for nthreads in (1, 4, 23, 100):
+        out = np.zeros((23, 3, 37))
+        rc = _hip.lib().rr_gather_rows(ptrs.ctypes.data, 23, rows[0].nbytes, out.ctypes.data, nthreads)
+        assert rc == 0
+        assert np.array_equal(out, np.stack(rows))
+    assert _hip.lib().rr_gather_rows(ptrs.ctypes.data, 0, 8, None, 4) == 0
+    bad = ptrs.copy()
+    bad[5] = 0
+    out = np.zeros((23, 3, 37))
+    assert _hip.lib().rr_gather_rows(bad.ctypes.data, 23, rows[0].nbytes, out.ctypes.data, 4) < 0
+    assert b'row 5' in _hip.lib().rr_last_error()
+    assert _hip.lib().rr_gather_rows(ptrs.ctypes.data, -1, 8, out.ctypes.data, 4) < 0

@@ -41,6 +41,15 @@ def test_upload_rows_matches_host(gpu, monkeypatch):
     zscan._upload_rows(dst, rows)
     want = np.stack(rows).astype(np.float64)
     assert np.array_equal(dst.cpu().numpy(), want)
+    # float64 rows: copied by the library's threads (rr_gather_rows), several chunks
+    rows64 = [rng.normal(size=(11, 97)) for _ in range(301)]
+    dst64 = torch.empty((301, 11, 97), dtype=torch.float64, device='cuda')
+    zscan._upload_rows(dst64, rows64)
+    assert np.array_equal(dst64.cpu().numpy(), np.stack(rows64))
+    # a non-contiguous float64 row among them: every row through numpy
+    rows64[7] = np.asfortranarray(rows64[7])
+    zscan._upload_rows(dst64, rows64)
+    assert np.array_equal(dst64.cpu().numpy(), np.stack(rows64))
     # a 2-D host array of rows, and rows larger than a staging buffer
     big = rng.normal(size=(3, 9000))
     dst2 = torch.empty((3, 9000), dtype=torch.float64, device='cuda')
