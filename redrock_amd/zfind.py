@@ -365,17 +365,21 @@ def _assemble_table(allresults, target_ids, templates, nminima, blocks=None):
         blocks[ft] = (ids, nacc, cols)
     keys = None
     parts = []                            # (target position, template rank, candidate, ft, cols)
+    # a template's rank among each target's templates: one number when every
+    # target holds the templates in the same order (zfind fills them so)
+    ref = tuple(fts)
+    uniform = all(tuple(k for k in allresults[tid] if k != 'meta') == ref for tid in target_ids)
     for ft in fts:
         ids, nacc, cols = blocks[ft]
-        for tid, n in zip(ids, nacc):
-            if n == 0:
-                print(f'WARNING: no {ft} chi2 vs. z minima for targetid {tid}')
+        for i in np.flatnonzero(np.asarray(nacc) == 0):
+            print(f'WARNING: no {ft} chi2 vs. z minima for targetid {ids[i]}')
         if int(np.sum(nacc)) == 0:
             continue
         if keys is None:
             keys = list(cols.keys())
         tpos_b = np.repeat(np.array([pos[tid] for tid in ids], dtype=np.int64), nacc)
-        rank = np.array([list(k for k in allresults[tid] if k != 'meta').index(ft) for tid in ids])
+        rank = (np.full(len(ids), fts.index(ft), dtype=np.int64) if uniform else
+                np.array([list(k for k in allresults[tid] if k != 'meta').index(ft) for tid in ids]))
         cand = np.arange(tpos_b.size) - np.repeat(np.cumsum(nacc) - nacc, nacc)
         parts.append((tpos_b, np.repeat(rank, nacc), cand, ft, cols))
     if keys is None:

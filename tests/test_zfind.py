@@ -139,10 +139,13 @@ def test_assemble_table_equals_per_target_assembly(seed):
     """The all-targets-at-once assembly (zfind without archetypes) gives the
     per-target assembly's table: same columns in the same order, dtypes, rows
     and row order, also with exact chi2 ties (HUGE_CHI2 rows) across templates
-    and candidates, and for every nminima."""
+    and candidates, for every nminima, and with one target holding its
+    templates in another order (the rank of a template is per target)."""
     import copy
     res, tids, templates = _fake_results(seed)
     rng = np.random.default_rng(100 + seed)
+    if seed % 3 == 2:
+        res[tids[1]] = dict(reversed(list(res[tids[1]].items())))
     if seed % 2:
         for tid in tids:
             for ft in res[tid]:
